@@ -1,0 +1,337 @@
+/*
+ * pedalsky_hip.h -- C ABI of the MI355X (gfx950) sky + cloud transport library.
+ *
+ * This is the drop-in boundary for PedalSky's atmosphere + cloud passes. The
+ * reference reaches those passes through the Rust `RenderSubsystem` trait
+ * (crates/ps-core/src/subsystem.rs:66-118) implemented by
+ * `AtmosphereSubsystem` (crates/ps-atmosphere/src/lib.rs:119-700) and
+ * `CloudsSubsystem` (crates/ps-clouds/src/lib.rs:95-1279). Each function below
+ * names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Plain C: POD structs, pointers and sizes, no exceptions cross the ABI.
+ *   - Every call returns ps_status (0 = PS_OK). On failure ps_last_error()
+ *     returns a thread-local message describing the last failing call.
+ *   - A ps_ctx is not re-entrant: one context per thread, or serialise calls.
+ *   - "device" pointers are HIP device allocations on the context's device;
+ *     "host" pointers are ordinary CPU memory.
+ *   - Units are metres, radians, and the reference's photometric proxy units.
+ */
+#ifndef PEDALSKY_HIP_H
+#define PEDALSKY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS_ABI_VERSION 1
+/* MAX_CLOUD_LAYERS: crates/ps-clouds/src/params.rs:9, cloud_uniforms.wgsl:8 */
+#define PS_MAX_CLOUD_LAYERS 8
+
+typedef int32_t ps_status;
+enum {
+    PS_OK = 0,
+    PS_ERR_INVALID_ARGUMENT = 1,
+    PS_ERR_DEVICE = 2,
+    PS_ERR_OUT_OF_MEMORY = 3,
+    PS_ERR_NOT_READY = 4,            /* a prerequisite upload or bake is missing */
+    PS_ERR_OVERLAPPING_LAYERS = 5,   /* SynthesisError::OverlappingCloudLayers, ps-synthesis/src/state.rs:35-46 */
+    PS_ERR_UNSUPPORTED = 6
+};
+
+/* ------------------------------------------------------------------------ */
+/* Uniform structs, byte-identical to the reference's GPU uploads.           */
+/* ------------------------------------------------------------------------ */
+
+/* FrameUniformsGpu, crates/ps-core/src/frame_uniforms.rs:123-162 and
+ * shaders/common/uniforms.wgsl:11-33. 448 bytes; matrices column-major. */
+typedef struct ps_frame_uniforms {
+    float view[16];
+    float proj[16];
+    float view_proj[16];
+    float inv_view_proj[16];
+    float camera_position_world[4];
+    float camera_velocity_world[4];
+    float sun_direction[4];          /* xyz unit direction, w angular radius (rad) */
+    float sun_illuminance[4];        /* rgb illuminance proxy, w TOA lux */
+    float lightning_illuminance[4];  /* rgb emission, w falloff radius (m) */
+    float lightning_origin_world[4];
+    float viewport_size[4];          /* w, h, 1/w, 1/h */
+    float time_seconds;
+    float simulated_seconds;
+    uint32_t frame_index;
+    float ev100;
+    float prev_view_proj[16];
+} ps_frame_uniforms;
+
+/* AtmosphereParams (= WorldUniformsGpu), crates/ps-core/src/weather.rs:21-57,
+ * shaders/common/uniforms.wgsl:39-58. 144 bytes. */
+typedef struct ps_atmosphere_params {
+    float planet_radius_m;
+    float atmosphere_top_m;
+    float rayleigh_scale_height_m;
+    float mie_scale_height_m;
+    float rayleigh_scattering[4];
+    float mie_scattering[4];
+    float mie_absorption[4];
+    float mie_g;
+    float _pad_after_mie_g[3];
+    float ozone_absorption[4];
+    float ozone_center_m;
+    float ozone_thickness_m;
+    float _pad_after_ozone[2];
+    float ground_albedo[4];
+    float haze_extinction_per_m[4];
+} ps_atmosphere_params;
+
+/* CloudParamsGpu, crates/ps-clouds/src/params.rs:31-140 and
+ * shaders/clouds/cloud_uniforms.wgsl:23-83. 112 bytes. */
+typedef struct ps_cloud_params {
+    float sigma_s[3];
+    float _pad_after_sigma_s;
+    float sigma_a[3];
+    float droplet_diameter_bias;
+    uint32_t cone_light_sampling;
+    float forward_bias;
+    float detail_strength;
+    float curl_strength;
+    float powder_strength;
+    float multi_scatter_a;
+    float multi_scatter_b;
+    float multi_scatter_c;
+    float ambient_strength;
+    float base_scale_m;
+    float detail_scale_m;
+    float weather_scale_m;
+    uint32_t light_steps;
+    uint32_t cloud_steps;
+    uint32_t multi_scatter_octaves;
+    uint32_t cloud_layer_count;
+    uint32_t temporal_jitter;
+    float wind_drift_strength;
+    float wind_skew_strength;
+    float diurnal_strength;
+} ps_cloud_params;
+
+/* CloudLayerGpu, crates/ps-core/src/weather.rs:98-134 and
+ * shaders/clouds/cloud_uniforms.wgsl:85-106. 48 bytes. */
+typedef struct ps_cloud_layer {
+    float base_m;
+    float top_m;
+    float coverage;
+    float density_scale;
+    uint32_t cloud_type;             /* 0 Cu, 1 St, 2 Sc, 3 Ac, 4 As, 5 Ci, 6 Cs, 7 Cb */
+    float shape_bias;
+    float detail_bias;
+    float anvil_bias;
+    float droplet_diameter_um;
+    float _pad_after_droplets[3];
+} ps_cloud_layer;
+
+/* LUT resolutions. The reference fixes them in
+ * crates/ps-core/src/atmosphere_luts.rs:15-27 (256x64, 32x32, 192x108,
+ * 32x32x64); they are a run-time choice here so the 32^3 froxel volume the
+ * benchmark workloads use is expressible. */
+typedef struct ps_lut_dims {
+    int32_t transmittance_w, transmittance_h;
+    int32_t multiscatter_w, multiscatter_h;
+    int32_t skyview_w, skyview_h;
+    int32_t ap_w, ap_h, ap_d;
+} ps_lut_dims;
+
+/* Noise storage formats (crates/ps-clouds/src/noise.rs:75-88 uses Rgba8Unorm /
+ * Rg8Unorm; FP16 is the half-precision alternative). */
+enum { PS_NOISE_UNORM8 = 0, PS_NOISE_FP16 = 1 };
+
+/* Output pixel formats. */
+enum { PS_PIXEL_F32 = 0, PS_PIXEL_F16 = 1 };
+
+/* LUT selectors and bake flags (the four compute passes of
+ * crates/ps-atmosphere/src/lib.rs:464-492). */
+enum {
+    PS_LUT_TRANSMITTANCE = 1,
+    PS_LUT_MULTISCATTER = 2,
+    PS_LUT_SKYVIEW = 4,
+    PS_LUT_AERIAL_PERSPECTIVE = 8,
+    PS_LUT_ALL = 15,
+    /* multi_scattering tuning off: the MS LUT is cleared instead of baked
+     * (crates/ps-atmosphere/src/lib.rs:527-556) */
+    PS_LUT_MULTISCATTER_DISABLED = 16
+};
+
+/* Noise volume selectors for ps_read_noise. */
+enum { PS_NOISE_BASE = 0, PS_NOISE_DETAIL = 1, PS_NOISE_CURL = 2 };
+
+/* Render outputs; each pointer is a device buffer of W*H RGBA texels in
+ * `format`, or NULL to skip that output.
+ *   composite          sky pass followed by the dual-source cloud composite
+ *                      (sky.wgsl:159-220, cloud_composite.wgsl:47-57):
+ *                      rgb = cloud_lum + sky * cloud_trans, a = 1
+ *   cloud_luminance    attachment 0 of cloud_march.wgsl fs_main (AP applied)
+ *   cloud_transmittance attachment 1 of cloud_march.wgsl fs_main            */
+typedef struct ps_image_out {
+    void* composite;
+    void* cloud_luminance;
+    void* cloud_transmittance;
+    int32_t format;
+    int32_t _reserved;
+} ps_image_out;
+
+/* Algorithmic event counts of one render (for the roofline model). */
+typedef struct ps_work_counters {
+    uint64_t pixels;          /* pixels rendered */
+    uint64_t layer_marches;   /* (pixel, layer) segments marched */
+    uint64_t primary_steps;   /* primary march steps executed */
+    uint64_t covered_steps;   /* steps past the coverage gate (base noise fetched) */
+    uint64_t eroded_steps;    /* steps with cloud > 0 (curl + detail fetched) */
+    uint64_t dense_samples;   /* steps with density > 1e-3 (light march + shading) */
+    uint64_t light_steps;     /* light-march density evaluations */
+    uint64_t light_covered;   /* light-march evaluations past the gates (base fetched) */
+    uint64_t octaves;         /* multi-scatter octave evaluations */
+    uint64_t ap_taps;         /* pixels whose cloud luminance received AP */
+} ps_work_counters;
+
+/* Per-kernel device times of the last timed frame sequence, averaged per
+ * frame, in milliseconds (HIP events on the context stream). */
+typedef struct ps_frame_timing {
+    float transmittance_ms;
+    float multiscatter_ms;
+    float skyview_ms;
+    float aerial_perspective_ms;
+    float render_ms;
+    float frame_ms;           /* event-to-event time of one whole frame */
+    int32_t frames;
+    int32_t _reserved;
+} ps_frame_timing;
+
+typedef struct ps_ctx ps_ctx;
+
+/* ------------------------------------------------------------------------ */
+/* Context lifecycle. Replaces AtmosphereFactory::build / CloudsFactory::build
+ * (crates/ps-atmosphere/src/lib.rs:684-699, crates/ps-clouds/src/lib.rs:1258-1278),
+ * which allocate the subsystems' GPU resources.                             */
+/* ------------------------------------------------------------------------ */
+ps_status ps_ctx_create(int32_t device, ps_ctx** out_ctx);
+void ps_ctx_destroy(ps_ctx* ctx);
+/* Thread-local description of the last failure (never NULL). */
+const char* ps_last_error(void);
+int32_t ps_abi_version(void);
+/* Use an existing hipStream_t (NULL = the context's own stream). */
+ps_status ps_ctx_set_stream(ps_ctx* ctx, void* hip_stream);
+ps_status ps_ctx_synchronize(ps_ctx* ctx);
+/* LUT resolutions; reallocates the LUTs (atmosphere_luts.rs:15-27). */
+ps_status ps_ctx_set_lut_dims(ps_ctx* ctx, const ps_lut_dims* dims);
+ps_status ps_ctx_get_lut_dims(const ps_ctx* ctx, ps_lut_dims* out);
+
+/* ------------------------------------------------------------------------ */
+/* Per-frame state uploads. Replaces the bind-group-0/1 uniform writes
+ * (FrameWorldBindings::write, crates/ps-core/src/bind_groups.rs:140) and
+ * CloudsSubsystem::prepare (crates/ps-clouds/src/lib.rs:1105-1148).        */
+/* ------------------------------------------------------------------------ */
+ps_status ps_set_atmosphere(ps_ctx* ctx, const ps_atmosphere_params* atm);
+ps_status ps_set_frame(ps_ctx* ctx, const ps_frame_uniforms* frame);
+/* `layer_wind_xz`: for each layer, the (u east, v south) wind in m/s that
+ * layer_wind_mps (cloud_density_common.wgsl:264-271) reads from the wind
+ * field at the layer's mid altitude; NULL = calm. Layers must not overlap
+ * (ps-synthesis/src/cloud_layers.rs check_non_overlap). */
+ps_status ps_set_clouds(ps_ctx* ctx, const ps_cloud_params* params,
+                        const ps_cloud_layer* layers, int32_t n_layers,
+                        const float* layer_wind_xz);
+
+/* ------------------------------------------------------------------------ */
+/* Cloud inputs. Replaces CloudNoise::bake (crates/ps-clouds/src/noise.rs:71-216)
+ * and the weather textures ps_synthesis::synthesise uploads
+ * (crates/ps-synthesis/src/state.rs:50-185).                               */
+/* ------------------------------------------------------------------------ */
+/* Bake the 128^3 base, 32^3 detail and 128^2 curl volumes on the GPU
+ * (noise_base/detail/curl.comp.wgsl). Seed 0 reproduces the reference hash. */
+ps_status ps_generate_noise(ps_ctx* ctx, uint32_t seed, int32_t format);
+/* Upload pre-baked volumes: base 128^3 RGBA, detail 32^3 RGBA, curl 128^2 RG,
+ * each texel in `format` (uint8 UNORM or IEEE binary16 bits). */
+ps_status ps_upload_noise(ps_ctx* ctx, int32_t format, const void* base_rgba,
+                          const void* detail_rgba, const void* curl_rg);
+/* Read back a stored volume in its storage format (same layouts as upload). */
+ps_status ps_read_noise(ps_ctx* ctx, int32_t which, void* host_out, size_t bytes);
+/* 64x64 blue-noise ranks (R8Unorm, noise.rs:646-704). */
+ps_status ps_upload_blue_noise(ps_ctx* ctx, const uint8_t* ranks_64x64);
+/* Weather map: w*h texels of `channels` binary16 values; channel 0 is the
+ * coverage the march reads (cloud_march.wgsl:872-875). `cloud_type_grid`:
+ * 128x128 u8 (255 = use the layer's type) or NULL for all-sentinel.
+ * `overcast_mask`: 128x128 R8Unorm top-down mask (sky.wgsl:120-140) or NULL
+ * for zero. */
+ps_status ps_upload_weather(ps_ctx* ctx, const uint16_t* weather_f16, int32_t w,
+                            int32_t h, int32_t channels,
+                            const uint8_t* cloud_type_grid_128,
+                            const uint8_t* overcast_mask_128);
+
+/* ------------------------------------------------------------------------ */
+/* The four LUT passes (AtmosphereSubsystem::dispatch_pass,
+ * crates/ps-atmosphere/src/lib.rs:494-584).                                */
+/* ------------------------------------------------------------------------ */
+ps_status ps_bake_luts(ps_ctx* ctx, uint32_t flags);
+/* Copy one LUT to host as float RGBA (w*h[*d]*4 floats). */
+ps_status ps_read_lut(ps_ctx* ctx, int32_t which, float* host_out, size_t floats);
+
+/* ------------------------------------------------------------------------ */
+/* Sky pass + cloud march + composite (atmosphere::sky, clouds::march,
+ * clouds::composite: crates/ps-atmosphere/src/lib.rs:585-642,
+ * crates/ps-clouds/src/lib.rs:736-994, 1054-1097). `depth` is a device
+ * buffer of W*H reverse-Z NDC depths (0 = far, Depth32Float semantics of
+ * crates/ps-core/src/framebuffer.rs) or NULL for all-far.                  */
+/* ------------------------------------------------------------------------ */
+ps_status ps_render(ps_ctx* ctx, const float* depth, const ps_image_out* out);
+/* One whole frame: ps_bake_luts(flags) then ps_render. */
+ps_status ps_frame(ps_ctx* ctx, uint32_t lut_flags, const float* depth,
+                   const ps_image_out* out);
+/* Run `frames` frames back to back on the context stream and report average
+ * per-kernel device times measured with HIP events. */
+ps_status ps_time_frames(ps_ctx* ctx, int32_t frames, uint32_t lut_flags,
+                         const float* depth, const ps_image_out* out,
+                         ps_frame_timing* timing);
+/* Re-run the render with event counting on; fills the algorithmic counts. */
+ps_status ps_count_work(ps_ctx* ctx, const float* depth, ps_work_counters* out);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side synthesis helpers (CPU, no device work).                       */
+/* ------------------------------------------------------------------------ */
+/* AtmosphereParams::default (weather.rs:60-84), CloudParamsGpu::default
+ * (params.rs:142-213). */
+void ps_default_atmosphere(ps_atmosphere_params* out);
+void ps_default_cloud_params(ps_cloud_params* out);
+/* remap_coverage_to_visible_band, ps-synthesis/src/cloud_layers.rs:98-120 */
+float ps_remap_coverage(float scene_coverage);
+/* default_density_scale / default_droplet_diameter_um, ps-core/src/scene.rs:293-354 */
+float ps_default_density_scale(uint32_t cloud_type);
+float ps_default_droplet_diameter_um(uint32_t cloud_type);
+/* synthesise_cloud_layers for one scene layer, cloud_layers.rs:44-73. A NaN
+ * density_scale / anvil_bias / droplet_diameter_um selects the per-type
+ * default. */
+void ps_synthesise_cloud_layer(uint32_t cloud_type, float base_m, float top_m,
+                               float scene_coverage, float density_scale,
+                               float shape_bias, float detail_bias,
+                               float anvil_bias, float droplet_diameter_um,
+                               ps_cloud_layer* out);
+/* check_non_overlap, cloud_layers.rs:134-148: returns 0 when disjoint, else
+ * 1 and the first overlapping pair in out_i/out_j. */
+int32_t ps_check_layers_overlap(const ps_cloud_layer* layers, int32_t n,
+                                int32_t* out_i, int32_t* out_j);
+/* The wind-field tap layer_wind_mps performs for a layer spanning
+ * [base_m, top_m] under the procedural profile of
+ * ps-synthesis/src/wind_field.rs:71-178 (power law + Ekman veer, f16
+ * storage, linear filtering with repeat addressing). */
+void ps_layer_wind(float surface_speed_mps, float wind_dir_deg, float base_m,
+                   float top_m, float out_xz[2]);
+/* Uniform top-down overcast mask value, density_mask.rs:134-165 with the
+ * CPU NDF of ps-synthesis/src/ndf.rs:20-58. */
+uint8_t ps_overcast_mask_value(const ps_cloud_layer* layers, int32_t n);
+/* generate_blue_noise(64), crates/ps-clouds/src/noise.rs:646-704. */
+void ps_generate_blue_noise(uint8_t* out_64x64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PEDALSKY_HIP_H */

@@ -1,0 +1,289 @@
+// Oracle (test infrastructure, not product code): extern "C" entry points of
+// the CPU restatement, loaded by tests/ through ctypes and by bench.py's CPU
+// baseline. `or_*` run in double precision (the parity oracle); `orf_*` are
+// the float instantiation of the same code (the CPU port of the reference
+// used as the timing baseline).
+#include <omp.h>
+#include <vector>
+#include <cstring>
+#include "oracle_clouds.hpp"
+#include "oracle_noise.hpp"
+
+using namespace oracle;
+
+extern "C" {
+
+typedef struct or_inputs {
+    const ps_frame_uniforms* frame;
+    const ps_atmosphere_params* atm;
+    const ps_cloud_params* clouds;
+    const ps_cloud_layer* layers;
+    int32_t n_layers;
+    const float* layer_wind_xz;  // n_layers * 2 or null
+    const void* T; int32_t tw, th;           // LUTs, RGBA in the instantiation's scalar type
+    const void* SV; int32_t sw, sh;
+    const void* AP; int32_t aw, ah, ad;
+    const float* base;           // 128^3 RGBA decoded
+    const float* detail;         // 32^3 RGBA decoded
+    const float* curl;           // 128^2 RG decoded
+    const float* weather; int32_t ww, wh, wch;
+    const uint8_t* type_grid;    // 128^2 or null
+    const uint8_t* overcast;     // 128^2 or null
+    const uint8_t* blue;         // 64^2
+    const float* depth;          // W*H or null
+} or_inputs;
+
+}  // extern "C"
+
+namespace {
+
+template <class R>
+Scene<R> make_scene(const or_inputs* in) {
+    Scene<R> s;
+    s.f = frame_from<R>(*in->frame);
+    s.a = atm_from<R>(*in->atm);
+    s.c = clouds_from<R>(*in->clouds);
+    s.n_layers = std::min<int>(in->n_layers, PS_MAX_CLOUD_LAYERS);
+    for (int i = 0; i < s.n_layers; ++i) {
+        s.layers[i] = layer_from<R>(in->layers[i]);
+        if (in->layer_wind_xz) s.layer_wind[i] = V3<R>(in->layer_wind_xz[2 * i], 0, in->layer_wind_xz[2 * i + 1]);
+        else s.layer_wind[i] = V3<R>(0);
+    }
+    s.T.data = static_cast<const R*>(in->T); s.T.w = in->tw; s.T.h = in->th; s.T.ch = 4;
+    s.SV.data = static_cast<const R*>(in->SV); s.SV.w = in->sw; s.SV.h = in->sh; s.SV.ch = 4;
+    s.AP.data = static_cast<const R*>(in->AP); s.AP.w = in->aw; s.AP.h = in->ah; s.AP.d = in->ad; s.AP.ch = 4;
+    s.base.data = in->base; s.base.w = s.base.h = s.base.d = 128; s.base.ch = 4;
+    s.detail.data = in->detail; s.detail.w = s.detail.h = s.detail.d = 32; s.detail.ch = 4;
+    s.curl.data = in->curl; s.curl.w = s.curl.h = 128; s.curl.ch = 2;
+    s.weather.data = in->weather; s.weather.w = in->ww; s.weather.h = in->wh; s.weather.ch = in->wch;
+    s.type_grid = in->type_grid;
+    s.overcast = in->overcast;
+    s.blue = in->blue;
+    s.depth = in->depth;
+    return s;
+}
+
+template <class R>
+void render_region(const or_inputs* in, int x0, int y0, int w, int h, R* comp, R* lum, R* trn,
+                   uint64_t* counters, int nthreads) {
+    Scene<R> s = make_scene<R>(in);
+    const int n = w * h;
+    Counters total;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+    {
+        Counters local;
+        Counters* cp = counters ? &local : nullptr;
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < n; ++i) {
+            int ix = x0 + i % w, iy = y0 + i / w;
+            R c4[4], l4[4], t4[4];
+            render_pixel(s, ix, iy, c4, l4, t4, cp);
+            for (int k = 0; k < 4; ++k) {
+                if (comp) comp[size_t(i) * 4 + k] = c4[k];
+                if (lum) lum[size_t(i) * 4 + k] = l4[k];
+                if (trn) trn[size_t(i) * 4 + k] = t4[k];
+            }
+        }
+        if (counters) {
+#pragma omp critical
+            {
+                total.pixels += local.pixels; total.layer_marches += local.layer_marches;
+                total.primary_steps += local.primary_steps; total.covered_steps += local.covered_steps;
+                total.eroded_steps += local.eroded_steps; total.dense_samples += local.dense_samples;
+                total.light_steps += local.light_steps; total.light_covered += local.light_covered;
+                total.octaves += local.octaves; total.ap_taps += local.ap_taps;
+            }
+        }
+    }
+    if (counters) {
+        uint64_t v[10] = {total.pixels, total.layer_marches, total.primary_steps, total.covered_steps,
+                          total.eroded_steps, total.dense_samples, total.light_steps, total.light_covered,
+                          total.octaves, total.ap_taps};
+        std::memcpy(counters, v, sizeof(v));
+    }
+}
+
+template <class R>
+void bake_ms(const ps_atmosphere_params* atm, const R* T, int tw, int th, int W, int H, R* out, int nthreads) {
+    Atm<R> a = atm_from<R>(*atm);
+    Tex<R, R> t; t.data = T; t.w = tw; t.h = th; t.ch = 4;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < W * H; ++i) {
+        int x = i % W, y = i / W;
+        bake_multiscatter_texel(a, t, (R(x) + R(0.5)) / R(W), (R(y) + R(0.5)) / R(H), out + size_t(i) * 4);
+    }
+}
+
+template <class R>
+void bake_sv(const ps_frame_uniforms* fr, const ps_atmosphere_params* atm, const R* T, int tw, int th,
+             const R* MS, int mw, int mh, int W, int H, R* out, int nthreads) {
+    Atm<R> a = atm_from<R>(*atm);
+    Frame<R> f = frame_from<R>(*fr);
+    Tex<R, R> t; t.data = T; t.w = tw; t.h = th; t.ch = 4;
+    Tex<R, R> m; m.data = MS; m.w = mw; m.h = mh; m.ch = 4;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < W * H; ++i) {
+        int x = i % W, y = i / W;
+        bake_skyview_texel(f, a, t, m, (R(x) + R(0.5)) / R(W), (R(y) + R(0.5)) / R(H), out + size_t(i) * 4);
+    }
+}
+
+template <class R>
+void bake_aerial(const ps_frame_uniforms* fr, const ps_atmosphere_params* atm, const R* T, int tw, int th,
+                 const R* MS, int mw, int mh, int W, int H, int D, R* out, int nthreads) {
+    Atm<R> a = atm_from<R>(*atm);
+    Frame<R> f = frame_from<R>(*fr);
+    Tex<R, R> t; t.data = T; t.w = tw; t.h = th; t.ch = 4;
+    Tex<R, R> m; m.data = MS; m.w = mw; m.h = mh; m.ch = 4;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < W * H * D; ++i) {
+        int x = i % W, y = (i / W) % H, z = i / (W * H);
+        bake_ap_froxel(f, a, t, m, x, y, z, W, H, D, out + size_t(i) * 4);
+    }
+}
+
+template <class R>
+void bake_t(const ps_atmosphere_params* atm, int W, int H, R* out, int nthreads) {
+    Atm<R> a = atm_from<R>(*atm);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            R u = (R(x) + R(0.5)) / R(W), v = (R(y) + R(0.5)) / R(H);
+            V3<R> p, dir;
+            transmittance_lut_uv_to_pos_dir(a, u, v, p, dir);
+            V3<R> od = integrate_optical_depth(a, p, dir, 40);
+            V3<R> tt = vexp(-od);
+            R* o = out + (size_t(y) * W + x) * 4;
+            o[0] = tt.x; o[1] = tt.y; o[2] = tt.z; o[3] = R(1);
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- LUT chain -----------------------------------------------------------------
+void or_bake_transmittance(const ps_atmosphere_params* atm, int W, int H, double* out, int nthreads) {
+    bake_t<double>(atm, W, H, out, nthreads);
+}
+void or_bake_multiscatter(const ps_atmosphere_params* atm, const double* T, int tw, int th, int W, int H,
+                          double* out, int nthreads) {
+    bake_ms<double>(atm, T, tw, th, W, H, out, nthreads);
+}
+void or_bake_skyview(const ps_frame_uniforms* f, const ps_atmosphere_params* atm, const double* T, int tw, int th,
+                     const double* MS, int mw, int mh, int W, int H, double* out, int nthreads) {
+    bake_sv<double>(f, atm, T, tw, th, MS, mw, mh, W, H, out, nthreads);
+}
+void or_bake_ap(const ps_frame_uniforms* f, const ps_atmosphere_params* atm, const double* T, int tw, int th,
+                const double* MS, int mw, int mh, int W, int H, int D, double* out, int nthreads) {
+    bake_aerial<double>(f, atm, T, tw, th, MS, mw, mh, W, H, D, out, nthreads);
+}
+void orf_bake_transmittance(const ps_atmosphere_params* atm, int W, int H, float* out, int nthreads) {
+    bake_t<float>(atm, W, H, out, nthreads);
+}
+void orf_bake_multiscatter(const ps_atmosphere_params* atm, const float* T, int tw, int th, int W, int H,
+                           float* out, int nthreads) {
+    bake_ms<float>(atm, T, tw, th, W, H, out, nthreads);
+}
+void orf_bake_skyview(const ps_frame_uniforms* f, const ps_atmosphere_params* atm, const float* T, int tw, int th,
+                      const float* MS, int mw, int mh, int W, int H, float* out, int nthreads) {
+    bake_sv<float>(f, atm, T, tw, th, MS, mw, mh, W, H, out, nthreads);
+}
+void orf_bake_ap(const ps_frame_uniforms* f, const ps_atmosphere_params* atm, const float* T, int tw, int th,
+                 const float* MS, int mw, int mh, int W, int H, int D, float* out, int nthreads) {
+    bake_aerial<float>(f, atm, T, tw, th, MS, mw, mh, W, H, D, out, nthreads);
+}
+
+// ---- per-pixel sky + cloud path ------------------------------------------------------
+// Renders the w x h region at (x0, y0) of the frame; outputs are w*h RGBA.
+// counters (10 x u64, ps_work_counters order) may be null.
+void or_render(const or_inputs* in, int x0, int y0, int w, int h, double* comp, double* lum, double* trn,
+               uint64_t* counters, int nthreads) {
+    render_region<double>(in, x0, y0, w, h, comp, lum, trn, counters, nthreads);
+}
+void orf_render(const or_inputs* in, int x0, int y0, int w, int h, float* comp, float* lum, float* trn,
+                uint64_t* counters, int nthreads) {
+    render_region<float>(in, x0, y0, w, h, comp, lum, trn, counters, nthreads);
+}
+
+// ---- noise bakes (decoded float values before storage quantisation) ---------------------
+void or_noise_base(uint32_t seed, int z0, int z1, double* out, int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int per = 128 * 128;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < (z1 - z0) * per; ++i) {
+        int z = z0 + i / per, y = (i % per) / 128, x = i % 128;
+        base_texel<double>(x, y, z, seed, out + size_t(i) * 4);
+    }
+}
+void or_noise_detail(uint32_t seed, double* out, int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < 32 * 32 * 32; ++i) {
+        int z = i / 1024, y = (i % 1024) / 32, x = i % 32;
+        detail_texel<double>(x, y, z, seed, out + size_t(i) * 4);
+    }
+}
+void or_noise_curl(uint32_t seed, double* out, int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < 128 * 128; ++i) curl_texel<double>(i % 128, i / 128, seed, out + size_t(i) * 2);
+}
+
+// ---- single functions, for closed-form checks ------------------------------------------
+double or_henyey_greenstein(double c, double g) { return henyey_greenstein(c, g); }
+double or_draine_phase(double c, double g, double alpha) { return draine_phase(c, g, alpha); }
+double or_phase_rayleigh(double c) { return phase_rayleigh(c); }
+double or_phase_mie(double c, double g) { return phase_mie(c, g); }
+double or_cloud_phase(double c, double g_scale, double droplet_um, double bias, uint32_t type, double h_norm) {
+    Scene<double> s;
+    s.c.droplet_bias = bias;
+    Layer<double> L;
+    L.droplet = droplet_um; L.type = type;
+    PhaseConsts<double> k = cloud_phase_constants(s, L, h_norm);
+    return cloud_phase_with(c, g_scale, k);
+}
+double or_ndf(double h, uint32_t type, double anvil) { return ndf(h, type, anvil); }
+void or_transmittance_uv(const ps_atmosphere_params* atm, double r, double mu, double* uv) {
+    Atm<double> a = atm_from<double>(*atm);
+    V3<double> p(0, r, 0), d(std::sqrt(std::max(0.0, 1 - mu * mu)), mu, 0);
+    transmittance_lut_uv(a, p, d, uv[0], uv[1]);
+}
+void or_transmittance_pos_dir(const ps_atmosphere_params* atm, double u, double v, double* r_mu) {
+    Atm<double> a = atm_from<double>(*atm);
+    V3<double> p, d;
+    transmittance_lut_uv_to_pos_dir(a, u, v, p, d);
+    r_mu[0] = p.y; r_mu[1] = d.y;
+}
+void or_optical_depth(const ps_atmosphere_params* atm, double r, double mu, int n_steps, double* od) {
+    Atm<double> a = atm_from<double>(*atm);
+    V3<double> p(0, r, 0), d(std::sqrt(std::max(0.0, 1 - mu * mu)), mu, 0);
+    V3<double> o = integrate_optical_depth(a, p, d, n_steps);
+    od[0] = o.x; od[1] = o.y; od[2] = o.z;
+}
+void or_skyview_uv(const ps_atmosphere_params* atm, const double* view_dir, const double* sun, double cam_r,
+                   double* uv) {
+    Atm<double> a = atm_from<double>(*atm);
+    skyview_uv(a, V3<double>(view_dir[0], view_dir[1], view_dir[2]), V3<double>(sun[0], sun[1], sun[2]), cam_r,
+               uv[0], uv[1]);
+}
+void or_sample3d_repeat(const float* data, int w, int h, int d, int ch, double u, double v, double ww, double* out) {
+    Tex<double, float> t; t.data = data; t.w = w; t.h = h; t.d = d; t.ch = ch;
+    double o[4];
+    sample3d(t, u, v, ww, true, o);
+    for (int c = 0; c < ch && c < 4; ++c) out[c] = o[c];
+}
+void or_sample2d(const float* data, int w, int h, int ch, double u, double v, int repeat, double* out) {
+    Tex<double, float> t; t.data = data; t.w = w; t.h = h; t.ch = ch;
+    double o[4];
+    sample2d(t, u, v, repeat != 0, o);
+    for (int c = 0; c < ch && c < 4; ++c) out[c] = o[c];
+}
+uint32_t or_hash3(uint32_t x, uint32_t y, uint32_t z, uint32_t seed) { return hash3(x, y, z, seed); }
+
+}  // extern "C"

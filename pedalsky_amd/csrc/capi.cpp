@@ -1,0 +1,681 @@
+// C ABI of the sky + cloud transport library: context lifecycle, uploads,
+// per-frame parameter folding (host FP64) and kernel dispatch on one HIP
+// stream. See include/pedalsky_hip.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "../../include/pedalsky_hip.h"
+#include "ps_device.h"
+#include "ps_launch.h"
+
+static_assert(sizeof(ps_frame_uniforms) == 448, "FrameUniformsGpu is 448 B (frame_uniforms.rs:195-201)");
+static_assert(sizeof(ps_atmosphere_params) == 144, "AtmosphereParams is 144 B (weather.rs:544-555)");
+static_assert(sizeof(ps_cloud_params) == 112, "CloudParamsGpu is 112 B (params.rs:221-231)");
+static_assert(sizeof(ps_cloud_layer) == 48, "CloudLayerGpu is 48 B (weather.rs:544-555)");
+static_assert(offsetof(ps_frame_uniforms, inv_view_proj) == 192, "inv_view_proj offset");
+static_assert(offsetof(ps_frame_uniforms, camera_position_world) == 256, "camera offset");
+static_assert(offsetof(ps_frame_uniforms, sun_direction) == 288, "sun_direction offset");
+static_assert(offsetof(ps_frame_uniforms, viewport_size) == 352, "viewport offset");
+static_assert(offsetof(ps_frame_uniforms, time_seconds) == 368, "time offset");
+static_assert(offsetof(ps_frame_uniforms, prev_view_proj) == 384, "prev_view_proj offset");
+static_assert(offsetof(ps_atmosphere_params, rayleigh_scattering) == 16, "rayleigh offset");
+static_assert(offsetof(ps_atmosphere_params, mie_g) == 64, "mie_g offset");
+static_assert(offsetof(ps_atmosphere_params, ozone_absorption) == 80, "ozone offset");
+static_assert(offsetof(ps_atmosphere_params, ozone_center_m) == 96, "ozone centre offset");
+static_assert(offsetof(ps_atmosphere_params, ground_albedo) == 112, "albedo offset");
+static_assert(offsetof(ps_cloud_params, sigma_a) == 16, "sigma_a offset");
+static_assert(offsetof(ps_cloud_params, droplet_diameter_bias) == 28, "droplet bias offset");
+static_assert(offsetof(ps_cloud_params, cone_light_sampling) == 32, "cone offset");
+static_assert(offsetof(ps_cloud_params, powder_strength) == 48, "powder offset");
+static_assert(offsetof(ps_cloud_params, ambient_strength) == 64, "ambient offset");
+static_assert(offsetof(ps_cloud_params, light_steps) == 80, "light_steps offset");
+static_assert(offsetof(ps_cloud_params, temporal_jitter) == 96, "temporal_jitter offset");
+static_assert(offsetof(ps_cloud_layer, cloud_type) == 16, "cloud_type offset");
+static_assert(offsetof(ps_cloud_layer, droplet_diameter_um) == 32, "droplet offset");
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+ps_status fail(ps_status code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define PS_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? PS_ERR_OUT_OF_MEMORY : PS_ERR_DEVICE,      \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+template <class T>
+ps_status dev_alloc(T** p, size_t bytes, const char* what) {
+    if (*p) return PS_OK;
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? PS_ERR_OUT_OF_MEMORY : PS_ERR_DEVICE,
+                    std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
+    }
+    return PS_OK;
+}
+template <class T>
+void dev_free(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct ps_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    ps_lut_dims dims{256, 64, 32, 32, 192, 108, 32, 32, 64};
+
+    ps_atmosphere_params atm{};
+    ps_frame_uniforms frame{};
+    ps_cloud_params clouds{};
+    ps_cloud_layer layers[PS_MAX_CLOUD_LAYERS]{};
+    float wind[2 * PS_MAX_CLOUD_LAYERS]{};
+    int n_layers = 0;
+    bool have_atm = false, have_frame = false, have_clouds = false;
+
+    float4 *T = nullptr, *MS = nullptr, *SV = nullptr, *AP = nullptr;
+    uint32_t lut_baked = 0;  // LUTs baked since the last atmosphere / size change
+    bool luts_valid = false;
+
+    int noise_format = -1;
+    void *base_store = nullptr, *detail_store = nullptr, *curl_store = nullptr;
+    float2* base_rl = nullptr;
+    float* detail_hf = nullptr;
+    float2* curl2 = nullptr;
+    bool have_noise = false;
+
+    float* weather = nullptr;
+    int ww = 0, wh = 0;
+    uint8_t* type_grid = nullptr;
+    bool type_grid_active = false;
+    uint8_t* overcast = nullptr;
+    bool overcast_active = false;
+    bool have_weather = false;
+    uint8_t* blue = nullptr;
+    bool have_blue = false;
+
+    RenderParams* d_params = nullptr;
+    RenderParams* h_params = nullptr;  // pinned staging of the last uploaded block
+    bool params_uploaded = false;
+    unsigned long long* d_counters = nullptr;
+    std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+size_t noise_texel_bytes(int format, int channels) { return (format == PS_NOISE_UNORM8 ? 1u : 2u) * channels; }
+
+ps_status ensure_luts(ps_ctx* c) {
+    const ps_lut_dims& d = c->dims;
+    ps_status s;
+    if ((s = dev_alloc(&c->T, sizeof(float4) * d.transmittance_w * d.transmittance_h, "transmittance"))) return s;
+    if ((s = dev_alloc(&c->MS, sizeof(float4) * d.multiscatter_w * d.multiscatter_h, "multiscatter"))) return s;
+    if ((s = dev_alloc(&c->SV, sizeof(float4) * d.skyview_w * d.skyview_h, "skyview"))) return s;
+    if ((s = dev_alloc(&c->AP, sizeof(float4) * d.ap_w * d.ap_h * d.ap_d, "aerial perspective"))) return s;
+    return PS_OK;
+}
+
+void fill_lut_params(const ps_ctx* c, LutParams& P) {
+    const ps_atmosphere_params& a = c->atm;
+    P.a.R = a.planet_radius_m; P.a.top = a.atmosphere_top_m;
+    P.a.hr = a.rayleigh_scale_height_m; P.a.hm = a.mie_scale_height_m;
+    for (int i = 0; i < 3; ++i) {
+        P.a.ray_s[i] = a.rayleigh_scattering[i];
+        P.a.mie_s[i] = a.mie_scattering[i];
+        P.a.mie_a[i] = a.mie_absorption[i];
+        P.a.ozone[i] = a.ozone_absorption[i];
+        P.a.albedo[i] = a.ground_albedo[i];
+    }
+    P.a.mie_g = a.mie_g; P.a.oz_c = a.ozone_center_m; P.a.oz_t = a.ozone_thickness_m;
+    const ps_frame_uniforms& f = c->frame;
+    for (int i = 0; i < 16; ++i) P.f.inv_vp[i] = f.inv_view_proj[i];
+    for (int i = 0; i < 3; ++i) {
+        P.f.cam[i] = f.camera_position_world[i];
+        P.f.sun[i] = f.sun_direction[i];
+        P.f.E[i] = f.sun_illuminance[i];
+    }
+    const ps_lut_dims& d = c->dims;
+    P.tw = d.transmittance_w; P.th = d.transmittance_h;
+    P.mw = d.multiscatter_w; P.mh = d.multiscatter_h;
+    P.sw = d.skyview_w; P.sh = d.skyview_h;
+    P.aw = d.ap_w; P.ah = d.ap_h; P.ad = d.ap_d;
+    P.ms_disabled = 0;
+}
+
+double smoothstep_d(double e0, double e1, double x) {
+    double t = std::min(std::max((x - e0) / (e1 - e0), 0.0), 1.0);
+    return t * t * (3.0 - 2.0 * t);
+}
+
+ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image_out* out, RenderParams& P) {
+    const ps_frame_uniforms& f = c->frame;
+    const ps_atmosphere_params& a = c->atm;
+    const ps_cloud_params& cp = c->clouds;
+    std::memset(&P, 0, sizeof(P));
+    float vw = f.viewport_size[0], vh = f.viewport_size[1];
+    if (!(vw >= 1.f && vh >= 1.f && vw <= 32768.f && vh <= 32768.f) || vw != std::floor(vw) || vh != std::floor(vh))
+        return fail(PS_ERR_INVALID_ARGUMENT, "frame.viewport_size must hold integer width/height in [1, 32768]");
+    P.W = (int)vw; P.H = (int)vh;
+    P.inv_W = 1.f / vw; P.inv_H = 1.f / vh;
+    for (int i = 0; i < 16; ++i) P.inv_vp[i] = f.inv_view_proj[i];
+    const double R = a.planet_radius_m, top = a.atmosphere_top_m;
+    double cam[3] = {f.camera_position_world[0], f.camera_position_world[1], f.camera_position_world[2]};
+    double sun[3] = {f.sun_direction[0], f.sun_direction[1], f.sun_direction[2]};
+    for (int i = 0; i < 3; ++i) {
+        P.cam[i] = cam[i];
+        P.sun_d[i] = sun[i];
+        P.sun[i] = f.sun_direction[i];
+        P.E[i] = f.sun_illuminance[i];
+        P.cam_f[i] = f.camera_position_world[i];
+    }
+    P.sun_ang = f.sun_direction[3];
+    P.cos_disk = std::cos((double)f.sun_direction[3]);
+    P.toa = f.sun_illuminance[3];
+    P.R = (float)R; P.top = (float)top;
+    P.big_h = (float)std::sqrt(std::max(top * top - R * R, 0.0));
+    P.top_minus_R = (float)(top - R);
+    // camera radius and altitude without cancellation
+    double oy = cam[1] + R;
+    double r_cam = std::sqrt(cam[0] * cam[0] + oy * oy + cam[2] * cam[2]);
+    double h_cam = (cam[0] * cam[0] + cam[2] * cam[2] + cam[1] * (cam[1] + 2.0 * R)) / (r_cam + R);
+    P.r0 = r_cam;
+    P.h0 = h_cam;
+    P.cam_alt = (float)h_cam;
+    double cam_r = std::max(r_cam, R + 1.0);
+    P.cam_zha = (float)(PS_PI - std::asin(std::min(std::max(R / cam_r, 0.0), 1.0)));
+    P.sun_az = (float)std::atan2(sun[0], sun[2]);
+    {
+        double hf = smoothstep_d(-0.09, 0.05, sun[1]);
+        double saf = std::sqrt(std::min(std::max(sun[1], 0.05), 1.0)) * hf;
+        P.overcast_lum = (float)((double)f.sun_illuminance[3] * saf * 0.25);
+    }
+    // clouds
+    int lc = (int)std::min<uint32_t>(cp.cloud_layer_count, PS_MAX_CLOUD_LAYERS);
+    lc = std::min(lc, c->n_layers);
+    P.n_layers = lc;
+    P.steps = (int)cp.cloud_steps;
+    P.light_steps = (int)cp.light_steps;
+    P.octaves = (int)cp.multi_scatter_octaves;
+    P.cone = cp.cone_light_sampling ? 1 : 0;
+    P.temporal_jitter = cp.temporal_jitter ? 1 : 0;
+    P.frame_index = f.frame_index;
+    P.powder = cp.powder_strength;
+    P.forward_bias = cp.forward_bias;
+    P.ambient = cp.ambient_strength;
+    P.curl_off_scale = cp.curl_strength * cp.detail_scale_m;
+    P.inv_detail_scale = (float)(1.0 / std::max((double)cp.detail_scale_m, 1.0));
+    P.inv_base_scale = (float)(1.0 / std::max((double)cp.base_scale_m, 1.0));
+    P.inv_weather_scale = (float)(1.0 / std::max((double)cp.weather_scale_m, 1.0));
+    for (int i = 0; i < 3; ++i) P.sigma_s[i] = cp.sigma_s[i];
+    P.ms_a = cp.multi_scatter_a; P.ms_b = cp.multi_scatter_b; P.ms_c = cp.multi_scatter_c;
+    P.droplet_bias = cp.droplet_diameter_bias;
+    {
+        double an = 1, bn = 1, cn = 1;
+        for (int n = 0; n < PS_MAX_OCTAVES; ++n) {
+            P.a_pow[n] = (float)an; P.b_pow[n] = (float)bn; P.c_pow[n] = (float)cn;
+            an *= cp.multi_scatter_a; bn *= cp.multi_scatter_b; cn *= cp.multi_scatter_c;
+        }
+    }
+    for (int i = 0; i < 3; ++i) P.light_rgb[i] = f.lightning_illuminance[i];
+    P.light_radius = f.lightning_illuminance[3];
+    P.light_x = f.lightning_origin_world[0];
+    P.light_z = f.lightning_origin_world[2];
+    const double drift = (double)cp.wind_drift_strength * (double)f.simulated_seconds;
+    for (int l = 0; l < lc; ++l) {
+        const ps_cloud_layer& L = c->layers[l];
+        LayerC& o = P.layers[l];
+        o.base = L.base_m; o.top = L.top_m;
+        o.inv_thick = (float)(1.0 / std::max((double)L.top_m - (double)L.base_m, 1.0));
+        o.coverage = L.coverage; o.density_scale = L.density_scale;
+        double diurnal = 0.0;
+        if (cp.diurnal_strength != 0.f) {
+            uint32_t t = L.cloud_type;
+            bool conv = t == 0 || t == 2 || t == 3 || t == 7;
+            if (conv) diurnal = smoothstep_d(-0.1, 0.4, sun[1]) * cp.diurnal_strength;
+        }
+        o.esb = (float)(L.shape_bias * diurnal);
+        o.ld = (float)std::min(std::max((double)cp.detail_strength + L.detail_bias * diurnal, 0.0), 1.0);
+        o.anvil = L.anvil_bias;
+        // chromatic_mie_modulation (cloud_march.wgsl:122-145) on sigma_s + sigma_a
+        double t = std::min(std::max(1.0 - L.droplet_diameter_um / 20.0, 0.0), 1.0);
+        double mod[3] = {1, 1, 1};
+        if (t > 0.0) {
+            double st = 0.25 * t * t;
+            mod[0] = 1.0 - 0.5 * st; mod[2] = 1.0 + st;
+        }
+        for (int k = 0; k < 3; ++k) o.sigma_t[k] = (float)(((double)cp.sigma_s[k] + cp.sigma_a[k]) * mod[k]);
+        double wx = c->wind[2 * l], wz = c->wind[2 * l + 1];
+        o.wind_off_x = (float)(wx * drift);
+        o.wind_off_z = (float)(wz * drift);
+        o.skew_x = 0.f; o.skew_z = 0.f;
+        if (cp.wind_skew_strength != 0.f) {
+            double speed = std::sqrt(wx * wx + wz * wz);
+            if (speed >= 0.1) {
+                double thick = std::max((double)L.top_m - L.base_m, 1.0) * cp.wind_skew_strength;
+                o.skew_x = (float)(wx / speed * thick);
+                o.skew_z = (float)(wz / speed * thick);
+            }
+        }
+        o.type = L.cloud_type;
+        o.is_cb = L.cloud_type == 7u ? 1u : 0u;
+        o.droplet = L.droplet_diameter_um;
+        o.r_bottom = R + (double)L.base_m;
+        o.r_top = R + (double)L.top_m;
+        double d = std::min(std::max((double)L.droplet_diameter_um * cp.droplet_diameter_bias, 5.0), 50.0);
+        o.g_hg = std::exp(-0.0990567 / (d - 1.67154));
+        o.g_d = std::exp(-2.20679 / (d + 3.91029)) - 0.428934;
+        o.alpha = std::exp(3.62489 - 8.29288 / (d + 5.52825));
+        o.w_d = std::exp(-0.599085 / (d - 0.641583) - 0.665888);
+        o.ice = smoothstep_d(35.0, 50.0, d);
+    }
+    // resources
+    const ps_lut_dims& dm = c->dims;
+    P.T = c->T; P.tw = dm.transmittance_w; P.th = dm.transmittance_h;
+    P.SV = c->SV; P.sw = dm.skyview_w; P.sh = dm.skyview_h;
+    P.AP = c->AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
+    P.base_rl = c->base_rl; P.detail_hf = c->detail_hf; P.curl = c->curl2;
+    P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
+    P.type_grid = c->type_grid; P.use_type_grid = c->type_grid_active ? 1 : 0;
+    P.overcast = c->overcast_active ? c->overcast : nullptr;
+    P.blue = c->blue;
+    P.depth = depth; P.has_depth = depth ? 1 : 0;
+    if (out) {
+        P.out_comp = out->composite; P.out_lum = out->cloud_luminance; P.out_trans = out->cloud_transmittance;
+        P.out_f16 = out->format == PS_PIXEL_F16 ? 1 : 0;
+    }
+    P.counters = c->d_counters;
+    return PS_OK;
+}
+
+ps_status check_ready(const ps_ctx* c, bool for_render) {
+    if (!c->have_atm) return fail(PS_ERR_NOT_READY, "ps_set_atmosphere has not been called");
+    if (!c->have_frame) return fail(PS_ERR_NOT_READY, "ps_set_frame has not been called");
+    if (!for_render) return PS_OK;
+    if (!c->have_clouds) return fail(PS_ERR_NOT_READY, "ps_set_clouds has not been called");
+    if (!c->have_noise) return fail(PS_ERR_NOT_READY, "no noise volumes: call ps_generate_noise or ps_upload_noise");
+    if (!c->have_weather) return fail(PS_ERR_NOT_READY, "no weather map: call ps_upload_weather");
+    if (!c->have_blue) return fail(PS_ERR_NOT_READY, "no blue noise: call ps_upload_blue_noise");
+    if (!c->luts_valid) return fail(PS_ERR_NOT_READY, "LUTs not baked: call ps_bake_luts");
+    return PS_OK;
+}
+
+ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
+    ps_status s;
+    if ((s = check_ready(c, false))) return s;
+    if ((s = ensure_luts(c))) return s;
+    LutParams P;
+    fill_lut_params(c, P);
+    uint32_t k = flags & 15u;
+    if (flags & PS_LUT_MULTISCATTER_DISABLED) {
+        P.ms_disabled = 1;
+        k |= 2u;
+    }
+    PS_HIP(ps_launch_luts(P, k, c->T, c->MS, c->SV, c->AP, c->stream, ev));
+    c->lut_baked |= (k & 15u);
+    c->luts_valid = c->lut_baked == 15u;
+    return PS_OK;
+}
+
+ps_status launch_render(ps_ctx* c, const float* depth, const ps_image_out* out, bool count) {
+    ps_status s;
+    if ((s = check_ready(c, true))) return s;
+    if (!out && !count) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render: out is NULL");
+    if ((s = dev_alloc(&c->d_params, sizeof(RenderParams), "render params"))) return s;
+    if (!c->h_params) PS_HIP(hipHostMalloc((void**)&c->h_params, sizeof(RenderParams), hipHostMallocDefault));
+    RenderParams next;
+    if ((s = fill_render_params(c, depth, out, next))) return s;
+    // Re-upload only when the block changed; the pinned staging buffer may
+    // still be read by an earlier queued copy, so drain the stream first.
+    if (!c->params_uploaded || std::memcmp(&next, c->h_params, sizeof(RenderParams)) != 0) {
+        PS_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(c->h_params, &next, sizeof(RenderParams));
+        PS_HIP(hipMemcpyAsync(c->d_params, c->h_params, sizeof(RenderParams), hipMemcpyHostToDevice, c->stream));
+        c->params_uploaded = true;
+    }
+    PS_HIP(ps_launch_render(c->d_params, next.W, next.H, count, c->stream));
+    return PS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ps_last_error(void) { return g_last_error.c_str(); }
+int32_t ps_abi_version(void) { return PS_ABI_VERSION; }
+
+ps_status ps_ctx_create(int32_t device, ps_ctx** out_ctx) {
+    if (!out_ctx) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_create: out_ctx is NULL");
+    *out_ctx = nullptr;
+    int n = 0;
+    PS_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_create: device index out of range");
+    PS_HIP(hipSetDevice(device));
+    ps_ctx* c = new ps_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->stream = c->own_stream;
+    e = hipMalloc((void**)&c->d_counters, 16 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->own_stream);
+        delete c;
+        return fail(PS_ERR_OUT_OF_MEMORY, std::string("hipMalloc(counters): ") + hipGetErrorString(e));
+    }
+    *out_ctx = c;
+    return PS_OK;
+}
+
+void ps_ctx_destroy(ps_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    dev_free(c->T); dev_free(c->MS); dev_free(c->SV); dev_free(c->AP);
+    dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
+    dev_free(c->base_rl); dev_free(c->detail_hf); dev_free(c->curl2);
+    dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
+    dev_free(c->d_params); dev_free(c->d_counters);
+    if (c->h_params) hipHostFree(c->h_params);
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+ps_status ps_ctx_set_stream(ps_ctx* c, void* hip_stream) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_set_stream: ctx is NULL");
+    c->stream = hip_stream ? (hipStream_t)hip_stream : c->own_stream;
+    return PS_OK;
+}
+
+ps_status ps_ctx_synchronize(ps_ctx* c) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_synchronize: ctx is NULL");
+    PS_HIP(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+ps_status ps_ctx_set_lut_dims(ps_ctx* c, const ps_lut_dims* d) {
+    if (!c || !d) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_set_lut_dims: NULL argument");
+    const int v[9] = {d->transmittance_w, d->transmittance_h, d->multiscatter_w, d->multiscatter_h,
+                      d->skyview_w, d->skyview_h, d->ap_w, d->ap_h, d->ap_d};
+    for (int i = 0; i < 9; ++i)
+        if (v[i] < 1 || v[i] > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_set_lut_dims: sizes must be in [1, 4096]");
+    if ((size_t)d->ap_w * d->ap_h * d->ap_d > (size_t)1 << 24)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_set_lut_dims: AP volume too large");
+    PS_HIP(hipStreamSynchronize(c->stream));
+    dev_free(c->T); dev_free(c->MS); dev_free(c->SV); dev_free(c->AP);
+    c->dims = *d;
+    c->lut_baked = 0;
+    c->luts_valid = false;
+    return ensure_luts(c);
+}
+
+ps_status ps_ctx_get_lut_dims(const ps_ctx* c, ps_lut_dims* out) {
+    if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_get_lut_dims: NULL argument");
+    *out = c->dims;
+    return PS_OK;
+}
+
+ps_status ps_set_atmosphere(ps_ctx* c, const ps_atmosphere_params* a) {
+    if (!c || !a) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_atmosphere: NULL argument");
+    if (!(a->planet_radius_m > 0.f) || !(a->atmosphere_top_m > a->planet_radius_m))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_atmosphere: need 0 < planet_radius_m < atmosphere_top_m");
+    if (!(a->rayleigh_scale_height_m > 0.f) || !(a->mie_scale_height_m > 0.f))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_atmosphere: scale heights must be positive");
+    c->atm = *a;
+    c->have_atm = true;
+    c->lut_baked = 0;
+    c->luts_valid = false;
+    return PS_OK;
+}
+
+ps_status ps_set_frame(ps_ctx* c, const ps_frame_uniforms* f) {
+    if (!c || !f) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_frame: NULL argument");
+    c->frame = *f;
+    c->have_frame = true;
+    return PS_OK;
+}
+
+ps_status ps_set_clouds(ps_ctx* c, const ps_cloud_params* p, const ps_cloud_layer* layers, int32_t n,
+                        const float* wind) {
+    if (!c || !p) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: NULL argument");
+    if (n < 0 || n > PS_MAX_CLOUD_LAYERS)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: n_layers must be in [0, 8]");
+    if (n > 0 && !layers) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: layers is NULL");
+    if (p->cloud_steps < 1 || p->cloud_steps > 4096)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: cloud_steps must be in [1, 4096]");
+    if (p->light_steps < 1 || p->light_steps > 256)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: light_steps must be in [1, 256]");
+    if (p->multi_scatter_octaves > PS_MAX_OCTAVES)
+        return fail(PS_ERR_UNSUPPORTED, "ps_set_clouds: at most 8 multi-scatter octaves are supported");
+    int i, j;
+    if (n > 0 && ps_check_layers_overlap(layers, n, &i, &j)) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "ps_set_clouds: layers %d and %d overlap ([%g, %g] vs [%g, %g] m)", i, j,
+                      layers[i].base_m, layers[i].top_m, layers[j].base_m, layers[j].top_m);
+        return fail(PS_ERR_OVERLAPPING_LAYERS, buf);
+    }
+    c->clouds = *p;
+    c->n_layers = n;
+    for (int l = 0; l < n; ++l) {
+        c->layers[l] = layers[l];
+        c->wind[2 * l] = wind ? wind[2 * l] : 0.f;
+        c->wind[2 * l + 1] = wind ? wind[2 * l + 1] : 0.f;
+    }
+    c->have_clouds = true;
+    return PS_OK;
+}
+
+static ps_status alloc_noise(ps_ctx* c, int format) {
+    if (c->noise_format != format) {
+        PS_HIP(hipStreamSynchronize(c->stream));
+        dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
+        c->noise_format = format;
+    }
+    ps_status s;
+    if ((s = dev_alloc(&c->base_store, noise_texel_bytes(format, 4) * 128 * 128 * 128, "base noise"))) return s;
+    if ((s = dev_alloc(&c->detail_store, noise_texel_bytes(format, 4) * 32 * 32 * 32, "detail noise"))) return s;
+    if ((s = dev_alloc(&c->curl_store, noise_texel_bytes(format, 2) * 128 * 128, "curl noise"))) return s;
+    if ((s = dev_alloc(&c->base_rl, sizeof(float2) * 128 * 128 * 128, "base fold"))) return s;
+    if ((s = dev_alloc(&c->detail_hf, sizeof(float) * 32 * 32 * 32, "detail fold"))) return s;
+    if ((s = dev_alloc(&c->curl2, sizeof(float2) * 128 * 128, "curl fold"))) return s;
+    return PS_OK;
+}
+
+ps_status ps_generate_noise(ps_ctx* c, uint32_t seed, int32_t format) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_generate_noise: ctx is NULL");
+    if (format != PS_NOISE_UNORM8 && format != PS_NOISE_FP16)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_generate_noise: format must be PS_NOISE_UNORM8 or PS_NOISE_FP16");
+    ps_status s;
+    if ((s = alloc_noise(c, format))) return s;
+    PS_HIP(ps_launch_noise_bake(seed, format, c->base_store, c->detail_store, c->curl_store, c->stream));
+    PS_HIP(ps_launch_noise_fold(format, c->base_store, c->detail_store, c->curl_store, c->base_rl, c->detail_hf,
+                                c->curl2, c->stream));
+    c->have_noise = true;
+    return PS_OK;
+}
+
+ps_status ps_upload_noise(ps_ctx* c, int32_t format, const void* base, const void* detail, const void* curl) {
+    if (!c || !base || !detail || !curl) return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_noise: NULL argument");
+    if (format != PS_NOISE_UNORM8 && format != PS_NOISE_FP16)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_noise: format must be PS_NOISE_UNORM8 or PS_NOISE_FP16");
+    ps_status s;
+    if ((s = alloc_noise(c, format))) return s;
+    PS_HIP(hipMemcpyAsync(c->base_store, base, noise_texel_bytes(format, 4) * 128 * 128 * 128, hipMemcpyHostToDevice,
+                          c->stream));
+    PS_HIP(hipMemcpyAsync(c->detail_store, detail, noise_texel_bytes(format, 4) * 32 * 32 * 32,
+                          hipMemcpyHostToDevice, c->stream));
+    PS_HIP(hipMemcpyAsync(c->curl_store, curl, noise_texel_bytes(format, 2) * 128 * 128, hipMemcpyHostToDevice,
+                          c->stream));
+    PS_HIP(ps_launch_noise_fold(format, c->base_store, c->detail_store, c->curl_store, c->base_rl, c->detail_hf,
+                                c->curl2, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
+    c->have_noise = true;
+    return PS_OK;
+}
+
+ps_status ps_read_noise(ps_ctx* c, int32_t which, void* host_out, size_t bytes) {
+    if (!c || !host_out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_noise: NULL argument");
+    if (!c->have_noise) return fail(PS_ERR_NOT_READY, "ps_read_noise: no noise volumes");
+    const void* src;
+    size_t need;
+    switch (which) {
+        case PS_NOISE_BASE: src = c->base_store; need = noise_texel_bytes(c->noise_format, 4) * 128 * 128 * 128; break;
+        case PS_NOISE_DETAIL: src = c->detail_store; need = noise_texel_bytes(c->noise_format, 4) * 32 * 32 * 32; break;
+        case PS_NOISE_CURL: src = c->curl_store; need = noise_texel_bytes(c->noise_format, 2) * 128 * 128; break;
+        default: return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_noise: unknown volume");
+    }
+    if (bytes < need) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_noise: host buffer too small");
+    PS_HIP(hipMemcpyAsync(host_out, src, need, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+ps_status ps_upload_blue_noise(ps_ctx* c, const uint8_t* ranks) {
+    if (!c || !ranks) return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_blue_noise: NULL argument");
+    ps_status s;
+    if ((s = dev_alloc(&c->blue, 64 * 64, "blue noise"))) return s;
+    PS_HIP(hipMemcpyAsync(c->blue, ranks, 64 * 64, hipMemcpyHostToDevice, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
+    c->have_blue = true;
+    return PS_OK;
+}
+
+ps_status ps_upload_weather(ps_ctx* c, const uint16_t* weather, int32_t w, int32_t h, int32_t channels,
+                            const uint8_t* type_grid, const uint8_t* overcast) {
+    if (!c || !weather) return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_weather: NULL argument");
+    if (w < 1 || h < 1 || w > 8192 || h > 8192 || channels < 1 || channels > 4)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_weather: need 1 <= w, h <= 8192 and 1 <= channels <= 4");
+    PS_HIP(hipStreamSynchronize(c->stream));
+    if (c->ww != w || c->wh != h) dev_free(c->weather);
+    ps_status s;
+    if ((s = dev_alloc(&c->weather, sizeof(float) * w * h, "weather"))) return s;
+    if ((s = dev_alloc(&c->type_grid, 128 * 128, "cloud type grid"))) return s;
+    if ((s = dev_alloc(&c->overcast, 128 * 128, "overcast mask"))) return s;
+    uint16_t* staging = nullptr;
+    size_t bytes = sizeof(uint16_t) * (size_t)w * h * channels;
+    PS_HIP(hipMalloc((void**)&staging, bytes));
+    hipError_t e = hipMemcpy(staging, weather, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ps_launch_weather_fold(staging, w, h, channels, c->weather, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(staging);
+    if (e != hipSuccess) return fail(PS_ERR_DEVICE, std::string("ps_upload_weather: ") + hipGetErrorString(e));
+    c->ww = w; c->wh = h;
+    c->type_grid_active = false;
+    if (type_grid) {
+        for (int i = 0; i < 128 * 128; ++i)
+            if (type_grid[i] != 255u) { c->type_grid_active = true; break; }
+        PS_HIP(hipMemcpy(c->type_grid, type_grid, 128 * 128, hipMemcpyHostToDevice));
+    }
+    c->overcast_active = overcast != nullptr;
+    if (overcast) PS_HIP(hipMemcpy(c->overcast, overcast, 128 * 128, hipMemcpyHostToDevice));
+    c->have_weather = true;
+    return PS_OK;
+}
+
+ps_status ps_bake_luts(ps_ctx* c, uint32_t flags) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_bake_luts: ctx is NULL");
+    return launch_luts(c, flags, nullptr);
+}
+
+ps_status ps_read_lut(ps_ctx* c, int32_t which, float* host_out, size_t floats) {
+    if (!c || !host_out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_lut: NULL argument");
+    if (!c->T) return fail(PS_ERR_NOT_READY, "ps_read_lut: LUTs not allocated");
+    const ps_lut_dims& d = c->dims;
+    const float4* src;
+    size_t n;
+    switch (which) {
+        case PS_LUT_TRANSMITTANCE: src = c->T; n = (size_t)d.transmittance_w * d.transmittance_h; break;
+        case PS_LUT_MULTISCATTER: src = c->MS; n = (size_t)d.multiscatter_w * d.multiscatter_h; break;
+        case PS_LUT_SKYVIEW: src = c->SV; n = (size_t)d.skyview_w * d.skyview_h; break;
+        case PS_LUT_AERIAL_PERSPECTIVE: src = c->AP; n = (size_t)d.ap_w * d.ap_h * d.ap_d; break;
+        default: return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_lut: unknown LUT");
+    }
+    if (floats < 4 * n) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_lut: host buffer too small");
+    PS_HIP(hipMemcpyAsync(host_out, src, 4 * n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+ps_status ps_render(ps_ctx* c, const float* depth, const ps_image_out* out) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render: ctx is NULL");
+    return launch_render(c, depth, out, false);
+}
+
+ps_status ps_frame(ps_ctx* c, uint32_t lut_flags, const float* depth, const ps_image_out* out) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_frame: ctx is NULL");
+    ps_status s;
+    if ((s = launch_luts(c, lut_flags, nullptr))) return s;
+    return launch_render(c, depth, out, false);
+}
+
+ps_status ps_time_frames(ps_ctx* c, int32_t frames, uint32_t lut_flags, const float* depth, const ps_image_out* out,
+                         ps_frame_timing* timing) {
+    if (!c || !timing) return fail(PS_ERR_INVALID_ARGUMENT, "ps_time_frames: NULL argument");
+    if (frames < 1 || frames > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "ps_time_frames: frames must be in [1, 4096]");
+    const size_t need = (size_t)frames * 6;
+    while (c->events.size() < need) {
+        hipEvent_t e;
+        PS_HIP(hipEventCreate(&e));
+        c->events.push_back(e);
+    }
+    ps_status s;
+    for (int f = 0; f < frames; ++f) {
+        hipEvent_t* ev = &c->events[(size_t)f * 6];
+        if ((s = launch_luts(c, lut_flags, ev))) return s;
+        if ((s = launch_render(c, depth, out, false))) return s;
+        PS_HIP(hipEventRecord(ev[5], c->stream));
+    }
+    PS_HIP(hipStreamSynchronize(c->stream));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int f = 0; f < frames; ++f) {
+        hipEvent_t* ev = &c->events[(size_t)f * 6];
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0.f;
+            PS_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    float total = 0.f;
+    PS_HIP(hipEventElapsedTime(&total, c->events[0], c->events[(size_t)(frames - 1) * 6 + 5]));
+    timing->transmittance_ms = (float)(acc[0] / frames);
+    timing->multiscatter_ms = (float)(acc[1] / frames);
+    timing->skyview_ms = (float)(acc[2] / frames);
+    timing->aerial_perspective_ms = (float)(acc[3] / frames);
+    timing->render_ms = (float)(acc[4] / frames);
+    timing->frame_ms = total / frames;
+    timing->frames = frames;
+    timing->_reserved = 0;
+    return PS_OK;
+}
+
+ps_status ps_count_work(ps_ctx* c, const float* depth, ps_work_counters* out) {
+    if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_count_work: NULL argument");
+    PS_HIP(hipMemsetAsync(c->d_counters, 0, 16 * sizeof(unsigned long long), c->stream));
+    ps_status s;
+    if ((s = launch_render(c, depth, nullptr, true))) return s;
+    unsigned long long h[16];
+    PS_HIP(hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
+    out->pixels = h[0]; out->layer_marches = h[1]; out->primary_steps = h[2]; out->covered_steps = h[3];
+    out->eroded_steps = h[4]; out->dense_samples = h[5]; out->light_steps = h[6]; out->light_covered = h[7];
+    out->octaves = h[8]; out->ap_taps = h[9];
+    return PS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// Cloud noise inputs for gfx950: the tileable Perlin-Worley / Worley-FBM /
+// curl bakes of shaders/clouds/noise_{base,detail,curl}.comp.wgsl (integer
+// hash exact, float work in FP32 as in the WGSL), stored as UNORM8 or FP16,
+// plus the fold kernels that turn the stored volumes into the layouts the
+// march reads. A seed is mixed into the hash as seed * 0x85ebca6b; seed 0 is
+// the reference's fixed hash.
+#include "ps_device.h"
+#include "ps_launch.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t hash3(uint32_t x0, uint32_t y0, uint32_t z0, uint32_t seed) {
+    uint32_t x = x0 * 0x27d4eb2du + y0 * 0x165667b1u + z0 * 0x9e3779b9u + seed * 0x85ebca6bu;
+    x = (x ^ (x >> 15u)) * 0x2c1b3c6du;
+    x = (x ^ (x >> 12u)) * 0x297a2d39u;
+    return x ^ (x >> 15u);
+}
+__device__ __forceinline__ uint32_t hash2(uint32_t x0, uint32_t y0, uint32_t seed) {
+    uint32_t x = x0 * 0x27d4eb2du + y0 * 0x165667b1u + seed * 0x85ebca6bu;
+    x = (x ^ (x >> 15u)) * 0x2c1b3c6du;
+    x = (x ^ (x >> 12u)) * 0x297a2d39u;
+    return x ^ (x >> 15u);
+}
+__device__ __forceinline__ float u16f(uint32_t h) { return (float)(h & 0xffffu) / 65535.f; }
+__device__ __forceinline__ int wrapf(int c, int f) { return ((c % f) + f) % f; }
+
+// noise_base.comp.wgsl:45-75 / noise_detail.comp.wgsl:170-199
+__device__ float worley_tile(float px, float py, float pz, int freq, uint32_t ox, uint32_t oy, uint32_t oz,
+                             uint32_t seed) {
+    float f = (float)freq;
+    float sx = px * f, sy = py * f, sz = pz * f;
+    float flx = floorf(sx), fly = floorf(sy), flz = floorf(sz);
+    int cx = (int)flx, cy = (int)fly, cz = (int)flz;
+    float frx = sx - flx, fry = sy - fly, frz = sz - flz;
+    float min_d = 1.f;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint32_t wx = (uint32_t)wrapf(cx + dx, freq) + ox;
+                uint32_t wy = (uint32_t)wrapf(cy + dy, freq) + oy;
+                uint32_t wz = (uint32_t)wrapf(cz + dz, freq) + oz;
+                float jx = u16f(hash3(wx, wy, wz, seed)), jy = u16f(hash3(wx + 1u, wy, wz, seed)),
+                      jz = u16f(hash3(wx, wy + 1u, wz, seed));
+                float ax = (float)dx + jx - frx, ay = (float)dy + jy - fry, az = (float)dz + jz - frz;
+                float d = sqrtf(ax * ax + ay * ay + az * az);
+                min_d = fminf(min_d, d);
+            }
+    return clampf(1.f - min_d, 0.f, 1.f);
+}
+__device__ float worley_fbm(float px, float py, float pz, int freq, uint32_t ox, uint32_t oy, uint32_t oz,
+                            uint32_t seed) {
+    return worley_tile(px, py, pz, freq, ox, oy, oz, seed) * 0.625f +
+           worley_tile(px, py, pz, freq * 2, ox, oy, oz, seed) * 0.25f +
+           worley_tile(px, py, pz, freq * 4, ox, oy, oz, seed) * 0.125f;
+}
+__device__ __forceinline__ float fade(float t) { return t * t * t * (t * (t * 6.f - 15.f) + 10.f); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a + (b - a) * t; }
+
+// noise_base.comp.wgsl:81-120
+__device__ float perlin_tile(float px, float py, float pz, int freq, uint32_t seed) {
+    float f = (float)freq;
+    float sx = px * f, sy = py * f, sz = pz * f;
+    float flx = floorf(sx), fly = floorf(sy), flz = floorf(sz);
+    int cx = (int)flx, cy = (int)fly, cz = (int)flz;
+    float fx = sx - flx, fy = sy - fly, fz = sz - flz;
+    float ux = fade(fx), uy = fade(fy), uz = fade(fz);
+    float dots[8];
+    for (int iz = 0; iz < 2; ++iz)
+        for (int iy = 0; iy < 2; ++iy)
+            for (int ix = 0; ix < 2; ++ix) {
+                uint32_t wx = (uint32_t)wrapf(cx + ix, freq), wy = (uint32_t)wrapf(cy + iy, freq),
+                         wz = (uint32_t)wrapf(cz + iz, freq);
+                float gx = u16f(hash3(wx, wy, wz, seed)) * 2.f - 1.f;
+                float gy = u16f(hash3(wx + 1u, wy, wz, seed)) * 2.f - 1.f;
+                float gz = u16f(hash3(wx, wy + 1u, wz, seed)) * 2.f - 1.f;
+                float il = 1.f / sqrtf(gx * gx + gy * gy + gz * gz);
+                dots[ix + iy * 2 + iz * 4] = gx * il * (fx - ix) + gy * il * (fy - iy) + gz * il * (fz - iz);
+            }
+    float x00 = mixf(dots[0], dots[1], ux), x10 = mixf(dots[2], dots[3], ux);
+    float x01 = mixf(dots[4], dots[5], ux), x11 = mixf(dots[6], dots[7], ux);
+    float y0 = mixf(x00, x10, uy), y1 = mixf(x01, x11, uy);
+    return mixf(y0, y1, uz) * 0.5f + 0.5f;
+}
+
+__device__ __forceinline__ uint8_t to_unorm8(float v) { return (uint8_t)rintf(clampf(v, 0.f, 1.f) * 255.f); }
+
+template <typename Store>
+__device__ __forceinline__ void store4(void* out, size_t i, int format, float a, float b, float c, float d) {
+    if (format == 0) {
+        uchar4 q = make_uchar4(to_unorm8(a), to_unorm8(b), to_unorm8(c), to_unorm8(d));
+        ((uchar4*)out)[i] = q;
+    } else {
+        __half2* q = (__half2*)out + 2 * i;
+        q[0] = __floats2half2_rn(a, b);
+        q[1] = __floats2half2_rn(c, d);
+    }
+}
+
+// noise_base.comp.wgsl:122-139
+__global__ __launch_bounds__(64) void k_noise_base(uint32_t seed, int format, void* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 128 * 128 * 128) return;
+    int x = i & 127, y = (i >> 7) & 127, z = i >> 14;
+    float px = (x + 0.5f) / 128.f, py = (y + 0.5f) / 128.f, pz = (z + 0.5f) / 128.f;
+    float perlin = perlin_tile(px, py, pz, 4, seed) * 0.5f + perlin_tile(px, py, pz, 8, seed) * 0.25f +
+                   perlin_tile(px, py, pz, 16, seed) * 0.125f;
+    float wl = worley_fbm(px, py, pz, 4, 0, 0, 0, seed);
+    float pw = clampf((perlin - (wl - 1.f)) / fmaxf(1.f - (wl - 1.f), 1e-5f), 0.f, 1.f);
+    float g = worley_fbm(px, py, pz, 2, 0, 0, 0, seed);
+    float b = worley_fbm(px, py, pz, 8, 0, 0, 0, seed);
+    float a = worley_fbm(px, py, pz, 14, 0, 0, 0, seed);
+    store4<int>(out, (size_t)i, format, pw, g, b, a);
+}
+
+// noise_detail.comp.wgsl:201-211
+__global__ __launch_bounds__(64) void k_noise_detail(uint32_t seed, int format, void* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 32 * 32 * 32) return;
+    int x = i & 31, y = (i >> 5) & 31, z = i >> 10;
+    float px = (x + 0.5f) / 32.f, py = (y + 0.5f) / 32.f, pz = (z + 0.5f) / 32.f;
+    float r = worley_fbm(px, py, pz, 2, 73u, 19u, 41u, seed);
+    float g = worley_fbm(px, py, pz, 8, 73u, 19u, 41u, seed);
+    float b = worley_fbm(px, py, pz, 16, 73u, 19u, 41u, seed);
+    store4<int>(out, (size_t)i, format, r, g, b, 0.f);
+}
+
+// noise_curl.comp.wgsl:230-285
+__device__ float perlin2(float px, float py, int freq, uint32_t seed) {
+    float f = (float)freq;
+    float sx = px * f, sy = py * f;
+    float flx = floorf(sx), fly = floorf(sy);
+    int cx = (int)flx, cy = (int)fly;
+    float fx = sx - flx, fy = sy - fly;
+    float ux = fade(fx), uy = fade(fy);
+    float d[4];
+    for (int k = 0; k < 4; ++k) {
+        int ox = k & 1, oy = k >> 1;
+        uint32_t wx = (uint32_t)wrapf(cx + ox, freq), wy = (uint32_t)wrapf(cy + oy, freq);
+        float a = u16f(hash2(wx, wy, seed)) * 2.f - 1.f;
+        float b = u16f(hash2(wx + 1u, wy, seed)) * 2.f - 1.f;
+        float il = 1.f / sqrtf(a * a + b * b);
+        d[k] = a * il * (fx - ox) + b * il * (fy - oy);
+    }
+    float x0 = mixf(d[0], d[1], ux), x1 = mixf(d[2], d[3], ux);
+    return mixf(x0, x1, uy);
+}
+__device__ __forceinline__ float potential(float px, float py, uint32_t seed) {
+    return perlin2(px, py, 4, seed) + 0.5f * perlin2(px, py, 8, seed);
+}
+__global__ __launch_bounds__(64) void k_noise_curl(uint32_t seed, int format, void* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 128 * 128) return;
+    int x = i & 127, y = i >> 7;
+    float px = (x + 0.5f) / 128.f, py = (y + 0.5f) / 128.f, eps = 1.f / 128.f;
+    float dpdx = (potential(px + eps, py, seed) - potential(px - eps, py, seed)) / (2.f * eps);
+    float dpdy = (potential(px, py + eps, seed) - potential(px, py - eps, seed)) / (2.f * eps);
+    float r = clampf(dpdy * 0.5f + 0.5f, 0.f, 1.f), g = clampf(-dpdx * 0.5f + 0.5f, 0.f, 1.f);
+    if (format == 0) {
+        ((uchar2*)out)[i] = make_uchar2(to_unorm8(r), to_unorm8(g));
+    } else {
+        ((__half2*)out)[i] = __floats2half2_rn(r, g);
+    }
+}
+
+__device__ __forceinline__ void load4(const void* src, size_t i, int format, float& a, float& b, float& c, float& d) {
+    if (format == 0) {
+        uchar4 q = ((const uchar4*)src)[i];
+        a = (float)q.x / 255.f; b = (float)q.y / 255.f; c = (float)q.z / 255.f; d = (float)q.w / 255.f;
+    } else {
+        const __half2* q = (const __half2*)src + 2 * i;
+        float2 lo = __half22float2(q[0]), hi = __half22float2(q[1]);
+        a = lo.x; b = lo.y; c = hi.x; d = hi.y;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fold(int format, const void* base, const void* detail, const void* curl,
+                                              float2* base_rl, float* detail_hf, float2* curl2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 128 * 128 * 128) {
+        float r, g, b, a;
+        load4(base, (size_t)i, format, r, g, b, a);
+        base_rl[i] = make_float2(r, g * 0.625f + b * 0.25f + a * 0.125f);
+    }
+    if (i < 32 * 32 * 32) {
+        float r, g, b, a;
+        load4(detail, (size_t)i, format, r, g, b, a);
+        detail_hf[i] = r * 0.625f + g * 0.25f + b * 0.125f;
+    }
+    if (i < 128 * 128) {
+        float r, g;
+        if (format == 0) {
+            uchar2 q = ((const uchar2*)curl)[i];
+            r = (float)q.x / 255.f; g = (float)q.y / 255.f;
+        } else {
+            float2 v = __half22float2(((const __half2*)curl)[i]);
+            r = v.x; g = v.y;
+        }
+        curl2[i] = make_float2(r * 2.f - 1.f, g * 2.f - 1.f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_weather_fold(const uint16_t* src, int n, int channels, float* dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    __half h;
+    uint16_t bits = src[(size_t)i * channels];
+    memcpy(&h, &bits, 2);
+    dst[i] = __half2float(h);
+}
+
+}  // namespace
+
+hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* detail, void* curl, hipStream_t s) {
+    hipLaunchKernelGGL(k_noise_base, dim3(128 * 128 * 128 / 64), dim3(64), 0, s, seed, format, base);
+    hipLaunchKernelGGL(k_noise_detail, dim3(32 * 32 * 32 / 64), dim3(64), 0, s, seed, format, detail);
+    hipLaunchKernelGGL(k_noise_curl, dim3(128 * 128 / 64), dim3(64), 0, s, seed, format, curl);
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
+                                float2* base_rl, float* detail_hf, float2* curl2, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl, base_rl,
+                       detail_hf, curl2);
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, float* dst, hipStream_t s) {
+    int n = w * h;
+    hipLaunchKernelGGL(k_weather_fold, dim3((n + 255) / 256), dim3(256), 0, s, src, n, channels, dst);
+    return hipGetLastError();
+}

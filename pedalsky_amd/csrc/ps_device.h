@@ -1,0 +1,161 @@
+// Shared definitions for the gfx950 kernels of the sky + cloud transport
+// path: kernel parameter blocks (built on the host, read by every lane
+// through the scalar cache) and the device helpers common to the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#define PS_MAX_LAYERS 8
+#define PS_MAX_OCTAVES 8
+#define PS_PI 3.14159265358979
+
+// ---------------------------------------------------------------------------
+// Atmosphere parameters in double precision (the LUT kernels run in FP64;
+// there are ~1e6 LUT steps per frame against ~1e8 march steps).
+// shaders/common/uniforms.wgsl:39-58.
+// ---------------------------------------------------------------------------
+struct AtmD {
+    double R, top, hr, hm;
+    double ray_s[3], mie_s[3], mie_a[3], ozone[3], albedo[3];
+    double mie_g, oz_c, oz_t;
+};
+
+// Frame data the view-dependent LUTs (sky-view, aerial perspective) need.
+struct LutFrameD {
+    double inv_vp[16];
+    double cam[3];
+    double sun[3];
+    double E[3];
+};
+
+struct LutParams {
+    AtmD a;
+    LutFrameD f;
+    int tw, th, mw, mh, sw, sh, aw, ah, ad;
+    int ms_disabled;
+};
+
+// ---------------------------------------------------------------------------
+// Render parameters: one block per scene. Everything that is constant over a
+// frame is folded here by the host (in double, then rounded once).
+// ---------------------------------------------------------------------------
+struct LayerC {
+    // march-side (FP32)
+    float base, top, inv_thick, coverage;
+    float density_scale, esb, ld, anvil;
+    float sigma_t[3];
+    float wind_off_x, wind_off_z;     // drift offset (m)
+    float skew_x, skew_z;             // skew_dir * skew_thickness (m per unit h)
+    uint32_t type;
+    uint32_t is_cb;                   // layer type 7: phase depends on height
+    float droplet;                    // raw layer droplet diameter (um)
+    // setup-side (FP64)
+    double r_bottom, r_top;
+    // phase constants for non-Cb layers (cloud_march.wgsl:256-272)
+    double g_hg, g_d, alpha, w_d, ice;
+};
+
+struct RenderParams {
+    int W, H;
+    float inv_W, inv_H;
+    double inv_vp[16];
+    double cam[3];
+    double sun_d[3];
+    double cos_disk, sun_ang;
+    float sun[3];
+    float E[3];
+    float toa;
+    // atmosphere (FP32 copies)
+    float R, top, big_h, top_minus_R;
+    // sky pass
+    float cam_zha;                    // zenith-horizon angle at the camera radius
+    float sun_az;
+    float overcast_lum;               // overcast_diffuse_luminance (grey)
+    float cam_f[3];
+    float cam_alt;                    // max(|cam_atm|, R + 1) - R
+    // clouds
+    int n_layers, steps, light_steps, octaves;
+    int cone, temporal_jitter, use_type_grid, has_depth;
+    uint32_t frame_index;
+    float powder, forward_bias, ambient;
+    float curl_off_scale;             // curl_strength * detail_scale_m
+    float inv_detail_scale, inv_base_scale, inv_weather_scale;
+    float sigma_s[3];
+    float ms_a, ms_b, ms_c, droplet_bias;
+    float b_pow[PS_MAX_OCTAVES];      // multi_scatter_b^n
+    float a_pow[PS_MAX_OCTAVES];      // multi_scatter_a^n
+    float c_pow[PS_MAX_OCTAVES];      // multi_scatter_c^n
+    float light_rgb[3], light_radius, light_x, light_z;
+    double r0;                        // camera radius |cam - centre|
+    double h0;                        // r0 - R
+    LayerC layers[PS_MAX_LAYERS];
+    // resources
+    const float4* T; int tw, th;
+    const float4* SV; int sw, sh;
+    const float4* AP; int aw, ah, ad;
+    const float2* base_rl;            // 128^3 (R, 0.625G + 0.25B + 0.125A)
+    const float* detail_hf;           // 32^3 (0.625R + 0.25G + 0.125B)
+    const float2* curl;               // 128^2 (2*rg - 1)
+    const float* weather;             // ww*wh coverage
+    int ww, wh;
+    const uint8_t* type_grid;         // 128^2
+    const uint8_t* overcast;          // 128^2
+    const uint8_t* blue;              // 64^2
+    const float* depth;               // W*H reverse-Z NDC or null
+    void* out_comp;
+    void* out_lum;
+    void* out_trans;
+    int out_f16;
+    int _pad;
+    unsigned long long* counters;     // 10 x u64 when counting
+};
+
+// ---------------------------------------------------------------------------
+// Small vector helpers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ double clampd(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+__device__ __forceinline__ double smoothstepd(double e0, double e1, double x) {
+    double t = clampd((x - e0) / (e1 - e0), 0.0, 1.0);
+    return t * t * (3.0 - 2.0 * t);
+}
+__device__ __forceinline__ int wrapi(int i, int n) { int m = i % n; return m < 0 ? m + n : m; }
+__device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
+// Bilinear ClampToEdge sample of an RGBA float LUT in float.
+__device__ __forceinline__ float4 lut2d(const float4* __restrict__ t, int w, int h, float u, float v) {
+    float cx = u * (float)w - 0.5f, cy = v * (float)h - 0.5f;
+    float fx0 = floorf(cx), fy0 = floorf(cy);
+    float fx = cx - fx0, fy = cy - fy0;
+    int ix = (int)fx0, iy = (int)fy0;
+    int x0 = clampi(ix, w), x1 = clampi(ix + 1, w), y0 = clampi(iy, h), y1 = clampi(iy + 1, h);
+    float4 a = t[y0 * w + x0], b = t[y0 * w + x1], c = t[y1 * w + x0], d = t[y1 * w + x1];
+    float4 r;
+    float tx, bx;
+    tx = a.x + (b.x - a.x) * fx; bx = c.x + (d.x - c.x) * fx; r.x = tx + (bx - tx) * fy;
+    tx = a.y + (b.y - a.y) * fx; bx = c.y + (d.y - c.y) * fx; r.y = tx + (bx - tx) * fy;
+    tx = a.z + (b.z - a.z) * fx; bx = c.z + (d.z - c.z) * fx; r.z = tx + (bx - tx) * fy;
+    tx = a.w + (b.w - a.w) * fx; bx = c.w + (d.w - c.w) * fx; r.w = tx + (bx - tx) * fy;
+    return r;
+}
+
+// Accurate 1 - exp(-x) for x >= 0 (the energy-conserving step factor).
+// For small x the direct form loses all digits in FP32; a short series is
+// used there, the exponential elsewhere.
+__device__ __forceinline__ float one_minus_exp_neg(float x) {
+    if (x < 0.0625f) {
+        // x - x^2/2 + x^3/6 - x^4/24 + x^5/120: error < x^6/720 ~ 8e-11 at x = 1/16
+        float p = 1.f / 120.f;
+        p = p * (-x) + 1.f / 24.f;
+        p = p * (-x) + 1.f / 6.f;
+        p = p * (-x) + 0.5f;
+        p = p * (-x) + 1.f;
+        return x * p;
+    }
+    return 1.f - __expf(-x);
+}

@@ -1,0 +1,709 @@
+// Fused per-pixel sky pass + volumetric cloud march + AP + composite for
+// gfx950 (shaders/atmosphere/sky.wgsl:159-220, shaders/clouds/cloud_march.wgsl:731-1054,
+// shaders/clouds/cloud_composite.wgsl:47-57 with the dual-source blend of
+// crates/ps-clouds/src/pipeline.rs:469-484).
+//
+// Design
+//  * One lane per pixel; each wave64 owns an 8x8 pixel tile so its rays stay
+//    coherent (neighbouring gathers hit the same cache lines) and its lanes
+//    leave the march loop together; a 256-thread block covers 16x16 pixels.
+//  * Per-pixel ray setup is FP64: the view ray from inv_view_proj, the
+//    ray/shell intersections (|o-c|^2 - r^2 at planet scale), the sun-disk
+//    test and the per-layer phase coefficients. The step loop is FP32.
+//  * Noise volumes are pre-folded: the base volume stores (R, 0.625G +
+//    0.25B + 0.125A) so the trilinear fetch interpolates two channels, the
+//    detail volume stores its FBM sum, the curl map stores 2*rg - 1.
+//    (Trilinear filtering is linear, so folding the channel weights before
+//    or after the fetch is the same function.)
+//  * Numerically stable restatements of FP32-fragile reference formulas:
+//    altitudes from h instead of |p| - R, vza via atan2 instead of acos near
+//    the zenith, 1 - exp(-x) via a series for thin steps.
+#include "ps_device.h"
+#include "ps_launch.h"
+
+namespace {
+
+struct Cnt {
+    uint32_t layer_marches = 0, primary = 0, covered = 0, eroded = 0, dense = 0, light = 0, light_cov = 0,
+             octaves = 0, ap = 0;
+};
+
+// Trilinear Repeat fetch of the folded base volume (128^3 float2).
+__device__ __forceinline__ float2 base_fetch(const float2* __restrict__ v, float cx, float cy, float cz) {
+    float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
+    float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
+    int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+    int x0 = ix & 127, x1 = (ix + 1) & 127;
+    int y0 = (iy & 127) << 7, y1 = ((iy + 1) & 127) << 7;
+    int z0 = (iz & 127) << 14, z1 = ((iz + 1) & 127) << 14;
+    float2 c000 = v[z0 + y0 + x0], c100 = v[z0 + y0 + x1];
+    float2 c010 = v[z0 + y1 + x0], c110 = v[z0 + y1 + x1];
+    float2 c001 = v[z1 + y0 + x0], c101 = v[z1 + y0 + x1];
+    float2 c011 = v[z1 + y1 + x0], c111 = v[z1 + y1 + x1];
+    float2 r;
+    {
+        float a0 = c000.x + (c100.x - c000.x) * fx, a1 = c010.x + (c110.x - c010.x) * fx;
+        float b0 = c001.x + (c101.x - c001.x) * fx, b1 = c011.x + (c111.x - c011.x) * fx;
+        float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
+        r.x = p0 + (p1 - p0) * fz;
+    }
+    {
+        float a0 = c000.y + (c100.y - c000.y) * fx, a1 = c010.y + (c110.y - c010.y) * fx;
+        float b0 = c001.y + (c101.y - c001.y) * fx, b1 = c011.y + (c111.y - c011.y) * fx;
+        float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
+        r.y = p0 + (p1 - p0) * fz;
+    }
+    return r;
+}
+
+// Trilinear Repeat fetch of the folded detail volume (32^3 float).
+__device__ __forceinline__ float detail_fetch(const float* __restrict__ v, float cx, float cy, float cz) {
+    float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
+    float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
+    int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+    int x0 = ix & 31, x1 = (ix + 1) & 31;
+    int y0 = (iy & 31) << 5, y1 = ((iy + 1) & 31) << 5;
+    int z0 = (iz & 31) << 10, z1 = ((iz + 1) & 31) << 10;
+    float c000 = v[z0 + y0 + x0], c100 = v[z0 + y0 + x1];
+    float c010 = v[z0 + y1 + x0], c110 = v[z0 + y1 + x1];
+    float c001 = v[z1 + y0 + x0], c101 = v[z1 + y0 + x1];
+    float c011 = v[z1 + y1 + x0], c111 = v[z1 + y1 + x1];
+    float a0 = c000 + (c100 - c000) * fx, a1 = c010 + (c110 - c010) * fx;
+    float b0 = c001 + (c101 - c001) * fx, b1 = c011 + (c111 - c011) * fx;
+    float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
+    return p0 + (p1 - p0) * fz;
+}
+
+// Bilinear Repeat fetch of the curl map (128^2 float2).
+__device__ __forceinline__ float2 curl_fetch(const float2* __restrict__ v, float cx, float cy) {
+    float fx0 = floorf(cx), fy0 = floorf(cy);
+    float fx = cx - fx0, fy = cy - fy0;
+    int ix = (int)fx0, iy = (int)fy0;
+    int x0 = ix & 127, x1 = (ix + 1) & 127, y0 = (iy & 127) << 7, y1 = ((iy + 1) & 127) << 7;
+    float2 a = v[y0 + x0], b = v[y0 + x1], c = v[y1 + x0], d = v[y1 + x1];
+    float2 r;
+    float t0 = a.x + (b.x - a.x) * fx, t1 = c.x + (d.x - c.x) * fx;
+    r.x = t0 + (t1 - t0) * fy;
+    t0 = a.y + (b.y - a.y) * fx; t1 = c.y + (d.y - c.y) * fx;
+    r.y = t0 + (t1 - t0) * fy;
+    return r;
+}
+
+// Bilinear Repeat fetch of the weather coverage plane (any size).
+__device__ __forceinline__ float weather_fetch(const float* __restrict__ v, int w, int h, float u, float vv) {
+    u = u - floorf(u);
+    vv = vv - floorf(vv);
+    float cx = u * (float)w - 0.5f, cy = vv * (float)h - 0.5f;
+    float fx0 = floorf(cx), fy0 = floorf(cy);
+    float fx = cx - fx0, fy = cy - fy0;
+    int ix = (int)fx0, iy = (int)fy0;
+    // u, vv in [0, 1] so ix, iy in [-1, n-1]; the final clamps only guard
+    // non-finite coordinates.
+    int x0 = clampi(ix < 0 ? w - 1 : ix, w), x1 = clampi(ix + 1 >= w ? ix + 1 - w : ix + 1, w);
+    int y0 = clampi(iy < 0 ? h - 1 : iy, h), y1 = clampi(iy + 1 >= h ? iy + 1 - h : iy + 1, h);
+    float a = v[y0 * w + x0], b = v[y0 * w + x1], c = v[y1 * w + x0], d = v[y1 * w + x1];
+    float t0 = a + (b - a) * fx, t1 = c + (d - c) * fx;
+    return t0 + (t1 - t0) * fy;
+}
+
+// cloud_density_common.wgsl:159-201 (WGSL smoothstep with reversed edges is
+// the same polynomial formula).
+__device__ __forceinline__ float ndf_f(float h, uint32_t t, float anvil) {
+    switch (t) {
+        case 0: return smoothstepf(0.f, 0.07f, h) * smoothstepf(1.f, 0.2f, h);
+        case 1: return smoothstepf(0.f, 0.10f, h) * (1.f - smoothstepf(0.6f, 1.f, h));
+        case 2: return smoothstepf(0.f, 0.15f, h) * smoothstepf(1.f, 0.4f, h);
+        case 3: return smoothstepf(0.f, 0.20f, h) * smoothstepf(1.f, 0.3f, h);
+        case 4: return smoothstepf(0.f, 0.30f, h) * (1.f - smoothstepf(0.7f, 1.f, h));
+        case 5: return smoothstepf(0.f, 0.40f, h) * (1.f - smoothstepf(0.6f, 1.f, h));
+        case 6: return smoothstepf(0.f, 0.50f, h) * (1.f - smoothstepf(0.8f, 1.f, h));
+        case 7: {
+            float base = smoothstepf(0.f, 0.05f, h);
+            float mid = smoothstepf(0.95f, 0.5f, h);
+            float anv = smoothstepf(0.7f, 0.9f, h) * 1.5f * clampf(anvil, 0.f, 2.f);
+            float mt = smoothstepf(0.65f, 0.8f, h);
+            return base * (mid + (anv - mid) * mt);
+        }
+        default: return 0.f;
+    }
+}
+
+// Base shape through the coverage remap (shared by sample_density and
+// sample_density_light). Returns the pre-erosion cloud value and writes the
+// advected position. `cov` >= 0.01 here, so remap(c, 1-cov, 1, 0, 1) * cov
+// equals c - (1 - cov) exactly.
+__device__ __forceinline__ float shape(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+                                       float h, float cov, uint32_t ctype, float& pax, float& pay, float& paz) {
+    pax = px - L.wind_off_x - L.skew_x * h;
+    pay = py;
+    paz = pz - L.wind_off_z - L.skew_z * h;
+    float s = P.inv_base_scale * 128.f;
+    float2 b = base_fetch(P.base_rl, pax * s - 0.5f, pay * s - 0.5f, paz * s - 0.5f);
+    float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
+    float base_cloud = (b.x + (1.f - lf)) / fmaxf(2.f - lf, 1e-5f);
+    float cloud = base_cloud * ndf_f(h, ctype, L.anvil);
+    return cloud - (1.f - cov);
+}
+
+// cloud_density_common.wgsl:376-415
+__device__ __forceinline__ float density_light(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+                                               float alt, float weather_r, uint32_t ctype, bool& covered) {
+    float h = (alt - L.base) * L.inv_thick;
+    covered = false;
+    if (h < 0.f || h > 1.f) return 0.f;
+    float cov = clampf(weather_r * L.coverage, 0.f, 1.f);
+    if (cov < 0.01f) return 0.f;
+    covered = true;
+    float ax, ay, az;
+    float cloud = shape(P, L, px, py, pz, h, cov, ctype, ax, ay, az);
+    return clampf(cloud, 0.f, 1.f) * L.density_scale;
+}
+
+// cloud_march.wgsl:432-530
+__device__ __forceinline__ float density_primary(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+                                                 float alt, float weather_r, uint32_t ctype, int& stage) {
+    stage = 0;
+    float h = (alt - L.base) * L.inv_thick;
+    if (h < 0.f || h > 1.f) return 0.f;
+    float cov = clampf(weather_r * L.coverage, 0.f, 1.f);
+    if (cov < 0.01f) return 0.f;
+    stage = 1;
+    float ax, ay, az;
+    float cloud = shape(P, L, px, py, pz, h, cov, ctype, ax, ay, az);
+    if (cloud > 0.f) {
+        stage = 2;
+        float ds = P.inv_detail_scale;
+        float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
+        float dx = ax + cu.x * P.curl_off_scale, dz = az + cu.y * P.curl_off_scale;
+        float hf = detail_fetch(P.detail_hf, dx * ds * 32.f - 0.5f, ay * ds * 32.f - 0.5f, dz * ds * 32.f - 0.5f);
+        float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
+        float e = dmod * L.ld;
+        cloud = (cloud - e) / fmaxf(1.f - e, 1e-5f);
+    }
+    return clampf(cloud, 0.f, 1.f) * L.density_scale;
+}
+
+// cloud_march.wgsl:543-578 (straight) and 599-666 (cone), FP32.
+__device__ __forceinline__ float march_light(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+                                             float alt, float weather_r, uint32_t ctype, Cnt& cnt) {
+    const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
+    float cos_sun = fmaxf(sy, 0.05f);
+    float dist = fmaxf((L.top - alt) / cos_sun, 1.f);
+    float r0 = P.R + alt;
+    float qy = py + P.R;
+    float cv = (px * sx + qy * sy + pz * sz) / fmaxf(r0, 1.f);
+    float k = 2.f * r0 * cv;
+    float od = 0.f;
+    bool cov;
+    if (P.cone) {
+        const float off[6][3] = {{0.38051305f, 0.92453449f, -0.02111345f}, {-0.50625799f, -0.03590792f, -0.86163418f},
+                                 {-0.32509218f, -0.94557439f, 0.01428793f}, {0.09026238f, -0.27376545f, 0.95755165f},
+                                 {0.28128598f, 0.42443639f, -0.86065785f}, {-0.16852403f, 0.14748697f, 0.97460106f}};
+        float step = dist / 5.f;
+        for (int i = 0; i < 6; ++i) {
+            float t, scale, w;
+            if (i < 5) { t = (i + 0.5f) * step; scale = step * (i + 1.f) / 5.f; w = step; }
+            else { t = dist * 3.f; scale = step * 3.f; w = step * 3.f; }
+            float qx = px + sx * t + off[i][0] * scale;
+            float qyy = py + sy * t + off[i][1] * scale;
+            float qz = pz + sz * t + off[i][2] * scale;
+            float yy = qyy + P.R;
+            float rr = sqrtf(qx * qx + yy * yy + qz * qz);
+            float a = alt + t * (k + t) / (fmaxf(r0, 1.f) + rr);
+            float d = density_light(P, L, qx, qyy, qz, a, weather_r, ctype, cov);
+            cnt.light++;
+            cnt.light_cov += cov ? 1u : 0u;
+            od += d * w;
+        }
+        return od;
+    }
+    float step = dist / (float)P.light_steps;
+    for (int i = 0; i < P.light_steps; ++i) {
+        float t = (i + 0.5f) * step;
+        float qx = px + sx * t, qyy = py + sy * t, qz = pz + sz * t;
+        float yy = qyy + P.R;
+        float rr = sqrtf(qx * qx + yy * yy + qz * qz);
+        float a = alt + t * (k + t) / (fmaxf(r0, 1.f) + rr);
+        float d = density_light(P, L, qx, qyy, qz, a, weather_r, ctype, cov);
+        cnt.light++;
+        cnt.light_cov += cov ? 1u : 0u;
+        od += d * step;
+    }
+    return od;
+}
+
+// Transmittance-LUT coordinates for a point at altitude h whose local up
+// makes cosine mu with the sun (cloud_march.wgsl:328-354), computed from h so
+// r^2 - R^2 never cancels. Returns false inside the Earth's shadow.
+__device__ __forceinline__ bool trans_uv_from_h(const RenderParams& P, float h, float mu, float& u, float& v) {
+    float r = P.R + h;
+    float rho = sqrtf(fmaxf(h * (2.f * P.R + h), 0.f));
+    if (mu <= -rho / r) return false;
+    float dtop = P.top_minus_R - h;                  // top - r
+    float disc = dtop * (P.top + r) + (r * mu) * (r * mu);
+    float sq = sqrtf(fmaxf(disc, 0.f));
+    float d = mu >= 0.f ? (dtop * (P.top + r)) / fmaxf(r * mu + sq, 1e-30f) : -r * mu + sq;
+    d = fmaxf(d, 0.f);
+    float d_min = dtop, d_max = rho + P.big_h;
+    u = clampf((d - d_min) / fmaxf(d_max - d_min, 1.f), 0.f, 1.f);
+    v = rho / fmaxf(P.big_h, 1.f);
+    return true;
+}
+
+// Sky-view LUT coordinates (sky.wgsl:73-102, cloud_march.wgsl:672-691) for a
+// direction given by its horizontal extent rxz, vertical component vy and
+// (x, z) for the azimuth; zha is the zenith-horizon angle of the viewer.
+__device__ __forceinline__ void skyview_uv_f(const RenderParams& P, float dx, float dy, float dz, float zha,
+                                             float& u, float& v) {
+    float rxz = sqrtf(dx * dx + dz * dz);
+    float vza = atan2f(rxz, dy);
+    if (vza < zha) {
+        v = sqrtf(clampf(vza / fmaxf(zha, 1e-6f), 0.f, 1.f)) * 0.5f;
+    } else {
+        float below = fmaxf((float)PS_PI - zha, 1e-6f);
+        v = 0.5f + sqrtf(clampf((vza - zha) / below, 0.f, 1.f)) * 0.5f;
+    }
+    float du = (atan2f(dx, dz) - P.sun_az) * (float)(1.0 / (2.0 * PS_PI));
+    u = du - floorf(du);
+}
+
+// Per-pixel, per-octave sun phase (cloud_march.wgsl:278-296) in FP64, in the
+// (1-g)^2 + 2g(1-cos) form of the HG/Draine denominator.
+__device__ __forceinline__ double phase_octave(double omc, double c, double g_scale, double g_hg, double g_d,
+                                               double alpha, double w_d, double ice) {
+    double g = g_hg * g_scale;
+    double omg = 1.0 - g;
+    double den = fmax(omg * omg + 2.0 * g * omc, 1e-4);
+    double p_hg = (1.0 - g * g) / (4.0 * PS_PI * den * sqrt(den));
+    double gd = g_d * g_scale;
+    double omgd = 1.0 - gd;
+    double dend = fmax(omgd * omgd + 2.0 * gd * omc, 1e-4);
+    double num = (1.0 - gd * gd) * (1.0 + alpha * c * c);
+    double dd = 4.0 * (1.0 + alpha * (1.0 + 2.0 * gd * gd) / 3.0) * PS_PI * dend * sqrt(dend);
+    double p = (1.0 - w_d) * p_hg + w_d * (num / fmax(dd, 1e-6));
+    if (ice > 0.0) {
+        double width = (0.5 * PS_PI / 180.0) / fmax(g_scale, 0.1);
+        double s22 = width * sqrt(fmax(1.0 - 0.92718 * 0.92718, 1e-4));
+        double s46 = width * sqrt(fmax(1.0 - 0.69466 * 0.69466, 1e-4));
+        double d22 = (c - 0.92718) / fmax(s22, 1e-4), d46 = (c - 0.69466) / fmax(s46, 1e-4);
+        p += ice * (0.08 * exp(-0.5 * d22 * d22) + 0.03 * exp(-0.5 * d46 * d46));
+    }
+    return p;
+}
+
+// FP32 per-sample phase for cumulonimbus layers (droplet size varies with h).
+__device__ __forceinline__ float phase_octave_f(float omc, float c, float g_scale, float g_hg, float omg_hg,
+                                                float g_d, float alpha, float w_d, float ice) {
+    float g = g_hg * g_scale;
+    float omg = g_scale == 1.f ? omg_hg : 1.f - g;
+    float den = fmaxf(omg * omg + 2.f * g * omc, 1e-4f);
+    float p_hg = (1.f - g * g) / (4.f * (float)PS_PI * den * sqrtf(den));
+    float gd = g_d * g_scale;
+    float omgd = 1.f - gd;
+    float dend = fmaxf(omgd * omgd + 2.f * gd * omc, 1e-4f);
+    float num = (1.f - gd * gd) * (1.f + alpha * c * c);
+    float dd = 4.f * (1.f + alpha * (1.f + 2.f * gd * gd) / 3.f) * (float)PS_PI * dend * sqrtf(dend);
+    float p = (1.f - w_d) * p_hg + w_d * (num / fmaxf(dd, 1e-6f));
+    if (ice > 0.f) {
+        float width = (float)(0.5 * PS_PI / 180.0) / fmaxf(g_scale, 0.1f);
+        float s22 = width * 0.3746002f, s46 = width * 0.7193335f;  // sqrt(1 - cos^2) at 22 and 46 degrees
+        float d22 = (c - 0.92718f) / fmaxf(s22, 1e-4f), d46 = (c - 0.69466f) / fmaxf(s46, 1e-4f);
+        p += ice * (0.08f * __expf(-0.5f * d22 * d22) + 0.03f * __expf(-0.5f * d46 * d46));
+    }
+    return p;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__ Pp) {
+    const RenderParams& P = *Pp;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (x >= P.W || y >= P.H) return;
+    Cnt cnt;
+
+    // ---- FP64 view ray (math.wgsl:23-33 / sky.wgsl:166-168) -------------------------
+    const double fx = x + 0.5, fy = y + 0.5;
+    const double ndx = fx / P.W * 2.0 - 1.0, ndy = 1.0 - fy / P.H * 2.0;
+    const double* m = P.inv_vp;
+    double nh0 = m[0] * ndx + m[4] * ndy + m[8] + m[12];
+    double nh1 = m[1] * ndx + m[5] * ndy + m[9] + m[13];
+    double nh2 = m[2] * ndx + m[6] * ndy + m[10] + m[14];
+    double nh3 = m[3] * ndx + m[7] * ndy + m[11] + m[15];
+    double dxd = nh0 / nh3 - P.cam[0], dyd = nh1 / nh3 - P.cam[1], dzd = nh2 / nh3 - P.cam[2];
+    double il = 1.0 / sqrt(dxd * dxd + dyd * dyd + dzd * dzd);
+    dxd *= il; dyd *= il; dzd *= il;
+    const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+    // cos(theta) against the sun direction as uploaded (not renormalised, as
+    // the reference uses it); 1 - cos in FP64 keeps the forward HG peak exact.
+    const double cos_vs = dxd * P.sun_d[0] + dyd * P.sun_d[1] + dzd * P.sun_d[2];
+    const double omc = 1.0 - cos_vs;
+
+    // ---- sky pass (sky.wgsl:159-220) ------------------------------------------------------
+    float3 sky;
+    {
+        float u, v;
+        skyview_uv_f(P, dx, dy, dz, P.cam_zha, u, v);
+        float4 s = lut2d(P.SV, P.sw, P.sh, u, v);
+        float cr = s.x * P.E[0], cg = s.y * P.E[1], cb = s.z * P.E[2];
+        if (dy > 0.f) {
+            float sxz0 = P.cam_f[0], sxz1 = P.cam_f[2];
+            if (dy > 1e-3f) {
+                float dh = fmaxf(1500.f - P.cam_f[1], 0.f);
+                float t = dh / dy;
+                sxz0 = P.cam_f[0] + dx * t;
+                sxz1 = P.cam_f[2] + dz * t;
+            }
+            float mu_ = clampf((sxz0 + 16000.f) / 32000.f, 0.f, 1.f), mv = clampf((sxz1 + 16000.f) / 32000.f, 0.f, 1.f);
+            float mask = 0.f;
+            if (P.overcast) {
+                float cx = mu_ * 128.f - 0.5f, cy = mv * 128.f - 0.5f;
+                float fx0 = floorf(cx), fy0 = floorf(cy);
+                float wx = cx - fx0, wy = cy - fy0;
+                int ix = (int)fx0, iy = (int)fy0;
+                int x0 = clampi(ix, 128), x1 = clampi(ix + 1, 128), y0 = clampi(iy, 128), y1 = clampi(iy + 1, 128);
+                float a = P.overcast[y0 * 128 + x0], b = P.overcast[y0 * 128 + x1];
+                float c = P.overcast[y1 * 128 + x0], d = P.overcast[y1 * 128 + x1];
+                float t0 = a + (b - a) * wx, t1 = c + (d - c) * wx;
+                mask = (t0 + (t1 - t0) * wy) * (1.f / 255.f);
+            }
+            float blocking = 1.f - __expf(-10.f * mask);
+            cr = cr + (P.overcast_lum - cr) * blocking;
+            cg = cg + (P.overcast_lum - cg) * blocking;
+            cb = cb + (P.overcast_lum - cb) * blocking;
+        }
+        if (cos_vs > P.cos_disk) {
+            double theta = acos(clampd(cos_vs, -1.0, 1.0));
+            double tn = theta / fmax(P.sun_ang, 1e-6);
+            double limb = 1.0 - 0.6 * (1.0 - sqrt(fmax(1.0 - tn * tn, 0.0)));
+            double omega = PS_PI * P.sun_ang * P.sun_ang;
+            double sc = limb / fmax(omega, 1e-6);
+            float mu_c = (float)(((P.cam[0]) * P.sun_d[0] + (P.cam[1] + P.R) * P.sun_d[1] + (P.cam[2]) * P.sun_d[2]) /
+                                 (P.R + (double)P.cam_alt));
+            float tu, tv;
+            float3 vis = make_float3(0.f, 0.f, 0.f);
+            if (trans_uv_from_h(P, P.cam_alt, mu_c, tu, tv)) {
+                float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
+                vis = make_float3(tt.x, tt.y, tt.z);
+            }
+            cr += (float)(P.E[0] * sc) * vis.x;
+            cg += (float)(P.E[1] * sc) * vis.y;
+            cb += (float)(P.E[2] * sc) * vis.z;
+        }
+        sky = make_float3(fmaxf(cr, 0.f), fmaxf(cg, 0.f), fmaxf(cb, 0.f));
+    }
+
+    // ---- blue-noise jitter (cloud_march.wgsl:745-756) ------------------------------------
+    int jx = x & 63, jy = y & 63;
+    if (P.temporal_jitter) {
+        uint32_t ph = P.frame_index & 15u;
+        jx = (jx ^ (int)((ph * 11u + 7u) & 63u)) & 63;
+        jy = (jy ^ (int)((ph * 23u + 13u) & 63u)) & 63;
+    }
+    const float jitter = (float)P.blue[jy * 64 + jx] * (1.f / 255.f);
+
+    // ---- depth clip (cloud_march.wgsl:764-775) --------------------------------------------
+    double depth_t_max = 1e30;
+    if (P.has_depth) {
+        float dn = P.depth[(size_t)y * P.W + x];
+        if (dn > 0.f) {
+            double w0 = m[0] * ndx + m[4] * ndy + m[8] * dn + m[12];
+            double w1 = m[1] * ndx + m[5] * ndy + m[9] * dn + m[13];
+            double w2 = m[2] * ndx + m[6] * ndy + m[10] * dn + m[14];
+            double w3 = m[3] * ndx + m[7] * ndy + m[11] * dn + m[15];
+            double qx = w0 / w3 - P.cam[0], qy = w1 / w3 - P.cam[1], qz = w2 / w3 - P.cam[2];
+            depth_t_max = fmax(sqrt(qx * qx + qy * qy + qz * qz), 0.0);
+        }
+    }
+
+    // ---- per-layer intervals in FP64 (cloud_march.wgsl:370-416, 777-806) ---------------
+    double t0s[PS_MAX_LAYERS], t1s[PS_MAX_LAYERS];
+    int idx[PS_MAX_LAYERS];
+    const int lc = P.n_layers;
+    const double ocx = P.cam[0], ocy = P.cam[1] + P.R, ocz = P.cam[2];
+    const double b_ = ocx * dxd + ocy * dyd + ocz * dzd;
+    const double oc2 = ocx * ocx + ocy * ocy + ocz * ocz;
+    const double r_cam = sqrt(oc2);
+    int nh = 0;
+    for (int l = 0; l < lc; ++l) {
+        const LayerC& L = P.layers[l];
+        // |oc|^2 - r^2 as (|oc| - r)(|oc| + r)
+        double ct = (r_cam - L.r_top) * (r_cam + L.r_top), cbt = (r_cam - L.r_bottom) * (r_cam + L.r_bottom);
+        double disc_t = b_ * b_ - ct, disc_b = b_ * b_ - cbt;
+        bool hit_top = disc_t >= 0.0, hit_bot = disc_b >= 0.0;
+        double sqt = hit_top ? sqrt(disc_t) : 0.0, sqb = hit_bot ? sqrt(disc_b) : 0.0;
+        double tt0 = -b_ - sqt, tt1 = -b_ + sqt, tb0 = -b_ - sqb, tb1 = -b_ + sqb;
+        bool ok = hit_top;
+        double ta = 0.0, tb = 0.0;
+        if (ok) {
+            if (r_cam < L.r_bottom) {
+                if (!hit_bot) ok = false;
+                ta = fmax(tb1, 0.0);
+                tb = fmax(tt1, 0.0);
+            } else if (r_cam > L.r_top) {
+                ta = fmax(tt0, 0.0);
+                tb = (hit_bot && tb0 > 0.0) ? fmax(tb0, 0.0) : fmax(tt1, 0.0);
+            } else {
+                ta = 0.0;
+                tb = (hit_bot && tb0 > 0.0) ? fmin(fmax(tt1, 0.0), tb0) : fmax(tt1, 0.0);
+            }
+            ok = ok && (tb > ta);
+        }
+        bool flag = false;
+        if (ok) {
+            tb = fmin(tb, depth_t_max);
+            flag = tb > ta;
+        }
+        // insertion by t0 (stable: equal keys keep layer order, as the reference sort)
+        if (flag) {
+            int j = nh++;
+            while (j > 0 && t0s[j - 1] > ta) {
+                t0s[j] = t0s[j - 1]; t1s[j] = t1s[j - 1]; idx[j] = idx[j - 1];
+                --j;
+            }
+            t0s[j] = ta;
+            t1s[j] = tb;
+            idx[j] = l;
+        }
+    }
+
+    // ---- march -------------------------------------------------------------------------------
+    float lr = 0.f, lg = 0.f, lb = 0.f;
+    float tr = 1.f, tg = 1.f, tbb = 1.f;
+    float tw = 0.f, twn = 0.f;
+    const float r0f = (float)P.r0;
+    const float kcam = (float)(2.0 * P.r0 * (ocx * dxd + ocy * dyd + ocz * dzd) / fmax(P.r0, 1.0));
+    const float h0f = (float)P.h0;
+    const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
+    const float fb_dir = smoothstepf(0.f, 1.f, fmaxf((float)cos_vs, 0.f));
+    const float omcf = (float)omc, cosf = (float)cos_vs;
+    for (int i = 0; i < nh; ++i) {
+        if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) break;
+        const LayerC& L = P.layers[idx[i]];
+        const double t0d = t0s[i];
+        const double seg = t1s[i] - t0d;
+        if (seg <= 0.0) continue;
+        cnt.layer_marches++;
+        const double stepd = seg / P.steps;
+        const float step = (float)stepd;
+        const float tstart = (float)(t0d + jitter * stepd);
+        const float sig0 = L.sigma_t[0], sig1 = L.sigma_t[1], sig2 = L.sigma_t[2];
+        // per-octave sun phase coefficients a^n * p_n(cos), per pixel and layer
+        float coef[PS_MAX_OCTAVES];
+        const int noct = P.octaves;
+        if (!L.is_cb) {
+#pragma unroll
+            for (int n = 0; n < PS_MAX_OCTAVES; ++n) {
+                coef[n] = n < noct ? (float)(P.a_pow[n] * phase_octave(omc, cos_vs, P.c_pow[n], L.g_hg, L.g_d,
+                                                                         L.alpha, L.w_d, L.ice))
+                                   : 0.f;
+            }
+        }
+        for (int s = 0; s < P.steps; ++s) {
+            cnt.primary++;
+            const float t = tstart + (float)s * step;
+            const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
+            const float qy = py + P.R;
+            const float rr = sqrtf(px * px + qy * qy + pz * pz);
+            const float alt = h0f + t * (kcam + t) / (fmaxf(r0f, 1.f) + rr);
+            const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
+            const float wr = weather_fetch(P.weather, P.ww, P.wh, wx * P.inv_weather_scale + 0.5f,
+                                           wz * P.inv_weather_scale + 0.5f);
+            uint32_t ctype = L.type;
+            if (P.use_type_grid) {
+                float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
+                int gx = (int)floorf(gu * 128.f), gy = (int)floorf(gv * 128.f);
+                gx = ((gx % 128) + 128) % 128;
+                gy = ((gy % 128) + 128) % 128;
+                uint32_t g = P.type_grid[gy * 128 + gx];
+                if (g != 255u) ctype = g;
+            }
+            int stage;
+            const float density = density_primary(P, L, px, py, pz, alt, wr, ctype, stage);
+            cnt.covered += stage >= 1 ? 1u : 0u;
+            cnt.eroded += stage >= 2 ? 1u : 0u;
+            if (density > 1e-3f) {
+                cnt.dense++;
+                const float od = march_light(P, L, px, py, pz, alt, wr, ctype, cnt);
+                // Beer-Powder (cloud_march.wgsl:888-892); exp(-2x) = exp(-x)^2
+                const float e0r = __expf(-sig0 * od), e0g = __expf(-sig1 * od), e0b = __expf(-sig2 * od);
+                const float pw = P.powder;
+                const float enr = e0r + (2.f * e0r * (1.f - e0r * e0r) - e0r) * pw;
+                const float eng = e0g + (2.f * e0g * (1.f - e0g * e0g) - e0g) * pw;
+                const float enb = e0b + (2.f * e0b * (1.f - e0b * e0b) - e0b) * pw;
+                const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
+                // sun transmittance (cloud_march.wgsl:328-354)
+                float str = 0.f, stg = 0.f, stb = 0.f;
+                {
+                    const float hh = fmaxf(alt, 1.f);
+                    const float rs = P.R + hh;
+                    const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / rs, -1.f, 1.f);
+                    float tu, tv;
+                    if (trans_uv_from_h(P, hh, mu, tu, tv)) {
+                        float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
+                        str = tt.x; stg = tt.y; stb = tt.z;
+                    }
+                }
+                // multi-octave sun term (cloud_march.wgsl:915-927)
+                float sir = 0.f, sig_ = 0.f, sib = 0.f;
+                if (!L.is_cb) {
+                    sir = coef[0] * e0r; sig_ = coef[0] * e0g; sib = coef[0] * e0b;
+#pragma unroll
+                    for (int n = 1; n < PS_MAX_OCTAVES; ++n) {
+                        if (n < noct) {
+                            const float bn = P.b_pow[n];
+                            sir += coef[n] * __expf(-sig0 * od * bn);
+                            sig_ += coef[n] * __expf(-sig1 * od * bn);
+                            sib += coef[n] * __expf(-sig2 * od * bn);
+                        }
+                    }
+                } else {
+                    // cumulonimbus mixed phase: droplet diameter blends to 50 um with height
+                    const float mt = smoothstepf(0.60f, 0.85f, hn);
+                    float d = L.droplet + (50.f - L.droplet) * mt;
+                    d = clampf(d * P.droplet_bias, 5.f, 50.f);
+                    const float xg = -0.0990567f / (d - 1.67154f);
+                    const float g_hg = __expf(xg), omg = -expm1f(xg);
+                    const float g_d = __expf(-2.20679f / (d + 3.91029f)) - 0.428934f;
+                    const float alpha = __expf(3.62489f - 8.29288f / (d + 5.52825f));
+                    const float w_d = __expf(-0.599085f / (d - 0.641583f) - 0.665888f);
+                    const float ice = smoothstepf(35.f, 50.f, d);
+                    for (int n = 0; n < noct && n < PS_MAX_OCTAVES; ++n) {
+                        const float cf = P.a_pow[n] * phase_octave_f(omcf, cosf, P.c_pow[n], g_hg, omg, g_d,
+                                                                     alpha, w_d, ice);
+                        const float bn = P.b_pow[n];
+                        sir += cf * __expf(-sig0 * od * bn);
+                        sig_ += cf * __expf(-sig1 * od * bn);
+                        sib += cf * __expf(-sig2 * od * bn);
+                    }
+                }
+                cnt.octaves += (uint32_t)noct;
+                sir *= P.E[0] * str; sig_ *= P.E[1] * stg; sib *= P.E[2] * stb;
+                if (P.forward_bias > 0.f) {
+                    const float fbm = 1.f + P.forward_bias * fb_dir;
+                    sir *= fbm; sig_ *= fbm; sib *= fbm;
+                }
+                // sky ambient at the local up (cloud_march.wgsl:693-700, 951-955)
+                float ar, ag, ab;
+                {
+                    const float hh = alt;
+                    const float rho = sqrtf(fmaxf(hh * (2.f * P.R + hh), 0.f));
+                    const float zha = (float)PS_PI - atan2f(P.R, rho);
+                    float u, v;
+                    skyview_uv_f(P, px, qy, pz, zha, u, v);
+                    const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
+                    const float g = P.ambient * (0.3f + 0.7f * hn);
+                    ar = sv.x * P.E[0] * g; ag = sv.y * P.E[1] * g; ab = sv.z * P.E[2] * g;
+                }
+                // lightning (cloud_march.wgsl:968-977)
+                if (P.light_radius > 0.f) {
+                    const float lx = px - P.light_x, lz = pz - P.light_z;
+                    const float fo = clampf(1.f - sqrtf(lx * lx + lz * lz) / P.light_radius, 0.f, 1.f);
+                    const float wgt = fo * fo * (3.f - 2.f * fo);
+                    ar += P.light_rgb[0] * wgt; ag += P.light_rgb[1] * wgt; ab += P.light_rgb[2] * wgt;
+                }
+                // energy-conserving step (cloud_march.wgsl:979-993)
+                const float inr = P.sigma_s[0] * density * (sir * enr + ar);
+                const float ing = P.sigma_s[1] * density * (sig_ * eng + ag);
+                const float inb = P.sigma_s[2] * density * (sib * enb + ab);
+                const float lsr = fmaxf(density * sig0, 1e-6f), lsg = fmaxf(density * sig1, 1e-6f),
+                            lsb = fmaxf(density * sig2, 1e-6f);
+                const float omr = one_minus_exp_neg(lsr * step), omg2 = one_minus_exp_neg(lsg * step),
+                            omb = one_minus_exp_neg(lsb * step);
+                const float slr = tr * (inr * omr / lsr), slg = tg * (ing * omg2 / lsg), slb = tbb * (inb * omb / lsb);
+                const float sw = slr * 0.2126f + slg * 0.7152f + slb * 0.0722f;
+                lr += slr; lg += slg; lb += slb;
+                tw += sw * t;
+                twn += sw;
+                tr *= 1.f - omr; tg *= 1.f - omg2; tbb *= 1.f - omb;
+            }
+            if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) break;
+        }
+    }
+
+    // ---- aerial perspective on the cloud luminance (cloud_march.wgsl:1016-1048) ----
+    const float t_lum = tr * 0.2126f + tg * 0.7152f + tbb * 0.0722f;
+    const float opacity = 1.f - t_lum;
+    float olr = lr, olg = lg, olb = lb;
+    if (opacity > 1e-4f) {
+        cnt.ap++;
+        const float tc = tw / fmaxf(twn, 1e-6f);
+        const float ds = fmaxf(tc, 50.f);
+        const float apz = clampf(__logf(ds / 50.f) * (float)(1.0 / 7.6009024595420822), 0.f, 1.f);
+        const float u = (float)fx * P.inv_W, v = (float)fy * P.inv_H;
+        // trilinear ClampToEdge on the froxel volume
+        const float cx = u * P.aw - 0.5f, cy = v * P.ah - 0.5f, cz = apz * P.ad - 0.5f;
+        const float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
+        const float wx = cx - fx0, wy = cy - fy0, wz = cz - fz0;
+        const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+        const int x0 = clampi(ix, P.aw), x1 = clampi(ix + 1, P.aw), y0 = clampi(iy, P.ah), y1 = clampi(iy + 1, P.ah),
+                  z0 = clampi(iz, P.ad), z1 = clampi(iz + 1, P.ad);
+        const int sxy = P.aw * P.ah;
+        const float4 c000 = P.AP[z0 * sxy + y0 * P.aw + x0], c100 = P.AP[z0 * sxy + y0 * P.aw + x1];
+        const float4 c010 = P.AP[z0 * sxy + y1 * P.aw + x0], c110 = P.AP[z0 * sxy + y1 * P.aw + x1];
+        const float4 c001 = P.AP[z1 * sxy + y0 * P.aw + x0], c101 = P.AP[z1 * sxy + y0 * P.aw + x1];
+        const float4 c011 = P.AP[z1 * sxy + y1 * P.aw + x0], c111 = P.AP[z1 * sxy + y1 * P.aw + x1];
+        float ap[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float v000 = ((const float*)&c000)[k], v100 = ((const float*)&c100)[k];
+            const float v010 = ((const float*)&c010)[k], v110 = ((const float*)&c110)[k];
+            const float v001 = ((const float*)&c001)[k], v101 = ((const float*)&c101)[k];
+            const float v011 = ((const float*)&c011)[k], v111 = ((const float*)&c111)[k];
+            const float a0 = v000 + (v100 - v000) * wx, a1 = v010 + (v110 - v010) * wx;
+            const float b0 = v001 + (v101 - v001) * wx, b1 = v011 + (v111 - v011) * wx;
+            const float p0 = a0 + (a1 - a0) * wy, p1 = b0 + (b1 - b0) * wy;
+            ap[k] = p0 + (p1 - p0) * wz;
+        }
+        olr = lr * ap[3] + ap[0] * opacity;
+        olg = lg * ap[3] + ap[1] * opacity;
+        olb = lb * ap[3] + ap[2] * opacity;
+    }
+
+    // ---- outputs: composite dst = lum + sky * T ------------------------------------------------
+    const size_t o = (size_t)y * P.W + x;
+    const float4 comp = make_float4(olr + sky.x * tr, olg + sky.y * tg, olb + sky.z * tbb, 1.f);
+    const float4 lum4 = make_float4(olr, olg, olb, 1.f);
+    const float4 tr4 = make_float4(tr, tg, tbb, 1.f);
+    if (P.out_f16) {
+        if (P.out_comp) {
+            __half2* q = (__half2*)P.out_comp + 2 * o;
+            q[0] = __floats2half2_rn(comp.x, comp.y); q[1] = __floats2half2_rn(comp.z, comp.w);
+        }
+        if (P.out_lum) {
+            __half2* q = (__half2*)P.out_lum + 2 * o;
+            q[0] = __floats2half2_rn(lum4.x, lum4.y); q[1] = __floats2half2_rn(lum4.z, lum4.w);
+        }
+        if (P.out_trans) {
+            __half2* q = (__half2*)P.out_trans + 2 * o;
+            q[0] = __floats2half2_rn(tr4.x, tr4.y); q[1] = __floats2half2_rn(tr4.z, tr4.w);
+        }
+    } else {
+        if (P.out_comp) ((float4*)P.out_comp)[o] = comp;
+        if (P.out_lum) ((float4*)P.out_lum)[o] = lum4;
+        if (P.out_trans) ((float4*)P.out_trans)[o] = tr4;
+    }
+
+    if (COUNT) {
+        unsigned long long* c = P.counters;
+        atomicAdd(c + 0, 1ull);
+        atomicAdd(c + 1, (unsigned long long)cnt.layer_marches);
+        atomicAdd(c + 2, (unsigned long long)cnt.primary);
+        atomicAdd(c + 3, (unsigned long long)cnt.covered);
+        atomicAdd(c + 4, (unsigned long long)cnt.eroded);
+        atomicAdd(c + 5, (unsigned long long)cnt.dense);
+        atomicAdd(c + 6, (unsigned long long)cnt.light);
+        atomicAdd(c + 7, (unsigned long long)cnt.light_cov);
+        atomicAdd(c + 8, (unsigned long long)cnt.octaves);
+        atomicAdd(c + 9, (unsigned long long)cnt.ap);
+    }
+}
+
+}  // namespace
+
+hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, hipStream_t s) {
+    dim3 grid((W + 15) / 16, (H + 15) / 16), block(256);
+    if (count) hipLaunchKernelGGL(k_render<true>, grid, block, 0, s, d_params);
+    else hipLaunchKernelGGL(k_render<false>, grid, block, 0, s, d_params);
+    return hipGetLastError();
+}
