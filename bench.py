@@ -1,0 +1,178 @@
+"""Benchmark of the sky + cloud path on one MI355X.
+
+One step = one full frame at 1280x720: the four LUT bakes (transmittance,
+multi-scattering, sky-view, 32^3 aerial perspective), then the fused sky
+pass + two-layer cloud march (cumulus 1500-2300 m, cirrus 8000-9000 m;
+128 primary / 6 light steps; FP16 noise) + AP + composite. Consecutive steps
+walk the seeded 16-camera sweep (pitch U[-10, 30] deg, yaw U[0, 360) deg).
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the round contract): value = Mpix/s, roofline of
+the dominant kernel (the render), and a CPU baseline (the oracle's float
+instantiation of the reference shaders on the host cores, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, base, detail, curl, budget_s: float, threads: int) -> dict:
+    import numpy as np  # noqa: F401
+    import oracle
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        oracle.render_f32_frame(w, base, detail, curl, nthreads=threads)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or frames >= 8:
+            break
+    mpix = frames * w.width * w.height / el / 1e6
+    return {"value": round(mpix, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} whole {w.width}x{w.height} frame(s) of sweep camera 0 (LUT chain + sky + "
+                      f"2-layer march + AP + composite), float instantiation of the CPU restatement, "
+                      f"{el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=720)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this path is benchmarked on a single MI355X "
+              "(no multi-GPU run is part of this build); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from pedalsky_amd import abi, cost_model, scene
+    from pedalsky_amd.renderer import Renderer
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    W, H = args.width, args.height
+    sweep = [scene.workload_full_frame(i, width=W, height=H) for i in range(16)]
+    w0 = sweep[0]
+    r = Renderer(0)
+    r.load_workload(w0)
+    depths = []
+    for w in sweep:
+        depths.append(torch.from_numpy(w.depth).to("cuda:0"))
+    bufs = r.alloc_outputs(W, H, abi.PS_PIXEL_F32, ("composite",))
+    out = r.image_out(bufs, abi.PS_PIXEL_F32)
+    lib = r.lib
+    import ctypes as C
+
+    def frame(k: int) -> None:
+        w = sweep[k % 16]
+        abi.check(lib.ps_set_frame(r.ctx, C.byref(w.frame)))
+        abi.check(lib.ps_frame(r.ctx, abi.PS_LUT_ALL, C.c_void_p(depths[k % 16].data_ptr()), C.byref(out)))
+
+    log(f"warmup {args.warmup} frames")
+    for k in range(args.warmup):
+        frame(k)
+    r.synchronize()
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} frames")
+    r.timing_begin(args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        frame(k)
+    r.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = r.timing_end()
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    mpix = args.steps * W * H / elapsed / 1e6
+
+    # algorithmic work over the sweep (counting pass, outside the timed region)
+    log("counting algorithmic work over the sweep")
+    flops, gbytes, cnt_sum = 0.0, 0.0, {}
+    used = [k % 16 for k in range(args.steps)]
+    per_cam = {}
+    for i in sorted(set(used)):
+        abi.check(lib.ps_set_frame(r.ctx, C.byref(sweep[i].frame)))
+        abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
+        cw = abi.WorkCounters()
+        abi.check(lib.ps_count_work(r.ctx, C.c_void_p(depths[i].data_ptr()), C.byref(cw)))
+        per_cam[i] = cw
+    for i in used:
+        cw = per_cam[i]
+        flops += cost_model.render_flops(cw)
+        gbytes += cost_model.render_bytes(cw)
+        for f, _ in abi.WorkCounters._fields_:
+            cnt_sum[f] = cnt_sum.get(f, 0) + getattr(cw, f)
+    flops_per_launch = flops / len(used)
+    bytes_per_launch = gbytes / len(used)
+    render_s = timing.render_ms * 1e-3
+    achieved_tflops = flops_per_launch / render_s / 1e12
+    achieved_gbps = bytes_per_launch / render_s / 1e9
+
+    result = {
+        "metric": "cloud+sky Mpix/s @1280x720 (fwd, fwd+adjoint, batched) at <=1e-3 rel. radiance",
+        "value": round(mpix, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded noise bake, PCG32 weather map, 16-camera seeded sweep)",
+        "config": {"workload": f"forward frame {W}x{H}: LUT chain + sky + cumulus/cirrus march "
+                               f"128/6 steps + AP 32^3 + composite, FP16 noise",
+                   "width": W, "height": H, "layers": 2, "primary_steps": 128, "light_steps": 6,
+                   "lut": "T 256x64, MS 32x32, SV 192x108, AP 32^3 (FP32)", "variant": "fwd"},
+        "kernel_ms": {"transmittance": round(timing.transmittance_ms, 5),
+                      "multiscatter": round(timing.multiscatter_ms, 5),
+                      "skyview": round(timing.skyview_ms, 5),
+                      "aerial_perspective": round(timing.aerial_perspective_ms, 5),
+                      "render": round(timing.render_ms, 5), "frame_events": round(timing.frame_ms, 5)},
+        "roofline": {"bound": "valu", "kernel": "k_render", "achieved": round(achieved_tflops, 3),
+                     "peak": cost_model.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tflops / cost_model.PEAK_FP32_VALU_TFLOPS, 4),
+                     "traffic": None,
+                     "gather_GBps": round(achieved_gbps, 1),
+                     "flop_per_launch": flops_per_launch, "gather_bytes_per_launch": bytes_per_launch},
+        "work_per_frame": {k: v / len(used) for k, v in cnt_sum.items()},
+    }
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        fmt = w0.noise_format
+        import oracle
+        base = oracle.decode(r.read_noise(abi.PS_NOISE_BASE, fmt), fmt)
+        det = oracle.decode(r.read_noise(abi.PS_NOISE_DETAIL, fmt), fmt)
+        curl = oracle.decode(r.read_noise(abi.PS_NOISE_CURL, fmt), fmt)
+        threads = min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(w0, base, det, curl, args.cpu_seconds, threads)
+    r.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

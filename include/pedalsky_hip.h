@@ -291,6 +291,13 @@ ps_status ps_frame(ps_ctx* ctx, uint32_t lut_flags, const float* depth,
 ps_status ps_time_frames(ps_ctx* ctx, int32_t frames, uint32_t lut_flags,
                          const float* depth, const ps_image_out* out,
                          ps_frame_timing* timing);
+/* In-loop timing: between ps_timing_begin and ps_timing_end, every ps_frame
+ * call (up to max_frames) records HIP events around its kernels; end
+ * synchronises and reports the per-kernel averages. No reference
+ * counterpart (the reference's GPU timestamp queries,
+ * crates/ps-core/src/app.rs:195-227, serve the same purpose). */
+ps_status ps_timing_begin(ps_ctx* ctx, int32_t max_frames);
+ps_status ps_timing_end(ps_ctx* ctx, ps_frame_timing* timing);
 /* Re-run the render with event counting on; fills the algorithmic counts. */
 ps_status ps_count_work(ps_ctx* ctx, const float* depth, ps_work_counters* out);
 

@@ -1,0 +1,248 @@
+"""Oracle: FP64 CPU restatement of PedalSky's WGSL sky + cloud path.
+
+This package is TEST INFRASTRUCTURE. Only tests/, ``__graft_entry__.smoke()``
+and bench.py's CPU-baseline leg may import it, and only as the checker (or
+the timed CPU baseline); the product path in ``pedalsky_amd`` never calls it.
+
+It wraps ``oracle/_build/liboracle.so`` (built from oracle/*.hpp by
+oracle/Makefile) with numpy-friendly functions. The C++ sources restate:
+  shaders/common/{atmosphere,atmosphere_lut_sampling,math}.wgsl,
+  shaders/atmosphere/{transmittance,multiscatter,skyview,aerialperspective}.comp.wgsl,
+  shaders/atmosphere/sky.wgsl, shaders/clouds/{cloud_density_common,cloud_march,
+  cloud_composite}.wgsl and shaders/clouds/noise_{base,detail,curl}.comp.wgsl.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+class OrInputs(C.Structure):
+    _fields_ = [
+        ("frame", C.c_void_p), ("atm", C.c_void_p), ("clouds", C.c_void_p), ("layers", C.c_void_p),
+        ("n_layers", C.c_int32), ("layer_wind_xz", C.c_void_p),
+        ("T", C.c_void_p), ("tw", C.c_int32), ("th", C.c_int32),
+        ("SV", C.c_void_p), ("sw", C.c_int32), ("sh", C.c_int32),
+        ("AP", C.c_void_p), ("aw", C.c_int32), ("ah", C.c_int32), ("ad", C.c_int32),
+        ("base", C.c_void_p), ("detail", C.c_void_p), ("curl", C.c_void_p),
+        ("weather", C.c_void_p), ("ww", C.c_int32), ("wh", C.c_int32), ("wch", C.c_int32),
+        ("type_grid", C.c_void_p), ("overcast", C.c_void_p), ("blue", C.c_void_p), ("depth", C.c_void_p),
+    ]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        import subprocess
+        subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    lib = C.CDLL(str(LIB_PATH))
+    d, i, P = C.c_double, C.c_int, C.c_void_p
+    sigs = {
+        "or_bake_transmittance": (None, [P, i, i, P, i]),
+        "or_bake_multiscatter": (None, [P, P, i, i, i, i, P, i]),
+        "or_bake_skyview": (None, [P, P, P, i, i, P, i, i, i, i, P, i]),
+        "or_bake_ap": (None, [P, P, P, i, i, P, i, i, i, i, i, P, i]),
+        "orf_bake_transmittance": (None, [P, i, i, P, i]),
+        "orf_bake_multiscatter": (None, [P, P, i, i, i, i, P, i]),
+        "orf_bake_skyview": (None, [P, P, P, i, i, P, i, i, i, i, P, i]),
+        "orf_bake_ap": (None, [P, P, P, i, i, P, i, i, i, i, i, P, i]),
+        "or_render": (None, [P, i, i, i, i, P, P, P, P, i]),
+        "orf_render": (None, [P, i, i, i, i, P, P, P, P, i]),
+        "or_render_pixels": (None, [P, P, i, P, P, P, i]),
+        "or_noise_base": (None, [C.c_uint32, i, i, P, i]),
+        "or_noise_detail": (None, [C.c_uint32, P, i]),
+        "or_noise_curl": (None, [C.c_uint32, P, i]),
+        "or_henyey_greenstein": (d, [d, d]),
+        "or_draine_phase": (d, [d, d, d]),
+        "or_phase_rayleigh": (d, [d]),
+        "or_phase_mie": (d, [d, d]),
+        "or_cloud_phase": (d, [d, d, d, d, C.c_uint32, d]),
+        "or_ndf": (d, [d, C.c_uint32, d]),
+        "or_transmittance_uv": (None, [P, d, d, P]),
+        "or_transmittance_pos_dir": (None, [P, d, d, P]),
+        "or_optical_depth": (None, [P, d, d, i, P]),
+        "or_skyview_uv": (None, [P, P, P, d, P]),
+        "or_sample3d_repeat": (None, [P, i, i, i, i, d, d, d, P]),
+        "or_sample2d": (None, [P, i, i, i, d, d, i, P]),
+        "or_hash3": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a) -> int:
+    return a.ctypes.data if a is not None else None
+
+
+def _ref(s) -> int:
+    return C.addressof(s)
+
+
+THREADS = int(os.environ.get("ORACLE_THREADS", "0"))
+
+
+# ---------------------------------------------------------------------------
+# LUT chain
+# ---------------------------------------------------------------------------
+def bake_transmittance(atm, w=256, h=64, dtype=np.float64):
+    out = np.zeros((h, w, 4), dtype=dtype)
+    fn = load().or_bake_transmittance if dtype == np.float64 else load().orf_bake_transmittance
+    fn(_ref(atm), w, h, _p(out), THREADS)
+    return out
+
+
+def bake_multiscatter(atm, T, w=32, h=32, dtype=np.float64):
+    T = np.ascontiguousarray(T, dtype=dtype)
+    out = np.zeros((h, w, 4), dtype=dtype)
+    fn = load().or_bake_multiscatter if dtype == np.float64 else load().orf_bake_multiscatter
+    fn(_ref(atm), _p(T), T.shape[1], T.shape[0], w, h, _p(out), THREADS)
+    return out
+
+
+def bake_skyview(frame, atm, T, MS, w=192, h=108, dtype=np.float64):
+    T = np.ascontiguousarray(T, dtype=dtype)
+    MS = np.ascontiguousarray(MS, dtype=dtype)
+    out = np.zeros((h, w, 4), dtype=dtype)
+    fn = load().or_bake_skyview if dtype == np.float64 else load().orf_bake_skyview
+    fn(_ref(frame), _ref(atm), _p(T), T.shape[1], T.shape[0], _p(MS), MS.shape[1], MS.shape[0], w, h, _p(out),
+       THREADS)
+    return out
+
+
+def bake_ap(frame, atm, T, MS, w=32, h=32, d=32, dtype=np.float64):
+    T = np.ascontiguousarray(T, dtype=dtype)
+    MS = np.ascontiguousarray(MS, dtype=dtype)
+    out = np.zeros((d, h, w, 4), dtype=dtype)
+    fn = load().or_bake_ap if dtype == np.float64 else load().orf_bake_ap
+    fn(_ref(frame), _ref(atm), _p(T), T.shape[1], T.shape[0], _p(MS), MS.shape[1], MS.shape[0], w, h, d, _p(out),
+       THREADS)
+    return out
+
+
+def bake_luts(frame, atm, dims, dtype=np.float64, ms_disabled=False):
+    T = bake_transmittance(atm, dims.transmittance_w, dims.transmittance_h, dtype)
+    if ms_disabled:
+        MS = np.zeros((dims.multiscatter_h, dims.multiscatter_w, 4), dtype=dtype)
+    else:
+        MS = bake_multiscatter(atm, T, dims.multiscatter_w, dims.multiscatter_h, dtype)
+    SV = bake_skyview(frame, atm, T, MS, dims.skyview_w, dims.skyview_h, dtype)
+    AP = bake_ap(frame, atm, T, MS, dims.ap_w, dims.ap_h, dims.ap_d, dtype)
+    return {"T": T, "MS": MS, "SV": SV, "AP": AP}
+
+
+# ---------------------------------------------------------------------------
+# noise
+# ---------------------------------------------------------------------------
+def noise_base(seed=0, z0=0, z1=128):
+    out = np.zeros((z1 - z0, 128, 128, 4), dtype=np.float64)
+    load().or_noise_base(seed, z0, z1, _p(out), THREADS)
+    return out
+
+
+def noise_detail(seed=0):
+    out = np.zeros((32, 32, 32, 4), dtype=np.float64)
+    load().or_noise_detail(seed, _p(out), THREADS)
+    return out
+
+
+def noise_curl(seed=0):
+    out = np.zeros((128, 128, 2), dtype=np.float64)
+    load().or_noise_curl(seed, _p(out), THREADS)
+    return out
+
+
+def quantize(values: np.ndarray, fmt: int) -> np.ndarray:
+    """Storage quantisation: 0 = UNORM8 (round(clamp(v)*255)), 1 = binary16 bits."""
+    if fmt == 0:
+        return np.rint(np.clip(values, 0.0, 1.0) * 255.0).astype(np.uint8)
+    return values.astype(np.float32).astype(np.float16).view(np.uint16)
+
+
+def decode(stored: np.ndarray, fmt: int) -> np.ndarray:
+    if fmt == 0:
+        return stored.astype(np.float32) / np.float32(255.0)
+    return stored.view(np.float16).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# per-pixel render
+# ---------------------------------------------------------------------------
+class Frame:
+    """Everything the oracle render reads, kept alive for the ctypes call."""
+
+    def __init__(self, workload, base, detail, curl, luts=None, depth="workload"):
+        self.w = workload
+        self.luts = luts if luts is not None else bake_luts(workload.frame, workload.atmosphere,
+                                                          workload.lut_dims)
+        self.base = np.ascontiguousarray(base, dtype=np.float32)
+        self.detail = np.ascontiguousarray(detail, dtype=np.float32)
+        self.curl = np.ascontiguousarray(curl, dtype=np.float32)
+        wm = workload.weather
+        self.weather = np.ascontiguousarray(wm.view(np.float16).astype(np.float32))
+        self.layers = (workload.layers[0].__class__ * max(len(workload.layers), 1))(*workload.layers)
+        self.winds = np.ascontiguousarray(workload.winds, dtype=np.float32)
+        self.overcast = np.ascontiguousarray(workload.overcast, dtype=np.uint8)
+        self.blue = np.ascontiguousarray(workload.blue, dtype=np.uint8)
+        d = workload.depth if isinstance(depth, str) else depth
+        self.depth = None if d is None else np.ascontiguousarray(d, dtype=np.float32)
+        T, SV, AP = self.luts["T"], self.luts["SV"], self.luts["AP"]
+        self.inputs = OrInputs(
+            _ref(workload.frame), _ref(workload.atmosphere), _ref(workload.clouds), C.addressof(self.layers),
+            len(workload.layers), _p(self.winds),
+            _p(T), T.shape[1], T.shape[0], _p(SV), SV.shape[1], SV.shape[0],
+            _p(AP), AP.shape[2], AP.shape[1], AP.shape[0],
+            _p(self.base), _p(self.detail), _p(self.curl),
+            _p(self.weather), wm.shape[1], wm.shape[0], wm.shape[2],
+            None, _p(self.overcast), _p(self.blue), _p(self.depth))
+
+    def render_region(self, x0, y0, w, h, counters=False):
+        comp = np.zeros((h, w, 4))
+        lum = np.zeros((h, w, 4))
+        trn = np.zeros((h, w, 4))
+        cnt = np.zeros(10, dtype=np.uint64) if counters else None
+        load().or_render(C.byref(self.inputs), x0, y0, w, h, _p(comp), _p(lum), _p(trn), _p(cnt), THREADS)
+        return comp, lum, trn, cnt
+
+    def render_pixels(self, xy: np.ndarray):
+        xy = np.ascontiguousarray(xy, dtype=np.int32)
+        n = xy.shape[0]
+        comp = np.zeros((n, 4))
+        lum = np.zeros((n, 4))
+        trn = np.zeros((n, 4))
+        load().or_render_pixels(C.byref(self.inputs), _p(xy), n, _p(comp), _p(lum), _p(trn), THREADS)
+        return comp, lum, trn
+
+
+def render_f32_frame(workload, base, detail, curl, nthreads=0):
+    """The float instantiation over the whole frame: LUT chain + render. Used
+    as the CPU baseline (a CPU port of the reference shaders)."""
+    luts = bake_luts(workload.frame, workload.atmosphere, workload.lut_dims, dtype=np.float32)
+    fr = Frame(workload, base, detail, curl, luts=luts)
+    W, H = workload.width, workload.height
+    comp = np.zeros((H, W, 4), dtype=np.float32)
+    load().orf_render(C.byref(fr.inputs), 0, 0, W, H, _p(comp), None, None, None, nthreads)
+    return comp
+
+
+def relative_error(got: np.ndarray, ref: np.ndarray, floor_frac: float = 1e-4) -> np.ndarray:
+    """Per-pixel relative radiance error: max over RGB of |a - b| /
+    max(|b|, floor_frac * max frame radiance)."""
+    got = np.asarray(got, dtype=np.float64)[..., :3]
+    ref = np.asarray(ref, dtype=np.float64)[..., :3]
+    floor = floor_frac * float(np.max(np.abs(ref)))
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), floor)
+    return err.max(axis=-1)

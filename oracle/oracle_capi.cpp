@@ -210,6 +210,22 @@ void orf_render(const or_inputs* in, int x0, int y0, int w, int h, float* comp, 
                 uint64_t* counters, int nthreads) {
     render_region<float>(in, x0, y0, w, h, comp, lum, trn, counters, nthreads);
 }
+// Renders an arbitrary set of n pixels given as (x, y) pairs; outputs n*4.
+void or_render_pixels(const or_inputs* in, const int32_t* xy, int n, double* comp, double* lum, double* trn,
+                      int nthreads) {
+    Scene<double> s = make_scene<double>(in);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+        double c4[4], l4[4], t4[4];
+        render_pixel(s, xy[2 * i], xy[2 * i + 1], c4, l4, t4, (Counters*)nullptr);
+        for (int k = 0; k < 4; ++k) {
+            if (comp) comp[size_t(i) * 4 + k] = c4[k];
+            if (lum) lum[size_t(i) * 4 + k] = l4[k];
+            if (trn) trn[size_t(i) * 4 + k] = t4[k];
+        }
+    }
+}
 
 // ---- noise bakes (decoded float values before storage quantisation) ---------------------
 void or_noise_base(uint32_t seed, int z0, int z1, double* out, int nthreads) {

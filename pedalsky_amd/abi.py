@@ -206,6 +206,8 @@ _SIGNATURES = [
     ("ps_frame", C.c_int32, [_P, C.c_uint32, _P, C.POINTER(ImageOut)]),
     ("ps_time_frames", C.c_int32, [_P, C.c_int32, C.c_uint32, _P, C.POINTER(ImageOut),
                                    C.POINTER(FrameTiming)]),
+    ("ps_timing_begin", C.c_int32, [_P, C.c_int32]),
+    ("ps_timing_end", C.c_int32, [_P, C.POINTER(FrameTiming)]),
     ("ps_count_work", C.c_int32, [_P, _P, C.POINTER(WorkCounters)]),
     ("ps_default_atmosphere", None, [C.POINTER(AtmosphereParams)]),
     ("ps_default_cloud_params", None, [C.POINTER(CloudParams)]),

@@ -108,11 +108,20 @@ struct ps_ctx {
     uint8_t* blue = nullptr;
     bool have_blue = false;
 
-    RenderParams* d_params = nullptr;
-    RenderParams* h_params = nullptr;  // pinned staging of the last uploaded block
-    bool params_uploaded = false;
+    // Render parameter blocks: a ring of device slots fed from pinned host
+    // staging, so a new camera each frame needs no host/device sync; a slot is
+    // rewritten only after the event of its last use has completed.
+    static constexpr int kParamSlots = 8;
+    RenderParams* d_params = nullptr;   // kParamSlots blocks
+    RenderParams* h_params = nullptr;   // kParamSlots pinned blocks
+    hipEvent_t slot_ev[kParamSlots]{};
+    bool slot_used[kParamSlots]{};
+    int cur_slot = -1;
     unsigned long long* d_counters = nullptr;
     std::vector<hipEvent_t> events;
+    // in-loop timing (ps_timing_begin / ps_timing_end)
+    bool timing = false;
+    int timing_max = 0, timing_n = 0;
 };
 
 namespace {
@@ -334,19 +343,26 @@ ps_status launch_render(ps_ctx* c, const float* depth, const ps_image_out* out, 
     ps_status s;
     if ((s = check_ready(c, true))) return s;
     if (!out && !count) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render: out is NULL");
-    if ((s = dev_alloc(&c->d_params, sizeof(RenderParams), "render params"))) return s;
-    if (!c->h_params) PS_HIP(hipHostMalloc((void**)&c->h_params, sizeof(RenderParams), hipHostMallocDefault));
+    const int NS = ps_ctx::kParamSlots;
+    if ((s = dev_alloc(&c->d_params, sizeof(RenderParams) * NS, "render params"))) return s;
+    if (!c->h_params) {
+        PS_HIP(hipHostMalloc((void**)&c->h_params, sizeof(RenderParams) * NS, hipHostMallocDefault));
+        for (int i = 0; i < NS; ++i) PS_HIP(hipEventCreateWithFlags(&c->slot_ev[i], hipEventDisableTiming));
+    }
     RenderParams next;
     if ((s = fill_render_params(c, depth, out, next))) return s;
-    // Re-upload only when the block changed; the pinned staging buffer may
-    // still be read by an earlier queued copy, so drain the stream first.
-    if (!c->params_uploaded || std::memcmp(&next, c->h_params, sizeof(RenderParams)) != 0) {
-        PS_HIP(hipStreamSynchronize(c->stream));
-        std::memcpy(c->h_params, &next, sizeof(RenderParams));
-        PS_HIP(hipMemcpyAsync(c->d_params, c->h_params, sizeof(RenderParams), hipMemcpyHostToDevice, c->stream));
-        c->params_uploaded = true;
+    int slot = c->cur_slot;
+    if (slot < 0 || std::memcmp(&next, &c->h_params[slot], sizeof(RenderParams)) != 0) {
+        slot = (slot + 1) % NS;
+        if (c->slot_used[slot]) PS_HIP(hipEventSynchronize(c->slot_ev[slot]));
+        std::memcpy(&c->h_params[slot], &next, sizeof(RenderParams));
+        PS_HIP(hipMemcpyAsync(&c->d_params[slot], &c->h_params[slot], sizeof(RenderParams), hipMemcpyHostToDevice,
+                              c->stream));
+        c->cur_slot = slot;
     }
-    PS_HIP(ps_launch_render(c->d_params, next.W, next.H, count, c->stream));
+    PS_HIP(ps_launch_render(&c->d_params[slot], next.W, next.H, count, c->stream));
+    PS_HIP(hipEventRecord(c->slot_ev[slot], c->stream));
+    c->slot_used[slot] = true;
     return PS_OK;
 }
 
@@ -391,7 +407,10 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->base_rl); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_params); dev_free(c->d_counters);
-    if (c->h_params) hipHostFree(c->h_params);
+    if (c->h_params) {
+        hipHostFree(c->h_params);
+        for (int i = 0; i < ps_ctx::kParamSlots; ++i) hipEventDestroy(c->slot_ev[i]);
+    }
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
@@ -620,48 +639,64 @@ ps_status ps_render(ps_ctx* c, const float* depth, const ps_image_out* out) {
 ps_status ps_frame(ps_ctx* c, uint32_t lut_flags, const float* depth, const ps_image_out* out) {
     if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_frame: ctx is NULL");
     ps_status s;
-    if ((s = launch_luts(c, lut_flags, nullptr))) return s;
-    return launch_render(c, depth, out, false);
+    hipEvent_t* ev = nullptr;
+    if (c->timing && c->timing_n < c->timing_max) ev = &c->events[(size_t)c->timing_n * 6];
+    if ((s = launch_luts(c, lut_flags, ev))) return s;
+    if ((s = launch_render(c, depth, out, false))) return s;
+    if (ev) {
+        PS_HIP(hipEventRecord(ev[5], c->stream));
+        c->timing_n++;
+    }
+    return PS_OK;
+}
+
+static ps_status ensure_events(ps_ctx* c, size_t need) {
+    while (c->events.size() < need) {
+        hipEvent_t e;
+        PS_HIP(hipEventCreate(&e));
+        c->events.push_back(e);
+    }
+    return PS_OK;
+}
+
+static ps_status reduce_timing(ps_ctx* c, int frames, ps_frame_timing* timing) {
+    return reduce_timing(c, frames, timing);
+}
+
+ps_status ps_timing_begin(ps_ctx* c, int32_t max_frames) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_timing_begin: ctx is NULL");
+    if (max_frames < 1 || max_frames > 65536)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_timing_begin: max_frames must be in [1, 65536]");
+    ps_status s;
+    if ((s = ensure_events(c, (size_t)max_frames * 6))) return s;
+    c->timing = true;
+    c->timing_max = max_frames;
+    c->timing_n = 0;
+    return PS_OK;
+}
+
+ps_status ps_timing_end(ps_ctx* c, ps_frame_timing* timing) {
+    if (!c || !timing) return fail(PS_ERR_INVALID_ARGUMENT, "ps_timing_end: NULL argument");
+    if (!c->timing) return fail(PS_ERR_NOT_READY, "ps_timing_end: timing was not started");
+    c->timing = false;
+    if (c->timing_n == 0) return fail(PS_ERR_NOT_READY, "ps_timing_end: no frame was recorded");
+    return reduce_timing(c, c->timing_n, timing);
 }
 
 ps_status ps_time_frames(ps_ctx* c, int32_t frames, uint32_t lut_flags, const float* depth, const ps_image_out* out,
                          ps_frame_timing* timing) {
     if (!c || !timing) return fail(PS_ERR_INVALID_ARGUMENT, "ps_time_frames: NULL argument");
     if (frames < 1 || frames > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "ps_time_frames: frames must be in [1, 4096]");
-    const size_t need = (size_t)frames * 6;
-    while (c->events.size() < need) {
-        hipEvent_t e;
-        PS_HIP(hipEventCreate(&e));
-        c->events.push_back(e);
-    }
     ps_status s;
+    if (c->timing) return fail(PS_ERR_NOT_READY, "ps_time_frames: in-loop timing is active");
+    if ((s = ensure_events(c, (size_t)frames * 6))) return s;
     for (int f = 0; f < frames; ++f) {
         hipEvent_t* ev = &c->events[(size_t)f * 6];
         if ((s = launch_luts(c, lut_flags, ev))) return s;
         if ((s = launch_render(c, depth, out, false))) return s;
         PS_HIP(hipEventRecord(ev[5], c->stream));
     }
-    PS_HIP(hipStreamSynchronize(c->stream));
-    double acc[5] = {0, 0, 0, 0, 0};
-    for (int f = 0; f < frames; ++f) {
-        hipEvent_t* ev = &c->events[(size_t)f * 6];
-        for (int k = 0; k < 5; ++k) {
-            float ms = 0.f;
-            PS_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-            acc[k] += ms;
-        }
-    }
-    float total = 0.f;
-    PS_HIP(hipEventElapsedTime(&total, c->events[0], c->events[(size_t)(frames - 1) * 6 + 5]));
-    timing->transmittance_ms = (float)(acc[0] / frames);
-    timing->multiscatter_ms = (float)(acc[1] / frames);
-    timing->skyview_ms = (float)(acc[2] / frames);
-    timing->aerial_perspective_ms = (float)(acc[3] / frames);
-    timing->render_ms = (float)(acc[4] / frames);
-    timing->frame_ms = total / frames;
-    timing->frames = frames;
-    timing->_reserved = 0;
-    return PS_OK;
+    return reduce_timing(c, frames, timing);
 }
 
 ps_status ps_count_work(ps_ctx* c, const float* depth, ps_work_counters* out) {

@@ -1,0 +1,196 @@
+"""Python handle on one library context (``ps_ctx``): every call goes through
+the C ABI in ``include/pedalsky_hip.h``. Device buffers are torch tensors
+(torch is plumbing for device memory here; all compute is in the HIP
+library)."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .abi import check
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Renderer:
+    def __init__(self, device: int = 0):
+        self.lib = abi.load()
+        self.device = device
+        h = C.c_void_p()
+        check(self.lib.ps_ctx_create(device, C.byref(h)))
+        self.ctx = h
+        self.workload = None
+        self._depth_dev = None
+
+    # lifecycle ---------------------------------------------------------------
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.ps_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self) -> None:
+        check(self.lib.ps_ctx_synchronize(self.ctx))
+
+    def set_stream(self, stream_handle: Optional[int]) -> None:
+        check(self.lib.ps_ctx_set_stream(self.ctx, stream_handle))
+
+    # state --------------------------------------------------------------------
+    def set_lut_dims(self, dims: abi.LutDims) -> None:
+        check(self.lib.ps_ctx_set_lut_dims(self.ctx, C.byref(dims)))
+
+    def lut_dims(self) -> abi.LutDims:
+        d = abi.LutDims()
+        check(self.lib.ps_ctx_get_lut_dims(self.ctx, C.byref(d)))
+        return d
+
+    def set_atmosphere(self, a: abi.AtmosphereParams) -> None:
+        check(self.lib.ps_set_atmosphere(self.ctx, C.byref(a)))
+
+    def set_frame(self, f: abi.FrameUniforms) -> None:
+        check(self.lib.ps_set_frame(self.ctx, C.byref(f)))
+
+    def set_clouds(self, params: abi.CloudParams, layers, winds=None) -> None:
+        n = len(layers)
+        arr = (abi.CloudLayer * max(n, 1))(*layers)
+        w = None
+        if winds is not None:
+            wa = np.ascontiguousarray(winds, dtype=np.float32).reshape(-1)
+            w = (C.c_float * len(wa))(*wa.tolist())
+        check(self.lib.ps_set_clouds(self.ctx, C.byref(params), arr, n, w))
+
+    # inputs -------------------------------------------------------------------
+    def generate_noise(self, seed: int = 0, fmt: int = abi.PS_NOISE_FP16) -> None:
+        check(self.lib.ps_generate_noise(self.ctx, seed, fmt))
+
+    def upload_noise(self, fmt: int, base: np.ndarray, detail: np.ndarray, curl: np.ndarray) -> None:
+        base, detail, curl = (np.ascontiguousarray(x) for x in (base, detail, curl))
+        check(self.lib.ps_upload_noise(self.ctx, fmt, base.ctypes.data, detail.ctypes.data, curl.ctypes.data))
+
+    def read_noise(self, which: int, fmt: int) -> np.ndarray:
+        dt = np.uint8 if fmt == abi.PS_NOISE_UNORM8 else np.uint16
+        shape = {abi.PS_NOISE_BASE: (128, 128, 128, 4), abi.PS_NOISE_DETAIL: (32, 32, 32, 4),
+                 abi.PS_NOISE_CURL: (128, 128, 2)}[which]
+        out = np.zeros(shape, dtype=dt)
+        check(self.lib.ps_read_noise(self.ctx, which, out.ctypes.data, out.nbytes))
+        return out
+
+    def upload_blue_noise(self, ranks: np.ndarray) -> None:
+        r = np.ascontiguousarray(ranks, dtype=np.uint8)
+        check(self.lib.ps_upload_blue_noise(self.ctx, r.ctypes.data))
+
+    def upload_weather(self, weather: np.ndarray, type_grid: Optional[np.ndarray] = None,
+                       overcast: Optional[np.ndarray] = None) -> None:
+        w = np.ascontiguousarray(weather, dtype=np.uint16)
+        tg = None if type_grid is None else np.ascontiguousarray(type_grid, dtype=np.uint8)
+        oc = None if overcast is None else np.ascontiguousarray(overcast, dtype=np.uint8)
+        check(self.lib.ps_upload_weather(self.ctx, w.ctypes.data, w.shape[1], w.shape[0], w.shape[2],
+                                         None if tg is None else tg.ctypes.data,
+                                         None if oc is None else oc.ctypes.data))
+
+    def set_depth(self, depth: Optional[np.ndarray]) -> None:
+        if depth is None:
+            self._depth_dev = None
+        else:
+            torch = _torch()
+            self._depth_dev = torch.from_numpy(np.ascontiguousarray(depth, dtype=np.float32)).to(
+                f"cuda:{self.device}")
+            torch.cuda.synchronize(self.device)
+
+    @property
+    def depth_ptr(self):
+        return None if self._depth_dev is None else self._depth_dev.data_ptr()
+
+    def load_workload(self, w) -> None:
+        """Upload every input of a scene.Workload (noise baked on the GPU)."""
+        self.set_lut_dims(w.lut_dims)
+        self.set_atmosphere(w.atmosphere)
+        self.set_frame(w.frame)
+        self.set_clouds(w.clouds, w.layers, w.winds)
+        self.generate_noise(w.noise_seed, w.noise_format)
+        self.upload_blue_noise(w.blue)
+        self.upload_weather(w.weather, None, w.overcast)
+        self.set_depth(w.depth)
+        self.workload = w
+
+    # passes -------------------------------------------------------------------
+    def bake_luts(self, flags: int = abi.PS_LUT_ALL) -> None:
+        check(self.lib.ps_bake_luts(self.ctx, flags))
+
+    def read_lut(self, which: int) -> np.ndarray:
+        d = self.lut_dims()
+        shape = {abi.PS_LUT_TRANSMITTANCE: (d.transmittance_h, d.transmittance_w, 4),
+                 abi.PS_LUT_MULTISCATTER: (d.multiscatter_h, d.multiscatter_w, 4),
+                 abi.PS_LUT_SKYVIEW: (d.skyview_h, d.skyview_w, 4),
+                 abi.PS_LUT_AERIAL_PERSPECTIVE: (d.ap_d, d.ap_h, d.ap_w, 4)}[which]
+        out = np.zeros(shape, dtype=np.float32)
+        check(self.lib.ps_read_lut(self.ctx, which, out.ctypes.data, out.size))
+        return out
+
+    def alloc_outputs(self, width: int, height: int, fmt: int = abi.PS_PIXEL_F32, which=("composite",)):
+        torch = _torch()
+        dt = torch.float32 if fmt == abi.PS_PIXEL_F32 else torch.float16
+        bufs = {k: torch.empty((height, width, 4), dtype=dt, device=f"cuda:{self.device}") for k in which}
+        torch.cuda.synchronize(self.device)
+        return bufs
+
+    def image_out(self, bufs, fmt: int = abi.PS_PIXEL_F32) -> abi.ImageOut:
+        o = abi.ImageOut()
+        o.composite = bufs["composite"].data_ptr() if "composite" in bufs else None
+        o.cloud_luminance = bufs["cloud_luminance"].data_ptr() if "cloud_luminance" in bufs else None
+        o.cloud_transmittance = bufs["cloud_transmittance"].data_ptr() if "cloud_transmittance" in bufs else None
+        o.format = fmt
+        return o
+
+    def render(self, out: abi.ImageOut) -> None:
+        check(self.lib.ps_render(self.ctx, self.depth_ptr, C.byref(out)))
+
+    def frame(self, out: abi.ImageOut, lut_flags: int = abi.PS_LUT_ALL) -> None:
+        check(self.lib.ps_frame(self.ctx, lut_flags, self.depth_ptr, C.byref(out)))
+
+    def time_frames(self, frames: int, out: abi.ImageOut, lut_flags: int = abi.PS_LUT_ALL) -> abi.FrameTiming:
+        t = abi.FrameTiming()
+        check(self.lib.ps_time_frames(self.ctx, frames, lut_flags, self.depth_ptr, C.byref(out), C.byref(t)))
+        return t
+
+    def timing_begin(self, max_frames: int) -> None:
+        check(self.lib.ps_timing_begin(self.ctx, max_frames))
+
+    def timing_end(self) -> abi.FrameTiming:
+        t = abi.FrameTiming()
+        check(self.lib.ps_timing_end(self.ctx, C.byref(t)))
+        return t
+
+    def count_work(self) -> abi.WorkCounters:
+        c = abi.WorkCounters()
+        check(self.lib.ps_count_work(self.ctx, self.depth_ptr, C.byref(c)))
+        return c
+
+    def render_numpy(self, which=("composite", "cloud_luminance", "cloud_transmittance"),
+                     fmt: int = abi.PS_PIXEL_F32, bake: bool = True) -> dict:
+        """Bake the LUTs, render, and return the outputs as numpy arrays."""
+        W, H = int(self.workload.width), int(self.workload.height)
+        bufs = self.alloc_outputs(W, H, fmt, which)
+        out = self.image_out(bufs, fmt)
+        if bake:
+            self.bake_luts()
+        self.render(out)
+        self.synchronize()
+        return {k: v.float().cpu().numpy() for k, v in bufs.items()}
