@@ -241,6 +241,14 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = Path(path) if path else LIB_PATH
+    # torch ships its own HIP runtime under the same SONAME (libamdhip64.so.7);
+    # whichever is loaded first serves the whole process. Load torch's first
+    # so a Python process that also uses torch for device buffers runs on one
+    # runtime (torch does not initialise on a runtime other than its own).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise FileNotFoundError(
             f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

@@ -660,7 +660,27 @@ static ps_status ensure_events(ps_ctx* c, size_t need) {
 }
 
 static ps_status reduce_timing(ps_ctx* c, int frames, ps_frame_timing* timing) {
-    return reduce_timing(c, frames, timing);
+    PS_HIP(hipStreamSynchronize(c->stream));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int f = 0; f < frames; ++f) {
+        hipEvent_t* ev = &c->events[(size_t)f * 6];
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0.f;
+            PS_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    float total = 0.f;
+    PS_HIP(hipEventElapsedTime(&total, c->events[0], c->events[(size_t)(frames - 1) * 6 + 5]));
+    timing->transmittance_ms = (float)(acc[0] / frames);
+    timing->multiscatter_ms = (float)(acc[1] / frames);
+    timing->skyview_ms = (float)(acc[2] / frames);
+    timing->aerial_perspective_ms = (float)(acc[3] / frames);
+    timing->render_ms = (float)(acc[4] / frames);
+    timing->frame_ms = total / frames;
+    timing->frames = frames;
+    timing->_reserved = 0;
+    return PS_OK;
 }
 
 ps_status ps_timing_begin(ps_ctx* c, int32_t max_frames) {

@@ -90,11 +90,20 @@ def test_noise_bake_matches_oracle(renderer, fmt):
             got = got[:16]
         if fmt == abi.PS_NOISE_UNORM8:
             d = np.abs(got.astype(np.int32) - exp.astype(np.int32))
+            frac = float((d > 0).mean())
+            _log(f"  volume {which}: max step diff {d.max()}, differing texels {frac:.2e}")
+            assert d.max() <= 1
         else:
-            d = np.abs(got.view(np.int16).astype(np.int32) - exp.view(np.int16).astype(np.int32))
-        frac = float((d > 0).mean())
-        _log(f"  volume {which}: max step diff {d.max()}, differing texels {frac:.2e}")
-        assert d.max() <= 1
+            # the GPU bake runs the WGSL's FP32 arithmetic: allow one binary16
+            # step at the value, or 1e-5 absolute where the value is near 0
+            # (the curl map's central difference scales FP32 noise by 64)
+            g = got.view(np.float16).astype(np.float64)
+            e = exp.view(np.float16).astype(np.float64)
+            step = np.maximum(np.spacing(np.abs(e).astype(np.float16)).astype(np.float64), 1e-5)
+            d = np.abs(g - e) / step
+            frac = float((d > 0).mean())
+            _log(f"  volume {which}: max diff {d.max():.3f} steps, differing texels {frac:.2e}")
+            assert d.max() <= 1.0 + 1e-9
         assert frac < 2e-2
 
 
