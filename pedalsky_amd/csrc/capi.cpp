@@ -93,7 +93,7 @@ struct ps_ctx {
 
     int noise_format = -1;
     void *base_store = nullptr, *detail_store = nullptr, *curl_store = nullptr;
-    float2* base_rl = nullptr;
+    uint32_t* base_rl = nullptr;
     float* detail_hf = nullptr;
     float2* curl2 = nullptr;
     bool have_noise = false;
@@ -246,7 +246,8 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     const double drift = (double)cp.wind_drift_strength * (double)f.simulated_seconds;
     for (int l = 0; l < lc; ++l) {
         const ps_cloud_layer& L = c->layers[l];
-        LayerC& o = P.layers[l];
+        LayerC& oc = P.layers[l];
+        LayerF& o = oc.f;
         o.base = L.base_m; o.top = L.top_m;
         o.inv_thick = (float)(1.0 / std::max((double)L.top_m - (double)L.base_m, 1.0));
         o.coverage = L.coverage; o.density_scale = L.density_scale;
@@ -280,16 +281,18 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
             }
         }
         o.type = L.cloud_type;
-        o.is_cb = L.cloud_type == 7u ? 1u : 0u;
-        o.droplet = L.droplet_diameter_um;
-        o.r_bottom = R + (double)L.base_m;
-        o.r_top = R + (double)L.top_m;
+        oc.is_cb = L.cloud_type == 7u ? 1u : 0u;
+        oc.droplet = L.droplet_diameter_um;
+        oc.r_bottom = R + (double)L.base_m;
+        oc.r_top = R + (double)L.top_m;
         double d = std::min(std::max((double)L.droplet_diameter_um * cp.droplet_diameter_bias, 5.0), 50.0);
-        o.g_hg = std::exp(-0.0990567 / (d - 1.67154));
-        o.g_d = std::exp(-2.20679 / (d + 3.91029)) - 0.428934;
-        o.alpha = std::exp(3.62489 - 8.29288 / (d + 5.52825));
-        o.w_d = std::exp(-0.599085 / (d - 0.641583) - 0.665888);
-        o.ice = smoothstep_d(35.0, 50.0, d);
+        double xg = -0.0990567 / (d - 1.67154);
+        oc.g_hg = (float)std::exp(xg);
+        oc.omg_hg = (float)-std::expm1(xg);
+        oc.g_d = (float)(std::exp(-2.20679 / (d + 3.91029)) - 0.428934);
+        oc.alpha = (float)std::exp(3.62489 - 8.29288 / (d + 5.52825));
+        oc.w_d = (float)std::exp(-0.599085 / (d - 0.641583) - 0.665888);
+        oc.ice = (float)smoothstep_d(35.0, 50.0, d);
     }
     // resources
     const ps_lut_dims& dm = c->dims;
@@ -510,7 +513,7 @@ static ps_status alloc_noise(ps_ctx* c, int format) {
     if ((s = dev_alloc(&c->base_store, noise_texel_bytes(format, 4) * 128 * 128 * 128, "base noise"))) return s;
     if ((s = dev_alloc(&c->detail_store, noise_texel_bytes(format, 4) * 32 * 32 * 32, "detail noise"))) return s;
     if ((s = dev_alloc(&c->curl_store, noise_texel_bytes(format, 2) * 128 * 128, "curl noise"))) return s;
-    if ((s = dev_alloc(&c->base_rl, sizeof(float2) * 128 * 128 * 128, "base fold"))) return s;
+    if ((s = dev_alloc(&c->base_rl, sizeof(uint32_t) * 128 * 128 * 128, "base fold"))) return s;
     if ((s = dev_alloc(&c->detail_hf, sizeof(float) * 32 * 32 * 32, "detail fold"))) return s;
     if ((s = dev_alloc(&c->curl2, sizeof(float2) * 128 * 128, "curl fold"))) return s;
     return PS_OK;

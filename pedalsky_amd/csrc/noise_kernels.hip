@@ -174,12 +174,18 @@ __device__ __forceinline__ void load4(const void* src, size_t i, int format, flo
 }
 
 __global__ __launch_bounds__(256) void k_fold(int format, const void* base, const void* detail, const void* curl,
-                                              float2* base_rl, float* detail_hf, float2* curl2) {
+                                              uint32_t* base_rl, float* detail_hf, float2* curl2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 128 * 128 * 128) {
         float r, g, b, a;
         load4(base, (size_t)i, format, r, g, b, a);
-        base_rl[i] = make_float2(r, g * 0.625f + b * 0.25f + a * 0.125f);
+        // R and the low-frequency FBM sum the march reads, as 16-bit UNORM
+        // (step 1.5e-5) in the bricked layout
+        float lf = g * 0.625f + b * 0.25f + a * 0.125f;
+        uint32_t qr = (uint32_t)rintf(clampf(r, 0.f, 1.f) * 65535.f);
+        uint32_t ql = (uint32_t)rintf(clampf(lf, 0.f, 1.f) * 65535.f);
+        int x = i & 127, y = (i >> 7) & 127, z = i >> 14;
+        base_rl[brick_x(x) + brick_y(y) + brick_z(z)] = qr | (ql << 16);
     }
     if (i < 32 * 32 * 32) {
         float r, g, b, a;
@@ -218,7 +224,7 @@ hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* det
 }
 
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
-                                float2* base_rl, float* detail_hf, float2* curl2, hipStream_t s) {
+                                uint32_t* base_rl, float* detail_hf, float2* curl2, hipStream_t s) {
     hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl, base_rl,
                        detail_hf, curl2);
     return hipGetLastError();

@@ -40,20 +40,26 @@ struct LutParams {
 // Render parameters: one block per scene. Everything that is constant over a
 // frame is folded here by the host (in double, then rounded once).
 // ---------------------------------------------------------------------------
-struct LayerC {
-    // march-side (FP32)
+// The per-layer values the density evaluation reads (copied into registers
+// when a lane enters a layer).
+struct LayerF {
     float base, top, inv_thick, coverage;
     float density_scale, esb, ld, anvil;
     float sigma_t[3];
     float wind_off_x, wind_off_z;     // drift offset (m)
     float skew_x, skew_z;             // skew_dir * skew_thickness (m per unit h)
     uint32_t type;
+};
+
+struct LayerC {
+    LayerF f;
     uint32_t is_cb;                   // layer type 7: phase depends on height
     float droplet;                    // raw layer droplet diameter (um)
+    // phase constants for non-Cb layers (cloud_march.wgsl:256-272), folded in
+    // FP64; omg_hg = 1 - g_hg without cancellation
+    float g_hg, omg_hg, g_d, alpha, w_d, ice;
     // setup-side (FP64)
     double r_bottom, r_top;
-    // phase constants for non-Cb layers (cloud_march.wgsl:256-272)
-    double g_hg, g_d, alpha, w_d, ice;
 };
 
 struct RenderParams {
@@ -94,7 +100,7 @@ struct RenderParams {
     const float4* T; int tw, th;
     const float4* SV; int sw, sh;
     const float4* AP; int aw, ah, ad;
-    const float2* base_rl;            // 128^3 (R, 0.625G + 0.25B + 0.125A)
+    const uint32_t* base_rl;          // 128^3 in 4x4x4 bricks: u16 R | u16 (0.625G + 0.25B + 0.125A) << 16
     const float* detail_hf;           // 32^3 (0.625R + 0.25G + 0.125B)
     const float2* curl;               // 128^2 (2*rg - 1)
     const float* weather;             // ww*wh coverage
@@ -143,6 +149,13 @@ __device__ __forceinline__ float4 lut2d(const float4* __restrict__ t, int w, int
     tx = a.w + (b.w - a.w) * fx; bx = c.w + (d.w - c.w) * fx; r.w = tx + (bx - tx) * fy;
     return r;
 }
+
+// Texel index in the 4x4x4-bricked 128^3 base volume: bricks are laid out
+// x fastest (32 per axis), 64 texels per brick x fastest; the index is the
+// sum of one term per axis.
+__device__ __forceinline__ int brick_x(int x) { return ((x >> 2) << 6) | (x & 3); }
+__device__ __forceinline__ int brick_y(int y) { return ((y >> 2) << 11) | ((y & 3) << 2); }
+__device__ __forceinline__ int brick_z(int z) { return ((z >> 2) << 16) | ((z & 3) << 4); }
 
 // Accurate 1 - exp(-x) for x >= 0 (the energy-conserving step factor).
 // For small x the direct form loses all digits in FP32; a short series is

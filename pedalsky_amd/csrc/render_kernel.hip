@@ -28,18 +28,22 @@ struct Cnt {
              octaves = 0, ap = 0;
 };
 
-// Trilinear Repeat fetch of the folded base volume (128^3 float2).
-__device__ __forceinline__ float2 base_fetch(const float2* __restrict__ v, float cx, float cy, float cz) {
+__device__ __forceinline__ float2 unpack_rl(uint32_t v) {
+    return make_float2((float)(v & 0xffffu) * (1.f / 65535.f), (float)(v >> 16) * (1.f / 65535.f));
+}
+
+// Trilinear Repeat fetch of the folded base volume (128^3, bricked, 2 x u16).
+__device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, float cx, float cy, float cz) {
     float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
     float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    int x0 = ix & 127, x1 = (ix + 1) & 127;
-    int y0 = (iy & 127) << 7, y1 = ((iy + 1) & 127) << 7;
-    int z0 = (iz & 127) << 14, z1 = ((iz + 1) & 127) << 14;
-    float2 c000 = v[z0 + y0 + x0], c100 = v[z0 + y0 + x1];
-    float2 c010 = v[z0 + y1 + x0], c110 = v[z0 + y1 + x1];
-    float2 c001 = v[z1 + y0 + x0], c101 = v[z1 + y0 + x1];
-    float2 c011 = v[z1 + y1 + x0], c111 = v[z1 + y1 + x1];
+    int x0 = brick_x(ix & 127), x1 = brick_x((ix + 1) & 127);
+    int y0 = brick_y(iy & 127), y1 = brick_y((iy + 1) & 127);
+    int z0 = brick_z(iz & 127), z1 = brick_z((iz + 1) & 127);
+    float2 c000 = unpack_rl(v[z0 + y0 + x0]), c100 = unpack_rl(v[z0 + y0 + x1]);
+    float2 c010 = unpack_rl(v[z0 + y1 + x0]), c110 = unpack_rl(v[z0 + y1 + x1]);
+    float2 c001 = unpack_rl(v[z1 + y0 + x0]), c101 = unpack_rl(v[z1 + y0 + x1]);
+    float2 c011 = unpack_rl(v[z1 + y1 + x0]), c111 = unpack_rl(v[z1 + y1 + x1]);
     float2 r;
     {
         float a0 = c000.x + (c100.x - c000.x) * fx, a1 = c010.x + (c110.x - c010.x) * fx;
@@ -132,7 +136,7 @@ __device__ __forceinline__ float ndf_f(float h, uint32_t t, float anvil) {
 // sample_density_light). Returns the pre-erosion cloud value and writes the
 // advected position. `cov` >= 0.01 here, so remap(c, 1-cov, 1, 0, 1) * cov
 // equals c - (1 - cov) exactly.
-__device__ __forceinline__ float shape(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+__device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, float px, float py, float pz,
                                        float h, float cov, uint32_t ctype, float& pax, float& pay, float& paz) {
     pax = px - L.wind_off_x - L.skew_x * h;
     pay = py;
@@ -146,7 +150,7 @@ __device__ __forceinline__ float shape(const RenderParams& P, const LayerC& L, f
 }
 
 // cloud_density_common.wgsl:376-415
-__device__ __forceinline__ float density_light(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+__device__ __forceinline__ float density_light(const RenderParams& P, const LayerF& L, float px, float py, float pz,
                                                float alt, float weather_r, uint32_t ctype, bool& covered) {
     float h = (alt - L.base) * L.inv_thick;
     covered = false;
@@ -160,7 +164,7 @@ __device__ __forceinline__ float density_light(const RenderParams& P, const Laye
 }
 
 // cloud_march.wgsl:432-530
-__device__ __forceinline__ float density_primary(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+__device__ __forceinline__ float density_primary(const RenderParams& P, const LayerF& L, float px, float py, float pz,
                                                  float alt, float weather_r, uint32_t ctype, int& stage) {
     stage = 0;
     float h = (alt - L.base) * L.inv_thick;
@@ -184,7 +188,7 @@ __device__ __forceinline__ float density_primary(const RenderParams& P, const La
 }
 
 // cloud_march.wgsl:543-578 (straight) and 599-666 (cone), FP32.
-__device__ __forceinline__ float march_light(const RenderParams& P, const LayerC& L, float px, float py, float pz,
+__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, float px, float py, float pz,
                                              float alt, float weather_r, uint32_t ctype, Cnt& cnt) {
     const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
     float cos_sun = fmaxf(sy, 0.05f);
@@ -250,9 +254,10 @@ __device__ __forceinline__ bool trans_uv_from_h(const RenderParams& P, float h, 
     return true;
 }
 
-// Sky-view LUT coordinates (sky.wgsl:73-102, cloud_march.wgsl:672-691) for a
-// direction given by its horizontal extent rxz, vertical component vy and
-// (x, z) for the azimuth; zha is the zenith-horizon angle of the viewer.
+// Sky-view LUT coordinates (sky.wgsl:73-102, cloud_march.wgsl:672-691) for the
+// direction (dx, dy, dz) (any length); zha is the viewer's zenith-horizon angle.
+// vza = atan2(|xz|, y) equals acos(y / |d|) without the cancellation near
+// the zenith.
 __device__ __forceinline__ void skyview_uv_f(const RenderParams& P, float dx, float dy, float dz, float zha,
                                              float& u, float& v) {
     float rxz = sqrtf(dx * dx + dz * dz);
@@ -267,63 +272,202 @@ __device__ __forceinline__ void skyview_uv_f(const RenderParams& P, float dx, fl
     u = du - floorf(du);
 }
 
-// Per-pixel, per-octave sun phase (cloud_march.wgsl:278-296) in FP64, in the
-// (1-g)^2 + 2g(1-cos) form of the HG/Draine denominator.
-__device__ __forceinline__ double phase_octave(double omc, double c, double g_scale, double g_hg, double g_d,
-                                               double alpha, double w_d, double ice) {
-    double g = g_hg * g_scale;
-    double omg = 1.0 - g;
-    double den = fmax(omg * omg + 2.0 * g * omc, 1e-4);
-    double p_hg = (1.0 - g * g) / (4.0 * PS_PI * den * sqrt(den));
-    double gd = g_d * g_scale;
-    double omgd = 1.0 - gd;
-    double dend = fmax(omgd * omgd + 2.0 * gd * omc, 1e-4);
-    double num = (1.0 - gd * gd) * (1.0 + alpha * c * c);
-    double dd = 4.0 * (1.0 + alpha * (1.0 + 2.0 * gd * gd) / 3.0) * PS_PI * dend * sqrt(dend);
-    double p = (1.0 - w_d) * p_hg + w_d * (num / fmax(dd, 1e-6));
-    if (ice > 0.0) {
-        double width = (0.5 * PS_PI / 180.0) / fmax(g_scale, 0.1);
-        double s22 = width * sqrt(fmax(1.0 - 0.92718 * 0.92718, 1e-4));
-        double s46 = width * sqrt(fmax(1.0 - 0.69466 * 0.69466, 1e-4));
-        double d22 = (c - 0.92718) / fmax(s22, 1e-4), d46 = (c - 0.69466) / fmax(s46, 1e-4);
-        p += ice * (0.08 * exp(-0.5 * d22 * d22) + 0.03 * exp(-0.5 * d46 * d46));
-    }
-    return p;
-}
-
-// FP32 per-sample phase for cumulonimbus layers (droplet size varies with h).
-__device__ __forceinline__ float phase_octave_f(float omc, float c, float g_scale, float g_hg, float omg_hg,
+// One octave of the approximate-Mie cloud phase (cloud_march.wgsl:151-168,
+// 278-296). The HG / Draine denominator 1 + g^2 - 2 g cos is evaluated as
+// (1 - g)^2 + 2 g (1 - cos) and 1 - g^2 as (1 - g)(1 + g): for water droplets
+// g ~ 0.995, where the textbook form loses ~4 digits in FP32. omg0 = 1 - g_hg
+// comes from the host in FP64; omc = 1 - cos from the FP64 ray setup.
+__device__ __forceinline__ float phase_octave_f(float omc, float c, float g_scale, float g_hg, float omg0,
                                                 float g_d, float alpha, float w_d, float ice) {
     float g = g_hg * g_scale;
-    float omg = g_scale == 1.f ? omg_hg : 1.f - g;
+    float omg = g_scale == 1.f ? omg0 : 1.f - g;
     float den = fmaxf(omg * omg + 2.f * g * omc, 1e-4f);
-    float p_hg = (1.f - g * g) / (4.f * (float)PS_PI * den * sqrtf(den));
+    float p_hg = omg * (1.f + g) / (4.f * (float)PS_PI * den * sqrtf(den));
     float gd = g_d * g_scale;
     float omgd = 1.f - gd;
     float dend = fmaxf(omgd * omgd + 2.f * gd * omc, 1e-4f);
-    float num = (1.f - gd * gd) * (1.f + alpha * c * c);
+    float num = omgd * (1.f + gd) * (1.f + alpha * c * c);
     float dd = 4.f * (1.f + alpha * (1.f + 2.f * gd * gd) / 3.f) * (float)PS_PI * dend * sqrtf(dend);
     float p = (1.f - w_d) * p_hg + w_d * (num / fmaxf(dd, 1e-6f));
     if (ice > 0.f) {
         float width = (float)(0.5 * PS_PI / 180.0) / fmaxf(g_scale, 0.1f);
-        float s22 = width * 0.3746002f, s46 = width * 0.7193335f;  // sqrt(1 - cos^2) at 22 and 46 degrees
+        float s22 = width * 0.37460022f, s46 = width * 0.71933556f;  // sqrt(1 - cos^2) at the halo angles
         float d22 = (c - 0.92718f) / fmaxf(s22, 1e-4f), d46 = (c - 0.69466f) / fmaxf(s46, 1e-4f);
         p += ice * (0.08f * __expf(-0.5f * d22 * d22) + 0.03f * __expf(-0.5f * d46 * d46));
     }
     return p;
 }
 
+// ---------------------------------------------------------------------------
+// Deferred dense-sample shading.
+//
+// Only luminance depends on the light march; transmittance, the early exit
+// and the step schedule depend on density alone. The primary march therefore
+// appends each dense sample (density > 1e-3) to a per-wave LDS queue with
+// the transmittance it saw, and whenever 64 samples are queued the whole
+// wave shades them together: light march, Beer-Powder, multi-octave phase,
+// sun transmittance and sky ambient, one sample per lane. Contributions go
+// to per-pixel accumulators in LDS. A wave never idles lanes on the light
+// march because only some of its pixels hit cloud.
+// ---------------------------------------------------------------------------
+constexpr int kQCap = 128;        // >= 64 + 63: a full flush plus one step's arrivals
+constexpr int kQFields = 13;
+enum { QF_PX = 0, QF_PY, QF_PZ, QF_ALT, QF_WR, QF_DENS, QF_TR, QF_TG, QF_TB, QF_T, QF_STEP, QF_OMC, QF_META };
+
+struct WaveQueue {
+    float f[kQFields][kQCap];
+    float acc[5][64];             // per-pixel luminance rgb, sum(w t), sum(w)
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Shade one queued dense sample (cloud_march.wgsl:886-993) and add its
+// luminance and depth-weight contributions to the owning pixel.
+__device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q, int j, Cnt& cnt) {
+    const float px = Q.f[QF_PX][j], py = Q.f[QF_PY][j], pz = Q.f[QF_PZ][j];
+    const float alt = Q.f[QF_ALT][j], wr = Q.f[QF_WR][j], density = Q.f[QF_DENS][j];
+    const float Tr = Q.f[QF_TR][j], Tg = Q.f[QF_TG][j], Tb = Q.f[QF_TB][j];
+    const float t = Q.f[QF_T][j], step = Q.f[QF_STEP][j], omc = Q.f[QF_OMC][j];
+    const uint32_t meta = __float_as_uint(Q.f[QF_META][j]);
+    const int owner = (int)(meta & 63u), li = (int)((meta >> 6) & 7u);
+    const uint32_t ctype = (meta >> 9) & 7u;
+    const LayerC& LC = P.layers[li];
+    const LayerF L = LC.f;
+    cnt.dense++;
+    const float od = march_light(P, L, px, py, pz, alt, wr, ctype, cnt);
+    const float sig0 = L.sigma_t[0], sig1 = L.sigma_t[1], sig2 = L.sigma_t[2];
+    // Beer-Powder (cloud_march.wgsl:888-892); exp(-2x) = exp(-x)^2
+    const float e0r = __expf(-sig0 * od), e0g = __expf(-sig1 * od), e0b = __expf(-sig2 * od);
+    const float pw = P.powder;
+    const float enr = e0r + (2.f * e0r * (1.f - e0r * e0r) - e0r) * pw;
+    const float eng = e0g + (2.f * e0g * (1.f - e0g * e0g) - e0g) * pw;
+    const float enb = e0b + (2.f * e0b * (1.f - e0b * e0b) - e0b) * pw;
+    const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
+    const float qy = py + P.R;
+    // sun transmittance (cloud_march.wgsl:328-354)
+    float str = 0.f, stg = 0.f, stb = 0.f;
+    {
+        const float hh = fmaxf(alt, 1.f);
+        const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / (P.R + hh), -1.f, 1.f);
+        float tu, tv;
+        if (trans_uv_from_h(P, hh, mu, tu, tv)) {
+            const float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
+            str = tt.x; stg = tt.y; stb = tt.z;
+        }
+    }
+    // phase constants (cloud_march.wgsl:256-272); cumulonimbus blends to ice with height
+    float g_hg = LC.g_hg, omg0 = LC.omg_hg, g_d = LC.g_d, alpha = LC.alpha, w_d = LC.w_d, ice = LC.ice;
+    if (LC.is_cb) {
+        const float mt = smoothstepf(0.60f, 0.85f, hn);
+        float d = LC.droplet + (50.f - LC.droplet) * mt;
+        d = clampf(d * P.droplet_bias, 5.f, 50.f);
+        const float xg = -0.0990567f / (d - 1.67154f);
+        g_hg = __expf(xg);
+        omg0 = -expm1f(xg);
+        g_d = __expf(-2.20679f / (d + 3.91029f)) - 0.428934f;
+        alpha = __expf(3.62489f - 8.29288f / (d + 5.52825f));
+        w_d = __expf(-0.599085f / (d - 0.641583f) - 0.665888f);
+        ice = smoothstepf(35.f, 50.f, d);
+    }
+    // multi-octave sun term (cloud_march.wgsl:915-927)
+    const float cosv = 1.f - omc;
+    float sir = 0.f, sig_ = 0.f, sib = 0.f;
+    const int noct = P.octaves;
+    for (int n = 0; n < noct; ++n) {
+        const float ph = P.a_pow[n] * phase_octave_f(omc, cosv, P.c_pow[n], g_hg, omg0, g_d, alpha, w_d, ice);
+        if (n == 0) {
+            sir += ph * e0r; sig_ += ph * e0g; sib += ph * e0b;
+        } else {
+            const float bn = P.b_pow[n];
+            sir += ph * __expf(-sig0 * od * bn);
+            sig_ += ph * __expf(-sig1 * od * bn);
+            sib += ph * __expf(-sig2 * od * bn);
+        }
+    }
+    cnt.octaves += (uint32_t)noct;
+    sir *= P.E[0] * str; sig_ *= P.E[1] * stg; sib *= P.E[2] * stb;
+    if (P.forward_bias > 0.f) {
+        const float fbm = 1.f + P.forward_bias * smoothstepf(0.f, 1.f, fmaxf(cosv, 0.f));
+        sir *= fbm; sig_ *= fbm; sib *= fbm;
+    }
+    // sky ambient at the local up (cloud_march.wgsl:693-700, 951-955)
+    float ar, ag, ab;
+    {
+        const float rho = sqrtf(fmaxf(alt * (2.f * P.R + alt), 0.f));
+        const float zha = (float)PS_PI - atan2f(P.R, rho);
+        float u, v;
+        skyview_uv_f(P, px, qy, pz, zha, u, v);
+        const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
+        const float g = P.ambient * (0.3f + 0.7f * hn);
+        ar = sv.x * P.E[0] * g; ag = sv.y * P.E[1] * g; ab = sv.z * P.E[2] * g;
+    }
+    // lightning (cloud_march.wgsl:968-977)
+    if (P.light_radius > 0.f) {
+        const float lx = px - P.light_x, lz = pz - P.light_z;
+        const float fo = clampf(1.f - sqrtf(lx * lx + lz * lz) / P.light_radius, 0.f, 1.f);
+        const float wgt = fo * fo * (3.f - 2.f * fo);
+        ar += P.light_rgb[0] * wgt; ag += P.light_rgb[1] * wgt; ab += P.light_rgb[2] * wgt;
+    }
+    // energy-conserving step (cloud_march.wgsl:979-993)
+    const float inr = P.sigma_s[0] * density * (sir * enr + ar);
+    const float ing = P.sigma_s[1] * density * (sig_ * eng + ag);
+    const float inb = P.sigma_s[2] * density * (sib * enb + ab);
+    const float lsr = fmaxf(density * sig0, 1e-6f), lsg = fmaxf(density * sig1, 1e-6f),
+                lsb = fmaxf(density * sig2, 1e-6f);
+    const float slr = Tr * (inr * one_minus_exp_neg(lsr * step) / lsr);
+    const float slg = Tg * (ing * one_minus_exp_neg(lsg * step) / lsg);
+    const float slb = Tb * (inb * one_minus_exp_neg(lsb * step) / lsb);
+    const float sw = slr * 0.2126f + slg * 0.7152f + slb * 0.0722f;
+    atomicAdd(&Q.acc[0][owner], slr);
+    atomicAdd(&Q.acc[1][owner], slg);
+    atomicAdd(&Q.acc[2][owner], slb);
+    atomicAdd(&Q.acc[3][owner], sw * t);
+    atomicAdd(&Q.acc[4][owner], sw);
+}
+
+// Shade the first n (<= 64) queued samples with the whole wave, then move
+// the entries [64, qn) to the front. Called with all 64 lanes active.
+__device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q, int lane, int n, int qn, Cnt& cnt) {
+    wave_sync_lds();
+    if (lane < n) shade_record(P, Q, lane, cnt);
+    wave_sync_lds();
+    const int rest = qn - n;
+    if (rest > 0) {
+        float v[kQFields];
+        if (lane < rest) {
+#pragma unroll
+            for (int k = 0; k < kQFields; ++k) v[k] = Q.f[k][64 + lane];
+        }
+        wave_sync_lds();
+        if (lane < rest) {
+#pragma unroll
+            for (int k = 0; k < kQFields; ++k) Q.f[k][lane] = v[k];
+        }
+        wave_sync_lds();
+    }
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__ Pp) {
+    __shared__ WaveQueue queues[4];
     const RenderParams& P = *Pp;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveQueue& Q = queues[wave];
     const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (x >= P.W || y >= P.H) return;
+    // Lanes outside the image stay in the wave (they help shade queued
+    // samples) but march nothing and write nothing.
+    const bool valid = x < P.W && y < P.H;
+    const int xc = min(x, P.W - 1), yc = min(y, P.H - 1);
     Cnt cnt;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Q.acc[k][lane] = 0.f;
 
     // ---- FP64 view ray (math.wgsl:23-33 / sky.wgsl:166-168) -------------------------
-    const double fx = x + 0.5, fy = y + 0.5;
+    const double fx = xc + 0.5, fy = yc + 0.5;
     const double ndx = fx / P.W * 2.0 - 1.0, ndy = 1.0 - fy / P.H * 2.0;
     const double* m = P.inv_vp;
     double nh0 = m[0] * ndx + m[4] * ndy + m[8] + m[12];
@@ -337,7 +481,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     // cos(theta) against the sun direction as uploaded (not renormalised, as
     // the reference uses it); 1 - cos in FP64 keeps the forward HG peak exact.
     const double cos_vs = dxd * P.sun_d[0] + dyd * P.sun_d[1] + dzd * P.sun_d[2];
-    const double omc = 1.0 - cos_vs;
+    const float omcf = (float)(1.0 - cos_vs);
 
     // ---- sky pass (sky.wgsl:159-220) ------------------------------------------------------
     float3 sky;
@@ -394,7 +538,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     }
 
     // ---- blue-noise jitter (cloud_march.wgsl:745-756) ------------------------------------
-    int jx = x & 63, jy = y & 63;
+    int jx = xc & 63, jy = yc & 63;
     if (P.temporal_jitter) {
         uint32_t ph = P.frame_index & 15u;
         jx = (jx ^ (int)((ph * 11u + 7u) & 63u)) & 63;
@@ -405,7 +549,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     // ---- depth clip (cloud_march.wgsl:764-775) --------------------------------------------
     double depth_t_max = 1e30;
     if (P.has_depth) {
-        float dn = P.depth[(size_t)y * P.W + x];
+        float dn = P.depth[(size_t)yc * P.W + xc];
         if (dn > 0.f) {
             double w0 = m[0] * ndx + m[4] * ndy + m[8] * dn + m[12];
             double w1 = m[1] * ndx + m[5] * ndy + m[9] * dn + m[13];
@@ -417,14 +561,15 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     }
 
     // ---- per-layer intervals in FP64 (cloud_march.wgsl:370-416, 777-806) ---------------
-    double t0s[PS_MAX_LAYERS], t1s[PS_MAX_LAYERS];
-    int idx[PS_MAX_LAYERS];
+    // Sorted by entry distance; kept as (start, step) pairs for the march.
+    float seg_t0[PS_MAX_LAYERS], seg_step[PS_MAX_LAYERS];
+    int seg_layer[PS_MAX_LAYERS];
+    double key[PS_MAX_LAYERS];
     const int lc = P.n_layers;
     const double ocx = P.cam[0], ocy = P.cam[1] + P.R, ocz = P.cam[2];
     const double b_ = ocx * dxd + ocy * dyd + ocz * dzd;
-    const double oc2 = ocx * ocx + ocy * ocy + ocz * ocz;
-    const double r_cam = sqrt(oc2);
-    int nh = 0;
+    const double r_cam = sqrt(ocx * ocx + ocy * ocy + ocz * ocz);
+    int nseg = 0;
     for (int l = 0; l < lc; ++l) {
         const LayerC& L = P.layers[l];
         // |oc|^2 - r^2 as (|oc| - r)(|oc| + r)
@@ -449,67 +594,52 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
             }
             ok = ok && (tb > ta);
         }
-        bool flag = false;
-        if (ok) {
-            tb = fmin(tb, depth_t_max);
-            flag = tb > ta;
-        }
-        // insertion by t0 (stable: equal keys keep layer order, as the reference sort)
-        if (flag) {
-            int j = nh++;
-            while (j > 0 && t0s[j - 1] > ta) {
-                t0s[j] = t0s[j - 1]; t1s[j] = t1s[j - 1]; idx[j] = idx[j - 1];
+        if (ok) tb = fmin(tb, depth_t_max);
+        if (valid && ok && tb > ta) {
+            // insertion by entry distance (stable: equal keys keep layer order)
+            const double stepd = (tb - ta) / P.steps;
+            int j = nseg++;
+            while (j > 0 && key[j - 1] > ta) {
+                key[j] = key[j - 1]; seg_t0[j] = seg_t0[j - 1]; seg_step[j] = seg_step[j - 1];
+                seg_layer[j] = seg_layer[j - 1];
                 --j;
             }
-            t0s[j] = ta;
-            t1s[j] = tb;
-            idx[j] = l;
+            key[j] = ta;
+            seg_t0[j] = (float)(ta + jitter * stepd);
+            seg_step[j] = (float)stepd;
+            seg_layer[j] = l;
         }
     }
 
-    // ---- march -------------------------------------------------------------------------------
-    float lr = 0.f, lg = 0.f, lb = 0.f;
-    float tr = 1.f, tg = 1.f, tbb = 1.f;
-    float tw = 0.f, twn = 0.f;
+    // ---- wave-uniform primary march ---------------------------------------------------
     const float r0f = (float)P.r0;
-    const float kcam = (float)(2.0 * P.r0 * (ocx * dxd + ocy * dyd + ocz * dzd) / fmax(P.r0, 1.0));
+    const float kcam = (float)(2.0 * P.r0 * b_ / fmax(P.r0, 1.0));
     const float h0f = (float)P.h0;
     const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
-    const float fb_dir = smoothstepf(0.f, 1.f, fmaxf((float)cos_vs, 0.f));
-    const float omcf = (float)omc, cosf = (float)cos_vs;
-    for (int i = 0; i < nh; ++i) {
-        if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) break;
-        const LayerC& L = P.layers[idx[i]];
-        const double t0d = t0s[i];
-        const double seg = t1s[i] - t0d;
-        if (seg <= 0.0) continue;
-        cnt.layer_marches++;
-        const double stepd = seg / P.steps;
-        const float step = (float)stepd;
-        const float tstart = (float)(t0d + jitter * stepd);
-        const float sig0 = L.sigma_t[0], sig1 = L.sigma_t[1], sig2 = L.sigma_t[2];
-        // per-octave sun phase coefficients a^n * p_n(cos), per pixel and layer
-        float coef[PS_MAX_OCTAVES];
-        const int noct = P.octaves;
-        if (!L.is_cb) {
-#pragma unroll
-            for (int n = 0; n < PS_MAX_OCTAVES; ++n) {
-                coef[n] = n < noct ? (float)(P.a_pow[n] * phase_octave(omc, cos_vs, P.c_pow[n], L.g_hg, L.g_d,
-                                                                         L.alpha, L.w_d, L.ice))
-                                   : 0.f;
-            }
-        }
-        for (int s = 0; s < P.steps; ++s) {
+    float tr = 1.f, tg = 1.f, tbb = 1.f;
+    int cur = 0, s = 0, li = 0;
+    float tstart = 0.f, step = 0.f;
+    LayerF L;
+    bool active = nseg > 0;
+    if (active) {
+        li = seg_layer[0]; tstart = seg_t0[0]; step = seg_step[0];
+        L = P.layers[li].f;
+    }
+    int qn = 0;  // queued samples (wave-uniform)
+    while (__ballot(active) != 0ull) {
+        bool dense = false;
+        float t = 0.f, px = 0.f, py = 0.f, pz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
+        uint32_t ctype = 0u;
+        if (active) {
             cnt.primary++;
-            const float t = tstart + (float)s * step;
-            const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
+            t = tstart + (float)s * step;
+            px = camx + dx * t; py = camy + dy * t; pz = camz + dz * t;
             const float qy = py + P.R;
             const float rr = sqrtf(px * px + qy * qy + pz * pz);
-            const float alt = h0f + t * (kcam + t) / (fmaxf(r0f, 1.f) + rr);
+            alt = h0f + t * (kcam + t) / (fmaxf(r0f, 1.f) + rr);
             const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
-            const float wr = weather_fetch(P.weather, P.ww, P.wh, wx * P.inv_weather_scale + 0.5f,
-                                           wz * P.inv_weather_scale + 0.5f);
-            uint32_t ctype = L.type;
+            wr = weather_fetch(P.weather, P.ww, P.wh, wx * P.inv_weather_scale + 0.5f, wz * P.inv_weather_scale + 0.5f);
+            ctype = L.type;
             if (P.use_type_grid) {
                 float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
                 int gx = (int)floorf(gu * 128.f), gy = (int)floorf(gv * 128.f);
@@ -519,113 +649,60 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
                 if (g != 255u) ctype = g;
             }
             int stage;
-            const float density = density_primary(P, L, px, py, pz, alt, wr, ctype, stage);
+            density = density_primary(P, L, px, py, pz, alt, wr, ctype, stage);
             cnt.covered += stage >= 1 ? 1u : 0u;
             cnt.eroded += stage >= 2 ? 1u : 0u;
-            if (density > 1e-3f) {
-                cnt.dense++;
-                const float od = march_light(P, L, px, py, pz, alt, wr, ctype, cnt);
-                // Beer-Powder (cloud_march.wgsl:888-892); exp(-2x) = exp(-x)^2
-                const float e0r = __expf(-sig0 * od), e0g = __expf(-sig1 * od), e0b = __expf(-sig2 * od);
-                const float pw = P.powder;
-                const float enr = e0r + (2.f * e0r * (1.f - e0r * e0r) - e0r) * pw;
-                const float eng = e0g + (2.f * e0g * (1.f - e0g * e0g) - e0g) * pw;
-                const float enb = e0b + (2.f * e0b * (1.f - e0b * e0b) - e0b) * pw;
-                const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
-                // sun transmittance (cloud_march.wgsl:328-354)
-                float str = 0.f, stg = 0.f, stb = 0.f;
-                {
-                    const float hh = fmaxf(alt, 1.f);
-                    const float rs = P.R + hh;
-                    const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / rs, -1.f, 1.f);
-                    float tu, tv;
-                    if (trans_uv_from_h(P, hh, mu, tu, tv)) {
-                        float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
-                        str = tt.x; stg = tt.y; stb = tt.z;
-                    }
-                }
-                // multi-octave sun term (cloud_march.wgsl:915-927)
-                float sir = 0.f, sig_ = 0.f, sib = 0.f;
-                if (!L.is_cb) {
-                    sir = coef[0] * e0r; sig_ = coef[0] * e0g; sib = coef[0] * e0b;
-#pragma unroll
-                    for (int n = 1; n < PS_MAX_OCTAVES; ++n) {
-                        if (n < noct) {
-                            const float bn = P.b_pow[n];
-                            sir += coef[n] * __expf(-sig0 * od * bn);
-                            sig_ += coef[n] * __expf(-sig1 * od * bn);
-                            sib += coef[n] * __expf(-sig2 * od * bn);
-                        }
-                    }
-                } else {
-                    // cumulonimbus mixed phase: droplet diameter blends to 50 um with height
-                    const float mt = smoothstepf(0.60f, 0.85f, hn);
-                    float d = L.droplet + (50.f - L.droplet) * mt;
-                    d = clampf(d * P.droplet_bias, 5.f, 50.f);
-                    const float xg = -0.0990567f / (d - 1.67154f);
-                    const float g_hg = __expf(xg), omg = -expm1f(xg);
-                    const float g_d = __expf(-2.20679f / (d + 3.91029f)) - 0.428934f;
-                    const float alpha = __expf(3.62489f - 8.29288f / (d + 5.52825f));
-                    const float w_d = __expf(-0.599085f / (d - 0.641583f) - 0.665888f);
-                    const float ice = smoothstepf(35.f, 50.f, d);
-                    for (int n = 0; n < noct && n < PS_MAX_OCTAVES; ++n) {
-                        const float cf = P.a_pow[n] * phase_octave_f(omcf, cosf, P.c_pow[n], g_hg, omg, g_d,
-                                                                     alpha, w_d, ice);
-                        const float bn = P.b_pow[n];
-                        sir += cf * __expf(-sig0 * od * bn);
-                        sig_ += cf * __expf(-sig1 * od * bn);
-                        sib += cf * __expf(-sig2 * od * bn);
-                    }
-                }
-                cnt.octaves += (uint32_t)noct;
-                sir *= P.E[0] * str; sig_ *= P.E[1] * stg; sib *= P.E[2] * stb;
-                if (P.forward_bias > 0.f) {
-                    const float fbm = 1.f + P.forward_bias * fb_dir;
-                    sir *= fbm; sig_ *= fbm; sib *= fbm;
-                }
-                // sky ambient at the local up (cloud_march.wgsl:693-700, 951-955)
-                float ar, ag, ab;
-                {
-                    const float hh = alt;
-                    const float rho = sqrtf(fmaxf(hh * (2.f * P.R + hh), 0.f));
-                    const float zha = (float)PS_PI - atan2f(P.R, rho);
-                    float u, v;
-                    skyview_uv_f(P, px, qy, pz, zha, u, v);
-                    const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
-                    const float g = P.ambient * (0.3f + 0.7f * hn);
-                    ar = sv.x * P.E[0] * g; ag = sv.y * P.E[1] * g; ab = sv.z * P.E[2] * g;
-                }
-                // lightning (cloud_march.wgsl:968-977)
-                if (P.light_radius > 0.f) {
-                    const float lx = px - P.light_x, lz = pz - P.light_z;
-                    const float fo = clampf(1.f - sqrtf(lx * lx + lz * lz) / P.light_radius, 0.f, 1.f);
-                    const float wgt = fo * fo * (3.f - 2.f * fo);
-                    ar += P.light_rgb[0] * wgt; ag += P.light_rgb[1] * wgt; ab += P.light_rgb[2] * wgt;
-                }
-                // energy-conserving step (cloud_march.wgsl:979-993)
-                const float inr = P.sigma_s[0] * density * (sir * enr + ar);
-                const float ing = P.sigma_s[1] * density * (sig_ * eng + ag);
-                const float inb = P.sigma_s[2] * density * (sib * enb + ab);
-                const float lsr = fmaxf(density * sig0, 1e-6f), lsg = fmaxf(density * sig1, 1e-6f),
-                            lsb = fmaxf(density * sig2, 1e-6f);
-                const float omr = one_minus_exp_neg(lsr * step), omg2 = one_minus_exp_neg(lsg * step),
-                            omb = one_minus_exp_neg(lsb * step);
-                const float slr = tr * (inr * omr / lsr), slg = tg * (ing * omg2 / lsg), slb = tbb * (inb * omb / lsb);
-                const float sw = slr * 0.2126f + slg * 0.7152f + slb * 0.0722f;
-                lr += slr; lg += slg; lb += slb;
-                tw += sw * t;
-                twn += sw;
-                tr *= 1.f - omr; tg *= 1.f - omg2; tbb *= 1.f - omb;
+            dense = density > 1e-3f;
+        }
+        const unsigned long long bal = __ballot(dense);
+        if (bal != 0ull) {
+            if (dense) {
+                const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                Q.f[QF_PX][pos] = px; Q.f[QF_PY][pos] = py; Q.f[QF_PZ][pos] = pz;
+                Q.f[QF_ALT][pos] = alt; Q.f[QF_WR][pos] = wr; Q.f[QF_DENS][pos] = density;
+                Q.f[QF_TR][pos] = tr; Q.f[QF_TG][pos] = tg; Q.f[QF_TB][pos] = tbb;
+                Q.f[QF_T][pos] = t; Q.f[QF_STEP][pos] = step; Q.f[QF_OMC][pos] = omcf;
+                Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9));
             }
-            if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) break;
+            qn += __popcll(bal);
+        }
+        if (active) {
+            if (dense) {
+                tr *= 1.f - one_minus_exp_neg(fmaxf(density * L.sigma_t[0], 1e-6f) * step);
+                tg *= 1.f - one_minus_exp_neg(fmaxf(density * L.sigma_t[1], 1e-6f) * step);
+                tbb *= 1.f - one_minus_exp_neg(fmaxf(density * L.sigma_t[2], 1e-6f) * step);
+            }
+            ++s;
+            if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) {
+                active = false;
+            } else if (s >= P.steps) {
+                ++cur;
+                if (cur < nseg) {
+                    li = seg_layer[cur]; tstart = seg_t0[cur]; step = seg_step[cur];
+                    L = P.layers[li].f;
+                    s = 0;
+                } else {
+                    active = false;
+                }
+            }
+        }
+        if (qn >= 64) {
+            flush_queue(P, Q, lane, 64, qn, cnt);
+            qn -= 64;
         }
     }
+    if (qn > 0) flush_queue(P, Q, lane, qn, qn, cnt);
+    wave_sync_lds();
+    cnt.layer_marches = (uint32_t)nseg;
 
     // ---- aerial perspective on the cloud luminance (cloud_march.wgsl:1016-1048) ----
+    const float lr = Q.acc[0][lane], lg = Q.acc[1][lane], lb = Q.acc[2][lane];
+    const float tw = Q.acc[3][lane], twn = Q.acc[4][lane];
     const float t_lum = tr * 0.2126f + tg * 0.7152f + tbb * 0.0722f;
     const float opacity = 1.f - t_lum;
     float olr = lr, olg = lg, olb = lb;
-    if (opacity > 1e-4f) {
+    if (valid && opacity > 1e-4f) {
         cnt.ap++;
         const float tc = tw / fmaxf(twn, 1e-6f);
         const float ds = fmaxf(tc, 50.f);
@@ -661,32 +738,34 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     }
 
     // ---- outputs: composite dst = lum + sky * T ------------------------------------------------
-    const size_t o = (size_t)y * P.W + x;
-    const float4 comp = make_float4(olr + sky.x * tr, olg + sky.y * tg, olb + sky.z * tbb, 1.f);
-    const float4 lum4 = make_float4(olr, olg, olb, 1.f);
-    const float4 tr4 = make_float4(tr, tg, tbb, 1.f);
-    if (P.out_f16) {
-        if (P.out_comp) {
-            __half2* q = (__half2*)P.out_comp + 2 * o;
-            q[0] = __floats2half2_rn(comp.x, comp.y); q[1] = __floats2half2_rn(comp.z, comp.w);
+    if (valid) {
+        const size_t o = (size_t)y * P.W + x;
+        const float4 comp = make_float4(olr + sky.x * tr, olg + sky.y * tg, olb + sky.z * tbb, 1.f);
+        const float4 lum4 = make_float4(olr, olg, olb, 1.f);
+        const float4 tr4 = make_float4(tr, tg, tbb, 1.f);
+        if (P.out_f16) {
+            if (P.out_comp) {
+                __half2* q = (__half2*)P.out_comp + 2 * o;
+                q[0] = __floats2half2_rn(comp.x, comp.y); q[1] = __floats2half2_rn(comp.z, comp.w);
+            }
+            if (P.out_lum) {
+                __half2* q = (__half2*)P.out_lum + 2 * o;
+                q[0] = __floats2half2_rn(lum4.x, lum4.y); q[1] = __floats2half2_rn(lum4.z, lum4.w);
+            }
+            if (P.out_trans) {
+                __half2* q = (__half2*)P.out_trans + 2 * o;
+                q[0] = __floats2half2_rn(tr4.x, tr4.y); q[1] = __floats2half2_rn(tr4.z, tr4.w);
+            }
+        } else {
+            if (P.out_comp) ((float4*)P.out_comp)[o] = comp;
+            if (P.out_lum) ((float4*)P.out_lum)[o] = lum4;
+            if (P.out_trans) ((float4*)P.out_trans)[o] = tr4;
         }
-        if (P.out_lum) {
-            __half2* q = (__half2*)P.out_lum + 2 * o;
-            q[0] = __floats2half2_rn(lum4.x, lum4.y); q[1] = __floats2half2_rn(lum4.z, lum4.w);
-        }
-        if (P.out_trans) {
-            __half2* q = (__half2*)P.out_trans + 2 * o;
-            q[0] = __floats2half2_rn(tr4.x, tr4.y); q[1] = __floats2half2_rn(tr4.z, tr4.w);
-        }
-    } else {
-        if (P.out_comp) ((float4*)P.out_comp)[o] = comp;
-        if (P.out_lum) ((float4*)P.out_lum)[o] = lum4;
-        if (P.out_trans) ((float4*)P.out_trans)[o] = tr4;
     }
 
     if (COUNT) {
         unsigned long long* c = P.counters;
-        atomicAdd(c + 0, 1ull);
+        if (valid) atomicAdd(c + 0, 1ull);
         atomicAdd(c + 1, (unsigned long long)cnt.layer_marches);
         atomicAdd(c + 2, (unsigned long long)cnt.primary);
         atomicAdd(c + 3, (unsigned long long)cnt.covered);
