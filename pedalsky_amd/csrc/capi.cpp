@@ -94,6 +94,7 @@ struct ps_ctx {
     int noise_format = -1;
     void *base_store = nullptr, *detail_store = nullptr, *curl_store = nullptr;
     uint32_t* base_rl = nullptr;
+    uint8_t* rmax = nullptr;
     float* detail_hf = nullptr;
     float2* curl2 = nullptr;
     bool have_noise = false;
@@ -299,7 +300,7 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     P.T = c->T; P.tw = dm.transmittance_w; P.th = dm.transmittance_h;
     P.SV = c->SV; P.sw = dm.skyview_w; P.sh = dm.skyview_h;
     P.AP = c->AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
-    P.base_rl = c->base_rl; P.detail_hf = c->detail_hf; P.curl = c->curl2;
+    P.base_rl = c->base_rl; P.rmax = c->rmax; P.detail_hf = c->detail_hf; P.curl = c->curl2;
     P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
     P.type_grid = c->type_grid; P.use_type_grid = c->type_grid_active ? 1 : 0;
     P.overcast = c->overcast_active ? c->overcast : nullptr;
@@ -407,7 +408,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     hipStreamSynchronize(c->stream);
     dev_free(c->T); dev_free(c->MS); dev_free(c->SV); dev_free(c->AP);
     dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
-    dev_free(c->base_rl); dev_free(c->detail_hf); dev_free(c->curl2);
+    dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_params); dev_free(c->d_counters);
     if (c->h_params) {
@@ -514,6 +515,7 @@ static ps_status alloc_noise(ps_ctx* c, int format) {
     if ((s = dev_alloc(&c->detail_store, noise_texel_bytes(format, 4) * 32 * 32 * 32, "detail noise"))) return s;
     if ((s = dev_alloc(&c->curl_store, noise_texel_bytes(format, 2) * 128 * 128, "curl noise"))) return s;
     if ((s = dev_alloc(&c->base_rl, sizeof(uint32_t) * 128 * 128 * 128, "base fold"))) return s;
+    if ((s = dev_alloc(&c->rmax, 32 * 32 * 32, "base max bricks"))) return s;
     if ((s = dev_alloc(&c->detail_hf, sizeof(float) * 32 * 32 * 32, "detail fold"))) return s;
     if ((s = dev_alloc(&c->curl2, sizeof(float2) * 128 * 128, "curl fold"))) return s;
     return PS_OK;
@@ -527,7 +529,7 @@ ps_status ps_generate_noise(ps_ctx* c, uint32_t seed, int32_t format) {
     if ((s = alloc_noise(c, format))) return s;
     PS_HIP(ps_launch_noise_bake(seed, format, c->base_store, c->detail_store, c->curl_store, c->stream));
     PS_HIP(ps_launch_noise_fold(format, c->base_store, c->detail_store, c->curl_store, c->base_rl, c->detail_hf,
-                                c->curl2, c->stream));
+                                c->curl2, c->rmax, c->stream));
     c->have_noise = true;
     return PS_OK;
 }
@@ -545,7 +547,7 @@ ps_status ps_upload_noise(ps_ctx* c, int32_t format, const void* base, const voi
     PS_HIP(hipMemcpyAsync(c->curl_store, curl, noise_texel_bytes(format, 2) * 128 * 128, hipMemcpyHostToDevice,
                           c->stream));
     PS_HIP(ps_launch_noise_fold(format, c->base_store, c->detail_store, c->curl_store, c->base_rl, c->detail_hf,
-                                c->curl2, c->stream));
+                                c->curl2, c->rmax, c->stream));
     PS_HIP(hipStreamSynchronize(c->stream));
     c->have_noise = true;
     return PS_OK;

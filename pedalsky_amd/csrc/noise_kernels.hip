@@ -205,6 +205,25 @@ __global__ __launch_bounds__(256) void k_fold(int format, const void* base, cons
     }
 }
 
+// Conservative per-brick bound for empty-space skipping: the maximum R over
+// texels [4b, 4b + 4] on each axis (the brick plus the +1 neighbour a
+// trilinear tap whose lower corner lies in the brick also reads), rounded up
+// to a multiple of 1/255.
+__global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_rl, uint8_t* rmax) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= 32 * 32 * 32) return;
+    int bx = b & 31, by = (b >> 5) & 31, bz = b >> 10;
+    uint32_t m = 0;
+    for (int k = 0; k <= 4; ++k)
+        for (int j = 0; j <= 4; ++j)
+            for (int i = 0; i <= 4; ++i) {
+                int x = (bx * 4 + i) & 127, y = (by * 4 + j) & 127, z = (bz * 4 + k) & 127;
+                m = max(m, base_rl[brick_x(x) + brick_y(y) + brick_z(z)] & 0xffffu);
+            }
+    // u16 -> u8 rounded up: ceil(m * 255 / 65535)
+    rmax[b] = (uint8_t)min((m * 255u + 65534u) / 65535u, 255u);
+}
+
 __global__ __launch_bounds__(256) void k_weather_fold(const uint16_t* src, int n, int channels, float* dst) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -224,9 +243,10 @@ hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* det
 }
 
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
-                                uint32_t* base_rl, float* detail_hf, float2* curl2, hipStream_t s) {
+                                uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s) {
     hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl, base_rl,
                        detail_hf, curl2);
+    hipLaunchKernelGGL(k_rmax, dim3(32 * 32 * 32 / 64), dim3(64), 0, s, base_rl, rmax);
     return hipGetLastError();
 }
 

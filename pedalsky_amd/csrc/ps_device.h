@@ -101,6 +101,7 @@ struct RenderParams {
     const float4* SV; int sw, sh;
     const float4* AP; int aw, ah, ad;
     const uint32_t* base_rl;          // 128^3 in 4x4x4 bricks: u16 R | u16 (0.625G + 0.25B + 0.125A) << 16
+    const uint8_t* rmax;              // 32^3: per brick, max R over the brick and its +1 halo (ceil to 1/255)
     const float* detail_hf;           // 32^3 (0.625R + 0.25G + 0.125B)
     const float2* curl;               // 128^2 (2*rg - 1)
     const float* weather;             // ww*wh coverage

@@ -136,16 +136,33 @@ __device__ __forceinline__ float ndf_f(float h, uint32_t t, float anvil) {
 // sample_density_light). Returns the pre-erosion cloud value and writes the
 // advected position. `cov` >= 0.01 here, so remap(c, 1-cov, 1, 0, 1) * cov
 // equals c - (1 - cov) exactly.
+//
+// Empty-space skipping. base_cloud = (r + 1 - lf) / (2 - lf) is increasing in
+// r and non-increasing in lf, so base_cloud <= (r_max + 1) / 2 for any LF-FBM
+// value. When (r_max + 1)/2 * ndf <= (1 - cov) the shape is <= 0 and the
+// density is exactly 0 whatever the fetch would return; the bound is checked
+// first with r_max = 1 (no memory access), then with the brick's r_max. The
+// 1e-5 margin covers FP32 rounding of the skipped arithmetic, so skipping
+// never changes a result. Returns -1 when skipped.
 __device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, float px, float py, float pz,
                                        float h, float cov, uint32_t ctype, float& pax, float& pay, float& paz) {
     pax = px - L.wind_off_x - L.skew_x * h;
     pay = py;
     paz = pz - L.wind_off_z - L.skew_z * h;
+    const float ndfv = ndf_f(h, ctype, L.anvil);
+    const float thr = (1.f - cov) * (1.f - 1e-5f);
+    if (ndfv <= thr) return -1.f;
     float s = P.inv_base_scale * 128.f;
-    float2 b = base_fetch(P.base_rl, pax * s - 0.5f, pay * s - 0.5f, paz * s - 0.5f);
+    const float cx = pax * s - 0.5f, cy = pay * s - 0.5f, cz = paz * s - 0.5f;
+    {
+        const int bx = ((int)floorf(cx) >> 2) & 31, by = ((int)floorf(cy) >> 2) & 31, bz = ((int)floorf(cz) >> 2) & 31;
+        const float rm = (float)P.rmax[(bz << 10) | (by << 5) | bx] * (1.f / 255.f);
+        if ((rm + 1.f) * 0.5f * ndfv <= thr) return -1.f;
+    }
+    float2 b = base_fetch(P.base_rl, cx, cy, cz);
     float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
     float base_cloud = (b.x + (1.f - lf)) / fmaxf(2.f - lf, 1e-5f);
-    float cloud = base_cloud * ndf_f(h, ctype, L.anvil);
+    float cloud = base_cloud * ndfv;
     return cloud - (1.f - cov);
 }
 
