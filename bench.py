@@ -55,6 +55,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (a known-size reference for the "
+                         "FETCH_SIZE / WRITE_SIZE hardware counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,6 +89,14 @@ def main() -> int:
         w = sweep[k % 16]
         abi.check(lib.ps_set_frame(r.ctx, C.byref(w.frame)))
         abi.check(lib.ps_frame(r.ctx, abi.PS_LUT_ALL, C.c_void_p(depths[k % 16].data_ptr()), C.byref(out)))
+
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+        del src, dst
 
     log(f"warmup {args.warmup} frames")
     for k in range(args.warmup):

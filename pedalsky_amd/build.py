@@ -30,11 +30,18 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 
+# The march runs FP32 divides and square roots in the loop body; their
+# correctly-rounded expansions are ~10 instructions each, the hardware
+# approximations (<= 2.5 ulp) one or two. The LUT kernels are FP64 and the
+# noise bakes keep the WGSL's IEEE FP32 semantics.
+PER_FILE = {"render_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+
+
 def _compile(src: str) -> Path:
     obj = OBJ_DIR / (Path(src).stem + ".o")
     path = CSRC / src
     if src.endswith(".hip"):
-        cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", str(path), "-o", str(obj)]
+        cmd = [HIPCC, *COMMON, *DEVICE, *PER_FILE.get(src, []), "-x", "hip", "-c", str(path), "-o", str(obj)]
     else:
         # host-only translation units still go through hipcc for the HIP runtime headers
         cmd = [HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", str(path), "-o", str(obj)]

@@ -302,6 +302,7 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     P.AP = c->AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
     P.base_rl = c->base_rl; P.rmax = c->rmax; P.detail_hf = c->detail_hf; P.curl = c->curl2;
     P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
+    P.weather_pow2 = (c->ww & (c->ww - 1)) == 0 && (c->wh & (c->wh - 1)) == 0 ? 1 : 0;
     P.type_grid = c->type_grid; P.use_type_grid = c->type_grid_active ? 1 : 0;
     P.overcast = c->overcast_active ? c->overcast : nullptr;
     P.blue = c->blue;

@@ -106,6 +106,8 @@ struct RenderParams {
     const float2* curl;               // 128^2 (2*rg - 1)
     const float* weather;             // ww*wh coverage
     int ww, wh;
+    int weather_pow2;                 // ww and wh are powers of two
+    int _pad2;
     const uint8_t* type_grid;         // 128^2
     const uint8_t* overcast;          // 128^2
     const uint8_t* blue;              // 64^2

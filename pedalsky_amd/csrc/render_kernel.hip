@@ -28,8 +28,10 @@ struct Cnt {
              octaves = 0, ap = 0;
 };
 
+// The 16-bit channels are interpolated as integers-in-float and scaled by
+// 1/65535 once after the trilinear blend (filtering is linear).
 __device__ __forceinline__ float2 unpack_rl(uint32_t v) {
-    return make_float2((float)(v & 0xffffu) * (1.f / 65535.f), (float)(v >> 16) * (1.f / 65535.f));
+    return make_float2((float)(v & 0xffffu), (float)(v >> 16));
 }
 
 // Trilinear Repeat fetch of the folded base volume (128^3, bricked, 2 x u16).
@@ -57,6 +59,8 @@ __device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, flo
         float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
         r.y = p0 + (p1 - p0) * fz;
     }
+    r.x *= 1.f / 65535.f;
+    r.y *= 1.f / 65535.f;
     return r;
 }
 
@@ -91,6 +95,18 @@ __device__ __forceinline__ float2 curl_fetch(const float2* __restrict__ v, float
     t0 = a.y + (b.y - a.y) * fx; t1 = c.y + (d.y - c.y) * fx;
     r.y = t0 + (t1 - t0) * fy;
     return r;
+}
+
+// Bilinear Repeat fetch of a power-of-two weather plane: wrap by masking.
+__device__ __forceinline__ float weather_fetch_pow2(const float* __restrict__ v, int w, int h, float u, float vv) {
+    float cx = u * (float)w - 0.5f, cy = vv * (float)h - 0.5f;
+    float fx0 = floorf(cx), fy0 = floorf(cy);
+    float fx = cx - fx0, fy = cy - fy0;
+    int ix = (int)fx0, iy = (int)fy0;
+    int x0 = ix & (w - 1), x1 = (ix + 1) & (w - 1), y0 = (iy & (h - 1)) * w, y1 = ((iy + 1) & (h - 1)) * w;
+    float a = v[y0 + x0], b = v[y0 + x1], c = v[y1 + x0], d = v[y1 + x1];
+    float t0 = a + (b - a) * fx, t1 = c + (d - c) * fx;
+    return t0 + (t1 - t0) * fy;
 }
 
 // Bilinear Repeat fetch of the weather coverage plane (any size).
@@ -655,7 +671,9 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
             const float rr = sqrtf(px * px + qy * qy + pz * pz);
             alt = h0f + t * (kcam + t) / (fmaxf(r0f, 1.f) + rr);
             const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
-            wr = weather_fetch(P.weather, P.ww, P.wh, wx * P.inv_weather_scale + 0.5f, wz * P.inv_weather_scale + 0.5f);
+            const float wu = wx * P.inv_weather_scale + 0.5f, wv = wz * P.inv_weather_scale + 0.5f;
+            wr = P.weather_pow2 ? weather_fetch_pow2(P.weather, P.ww, P.wh, wu, wv)
+                                : weather_fetch(P.weather, P.ww, P.wh, wu, wv);
             ctype = L.type;
             if (P.use_type_grid) {
                 float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
@@ -686,9 +704,9 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
         }
         if (active) {
             if (dense) {
-                tr *= 1.f - one_minus_exp_neg(fmaxf(density * L.sigma_t[0], 1e-6f) * step);
-                tg *= 1.f - one_minus_exp_neg(fmaxf(density * L.sigma_t[1], 1e-6f) * step);
-                tbb *= 1.f - one_minus_exp_neg(fmaxf(density * L.sigma_t[2], 1e-6f) * step);
+                tr *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * step);
+                tg *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * step);
+                tbb *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * step);
             }
             ++s;
             if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) {
