@@ -99,7 +99,7 @@ struct ps_ctx {
     float2* curl2 = nullptr;
     bool have_noise = false;
 
-    float* weather = nullptr;
+    uint2* weather = nullptr;         // binary16 2x2 quads
     int ww = 0, wh = 0;
     uint8_t* type_grid = nullptr;
     bool type_grid_active = false;
@@ -282,6 +282,11 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
             }
         }
         o.type = L.cloud_type;
+        {
+            static constexpr float ndf_rows[9][5] = PS_NDF_ROWS;
+            const float* row = ndf_rows[L.cloud_type < 8u ? L.cloud_type : 8u];
+            o.ndf_ia = row[0]; o.ndf_e0 = row[1]; o.ndf_ie = row[2]; o.ndf_a = row[3]; o.ndf_b = row[4];
+        }
         oc.is_cb = L.cloud_type == 7u ? 1u : 0u;
         oc.droplet = L.droplet_diameter_um;
         oc.r_bottom = R + (double)L.base_m;
@@ -303,6 +308,9 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     P.base_rl = c->base_rl; P.rmax = c->rmax; P.detail_hf = c->detail_hf; P.curl = c->curl2;
     P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
     P.weather_pow2 = (c->ww & (c->ww - 1)) == 0 && (c->wh & (c->wh - 1)) == 0 ? 1 : 0;
+    P.ww_log2 = 0;
+    while ((1 << P.ww_log2) < c->ww) ++P.ww_log2;
+    P.ww_f = (float)c->ww; P.wh_f = (float)c->wh;
     P.type_grid = c->type_grid; P.use_type_grid = c->type_grid_active ? 1 : 0;
     P.overcast = c->overcast_active ? c->overcast : nullptr;
     P.blue = c->blue;
@@ -589,7 +597,7 @@ ps_status ps_upload_weather(ps_ctx* c, const uint16_t* weather, int32_t w, int32
     PS_HIP(hipStreamSynchronize(c->stream));
     if (c->ww != w || c->wh != h) dev_free(c->weather);
     ps_status s;
-    if ((s = dev_alloc(&c->weather, sizeof(float) * w * h, "weather"))) return s;
+    if ((s = dev_alloc(&c->weather, sizeof(uint2) * w * h, "weather"))) return s;
     if ((s = dev_alloc(&c->type_grid, 128 * 128, "cloud type grid"))) return s;
     if ((s = dev_alloc(&c->overcast, 128 * 128, "overcast mask"))) return s;
     uint16_t* staging = nullptr;

@@ -224,13 +224,16 @@ __global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_r
     rmax[b] = (uint8_t)min((m * 255u + 65534u) / 65535u, 255u);
 }
 
-__global__ __launch_bounds__(256) void k_weather_fold(const uint16_t* src, int n, int channels, float* dst) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    __half h;
-    uint16_t bits = src[(size_t)i * channels];
-    memcpy(&h, &bits, 2);
-    dst[i] = __half2float(h);
+// Texel (x, y) of the output holds channel 0 of (x, y), (x+1, y) in .x and
+// of (x, y+1), (x+1, y+1) in .y (low half first), neighbours wrapped: the
+// four taps of a bilinear Repeat fetch whose lower-left texel is (x, y).
+__global__ __launch_bounds__(256) void k_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const int x = i % w, y = i / w;
+    const int x1 = x + 1 == w ? 0 : x + 1, y1 = y + 1 == h ? 0 : y + 1;
+    auto c0 = [&](int xx, int yy) { return (uint32_t)src[((size_t)yy * w + xx) * channels]; };
+    dst[i] = make_uint2(c0(x, y) | (c0(x1, y) << 16), c0(x, y1) | (c0(x1, y1) << 16));
 }
 
 }  // namespace
@@ -250,8 +253,8 @@ hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail
     return hipGetLastError();
 }
 
-hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, float* dst, hipStream_t s) {
+hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s) {
     int n = w * h;
-    hipLaunchKernelGGL(k_weather_fold, dim3((n + 255) / 256), dim3(256), 0, s, src, n, channels, dst);
+    hipLaunchKernelGGL(k_weather_fold, dim3((n + 255) / 256), dim3(256), 0, s, src, w, h, channels, dst);
     return hipGetLastError();
 }

@@ -49,6 +49,9 @@ struct LayerF {
     float wind_off_x, wind_off_z;     // drift offset (m)
     float skew_x, skew_z;             // skew_dir * skew_thickness (m per unit h)
     uint32_t type;
+    // NDF of types 0..6 as s(clamp(h * ndf_ia)) * (ndf_a + ndf_b * s(clamp((h - ndf_e0) * ndf_ie)))
+    // with s(t) = t^2 (3 - 2t); type 7 (cumulonimbus) has its own profile.
+    float ndf_ia, ndf_e0, ndf_ie, ndf_a, ndf_b;
 };
 
 struct LayerC {
@@ -104,10 +107,11 @@ struct RenderParams {
     const uint8_t* rmax;              // 32^3: per brick, max R over the brick and its +1 halo (ceil to 1/255)
     const float* detail_hf;           // 32^3 (0.625R + 0.25G + 0.125B)
     const float2* curl;               // 128^2 (2*rg - 1)
-    const float* weather;             // ww*wh coverage
+    const uint2* weather;             // ww*wh texels, each the binary16 2x2 quad (x, y), (x+1, y) | (x, y+1), (x+1, y+1)
     int ww, wh;
     int weather_pow2;                 // ww and wh are powers of two
-    int _pad2;
+    int ww_log2;                      // log2(ww) when weather_pow2
+    float ww_f, wh_f;
     const uint8_t* type_grid;         // 128^2
     const uint8_t* overcast;          // 128^2
     const uint8_t* blue;              // 64^2
@@ -123,6 +127,45 @@ struct RenderParams {
 // ---------------------------------------------------------------------------
 // Small vector helpers.
 // ---------------------------------------------------------------------------
+#define PS_GLOBAL __attribute__((address_space(1)))
+
+// Load a T at a 32-bit byte offset from a global-memory base. Resource
+// pointers are read from the parameter block, so the compiler sees them as
+// generic; this names the global address space (global_load with a scalar
+// base and a 32-bit vector offset instead of flat loads with 64-bit
+// address arithmetic).
+template <class T>
+__device__ __forceinline__ T ld_g(const void* base, uint32_t byte_off) {
+    return *(const PS_GLOBAL T*)((const PS_GLOBAL char*)base + byte_off);
+}
+typedef float ps_f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t ps_u32x2 __attribute__((ext_vector_type(2)));
+template <>
+__device__ __forceinline__ float2 ld_g<float2>(const void* base, uint32_t byte_off) {
+    const ps_f32x2 v = *(const PS_GLOBAL ps_f32x2*)((const PS_GLOBAL char*)base + byte_off);
+    return make_float2(v.x, v.y);
+}
+template <>
+__device__ __forceinline__ uint2 ld_g<uint2>(const void* base, uint32_t byte_off) {
+    const ps_u32x2 v = *(const PS_GLOBAL ps_u32x2*)((const PS_GLOBAL char*)base + byte_off);
+    return make_uint2(v.x, v.y);
+}
+
+// NDF coefficients of cloud types 0..6 (rising edge 1/a; falling part
+// e0, 1/(e1 - e0), a, b) and a zero row for every other type; one table for
+// the host (per-layer copies) and the device (per-sample type grid).
+#define PS_NDF_ROWS                                                                                        \
+    {{1.f / 0.07f, 1.f, 1.f / (0.2f - 1.f), 0.f, 1.f},  {1.f / 0.10f, 0.6f, 1.f / (1.f - 0.6f), 1.f, -1.f},  \
+     {1.f / 0.15f, 1.f, 1.f / (0.4f - 1.f), 0.f, 1.f},  {1.f / 0.20f, 1.f, 1.f / (0.3f - 1.f), 0.f, 1.f},    \
+     {1.f / 0.30f, 0.7f, 1.f / (1.f - 0.7f), 1.f, -1.f}, {1.f / 0.40f, 0.6f, 1.f / (1.f - 0.6f), 1.f, -1.f}, \
+     {1.f / 0.50f, 0.8f, 1.f / (1.f - 0.8f), 1.f, -1.f}, {0.f, 0.f, 0.f, 0.f, 0.f},                         \
+     {0.f, 0.f, 0.f, 0.f, 0.f}}
+
+// Hardware reciprocal and square root (1 ulp); used in the FP32 march where
+// the correctly rounded sequences would dominate the step cost.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float s3(float t) { return t * t * (3.f - 2.f * t); }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ double clampd(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {

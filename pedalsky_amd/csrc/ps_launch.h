@@ -18,5 +18,6 @@ hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool cou
 hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* detail, void* curl, hipStream_t s);
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
                                 uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s);
-// Weather: binary16 channel 0 of a w*h*channels map -> float coverage plane.
-hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, float* dst, hipStream_t s);
+// Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
+// (Repeat wrap) of the coverage plane, still binary16.
+hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s);

@@ -39,13 +39,13 @@ __device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, flo
     float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
     float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    int x0 = brick_x(ix & 127), x1 = brick_x((ix + 1) & 127);
-    int y0 = brick_y(iy & 127), y1 = brick_y((iy + 1) & 127);
-    int z0 = brick_z(iz & 127), z1 = brick_z((iz + 1) & 127);
-    float2 c000 = unpack_rl(v[z0 + y0 + x0]), c100 = unpack_rl(v[z0 + y0 + x1]);
-    float2 c010 = unpack_rl(v[z0 + y1 + x0]), c110 = unpack_rl(v[z0 + y1 + x1]);
-    float2 c001 = unpack_rl(v[z1 + y0 + x0]), c101 = unpack_rl(v[z1 + y0 + x1]);
-    float2 c011 = unpack_rl(v[z1 + y1 + x0]), c111 = unpack_rl(v[z1 + y1 + x1]);
+    uint32_t x0 = brick_x(ix & 127), x1 = brick_x((ix + 1) & 127);
+    uint32_t y0 = brick_y(iy & 127), y1 = brick_y((iy + 1) & 127);
+    uint32_t z0 = brick_z(iz & 127), z1 = brick_z((iz + 1) & 127);
+    float2 c000 = unpack_rl(ld_g<uint32_t>(v, (z0 + y0 + x0) * 4u)), c100 = unpack_rl(ld_g<uint32_t>(v, (z0 + y0 + x1) * 4u));
+    float2 c010 = unpack_rl(ld_g<uint32_t>(v, (z0 + y1 + x0) * 4u)), c110 = unpack_rl(ld_g<uint32_t>(v, (z0 + y1 + x1) * 4u));
+    float2 c001 = unpack_rl(ld_g<uint32_t>(v, (z1 + y0 + x0) * 4u)), c101 = unpack_rl(ld_g<uint32_t>(v, (z1 + y0 + x1) * 4u));
+    float2 c011 = unpack_rl(ld_g<uint32_t>(v, (z1 + y1 + x0) * 4u)), c111 = unpack_rl(ld_g<uint32_t>(v, (z1 + y1 + x1) * 4u));
     float2 r;
     {
         float a0 = c000.x + (c100.x - c000.x) * fx, a1 = c010.x + (c110.x - c010.x) * fx;
@@ -69,13 +69,13 @@ __device__ __forceinline__ float detail_fetch(const float* __restrict__ v, float
     float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
     float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    int x0 = ix & 31, x1 = (ix + 1) & 31;
-    int y0 = (iy & 31) << 5, y1 = ((iy + 1) & 31) << 5;
-    int z0 = (iz & 31) << 10, z1 = ((iz + 1) & 31) << 10;
-    float c000 = v[z0 + y0 + x0], c100 = v[z0 + y0 + x1];
-    float c010 = v[z0 + y1 + x0], c110 = v[z0 + y1 + x1];
-    float c001 = v[z1 + y0 + x0], c101 = v[z1 + y0 + x1];
-    float c011 = v[z1 + y1 + x0], c111 = v[z1 + y1 + x1];
+    uint32_t x0 = ix & 31, x1 = (ix + 1) & 31;
+    uint32_t y0 = (iy & 31) << 5, y1 = ((iy + 1) & 31) << 5;
+    uint32_t z0 = (iz & 31) << 10, z1 = ((iz + 1) & 31) << 10;
+    float c000 = ld_g<float>(v, (z0 + y0 + x0) * 4u), c100 = ld_g<float>(v, (z0 + y0 + x1) * 4u);
+    float c010 = ld_g<float>(v, (z0 + y1 + x0) * 4u), c110 = ld_g<float>(v, (z0 + y1 + x1) * 4u);
+    float c001 = ld_g<float>(v, (z1 + y0 + x0) * 4u), c101 = ld_g<float>(v, (z1 + y0 + x1) * 4u);
+    float c011 = ld_g<float>(v, (z1 + y1 + x0) * 4u), c111 = ld_g<float>(v, (z1 + y1 + x1) * 4u);
     float a0 = c000 + (c100 - c000) * fx, a1 = c010 + (c110 - c010) * fx;
     float b0 = c001 + (c101 - c001) * fx, b1 = c011 + (c111 - c011) * fx;
     float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
@@ -87,8 +87,9 @@ __device__ __forceinline__ float2 curl_fetch(const float2* __restrict__ v, float
     float fx0 = floorf(cx), fy0 = floorf(cy);
     float fx = cx - fx0, fy = cy - fy0;
     int ix = (int)fx0, iy = (int)fy0;
-    int x0 = ix & 127, x1 = (ix + 1) & 127, y0 = (iy & 127) << 7, y1 = ((iy + 1) & 127) << 7;
-    float2 a = v[y0 + x0], b = v[y0 + x1], c = v[y1 + x0], d = v[y1 + x1];
+    uint32_t x0 = ix & 127, x1 = (ix + 1) & 127, y0 = (iy & 127) << 7, y1 = ((iy + 1) & 127) << 7;
+    float2 a = ld_g<float2>(v, (y0 + x0) * 8u), b = ld_g<float2>(v, (y0 + x1) * 8u);
+    float2 c = ld_g<float2>(v, (y1 + x0) * 8u), d = ld_g<float2>(v, (y1 + x1) * 8u);
     float2 r;
     float t0 = a.x + (b.x - a.x) * fx, t1 = c.x + (d.x - c.x) * fx;
     r.x = t0 + (t1 - t0) * fy;
@@ -97,56 +98,66 @@ __device__ __forceinline__ float2 curl_fetch(const float2* __restrict__ v, float
     return r;
 }
 
-// Bilinear Repeat fetch of a power-of-two weather plane: wrap by masking.
-__device__ __forceinline__ float weather_fetch_pow2(const float* __restrict__ v, int w, int h, float u, float vv) {
-    float cx = u * (float)w - 0.5f, cy = vv * (float)h - 0.5f;
-    float fx0 = floorf(cx), fy0 = floorf(cy);
-    float fx = cx - fx0, fy = cy - fy0;
-    int ix = (int)fx0, iy = (int)fy0;
-    int x0 = ix & (w - 1), x1 = (ix + 1) & (w - 1), y0 = (iy & (h - 1)) * w, y1 = ((iy + 1) & (h - 1)) * w;
-    float a = v[y0 + x0], b = v[y0 + x1], c = v[y1 + x0], d = v[y1 + x1];
-    float t0 = a + (b - a) * fx, t1 = c + (d - c) * fx;
-    return t0 + (t1 - t0) * fy;
-}
-
-// Bilinear Repeat fetch of the weather coverage plane (any size).
-__device__ __forceinline__ float weather_fetch(const float* __restrict__ v, int w, int h, float u, float vv) {
-    u = u - floorf(u);
-    vv = vv - floorf(vv);
-    float cx = u * (float)w - 0.5f, cy = vv * (float)h - 0.5f;
-    float fx0 = floorf(cx), fy0 = floorf(cy);
-    float fx = cx - fx0, fy = cy - fy0;
-    int ix = (int)fx0, iy = (int)fy0;
-    // u, vv in [0, 1] so ix, iy in [-1, n-1]; the final clamps only guard
-    // non-finite coordinates.
-    int x0 = clampi(ix < 0 ? w - 1 : ix, w), x1 = clampi(ix + 1 >= w ? ix + 1 - w : ix + 1, w);
-    int y0 = clampi(iy < 0 ? h - 1 : iy, h), y1 = clampi(iy + 1 >= h ? iy + 1 - h : iy + 1, h);
-    float a = v[y0 * w + x0], b = v[y0 * w + x1], c = v[y1 * w + x0], d = v[y1 * w + x1];
-    float t0 = a + (b - a) * fx, t1 = c + (d - c) * fx;
-    return t0 + (t1 - t0) * fy;
-}
-
-// cloud_density_common.wgsl:159-201 (WGSL smoothstep with reversed edges is
-// the same polynomial formula).
-__device__ __forceinline__ float ndf_f(float h, uint32_t t, float anvil) {
-    switch (t) {
-        case 0: return smoothstepf(0.f, 0.07f, h) * smoothstepf(1.f, 0.2f, h);
-        case 1: return smoothstepf(0.f, 0.10f, h) * (1.f - smoothstepf(0.6f, 1.f, h));
-        case 2: return smoothstepf(0.f, 0.15f, h) * smoothstepf(1.f, 0.4f, h);
-        case 3: return smoothstepf(0.f, 0.20f, h) * smoothstepf(1.f, 0.3f, h);
-        case 4: return smoothstepf(0.f, 0.30f, h) * (1.f - smoothstepf(0.7f, 1.f, h));
-        case 5: return smoothstepf(0.f, 0.40f, h) * (1.f - smoothstepf(0.6f, 1.f, h));
-        case 6: return smoothstepf(0.f, 0.50f, h) * (1.f - smoothstepf(0.8f, 1.f, h));
-        case 7: {
-            float base = smoothstepf(0.f, 0.05f, h);
-            float mid = smoothstepf(0.95f, 0.5f, h);
-            float anv = smoothstepf(0.7f, 0.9f, h) * 1.5f * clampf(anvil, 0.f, 2.f);
-            float mt = smoothstepf(0.65f, 0.8f, h);
-            return base * (mid + (anv - mid) * mt);
-        }
-        default: return 0.f;
+// Byte offset and weights of a bilinear Repeat tap of the weather plane at
+// uv. The plane is stored as 2x2 texel quads, so one 8-byte load returns
+// all four taps. Power-of-two sizes wrap by masking; other sizes wrap u, v
+// into [0, 1) first (ix, iy in [-1, n-1]; the clamps only guard non-finite
+// coordinates).
+__device__ __forceinline__ uint32_t weather_tap(const RenderParams& P, float u, float v, float& fx, float& fy) {
+    if (P.weather_pow2) {
+        const float cx = u * P.ww_f - 0.5f, cy = v * P.wh_f - 0.5f;
+        const float fx0 = floorf(cx), fy0 = floorf(cy);
+        fx = cx - fx0; fy = cy - fy0;
+        const uint32_t x0 = (uint32_t)(int)fx0 & (uint32_t)(P.ww - 1), y0 = (uint32_t)(int)fy0 & (uint32_t)(P.wh - 1);
+        return ((y0 << P.ww_log2) | x0) * 8u;
     }
+    u = u - floorf(u);
+    v = v - floorf(v);
+    const float cx = u * P.ww_f - 0.5f, cy = v * P.wh_f - 0.5f;
+    const float fx0 = floorf(cx), fy0 = floorf(cy);
+    fx = cx - fx0; fy = cy - fy0;
+    const int ix = (int)fx0, iy = (int)fy0;
+    const int x0 = clampi(ix < 0 ? P.ww - 1 : ix, P.ww), y0 = clampi(iy < 0 ? P.wh - 1 : iy, P.wh);
+    return (uint32_t)(y0 * P.ww + x0) * 8u;
 }
+
+typedef _Float16 ps_half2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float weather_blend(uint2 q, float fx, float fy) {
+    const ps_half2 r0 = __builtin_bit_cast(ps_half2, q.x), r1 = __builtin_bit_cast(ps_half2, q.y);
+    const float a = (float)r0.x, b = (float)r0.y, c = (float)r1.x, d = (float)r1.y;
+    const float t0 = a + (b - a) * fx, t1 = c + (d - c) * fx;
+    return t0 + (t1 - t0) * fy;
+}
+
+// Vertical density profiles (cloud_density_common.wgsl:49-91). WGSL
+// smoothstep(e0, e1, x) = s(clamp((x - e0) / (e1 - e0), 0, 1)) with s(t) =
+// t^2 (3 - 2t), reversed edges included. Types 0..6 are a rising edge
+// smoothstep(0, a, h) times either smoothstep(1, b, h) or 1 - smoothstep(c,
+// 1, h); both are a + b s(clamp((h - e0) ie)) with (e0, ie, a, b) from the
+// table. Type 7 (cumulonimbus) has a mid/anvil blend; other values give 0.
+struct NdfC { float ia, e0, ie, a, b; };
+
+__device__ __forceinline__ NdfC ndf_coef(uint32_t t) {
+    constexpr float k[9][5] = PS_NDF_ROWS;
+    const uint32_t i = t < 8u ? t : 8u;
+    return NdfC{k[i][0], k[i][1], k[i][2], k[i][3], k[i][4]};
+}
+
+__device__ __forceinline__ float ndf_cb(float h, float anvil) {
+    const float base = s3(clampf(h * (1.f / 0.05f), 0.f, 1.f));
+    const float mid = s3(clampf((h - 0.95f) * (1.f / (0.5f - 0.95f)), 0.f, 1.f));
+    const float anv = s3(clampf((h - 0.7f) * (1.f / (0.9f - 0.7f)), 0.f, 1.f)) * 1.5f * clampf(anvil, 0.f, 2.f);
+    const float mt = s3(clampf((h - 0.65f) * (1.f / (0.8f - 0.65f)), 0.f, 1.f));
+    return base * (mid + (anv - mid) * mt);
+}
+
+__device__ __forceinline__ float ndf_eval(const NdfC& c, uint32_t t, float h, float anvil) {
+    if (t == 7u) return ndf_cb(h, anvil);
+    return s3(clampf(h * c.ia, 0.f, 1.f)) * (c.a + c.b * s3(clampf((h - c.e0) * c.ie, 0.f, 1.f)));
+}
+
+__device__ __forceinline__ NdfC layer_ndf(const LayerF& L) { return NdfC{L.ndf_ia, L.ndf_e0, L.ndf_ie, L.ndf_a, L.ndf_b}; }
 
 // Base shape through the coverage remap (shared by sample_density and
 // sample_density_light). Returns the pre-erosion cloud value and writes the
@@ -160,75 +171,72 @@ __device__ __forceinline__ float ndf_f(float h, uint32_t t, float anvil) {
 // first with r_max = 1 (no memory access), then with the brick's r_max. The
 // 1e-5 margin covers FP32 rounding of the skipped arithmetic, so skipping
 // never changes a result. Returns -1 when skipped.
-__device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, float px, float py, float pz,
-                                       float h, float cov, uint32_t ctype, float& pax, float& pay, float& paz) {
-    pax = px - L.wind_off_x - L.skew_x * h;
-    pay = py;
-    paz = pz - L.wind_off_z - L.skew_z * h;
-    const float ndfv = ndf_f(h, ctype, L.anvil);
+//
+// `rm` is the brick maximum when the caller has already loaded it (negative:
+// load it here).
+__device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, float cx, float cy, float cz,
+                                       float ndfv, float cov, float rm) {
     const float thr = (1.f - cov) * (1.f - 1e-5f);
     if (ndfv <= thr) return -1.f;
-    float s = P.inv_base_scale * 128.f;
-    const float cx = pax * s - 0.5f, cy = pay * s - 0.5f, cz = paz * s - 0.5f;
-    {
-        const int bx = ((int)floorf(cx) >> 2) & 31, by = ((int)floorf(cy) >> 2) & 31, bz = ((int)floorf(cz) >> 2) & 31;
-        const float rm = (float)P.rmax[(bz << 10) | (by << 5) | bx] * (1.f / 255.f);
-        if ((rm + 1.f) * 0.5f * ndfv <= thr) return -1.f;
+    if (rm < 0.f) {
+        const uint32_t bx = ((int)floorf(cx) >> 2) & 31, by = ((int)floorf(cy) >> 2) & 31,
+                       bz = ((int)floorf(cz) >> 2) & 31;
+        rm = (float)ld_g<uint8_t>(P.rmax, (bz << 10) | (by << 5) | bx) * (1.f / 255.f);
     }
+    if ((rm + 1.f) * 0.5f * ndfv <= thr) return -1.f;
     float2 b = base_fetch(P.base_rl, cx, cy, cz);
     float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
-    float base_cloud = (b.x + (1.f - lf)) / fmaxf(2.f - lf, 1e-5f);
-    float cloud = base_cloud * ndfv;
-    return cloud - (1.f - cov);
+    float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
+    return base_cloud * ndfv - (1.f - cov);
 }
 
-// cloud_density_common.wgsl:376-415
-__device__ __forceinline__ float density_light(const RenderParams& P, const LayerF& L, float px, float py, float pz,
-                                               float alt, float weather_r, uint32_t ctype, bool& covered) {
+// Advected position of a sample (cloud_march.wgsl:448-458) and its texel
+// coordinates in the base volume.
+__device__ __forceinline__ void advect(const RenderParams& P, const LayerF& L, float px, float py, float pz, float h,
+                                       float& ax, float& ay, float& az, float& cx, float& cy, float& cz) {
+    ax = px - L.wind_off_x - L.skew_x * h; ay = py; az = pz - L.wind_off_z - L.skew_z * h;
+    const float s = P.inv_base_scale * 128.f;
+    cx = ax * s - 0.5f; cy = ay * s - 0.5f; cz = az * s - 0.5f;
+}
+
+// cloud_density_common.wgsl:266-305
+__device__ __forceinline__ float density_light(const RenderParams& P, const LayerF& L, const NdfC& nc, float px,
+                                               float py, float pz, float alt, float weather_r, uint32_t ctype,
+                                               bool& covered) {
     float h = (alt - L.base) * L.inv_thick;
     covered = false;
     if (h < 0.f || h > 1.f) return 0.f;
     float cov = clampf(weather_r * L.coverage, 0.f, 1.f);
     if (cov < 0.01f) return 0.f;
     covered = true;
-    float ax, ay, az;
-    float cloud = shape(P, L, px, py, pz, h, cov, ctype, ax, ay, az);
+    float ax, ay, az, cx, cy, cz;
+    advect(P, L, px, py, pz, h, ax, ay, az, cx, cy, cz);
+    float cloud = shape(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, -1.f);
     return clampf(cloud, 0.f, 1.f) * L.density_scale;
 }
 
-// cloud_march.wgsl:432-530
-__device__ __forceinline__ float density_primary(const RenderParams& P, const LayerF& L, float px, float py, float pz,
-                                                 float alt, float weather_r, uint32_t ctype, int& stage) {
-    stage = 0;
-    float h = (alt - L.base) * L.inv_thick;
-    if (h < 0.f || h > 1.f) return 0.f;
-    float cov = clampf(weather_r * L.coverage, 0.f, 1.f);
-    if (cov < 0.01f) return 0.f;
-    stage = 1;
-    float ax, ay, az;
-    float cloud = shape(P, L, px, py, pz, h, cov, ctype, ax, ay, az);
-    if (cloud > 0.f) {
-        stage = 2;
-        float ds = P.inv_detail_scale;
-        float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
-        float dx = ax + cu.x * P.curl_off_scale, dz = az + cu.y * P.curl_off_scale;
-        float hf = detail_fetch(P.detail_hf, dx * ds * 32.f - 0.5f, ay * ds * 32.f - 0.5f, dz * ds * 32.f - 0.5f);
-        float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
-        float e = dmod * L.ld;
-        cloud = (cloud - e) / fmaxf(1.f - e, 1e-5f);
-    }
-    return clampf(cloud, 0.f, 1.f) * L.density_scale;
+// Detail erosion of a positive base shape (cloud_march.wgsl:502-527).
+__device__ __forceinline__ float erode(const RenderParams& P, const LayerF& L, float cloud, float ax, float ay,
+                                      float az, float h) {
+    const float ds = P.inv_detail_scale;
+    const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
+    const float dx = ax + cu.x * P.curl_off_scale, dz = az + cu.y * P.curl_off_scale;
+    const float hf = detail_fetch(P.detail_hf, dx * ds * 32.f - 0.5f, ay * ds * 32.f - 0.5f, dz * ds * 32.f - 0.5f);
+    const float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
+    const float e = dmod * L.ld;
+    return (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
 }
 
 // cloud_march.wgsl:543-578 (straight) and 599-666 (cone), FP32.
-__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, float px, float py, float pz,
-                                             float alt, float weather_r, uint32_t ctype, Cnt& cnt) {
+__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, const NdfC& nc, float px,
+                                             float py, float pz, float alt, float weather_r, uint32_t ctype, Cnt& cnt) {
     const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
     float cos_sun = fmaxf(sy, 0.05f);
     float dist = fmaxf((L.top - alt) / cos_sun, 1.f);
     float r0 = P.R + alt;
     float qy = py + P.R;
-    float cv = (px * sx + qy * sy + pz * sz) / fmaxf(r0, 1.f);
+    const float r0c = fmaxf(r0, 1.f);
+    float cv = (px * sx + qy * sy + pz * sz) * frcp(r0c);
     float k = 2.f * r0 * cv;
     float od = 0.f;
     bool cov;
@@ -245,9 +253,9 @@ __device__ __forceinline__ float march_light(const RenderParams& P, const LayerF
             float qyy = py + sy * t + off[i][1] * scale;
             float qz = pz + sz * t + off[i][2] * scale;
             float yy = qyy + P.R;
-            float rr = sqrtf(qx * qx + yy * yy + qz * qz);
-            float a = alt + t * (k + t) / (fmaxf(r0, 1.f) + rr);
-            float d = density_light(P, L, qx, qyy, qz, a, weather_r, ctype, cov);
+            float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+            float a = alt + t * (k + t) * frcp(r0c + rr);
+            float d = density_light(P, L, nc, qx, qyy, qz, a, weather_r, ctype, cov);
             cnt.light++;
             cnt.light_cov += cov ? 1u : 0u;
             od += d * w;
@@ -259,9 +267,9 @@ __device__ __forceinline__ float march_light(const RenderParams& P, const LayerF
         float t = (i + 0.5f) * step;
         float qx = px + sx * t, qyy = py + sy * t, qz = pz + sz * t;
         float yy = qyy + P.R;
-        float rr = sqrtf(qx * qx + yy * yy + qz * qz);
-        float a = alt + t * (k + t) / (fmaxf(r0, 1.f) + rr);
-        float d = density_light(P, L, qx, qyy, qz, a, weather_r, ctype, cov);
+        float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+        float a = alt + t * (k + t) * frcp(r0c + rr);
+        float d = density_light(P, L, nc, qx, qyy, qz, a, weather_r, ctype, cov);
         cnt.light++;
         cnt.light_cov += cov ? 1u : 0u;
         od += d * step;
@@ -370,7 +378,7 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
     const LayerC& LC = P.layers[li];
     const LayerF L = LC.f;
     cnt.dense++;
-    const float od = march_light(P, L, px, py, pz, alt, wr, ctype, cnt);
+    const float od = march_light(P, L, ndf_coef(ctype), px, py, pz, alt, wr, ctype, cnt);
     const float sig0 = L.sigma_t[0], sig1 = L.sigma_t[1], sig2 = L.sigma_t[2];
     // Beer-Powder (cloud_march.wgsl:888-892); exp(-2x) = exp(-x)^2
     const float e0r = __expf(-sig0 * od), e0g = __expf(-sig1 * od), e0b = __expf(-sig2 * od);
@@ -481,6 +489,33 @@ __device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q,
         }
         wave_sync_lds();
     }
+}
+
+// Ray / layer-shell interval (cloud_march.wgsl:370-416) in FP64 with
+// |oc|^2 - r^2 as (|oc| - r)(|oc| + r), clipped to the scene depth
+// (cloud_march.wgsl:777-806). Returns false when the interval is empty.
+__device__ __forceinline__ bool layer_interval(const LayerC& L, double r_cam, double b_, double depth_t_max,
+                                              double& ta, double& tb) {
+    const double ct = (r_cam - L.r_top) * (r_cam + L.r_top), cbt = (r_cam - L.r_bottom) * (r_cam + L.r_bottom);
+    const double disc_t = b_ * b_ - ct, disc_b = b_ * b_ - cbt;
+    const bool hit_top = disc_t >= 0.0, hit_bot = disc_b >= 0.0;
+    if (!hit_top) return false;
+    const double sqt = sqrt(disc_t), sqb = hit_bot ? sqrt(disc_b) : 0.0;
+    const double tt0 = -b_ - sqt, tt1 = -b_ + sqt, tb0 = -b_ - sqb, tb1 = -b_ + sqb;
+    if (r_cam < L.r_bottom) {
+        if (!hit_bot) return false;
+        ta = fmax(tb1, 0.0);
+        tb = fmax(tt1, 0.0);
+    } else if (r_cam > L.r_top) {
+        ta = fmax(tt0, 0.0);
+        tb = (hit_bot && tb0 > 0.0) ? fmax(tb0, 0.0) : fmax(tt1, 0.0);
+    } else {
+        ta = 0.0;
+        tb = (hit_bot && tb0 > 0.0) ? fmin(fmax(tt1, 0.0), tb0) : fmax(tt1, 0.0);
+    }
+    if (!(tb > ta)) return false;
+    tb = fmin(tb, depth_t_max);
+    return tb > ta;
 }
 
 template <bool COUNT>
@@ -594,70 +629,57 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     }
 
     // ---- per-layer intervals in FP64 (cloud_march.wgsl:370-416, 777-806) ---------------
-    // Sorted by entry distance; kept as (start, step) pairs for the march.
-    float seg_t0[PS_MAX_LAYERS], seg_step[PS_MAX_LAYERS];
-    int seg_layer[PS_MAX_LAYERS];
-    double key[PS_MAX_LAYERS];
-    const int lc = P.n_layers;
+    // Layers hit are sorted by entry distance (stable: equal keys keep layer
+    // order). Only the order is kept (3 bits per entry); each interval is
+    // recomputed when the march reaches it, which keeps the per-lane state
+    // small.
     const double ocx = P.cam[0], ocy = P.cam[1] + P.R, ocz = P.cam[2];
     const double b_ = ocx * dxd + ocy * dyd + ocz * dzd;
-    const double r_cam = sqrt(ocx * ocx + ocy * ocy + ocz * ocz);
+    const double r_cam = P.r0;
+    uint32_t order = 0;
     int nseg = 0;
-    for (int l = 0; l < lc; ++l) {
-        const LayerC& L = P.layers[l];
-        // |oc|^2 - r^2 as (|oc| - r)(|oc| + r)
-        double ct = (r_cam - L.r_top) * (r_cam + L.r_top), cbt = (r_cam - L.r_bottom) * (r_cam + L.r_bottom);
-        double disc_t = b_ * b_ - ct, disc_b = b_ * b_ - cbt;
-        bool hit_top = disc_t >= 0.0, hit_bot = disc_b >= 0.0;
-        double sqt = hit_top ? sqrt(disc_t) : 0.0, sqb = hit_bot ? sqrt(disc_b) : 0.0;
-        double tt0 = -b_ - sqt, tt1 = -b_ + sqt, tb0 = -b_ - sqb, tb1 = -b_ + sqb;
-        bool ok = hit_top;
-        double ta = 0.0, tb = 0.0;
-        if (ok) {
-            if (r_cam < L.r_bottom) {
-                if (!hit_bot) ok = false;
-                ta = fmax(tb1, 0.0);
-                tb = fmax(tt1, 0.0);
-            } else if (r_cam > L.r_top) {
-                ta = fmax(tt0, 0.0);
-                tb = (hit_bot && tb0 > 0.0) ? fmax(tb0, 0.0) : fmax(tt1, 0.0);
-            } else {
-                ta = 0.0;
-                tb = (hit_bot && tb0 > 0.0) ? fmin(fmax(tt1, 0.0), tb0) : fmax(tt1, 0.0);
-            }
-            ok = ok && (tb > ta);
-        }
-        if (ok) tb = fmin(tb, depth_t_max);
-        if (valid && ok && tb > ta) {
-            // insertion by entry distance (stable: equal keys keep layer order)
-            const double stepd = (tb - ta) / P.steps;
+    if (valid) {
+        double key[PS_MAX_LAYERS];
+        const int lc = P.n_layers;
+        for (int l = 0; l < lc; ++l) {
+            double ta, tb;
+            if (!layer_interval(P.layers[l], r_cam, b_, depth_t_max, ta, tb)) continue;
             int j = nseg++;
             while (j > 0 && key[j - 1] > ta) {
-                key[j] = key[j - 1]; seg_t0[j] = seg_t0[j - 1]; seg_step[j] = seg_step[j - 1];
-                seg_layer[j] = seg_layer[j - 1];
-                --j;
+                key[j] = key[j - 1];
+                j--;
             }
+            // insert l at rank j: shift ranks >= j up by one 3-bit slot
+            const uint32_t lo = order & ((1u << (3 * j)) - 1u);
+            order = lo | ((order & ~((1u << (3 * j)) - 1u)) << 3) | ((uint32_t)l << (3 * j));
             key[j] = ta;
-            seg_t0[j] = (float)(ta + jitter * stepd);
-            seg_step[j] = (float)stepd;
-            seg_layer[j] = l;
         }
     }
 
     // ---- wave-uniform primary march ---------------------------------------------------
-    const float r0f = (float)P.r0;
+    const float r0c = fmaxf((float)P.r0, 1.f);
     const float kcam = (float)(2.0 * P.r0 * b_ / fmax(P.r0, 1.0));
     const float h0f = (float)P.h0;
     const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
+    const float wscale = P.inv_weather_scale, nscale = P.inv_base_scale * 128.f;
     float tr = 1.f, tg = 1.f, tbb = 1.f;
     int cur = 0, s = 0, li = 0;
     float tstart = 0.f, step = 0.f;
     LayerF L;
     bool active = nseg > 0;
-    if (active) {
-        li = seg_layer[0]; tstart = seg_t0[0]; step = seg_step[0];
-        L = P.layers[li].f;
-    }
+    // enter the march's next layer interval (the cur-th in entry order)
+    auto enter = [&](int k) {
+        li = (int)((order >> (3 * k)) & 7u);
+        const LayerC& LC = P.layers[li];
+        L = LC.f;
+        double ta, tb;
+        layer_interval(LC, r_cam, b_, depth_t_max, ta, tb);
+        const double stepd = (tb - ta) / P.steps;
+        tstart = (float)(ta + jitter * stepd);
+        step = (float)stepd;
+        s = 0;
+    };
+    if (active) enter(0);
     int qn = 0;  // queued samples (wave-uniform)
     while (__ballot(active) != 0ull) {
         bool dense = false;
@@ -668,23 +690,44 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
             t = tstart + (float)s * step;
             px = camx + dx * t; py = camy + dy * t; pz = camz + dz * t;
             const float qy = py + P.R;
-            const float rr = sqrtf(px * px + qy * qy + pz * pz);
-            alt = h0f + t * (kcam + t) / (fmaxf(r0f, 1.f) + rr);
+            const float rr = fsqrt(px * px + qy * qy + pz * pz);
+            alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+            const float h = (alt - L.base) * L.inv_thick;
+            // the weather quad and the brick maximum depend on the position
+            // only: both loads are issued before either is used
             const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
-            const float wu = wx * P.inv_weather_scale + 0.5f, wv = wz * P.inv_weather_scale + 0.5f;
-            wr = P.weather_pow2 ? weather_fetch_pow2(P.weather, P.ww, P.wh, wu, wv)
-                                : weather_fetch(P.weather, P.ww, P.wh, wu, wv);
+            float wfx, wfy;
+            const uint32_t woff = weather_tap(P, wx * wscale + 0.5f, wz * wscale + 0.5f, wfx, wfy);
+            const float ax = wx - L.skew_x * h, ay = py, az = wz - L.skew_z * h;
+            const float cx = ax * nscale - 0.5f, cy = ay * nscale - 0.5f, cz = az * nscale - 0.5f;
+            const uint32_t roff = ((((int)floorf(cz) >> 2) & 31) << 10) | ((((int)floorf(cy) >> 2) & 31) << 5) |
+                                  (((int)floorf(cx) >> 2) & 31);
+            const uint2 wq = ld_g<uint2>(P.weather, woff);
+            const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
+            wr = weather_blend(wq, wfx, wfy);
             ctype = L.type;
+            NdfC nc = layer_ndf(L);
             if (P.use_type_grid) {
                 float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
-                int gx = (int)floorf(gu * 128.f), gy = (int)floorf(gv * 128.f);
-                gx = ((gx % 128) + 128) % 128;
-                gy = ((gy % 128) + 128) % 128;
-                uint32_t g = P.type_grid[gy * 128 + gx];
-                if (g != 255u) ctype = g;
+                const uint32_t gx = (uint32_t)(int)floorf(gu * 128.f) & 127u, gy = (uint32_t)(int)floorf(gv * 128.f) & 127u;
+                const uint32_t g = ld_g<uint8_t>(P.type_grid, gy * 128u + gx);
+                if (g != 255u) { ctype = g; nc = ndf_coef(g); }
             }
-            int stage;
-            density = density_primary(P, L, px, py, pz, alt, wr, ctype, stage);
+            // sample_density (cloud_march.wgsl:432-530)
+            int stage = 0;
+            if (h >= 0.f && h <= 1.f) {
+                const float cov = clampf(wr * L.coverage, 0.f, 1.f);
+                if (cov >= 0.01f) {
+                    stage = 1;
+                    float cloud = shape(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov,
+                                        (float)rmb * (1.f / 255.f));
+                    if (cloud > 0.f) {
+                        stage = 2;
+                        cloud = erode(P, L, cloud, ax, ay, az, h);
+                    }
+                    density = clampf(cloud, 0.f, 1.f) * L.density_scale;
+                }
+            }
             cnt.covered += stage >= 1 ? 1u : 0u;
             cnt.eroded += stage >= 2 ? 1u : 0u;
             dense = density > 1e-3f;
@@ -713,13 +756,8 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
                 active = false;
             } else if (s >= P.steps) {
                 ++cur;
-                if (cur < nseg) {
-                    li = seg_layer[cur]; tstart = seg_t0[cur]; step = seg_step[cur];
-                    L = P.layers[li].f;
-                    s = 0;
-                } else {
-                    active = false;
-                }
+                if (cur < nseg) enter(cur);
+                else active = false;
             }
         }
         if (qn >= 64) {
