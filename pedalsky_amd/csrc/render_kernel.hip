@@ -6,7 +6,10 @@
 // Design
 //  * One lane per pixel; each wave64 owns an 8x8 pixel tile so its rays stay
 //    coherent (neighbouring gathers hit the same cache lines) and its lanes
-//    leave the march loop together; a 256-thread block covers 16x16 pixels.
+//    leave the march loop together. A block is one wave: per-tile cost
+//    varies by orders of magnitude (clear sky vs. horizon cloud), and
+//    single-wave blocks let the dispatcher balance the CUs at the finest
+//    grain (a four-wave block holds its slots until its slowest wave ends).
 //  * Per-pixel ray setup is FP64: the view ray from inv_view_proj, the
 //    ray/shell intersections (|o-c|^2 - r^2 at planet scale), the sun-disk
 //    test and the per-layer phase coefficients. The step loop is FP32.
@@ -519,13 +522,13 @@ __device__ __forceinline__ bool layer_interval(const LayerC& L, double r_cam, do
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__ Pp) {
-    __shared__ WaveQueue queues[4];
+__global__ __launch_bounds__(64) void k_render(const RenderParams* __restrict__ Pp) {
+    __shared__ WaveQueue Qs;
     const RenderParams& P = *Pp;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveQueue& Q = queues[wave];
-    const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int lane = threadIdx.x;
+    WaveQueue& Q = Qs;
+    const int x = blockIdx.x * 8 + (lane & 7);
+    const int y = blockIdx.y * 8 + (lane >> 3);
     // Lanes outside the image stay in the wave (they help shade queued
     // samples) but march nothing and write nothing.
     const bool valid = x < P.W && y < P.H;
@@ -719,11 +722,26 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
                 const float cov = clampf(wr * L.coverage, 0.f, 1.f);
                 if (cov >= 0.01f) {
                     stage = 1;
-                    float cloud = shape(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov,
-                                        (float)rmb * (1.f / 255.f));
-                    if (cloud > 0.f) {
-                        stage = 2;
-                        cloud = erode(P, L, cloud, ax, ay, az, h);
+                    const float ndfv = ndf_eval(nc, ctype, h, L.anvil);
+                    const float thr = (1.f - cov) * (1.f - 1e-5f);
+                    const float rm = (float)rmb * (1.f / 255.f);
+                    float cloud = -1.f;
+                    if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
+                        const float ds = P.inv_detail_scale;
+                        const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
+                        const float2 b = base_fetch(P.base_rl, cx, cy, cz);
+                        const float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
+                        const float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
+                        cloud = base_cloud * ndfv - (1.f - cov);
+                        if (cloud > 0.f) {
+                            stage = 2;
+                            const float ddx = ax + cu.x * P.curl_off_scale, ddz = az + cu.y * P.curl_off_scale;
+                            const float hf = detail_fetch(P.detail_hf, ddx * ds * 32.f - 0.5f, ay * ds * 32.f - 0.5f,
+                                                          ddz * ds * 32.f - 0.5f);
+                            const float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
+                            const float e = dmod * L.ld;
+                            cloud = (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
+                        }
                     }
                     density = clampf(cloud, 0.f, 1.f) * L.density_scale;
                 }
@@ -837,24 +855,26 @@ __global__ __launch_bounds__(256) void k_render(const RenderParams* __restrict__
     }
 
     if (COUNT) {
-        unsigned long long* c = P.counters;
-        if (valid) atomicAdd(c + 0, 1ull);
-        atomicAdd(c + 1, (unsigned long long)cnt.layer_marches);
-        atomicAdd(c + 2, (unsigned long long)cnt.primary);
-        atomicAdd(c + 3, (unsigned long long)cnt.covered);
-        atomicAdd(c + 4, (unsigned long long)cnt.eroded);
-        atomicAdd(c + 5, (unsigned long long)cnt.dense);
-        atomicAdd(c + 6, (unsigned long long)cnt.light);
-        atomicAdd(c + 7, (unsigned long long)cnt.light_cov);
-        atomicAdd(c + 8, (unsigned long long)cnt.octaves);
-        atomicAdd(c + 9, (unsigned long long)cnt.ap);
+        // wave sums first (all 64 lanes are converged here), then one
+        // atomic per counter per wave
+        uint32_t v[10] = {valid ? 1u : 0u, cnt.layer_marches, cnt.primary, cnt.covered, cnt.eroded,
+                          cnt.dense,       cnt.light,         cnt.light_cov, cnt.octaves, cnt.ap};
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o);
+        if (lane == 0) {
+            unsigned long long* c = P.counters;
+#pragma unroll
+            for (int k = 0; k < 10; ++k) atomicAdd(c + k, (unsigned long long)v[k]);
+        }
     }
 }
 
 }  // namespace
 
 hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, hipStream_t s) {
-    dim3 grid((W + 15) / 16, (H + 15) / 16), block(256);
+    dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
     if (count) hipLaunchKernelGGL(k_render<true>, grid, block, 0, s, d_params);
     else hipLaunchKernelGGL(k_render<false>, grid, block, 0, s, d_params);
     return hipGetLastError();
