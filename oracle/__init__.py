@@ -199,6 +199,8 @@ class Frame:
         self.blue = np.ascontiguousarray(workload.blue, dtype=np.uint8)
         d = workload.depth if isinstance(depth, str) else depth
         self.depth = None if d is None else np.ascontiguousarray(d, dtype=np.float32)
+        tg = getattr(workload, "type_grid", None)
+        self.type_grid = None if tg is None else np.ascontiguousarray(tg, dtype=np.uint8)
         T, SV, AP = self.luts["T"], self.luts["SV"], self.luts["AP"]
         self.inputs = OrInputs(
             _ref(workload.frame), _ref(workload.atmosphere), _ref(workload.clouds), C.addressof(self.layers),
@@ -207,7 +209,7 @@ class Frame:
             _p(AP), AP.shape[2], AP.shape[1], AP.shape[0],
             _p(self.base), _p(self.detail), _p(self.curl),
             _p(self.weather), wm.shape[1], wm.shape[0], wm.shape[2],
-            None, _p(self.overcast), _p(self.blue), _p(self.depth))
+            _p(self.type_grid), _p(self.overcast), _p(self.blue), _p(self.depth))
 
     def render_region(self, x0, y0, w, h, counters=False):
         comp = np.zeros((h, w, 4))

@@ -126,7 +126,7 @@ class Renderer:
         self.set_clouds(w.clouds, w.layers, w.winds)
         self.generate_noise(w.noise_seed, w.noise_format)
         self.upload_blue_noise(w.blue)
-        self.upload_weather(w.weather, None, w.overcast)
+        self.upload_weather(w.weather, w.type_grid, w.overcast)
         self.set_depth(w.depth)
         self.workload = w
 

@@ -254,13 +254,34 @@ def overcast_mask(layers: Sequence[abi.CloudLayer]) -> np.ndarray:
     return np.full(128 * 128, v, dtype=np.uint8)
 
 
+def ground_distance(frame: abi.FrameUniforms, planet_radius: float = 6_360_000.0) -> np.ndarray:
+    """Per pixel, the view distance to the first hit of the planet sphere of
+    radius `planet_radius` centred (0, -R, 0); inf where the ray misses.
+    float64 (H, W)."""
+    cam, d = pixel_rays(frame)
+    oc = cam + np.array([0.0, planet_radius, 0.0])
+    b = d @ oc
+    r_cam = np.linalg.norm(oc)
+    c = (r_cam - planet_radius) * (r_cam + planet_radius)
+    disc = b * b - c
+    hit = disc >= 0
+    t0 = -b - np.sqrt(np.where(hit, disc, 0.0))
+    hit &= t0 > 0
+    return np.where(hit, t0, np.inf)
+
+
 def ground_depth(frame: abi.FrameUniforms, planet_radius: float = 6_360_000.0) -> np.ndarray:
     """Reverse-Z NDC depth an opaque planet-sized ground writes: per pixel, the
     view ray's first hit of the sphere of radius `planet_radius` centred
     (0, -R, 0); 0 (far) where the ray misses. float32 (H, W)."""
+    return distance_depth(frame, ground_distance(frame, planet_radius))
+
+
+def pixel_rays(frame: abi.FrameUniforms):
+    """Camera position and the unit view ray through each pixel centre
+    (math.wgsl:23-33 in FP64): (cam (3,), d (H, W, 3))."""
     W, H = int(frame.viewport_size[0]), int(frame.viewport_size[1])
     inv = frame_inv_view_proj(frame).astype(np.float64)  # [col, row]
-    vp = frame_view_proj(frame).astype(np.float64)
     cam = np.array(frame.camera_position_world[:3], dtype=np.float64)
     xs = (np.arange(W) + 0.5) / W * 2 - 1
     ys = 1 - (np.arange(H) + 0.5) / H * 2
@@ -270,18 +291,19 @@ def ground_depth(frame: abi.FrameUniforms, planet_radius: float = 6_360_000.0) -
     p = near[..., :3] / near[..., 3:4]
     d = p - cam
     d /= np.linalg.norm(d, axis=-1, keepdims=True)
-    oc = cam + np.array([0.0, planet_radius, 0.0])
-    b = d @ oc
-    r_cam = np.linalg.norm(oc)
-    c = (r_cam - planet_radius) * (r_cam + planet_radius)
-    disc = b * b - c
-    hit = disc >= 0
-    t0 = -b - np.sqrt(np.where(hit, disc, 0.0))
-    hit &= t0 > 0
-    wp = cam + d * t0[..., None]
-    clip = np.concatenate([wp, np.ones_like(t0)[..., None]], axis=-1) @ vp
-    depth = np.where(hit, clip[..., 2] / clip[..., 3], 0.0)
-    return depth.astype(np.float32)
+    return cam, d
+
+
+def distance_depth(frame: abi.FrameUniforms, t: np.ndarray) -> np.ndarray:
+    """Reverse-Z NDC depth of opaque surfaces at view distance t (H, W) along
+    each pixel's ray; non-finite t is the far plane (0). float32 (H, W)."""
+    cam, d = pixel_rays(frame)
+    vp = frame_view_proj(frame).astype(np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    hit = np.isfinite(t)
+    wp = cam + d * np.where(hit, t, 1.0)[..., None]
+    clip = np.concatenate([wp, np.ones(t.shape)[..., None]], axis=-1) @ vp
+    return np.where(hit, clip[..., 2] / clip[..., 3], 0.0).astype(np.float32)
 
 
 # ---------------------------------------------------------------------------
@@ -302,6 +324,7 @@ class Workload:
     overcast: np.ndarray           # 128*128 uint8
     blue: np.ndarray               # 64*64 uint8
     depth: Optional[np.ndarray]    # (H, W) float32 or None (all far)
+    type_grid: Optional[np.ndarray] = None  # 128*128 uint8 cloud types (255 = layer type) or None
 
     @property
     def width(self) -> int:
