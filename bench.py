@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -44,6 +45,18 @@ def cpu_baseline(w, base, detail, curl, budget_s: float, threads: int) -> dict:
             "sample": f"{frames} whole {w.width}x{w.height} frame(s) of sweep camera 0 (LUT chain + sky + "
                       f"2-layer march + AP + composite), float instantiation of the CPU restatement, "
                       f"{el:.1f} s"}
+
+
+def measured_traffic():
+    """HBM bytes per k_render launch from the committed counter summary
+    (FETCH_SIZE x its copy-calibrated factor + WRITE_SIZE, rocprofv3 --pmc
+    runs of this bench); None when the summary is absent."""
+    p = Path(__file__).resolve().parent / "profiles" / "k_render_fullframe_1280x720_counters.md"
+    try:
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", p.read_text(), re.M)
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
 
 
 def main() -> int:
@@ -138,6 +151,7 @@ def main() -> int:
     flops_per_launch = flops / len(used)
     bytes_per_launch = gbytes / len(used)
     render_s = timing.render_ms * 1e-3
+    hbm_bytes = cost_model.render_hbm_bytes(W, H, out_bytes_per_px=16)
     achieved_tflops = flops_per_launch / render_s / 1e12
     achieved_gbps = bytes_per_launch / render_s / 1e9
 
@@ -166,9 +180,13 @@ def main() -> int:
         "roofline": {"bound": "valu", "kernel": "k_render", "achieved": round(achieved_tflops, 3),
                      "peak": cost_model.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / cost_model.PEAK_FP32_VALU_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(),
+                     "peak_unpacked": cost_model.PEAK_FP32_VALU_UNPACKED_TFLOPS,
+                     "frac_unpacked": round(achieved_tflops / cost_model.PEAK_FP32_VALU_UNPACKED_TFLOPS, 4),
                      "gather_GBps": round(achieved_gbps, 1),
-                     "flop_per_launch": flops_per_launch, "gather_bytes_per_launch": bytes_per_launch},
+                     "flop_per_launch": flops_per_launch, "gather_bytes_per_launch": bytes_per_launch,
+                     "algorithmic_hbm_bytes_per_launch": hbm_bytes,
+                     "hbm_GBps": round(hbm_bytes / render_s / 1e9, 1)},
         "work_per_frame": {k: v / len(used) for k, v in cnt_sum.items()},
     }
     if not args.no_cpu_baseline:

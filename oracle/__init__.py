@@ -10,6 +10,30 @@ oracle/Makefile) with numpy-friendly functions. The C++ sources restate:
   shaders/atmosphere/{transmittance,multiscatter,skyview,aerialperspective}.comp.wgsl,
   shaders/atmosphere/sky.wgsl, shaders/clouds/{cloud_density_common,cloud_march,
   cloud_composite}.wgsl and shaders/clouds/noise_{base,detail,curl}.comp.wgsl.
+
+The reference (Rust + wgpu + WGSL) cannot be built or run here and holds no
+golden vectors for this path, so the restatement is pinned by checks in
+tests/test_oracle.py and tests/test_abi.py:
+  optical depth / transmittance LUT   closed form vertical OD, h^2 convergence,
+                                      T texel = exp(-OD), uv round trip
+  phase functions                     normalisation (HG, Draine, Rayleigh, mix),
+                                      Draine -> HG / Rayleigh reductions,
+                                      published Jendersie-d'Eon fit values
+  multi-scatter / sky-view / AP       MS = 0 without scattering, sky-view uv
+                                      inverse + azimuthal symmetry + blue
+                                      dominance, AP monotone in depth
+  samplers                            scipy.ndimage.map_coordinates(order=1)
+  march + integration                 Beer-Lambert constant column, emission-only
+                                      telescoping, coverage/density 0 identity,
+                                      depth clip, NDF shape properties
+  noise hash                          the integer mix (exact)
+  host synthesis                      coverage remap / type defaults / blue-noise
+                                      histogram / wind profile pins
+Parity unpinned (checked only against the restatement itself): the FBM
+arithmetic of the noise bakes beyond the hash and the tiling/range checks,
+the sun-disk limb darkening of the sky pass, the cone-kernel offsets of the
+cone light march, and the reference hardware sampler's filter-weight and
+f16/UNORM8 rounding.
 """
 from __future__ import annotations
 

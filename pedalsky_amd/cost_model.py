@@ -59,9 +59,25 @@ BYTES = {
     "ap_taps": 0.0,
 }
 
-# MI355X published peaks
+# MI355X published peaks. The FP32 vector figure counts packed FP32 (two
+# lanes per VALU op); the kernels here are built without packed FP32, so
+# 256 CUs x 4 SIMD x 16 lanes x 2 FLOP x 2.4 GHz = 78.6 TFLOP/s is the rate
+# they can reach.
 PEAK_FP32_VALU_TFLOPS = 157.3
+PEAK_FP32_VALU_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
+
+# Inputs one frame reads at least once from HBM: base volume (128^3 x 2 x u16),
+# its brick maxima (32^3 u8), detail (32^3 f32), curl (128^2 x 2 f32), the
+# 512^2 weather quads (4 x binary16) and the four FP32 LUTs of the benchmark
+# dimensions, plus the per-pixel depth (f32).
+INPUT_BYTES = (128 ** 3 * 4 + 32 ** 3 + 32 ** 3 * 4 + 128 ** 2 * 8 + 512 ** 2 * 8
+               + (256 * 64 + 32 * 32 + 192 * 108 + 32 ** 3) * 16)
+
+
+def render_hbm_bytes(width: int, height: int, out_bytes_per_px: int = 16) -> float:
+    """Algorithmic HBM bytes of one render: inputs once, depth in, output out."""
+    return float(INPUT_BYTES + width * height * (4 + out_bytes_per_px))
 
 
 def lut_flops(dims) -> float:

@@ -100,6 +100,12 @@ def _variant(name):
         t[band, :] = np.minimum(t[band, :], 2500.0 + 4500.0 * (np.arange(W) / (W - 1))[None, :])
         w.depth = scene.distance_depth(w.frame, t)
         return w, "scene depth clipping the march mid-layer (cloud_march.wgsl:758-806)"
+    if name == "sun_disk_in_view":
+        # sun 20 deg above the horizon, 5 deg above the view centre; a 1.5 deg
+        # disk radius so the disk and its limb cover several pixels
+        w = _base(sun_zenith=70.0, pitch=15.0, yaw=180.0)
+        w.frame.sun_direction[3] = np.radians(1.5)
+        return w, "sun disk with limb darkening in the sky pass (sky.wgsl:186-215)"
     if name == "phase_and_scattering_knobs":
         layers = [scene.synth_layer(0, 1500.0, 2300.0, 0.5, density_scale=1.0, droplet_diameter_um=8.0)]
         w = _base(layers=layers)
@@ -113,7 +119,8 @@ def _variant(name):
 
 
 VARIANTS = ["cone_light_sampling", "cloud_type_grid", "cumulonimbus_ice", "three_layers_low_sun", "lightning",
-            "temporal_jitter", "drift_skew_diurnal", "depth_clip_mid_layer", "phase_and_scattering_knobs"]
+            "temporal_jitter", "drift_skew_diurnal", "depth_clip_mid_layer", "sun_disk_in_view",
+            "phase_and_scattering_knobs"]
 
 
 def _render_both(r: Renderer, w):
