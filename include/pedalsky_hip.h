@@ -302,6 +302,63 @@ ps_status ps_timing_end(ps_ctx* ctx, ps_frame_timing* timing);
 ps_status ps_count_work(ps_ctx* ctx, const float* depth, ps_work_counters* out);
 
 /* ------------------------------------------------------------------------ */
+/* Many scenes per launch. No reference counterpart: the reference renders
+ * one scene per App::frame (crates/ps-core/src/app.rs:294-322), running the
+ * same passes per scene. A batch bakes every scene's own LUT set (one launch
+ * per LUT kind for all scenes) and renders every wave tile of every scene
+ * in one tile launch; the sequence can be captured once into a hipGraph and
+ * replayed. Noise, weather map, cloud-type grid and blue noise are the
+ * context's (shared by all scenes), as are the LUT sizes; the overcast mask
+ * may be per scene. Scene parameters are read from the descriptors at create and
+ * update time; depth and outputs are device pointers.                      */
+/* ------------------------------------------------------------------------ */
+typedef struct ps_scene_desc {
+    const ps_frame_uniforms* frame;        /* viewport up to 4096 x 4096 */
+    const ps_atmosphere_params* atmosphere;
+    const ps_cloud_params* clouds;
+    const ps_cloud_layer* layers;
+    int32_t n_layers;
+    int32_t _reserved;
+    const float* layer_wind_xz;            /* 2 per layer, or NULL (calm) */
+    const uint8_t* overcast_mask_128;      /* host 128x128 R8Unorm mask of this scene,
+                                              or NULL for the context's */
+    const float* depth;                    /* device W*H reverse-Z NDC, or NULL */
+    ps_image_out out;                      /* device outputs (at least one) */
+} ps_scene_desc;
+
+/* Launch shape of a batch, chosen by ps_batch_autotune or by the caller. */
+typedef struct ps_tuning {
+    int32_t tile_w;          /* wave tile tile_w x (64 / tile_w) pixels: 4, 8 or 16 */
+    int32_t persistent;      /* 1: one-wave workers pull tiles from an atomic counter;
+                                0: one block per tile (hardware dispatch) */
+    int32_t workers_per_cu;  /* persistent workers per compute unit (1..64) */
+    int32_t order;           /* tile order: 0 scene order, 1 larger scenes first */
+} ps_tuning;
+
+typedef struct ps_batch ps_batch;
+
+ps_status ps_batch_create(ps_ctx* ctx, const ps_scene_desc* scenes, int32_t n, ps_batch** out);
+/* New parameters for the same scenes (same count and resolutions); a
+ * captured graph stays valid and reads them on its next replay. */
+ps_status ps_batch_update(ps_batch* batch, const ps_scene_desc* scenes, int32_t n);
+ps_status ps_batch_set_tuning(ps_batch* batch, const ps_tuning* tuning);
+ps_status ps_batch_get_tuning(const ps_batch* batch, ps_tuning* tuning);
+/* Time each candidate launch shape on this batch (`reps` runs each) and keep
+ * the fastest; a captured graph is re-captured with it. */
+ps_status ps_batch_autotune(ps_batch* batch, int32_t reps, ps_tuning* best);
+/* Enqueue one batch run on the context stream (direct launches). */
+ps_status ps_batch_run(ps_batch* batch);
+/* Record the run sequence into a hipGraph; ps_batch_replay launches it. */
+ps_status ps_batch_capture(ps_batch* batch);
+ps_status ps_batch_replay(ps_batch* batch);
+/* Run (or replay, when captured) `reps` times back to back; average device
+ * time per run from HIP events on the context stream. */
+ps_status ps_batch_time(ps_batch* batch, int32_t reps, float* ms_per_run);
+void ps_batch_destroy(ps_batch* batch);
+/* One-shot: create, run with `tuning` (NULL = default), wait, destroy. */
+ps_status ps_render_batch(ps_ctx* ctx, const ps_scene_desc* scenes, int32_t n, const ps_tuning* tuning);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side synthesis helpers (CPU, no device work).                       */
 /* ------------------------------------------------------------------------ */
 /* AtmosphereParams::default (weather.rs:60-84), CloudParamsGpu::default

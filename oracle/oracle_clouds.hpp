@@ -175,12 +175,22 @@ inline R sample_density_light(const Scene<R>& s, const V3<R>& p, R p_alt, const 
     return clampr<R>(cloud, 0, 1) * L.density_scale;
 }
 
+// Upper tolerance of the primary march's altitude gate. With blue-noise
+// jitter 255/255 the last sample of a layer sits exactly at the interval end
+// t1 = t0 + steps * step, i.e. exactly on the top shell (h = 1) for rays
+// leaving through the top; the literal `h > 1` test then decides it by
+// rounding noise in any precision (the WGSL's FP32 included). h = 1 is
+// inside in exact arithmetic, and only the cumulonimbus profile is non-zero
+// there (anvil), so the gate accepts h <= 1 + 1e-5 (3 cm of a 3 km layer);
+// the GPU kernel uses the same bound.
+constexpr double kPrimaryGateTop = 1.0 + 1e-5;
+
 // ---- cloud_march.wgsl:432-530 --------------------------------------------------------------
 template <class R>
 inline R sample_density(const Scene<R>& s, const V3<R>& p, R p_alt, const Layer<R>& L, R weather_r,
                         const Wind<R>& w, uint32_t ctype, Counters* cnt) {
     R h = (p_alt - L.base) / std::max(L.top - L.base, R(1));
-    if (h < R(0) || h > R(1)) return R(0);
+    if (h < R(0) || h > R(kPrimaryGateTop)) return R(0);
     R eff = clampr<R>(weather_r * L.coverage, 0, 1);
     if (eff < R(0.01)) return R(0);
     if (cnt) cnt->covered_steps++;

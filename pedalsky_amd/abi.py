@@ -119,6 +119,25 @@ class ImageOut(C.Structure):
     ]
 
 
+class SceneDesc(C.Structure):
+    """ps_scene_desc: one scene of a batch (host pointers to its uniform
+    blocks, device pointers for depth and outputs)."""
+    _fields_ = [
+        ("frame", C.c_void_p), ("atmosphere", C.c_void_p), ("clouds", C.c_void_p), ("layers", C.c_void_p),
+        ("n_layers", C.c_int32), ("_reserved", C.c_int32), ("layer_wind_xz", C.c_void_p),
+        ("overcast_mask_128", C.c_void_p), ("depth", C.c_void_p), ("out", ImageOut),
+    ]
+
+
+class Tuning(C.Structure):
+    """ps_tuning: launch shape of a batch."""
+    _fields_ = [("tile_w", C.c_int32), ("persistent", C.c_int32), ("workers_per_cu", C.c_int32),
+                ("order", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class WorkCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "pixels", "layer_marches", "primary_steps", "covered_steps", "eroded_steps",
@@ -209,6 +228,17 @@ _SIGNATURES = [
     ("ps_timing_begin", C.c_int32, [_P, C.c_int32]),
     ("ps_timing_end", C.c_int32, [_P, C.POINTER(FrameTiming)]),
     ("ps_count_work", C.c_int32, [_P, _P, C.POINTER(WorkCounters)]),
+    ("ps_batch_create", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32, C.POINTER(C.c_void_p)]),
+    ("ps_batch_update", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32]),
+    ("ps_batch_set_tuning", C.c_int32, [_P, C.POINTER(Tuning)]),
+    ("ps_batch_get_tuning", C.c_int32, [_P, C.POINTER(Tuning)]),
+    ("ps_batch_autotune", C.c_int32, [_P, C.c_int32, C.POINTER(Tuning)]),
+    ("ps_batch_run", C.c_int32, [_P]),
+    ("ps_batch_capture", C.c_int32, [_P]),
+    ("ps_batch_replay", C.c_int32, [_P]),
+    ("ps_batch_time", C.c_int32, [_P, C.c_int32, C.POINTER(C.c_float)]),
+    ("ps_batch_destroy", None, [_P]),
+    ("ps_render_batch", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32, C.POINTER(Tuning)]),
     ("ps_default_atmosphere", None, [C.POINTER(AtmosphereParams)]),
     ("ps_default_cloud_params", None, [C.POINTER(CloudParams)]),
     ("ps_remap_coverage", C.c_float, [C.c_float]),

@@ -8,9 +8,23 @@
 #include <string>
 #include <vector>
 #include <algorithm>
-#include "../../include/pedalsky_hip.h"
-#include "ps_device.h"
+#include "ps_host.h"
 #include "ps_launch.h"
+
+namespace psi {
+
+namespace {
+thread_local std::string g_last_error = "no error";
+}
+
+ps_status fail(ps_status code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace psi
+
+using namespace psi;
 
 static_assert(sizeof(ps_frame_uniforms) == 448, "FrameUniformsGpu is 448 B (frame_uniforms.rs:195-201)");
 static_assert(sizeof(ps_atmosphere_params) == 144, "AtmosphereParams is 144 B (weather.rs:544-555)");
@@ -37,93 +51,7 @@ static_assert(offsetof(ps_cloud_params, temporal_jitter) == 96, "temporal_jitter
 static_assert(offsetof(ps_cloud_layer, cloud_type) == 16, "cloud_type offset");
 static_assert(offsetof(ps_cloud_layer, droplet_diameter_um) == 32, "droplet offset");
 
-namespace {
 
-thread_local std::string g_last_error = "no error";
-
-ps_status fail(ps_status code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-
-#define PS_HIP(call)                                                                           \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(e_ == hipErrorOutOfMemory ? PS_ERR_OUT_OF_MEMORY : PS_ERR_DEVICE,      \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                    \
-    } while (0)
-
-template <class T>
-ps_status dev_alloc(T** p, size_t bytes, const char* what) {
-    if (*p) return PS_OK;
-    hipError_t e = hipMalloc((void**)p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(e == hipErrorOutOfMemory ? PS_ERR_OUT_OF_MEMORY : PS_ERR_DEVICE,
-                    std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
-    }
-    return PS_OK;
-}
-template <class T>
-void dev_free(T*& p) {
-    if (p) hipFree((void*)p);
-    p = nullptr;
-}
-
-}  // namespace
-
-struct ps_ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    ps_lut_dims dims{256, 64, 32, 32, 192, 108, 32, 32, 64};
-
-    ps_atmosphere_params atm{};
-    ps_frame_uniforms frame{};
-    ps_cloud_params clouds{};
-    ps_cloud_layer layers[PS_MAX_CLOUD_LAYERS]{};
-    float wind[2 * PS_MAX_CLOUD_LAYERS]{};
-    int n_layers = 0;
-    bool have_atm = false, have_frame = false, have_clouds = false;
-
-    float4 *T = nullptr, *MS = nullptr, *SV = nullptr, *AP = nullptr;
-    uint32_t lut_baked = 0;  // LUTs baked since the last atmosphere / size change
-    bool luts_valid = false;
-
-    int noise_format = -1;
-    void *base_store = nullptr, *detail_store = nullptr, *curl_store = nullptr;
-    uint32_t* base_rl = nullptr;
-    uint8_t* rmax = nullptr;
-    float* detail_hf = nullptr;
-    float2* curl2 = nullptr;
-    bool have_noise = false;
-
-    uint2* weather = nullptr;         // binary16 2x2 quads
-    int ww = 0, wh = 0;
-    uint8_t* type_grid = nullptr;
-    bool type_grid_active = false;
-    uint8_t* overcast = nullptr;
-    bool overcast_active = false;
-    bool have_weather = false;
-    uint8_t* blue = nullptr;
-    bool have_blue = false;
-
-    // Render parameter blocks: a ring of device slots fed from pinned host
-    // staging, so a new camera each frame needs no host/device sync; a slot is
-    // rewritten only after the event of its last use has completed.
-    static constexpr int kParamSlots = 8;
-    RenderParams* d_params = nullptr;   // kParamSlots blocks
-    RenderParams* h_params = nullptr;   // kParamSlots pinned blocks
-    hipEvent_t slot_ev[kParamSlots]{};
-    bool slot_used[kParamSlots]{};
-    int cur_slot = -1;
-    unsigned long long* d_counters = nullptr;
-    std::vector<hipEvent_t> events;
-    // in-loop timing (ps_timing_begin / ps_timing_end)
-    bool timing = false;
-    int timing_max = 0, timing_n = 0;
-};
 
 namespace {
 
@@ -132,15 +60,17 @@ size_t noise_texel_bytes(int format, int channels) { return (format == PS_NOISE_
 ps_status ensure_luts(ps_ctx* c) {
     const ps_lut_dims& d = c->dims;
     ps_status s;
-    if ((s = dev_alloc(&c->T, sizeof(float4) * d.transmittance_w * d.transmittance_h, "transmittance"))) return s;
-    if ((s = dev_alloc(&c->MS, sizeof(float4) * d.multiscatter_w * d.multiscatter_h, "multiscatter"))) return s;
-    if ((s = dev_alloc(&c->SV, sizeof(float4) * d.skyview_w * d.skyview_h, "skyview"))) return s;
-    if ((s = dev_alloc(&c->AP, sizeof(float4) * d.ap_w * d.ap_h * d.ap_d, "aerial perspective"))) return s;
+    if ((s = dev_alloc(&c->luts.T, sizeof(float4) * d.transmittance_w * d.transmittance_h, "transmittance"))) return s;
+    if ((s = dev_alloc(&c->luts.MS, sizeof(float4) * d.multiscatter_w * d.multiscatter_h, "multiscatter"))) return s;
+    if ((s = dev_alloc(&c->luts.SV, sizeof(float4) * d.skyview_w * d.skyview_h, "skyview"))) return s;
+    if ((s = dev_alloc(&c->luts.AP, sizeof(float4) * d.ap_w * d.ap_h * d.ap_d, "aerial perspective"))) return s;
     return PS_OK;
 }
 
-void fill_lut_params(const ps_ctx* c, LutParams& P) {
-    const ps_atmosphere_params& a = c->atm;
+}  // namespace
+
+void psi::fill_lut_params(const SceneIn& sc, const ps_lut_dims& d, LutParams& P) {
+    const ps_atmosphere_params& a = sc.atm;
     P.a.R = a.planet_radius_m; P.a.top = a.atmosphere_top_m;
     P.a.hr = a.rayleigh_scale_height_m; P.a.hm = a.mie_scale_height_m;
     for (int i = 0; i < 3; ++i) {
@@ -151,14 +81,13 @@ void fill_lut_params(const ps_ctx* c, LutParams& P) {
         P.a.albedo[i] = a.ground_albedo[i];
     }
     P.a.mie_g = a.mie_g; P.a.oz_c = a.ozone_center_m; P.a.oz_t = a.ozone_thickness_m;
-    const ps_frame_uniforms& f = c->frame;
+    const ps_frame_uniforms& f = sc.frame;
     for (int i = 0; i < 16; ++i) P.f.inv_vp[i] = f.inv_view_proj[i];
     for (int i = 0; i < 3; ++i) {
         P.f.cam[i] = f.camera_position_world[i];
         P.f.sun[i] = f.sun_direction[i];
         P.f.E[i] = f.sun_illuminance[i];
     }
-    const ps_lut_dims& d = c->dims;
     P.tw = d.transmittance_w; P.th = d.transmittance_h;
     P.mw = d.multiscatter_w; P.mh = d.multiscatter_h;
     P.sw = d.skyview_w; P.sh = d.skyview_h;
@@ -166,15 +95,20 @@ void fill_lut_params(const ps_ctx* c, LutParams& P) {
     P.ms_disabled = 0;
 }
 
+namespace {
+
 double smoothstep_d(double e0, double e1, double x) {
     double t = std::min(std::max((x - e0) / (e1 - e0), 0.0), 1.0);
     return t * t * (3.0 - 2.0 * t);
 }
 
-ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image_out* out, RenderParams& P) {
-    const ps_frame_uniforms& f = c->frame;
-    const ps_atmosphere_params& a = c->atm;
-    const ps_cloud_params& cp = c->clouds;
+}  // namespace
+
+ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutSet& luts, const float* depth,
+                                  const ps_image_out* out, RenderParams& P) {
+    const ps_frame_uniforms& f = sc.frame;
+    const ps_atmosphere_params& a = sc.atm;
+    const ps_cloud_params& cp = sc.clouds;
     std::memset(&P, 0, sizeof(P));
     float vw = f.viewport_size[0], vh = f.viewport_size[1];
     if (!(vw >= 1.f && vh >= 1.f && vw <= 32768.f && vh <= 32768.f) || vw != std::floor(vw) || vh != std::floor(vh))
@@ -215,7 +149,7 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     }
     // clouds
     int lc = (int)std::min<uint32_t>(cp.cloud_layer_count, PS_MAX_CLOUD_LAYERS);
-    lc = std::min(lc, c->n_layers);
+    lc = std::min(lc, sc.n_layers);
     P.n_layers = lc;
     P.steps = (int)cp.cloud_steps;
     P.light_steps = (int)cp.light_steps;
@@ -246,7 +180,7 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     P.light_z = f.lightning_origin_world[2];
     const double drift = (double)cp.wind_drift_strength * (double)f.simulated_seconds;
     for (int l = 0; l < lc; ++l) {
-        const ps_cloud_layer& L = c->layers[l];
+        const ps_cloud_layer& L = sc.layers[l];
         LayerC& oc = P.layers[l];
         LayerF& o = oc.f;
         o.base = L.base_m; o.top = L.top_m;
@@ -269,7 +203,7 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
             mod[0] = 1.0 - 0.5 * st; mod[2] = 1.0 + st;
         }
         for (int k = 0; k < 3; ++k) o.sigma_t[k] = (float)(((double)cp.sigma_s[k] + cp.sigma_a[k]) * mod[k]);
-        double wx = c->wind[2 * l], wz = c->wind[2 * l + 1];
+        double wx = sc.wind[2 * l], wz = sc.wind[2 * l + 1];
         o.wind_off_x = (float)(wx * drift);
         o.wind_off_z = (float)(wz * drift);
         o.skew_x = 0.f; o.skew_z = 0.f;
@@ -302,9 +236,9 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     }
     // resources
     const ps_lut_dims& dm = c->dims;
-    P.T = c->T; P.tw = dm.transmittance_w; P.th = dm.transmittance_h;
-    P.SV = c->SV; P.sw = dm.skyview_w; P.sh = dm.skyview_h;
-    P.AP = c->AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
+    P.T = luts.T; P.tw = dm.transmittance_w; P.th = dm.transmittance_h;
+    P.SV = luts.SV; P.sw = dm.skyview_w; P.sh = dm.skyview_h;
+    P.AP = luts.AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
     P.base_rl = c->base_rl; P.rmax = c->rmax; P.detail_hf = c->detail_hf; P.curl = c->curl2;
     P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
     P.weather_pow2 = (c->ww & (c->ww - 1)) == 0 && (c->wh & (c->wh - 1)) == 0 ? 1 : 0;
@@ -323,14 +257,22 @@ ps_status fill_render_params(const ps_ctx* c, const float* depth, const ps_image
     return PS_OK;
 }
 
+ps_status psi::check_shared_inputs(const ps_ctx* c) {
+    if (!c->have_noise) return fail(PS_ERR_NOT_READY, "no noise volumes: call ps_generate_noise or ps_upload_noise");
+    if (!c->have_weather) return fail(PS_ERR_NOT_READY, "no weather map: call ps_upload_weather");
+    if (!c->have_blue) return fail(PS_ERR_NOT_READY, "no blue noise: call ps_upload_blue_noise");
+    return PS_OK;
+}
+
+namespace {
+
 ps_status check_ready(const ps_ctx* c, bool for_render) {
     if (!c->have_atm) return fail(PS_ERR_NOT_READY, "ps_set_atmosphere has not been called");
     if (!c->have_frame) return fail(PS_ERR_NOT_READY, "ps_set_frame has not been called");
     if (!for_render) return PS_OK;
     if (!c->have_clouds) return fail(PS_ERR_NOT_READY, "ps_set_clouds has not been called");
-    if (!c->have_noise) return fail(PS_ERR_NOT_READY, "no noise volumes: call ps_generate_noise or ps_upload_noise");
-    if (!c->have_weather) return fail(PS_ERR_NOT_READY, "no weather map: call ps_upload_weather");
-    if (!c->have_blue) return fail(PS_ERR_NOT_READY, "no blue noise: call ps_upload_blue_noise");
+    ps_status s;
+    if ((s = check_shared_inputs(c))) return s;
     if (!c->luts_valid) return fail(PS_ERR_NOT_READY, "LUTs not baked: call ps_bake_luts");
     return PS_OK;
 }
@@ -340,13 +282,16 @@ ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
     if ((s = check_ready(c, false))) return s;
     if ((s = ensure_luts(c))) return s;
     LutParams P;
-    fill_lut_params(c, P);
+    fill_lut_params(c->sc, c->dims, P);
     uint32_t k = flags & 15u;
     if (flags & PS_LUT_MULTISCATTER_DISABLED) {
         P.ms_disabled = 1;
         k |= 2u;
     }
-    PS_HIP(ps_launch_luts(P, k, c->T, c->MS, c->SV, c->AP, c->stream, ev));
+    LutParams* dP = nullptr;
+    if ((s = c->lut_ring.upload(P, c->stream, &dP))) return s;
+    PS_HIP(ps_launch_luts(dP, P, 1, k, c->luts.T, c->luts.MS, c->luts.SV, c->luts.AP, c->stream, ev));
+    if ((s = c->lut_ring.mark_used(c->stream))) return s;
     c->lut_baked |= (k & 15u);
     c->luts_valid = c->lut_baked == 15u;
     return PS_OK;
@@ -356,34 +301,47 @@ ps_status launch_render(ps_ctx* c, const float* depth, const ps_image_out* out, 
     ps_status s;
     if ((s = check_ready(c, true))) return s;
     if (!out && !count) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render: out is NULL");
-    const int NS = ps_ctx::kParamSlots;
-    if ((s = dev_alloc(&c->d_params, sizeof(RenderParams) * NS, "render params"))) return s;
-    if (!c->h_params) {
-        PS_HIP(hipHostMalloc((void**)&c->h_params, sizeof(RenderParams) * NS, hipHostMallocDefault));
-        for (int i = 0; i < NS; ++i) PS_HIP(hipEventCreateWithFlags(&c->slot_ev[i], hipEventDisableTiming));
-    }
     RenderParams next;
-    if ((s = fill_render_params(c, depth, out, next))) return s;
-    int slot = c->cur_slot;
-    if (slot < 0 || std::memcmp(&next, &c->h_params[slot], sizeof(RenderParams)) != 0) {
-        slot = (slot + 1) % NS;
-        if (c->slot_used[slot]) PS_HIP(hipEventSynchronize(c->slot_ev[slot]));
-        std::memcpy(&c->h_params[slot], &next, sizeof(RenderParams));
-        PS_HIP(hipMemcpyAsync(&c->d_params[slot], &c->h_params[slot], sizeof(RenderParams), hipMemcpyHostToDevice,
-                              c->stream));
-        c->cur_slot = slot;
-    }
-    PS_HIP(ps_launch_render(&c->d_params[slot], next.W, next.H, count, c->stream));
-    PS_HIP(hipEventRecord(c->slot_ev[slot], c->stream));
-    c->slot_used[slot] = true;
-    return PS_OK;
+    if ((s = fill_render_params(c, c->sc, c->luts, depth, out, next))) return s;
+    RenderParams* dP = nullptr;
+    if ((s = c->render_ring.upload(next, c->stream, &dP))) return s;
+    PS_HIP(ps_launch_render(dP, next.W, next.H, count, c->stream));
+    return c->render_ring.mark_used(c->stream);
 }
 
 }  // namespace
 
+ps_status psi::check_atmosphere(const ps_atmosphere_params* a, const char* who) {
+    if (!(a->planet_radius_m > 0.f) || !(a->atmosphere_top_m > a->planet_radius_m))
+        return fail(PS_ERR_INVALID_ARGUMENT, std::string(who) + ": need 0 < planet_radius_m < atmosphere_top_m");
+    if (!(a->rayleigh_scale_height_m > 0.f) || !(a->mie_scale_height_m > 0.f))
+        return fail(PS_ERR_INVALID_ARGUMENT, std::string(who) + ": scale heights must be positive");
+    return PS_OK;
+}
+
+ps_status psi::check_clouds(const ps_cloud_params* p, const ps_cloud_layer* layers, int32_t n, const char* who) {
+    if (n < 0 || n > PS_MAX_CLOUD_LAYERS)
+        return fail(PS_ERR_INVALID_ARGUMENT, std::string(who) + ": n_layers must be in [0, 8]");
+    if (n > 0 && !layers) return fail(PS_ERR_INVALID_ARGUMENT, std::string(who) + ": layers is NULL");
+    if (p->cloud_steps < 1 || p->cloud_steps > 4096)
+        return fail(PS_ERR_INVALID_ARGUMENT, std::string(who) + ": cloud_steps must be in [1, 4096]");
+    if (p->light_steps < 1 || p->light_steps > 256)
+        return fail(PS_ERR_INVALID_ARGUMENT, std::string(who) + ": light_steps must be in [1, 256]");
+    if (p->multi_scatter_octaves > PS_MAX_OCTAVES)
+        return fail(PS_ERR_UNSUPPORTED, std::string(who) + ": at most 8 multi-scatter octaves are supported");
+    int i, j;
+    if (n > 0 && ps_check_layers_overlap(layers, n, &i, &j)) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "%s: layers %d and %d overlap ([%g, %g] vs [%g, %g] m)", who, i, j,
+                      layers[i].base_m, layers[i].top_m, layers[j].base_m, layers[j].top_m);
+        return fail(PS_ERR_OVERLAPPING_LAYERS, buf);
+    }
+    return PS_OK;
+}
+
 extern "C" {
 
-const char* ps_last_error(void) { return g_last_error.c_str(); }
+const char* ps_last_error(void) { return psi::g_last_error.c_str(); }
 int32_t ps_abi_version(void) { return PS_ABI_VERSION; }
 
 ps_status ps_ctx_create(int32_t device, ps_ctx** out_ctx) {
@@ -415,15 +373,13 @@ void ps_ctx_destroy(ps_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    dev_free(c->T); dev_free(c->MS); dev_free(c->SV); dev_free(c->AP);
+    dev_free(c->luts.T); dev_free(c->luts.MS); dev_free(c->luts.SV); dev_free(c->luts.AP);
     dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
     dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
-    dev_free(c->d_params); dev_free(c->d_counters);
-    if (c->h_params) {
-        hipHostFree(c->h_params);
-        for (int i = 0; i < ps_ctx::kParamSlots; ++i) hipEventDestroy(c->slot_ev[i]);
-    }
+    dev_free(c->d_counters);
+    c->render_ring.release();
+    c->lut_ring.release();
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
@@ -450,7 +406,7 @@ ps_status ps_ctx_set_lut_dims(ps_ctx* c, const ps_lut_dims* d) {
     if ((size_t)d->ap_w * d->ap_h * d->ap_d > (size_t)1 << 24)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_set_lut_dims: AP volume too large");
     PS_HIP(hipStreamSynchronize(c->stream));
-    dev_free(c->T); dev_free(c->MS); dev_free(c->SV); dev_free(c->AP);
+    dev_free(c->luts.T); dev_free(c->luts.MS); dev_free(c->luts.SV); dev_free(c->luts.AP);
     c->dims = *d;
     c->lut_baked = 0;
     c->luts_valid = false;
@@ -465,11 +421,9 @@ ps_status ps_ctx_get_lut_dims(const ps_ctx* c, ps_lut_dims* out) {
 
 ps_status ps_set_atmosphere(ps_ctx* c, const ps_atmosphere_params* a) {
     if (!c || !a) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_atmosphere: NULL argument");
-    if (!(a->planet_radius_m > 0.f) || !(a->atmosphere_top_m > a->planet_radius_m))
-        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_atmosphere: need 0 < planet_radius_m < atmosphere_top_m");
-    if (!(a->rayleigh_scale_height_m > 0.f) || !(a->mie_scale_height_m > 0.f))
-        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_atmosphere: scale heights must be positive");
-    c->atm = *a;
+    ps_status s;
+    if ((s = check_atmosphere(a, "ps_set_atmosphere"))) return s;
+    c->sc.atm = *a;
     c->have_atm = true;
     c->lut_baked = 0;
     c->luts_valid = false;
@@ -478,7 +432,7 @@ ps_status ps_set_atmosphere(ps_ctx* c, const ps_atmosphere_params* a) {
 
 ps_status ps_set_frame(ps_ctx* c, const ps_frame_uniforms* f) {
     if (!c || !f) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_frame: NULL argument");
-    c->frame = *f;
+    c->sc.frame = *f;
     c->have_frame = true;
     return PS_OK;
 }
@@ -486,28 +440,14 @@ ps_status ps_set_frame(ps_ctx* c, const ps_frame_uniforms* f) {
 ps_status ps_set_clouds(ps_ctx* c, const ps_cloud_params* p, const ps_cloud_layer* layers, int32_t n,
                         const float* wind) {
     if (!c || !p) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: NULL argument");
-    if (n < 0 || n > PS_MAX_CLOUD_LAYERS)
-        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: n_layers must be in [0, 8]");
-    if (n > 0 && !layers) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: layers is NULL");
-    if (p->cloud_steps < 1 || p->cloud_steps > 4096)
-        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: cloud_steps must be in [1, 4096]");
-    if (p->light_steps < 1 || p->light_steps > 256)
-        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_clouds: light_steps must be in [1, 256]");
-    if (p->multi_scatter_octaves > PS_MAX_OCTAVES)
-        return fail(PS_ERR_UNSUPPORTED, "ps_set_clouds: at most 8 multi-scatter octaves are supported");
-    int i, j;
-    if (n > 0 && ps_check_layers_overlap(layers, n, &i, &j)) {
-        char buf[160];
-        std::snprintf(buf, sizeof(buf), "ps_set_clouds: layers %d and %d overlap ([%g, %g] vs [%g, %g] m)", i, j,
-                      layers[i].base_m, layers[i].top_m, layers[j].base_m, layers[j].top_m);
-        return fail(PS_ERR_OVERLAPPING_LAYERS, buf);
-    }
-    c->clouds = *p;
-    c->n_layers = n;
+    ps_status s;
+    if ((s = check_clouds(p, layers, n, "ps_set_clouds"))) return s;
+    c->sc.clouds = *p;
+    c->sc.n_layers = n;
     for (int l = 0; l < n; ++l) {
-        c->layers[l] = layers[l];
-        c->wind[2 * l] = wind ? wind[2 * l] : 0.f;
-        c->wind[2 * l + 1] = wind ? wind[2 * l + 1] : 0.f;
+        c->sc.layers[l] = layers[l];
+        c->sc.wind[2 * l] = wind ? wind[2 * l] : 0.f;
+        c->sc.wind[2 * l + 1] = wind ? wind[2 * l + 1] : 0.f;
     }
     c->have_clouds = true;
     return PS_OK;
@@ -628,15 +568,15 @@ ps_status ps_bake_luts(ps_ctx* c, uint32_t flags) {
 
 ps_status ps_read_lut(ps_ctx* c, int32_t which, float* host_out, size_t floats) {
     if (!c || !host_out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_lut: NULL argument");
-    if (!c->T) return fail(PS_ERR_NOT_READY, "ps_read_lut: LUTs not allocated");
+    if (!c->luts.T) return fail(PS_ERR_NOT_READY, "ps_read_lut: LUTs not allocated");
     const ps_lut_dims& d = c->dims;
     const float4* src;
     size_t n;
     switch (which) {
-        case PS_LUT_TRANSMITTANCE: src = c->T; n = (size_t)d.transmittance_w * d.transmittance_h; break;
-        case PS_LUT_MULTISCATTER: src = c->MS; n = (size_t)d.multiscatter_w * d.multiscatter_h; break;
-        case PS_LUT_SKYVIEW: src = c->SV; n = (size_t)d.skyview_w * d.skyview_h; break;
-        case PS_LUT_AERIAL_PERSPECTIVE: src = c->AP; n = (size_t)d.ap_w * d.ap_h * d.ap_d; break;
+        case PS_LUT_TRANSMITTANCE: src = c->luts.T; n = (size_t)d.transmittance_w * d.transmittance_h; break;
+        case PS_LUT_MULTISCATTER: src = c->luts.MS; n = (size_t)d.multiscatter_w * d.multiscatter_h; break;
+        case PS_LUT_SKYVIEW: src = c->luts.SV; n = (size_t)d.skyview_w * d.skyview_h; break;
+        case PS_LUT_AERIAL_PERSPECTIVE: src = c->luts.AP; n = (size_t)d.ap_w * d.ap_h * d.ap_d; break;
         default: return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_lut: unknown LUT");
     }
     if (floats < 4 * n) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_lut: host buffer too small");

@@ -107,7 +107,7 @@ __device__ __forceinline__ void store4(float4* out, int idx, D3 v, double w) {
 }
 
 // ---- transmittance.comp.wgsl:18-31 with atmosphere.wgsl:162-214 ---------------
-__global__ __launch_bounds__(64) void k_transmittance(LutParams P, float4* __restrict__ out) {
+__device__ __forceinline__ void transmittance_body(const LutParams& P, float4* __restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.tw * P.th) return;
     const AtmD& a = P.a;
@@ -148,8 +148,8 @@ __global__ __launch_bounds__(64) void k_transmittance(LutParams P, float4* __res
 
 // ---- multiscatter.comp.wgsl:58-156: one wave64 per texel, one lane per
 // Fibonacci direction, directions summed by a wave reduction. ---------------------
-__global__ __launch_bounds__(64) void k_multiscatter(LutParams P, const float4* __restrict__ T,
-                                                     float4* __restrict__ out) {
+__device__ __forceinline__ void multiscatter_body(const LutParams& P, const float4* __restrict__ T,
+                                                  float4* __restrict__ out) {
     int texel = blockIdx.x;
     if (texel >= P.mw * P.mh) return;
     const AtmD& a = P.a;
@@ -261,8 +261,8 @@ __device__ __forceinline__ void inscatter_march(const LutParams& P, const float4
 }
 
 // ---- skyview.comp.wgsl:46-174 -----------------------------------------------------
-__global__ __launch_bounds__(64) void k_skyview(LutParams P, const float4* __restrict__ T,
-                                                const float4* __restrict__ MS, float4* __restrict__ out) {
+__device__ __forceinline__ void skyview_body(const LutParams& P, const float4* __restrict__ T,
+                                             const float4* __restrict__ MS, float4* __restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.sw * P.sh) return;
     const AtmD& a = P.a;
@@ -310,8 +310,8 @@ __global__ __launch_bounds__(64) void k_skyview(LutParams P, const float4* __res
 }
 
 // ---- aerialperspective.comp.wgsl:41-127 ----------------------------------------------
-__global__ __launch_bounds__(64) void k_aerial(LutParams P, const float4* __restrict__ T,
-                                               const float4* __restrict__ MS, float4* __restrict__ out) {
+__device__ __forceinline__ void aerial_body(const LutParams& P, const float4* __restrict__ T,
+                                            const float4* __restrict__ MS, float4* __restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.aw * P.ah * P.ad) return;
     const AtmD& a = P.a;
@@ -337,20 +337,50 @@ __global__ __launch_bounds__(64) void k_aerial(LutParams P, const float4* __rest
     store4(out, i, lum, t_lum);
 }
 
+// Batched entry points: blockIdx.y selects the scene; its parameter block
+// and its slice of each LUT array.
+__global__ __launch_bounds__(64) void k_transmittance(const LutParams* __restrict__ Ps, float4* __restrict__ T) {
+    const LutParams& P = Ps[blockIdx.y];
+    transmittance_body(P, T + (size_t)blockIdx.y * P.tw * P.th);
+}
+
+__global__ __launch_bounds__(64) void k_multiscatter(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
+                                                     float4* __restrict__ MS) {
+    const LutParams& P = Ps[blockIdx.y];
+    multiscatter_body(P, T + (size_t)blockIdx.y * P.tw * P.th, MS + (size_t)blockIdx.y * P.mw * P.mh);
+}
+
+__global__ __launch_bounds__(64) void k_skyview(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
+                                                const float4* __restrict__ MS, float4* __restrict__ SV) {
+    const LutParams& P = Ps[blockIdx.y];
+    const size_t s = blockIdx.y;
+    skyview_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, SV + s * P.sw * P.sh);
+}
+
+__global__ __launch_bounds__(64) void k_aerial(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
+                                               const float4* __restrict__ MS, float4* __restrict__ AP) {
+    const LutParams& P = Ps[blockIdx.y];
+    const size_t s = blockIdx.y;
+    aerial_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, AP + s * P.aw * P.ah * P.ad);
+}
+
 }  // namespace
 
-hipError_t ps_launch_luts(const LutParams& P, uint32_t flags, float4* T, float4* MS, float4* SV, float4* AP,
-                          hipStream_t s, hipEvent_t* ev) {
+hipError_t ps_launch_luts(const LutParams* d_params, const LutParams& P, int n, uint32_t flags, float4* T,
+                          float4* MS, float4* SV, float4* AP, hipStream_t s, hipEvent_t* ev) {
     // ev (optional): 5 events recorded before T, MS, SV, AP and after AP.
+    const unsigned ns = (unsigned)n;
     if (ev) hipEventRecord(ev[0], s);
-    if (flags & 1u) hipLaunchKernelGGL(k_transmittance, dim3((P.tw * P.th + 63) / 64), dim3(64), 0, s, P, T);
+    if (flags & 1u)
+        hipLaunchKernelGGL(k_transmittance, dim3((P.tw * P.th + 63) / 64, ns), dim3(64), 0, s, d_params, T);
     if (ev) hipEventRecord(ev[1], s);
-    if (flags & 2u) hipLaunchKernelGGL(k_multiscatter, dim3(P.mw * P.mh), dim3(64), 0, s, P, T, MS);
+    if (flags & 2u) hipLaunchKernelGGL(k_multiscatter, dim3(P.mw * P.mh, ns), dim3(64), 0, s, d_params, T, MS);
     if (ev) hipEventRecord(ev[2], s);
-    if (flags & 4u) hipLaunchKernelGGL(k_skyview, dim3((P.sw * P.sh + 63) / 64), dim3(64), 0, s, P, T, MS, SV);
+    if (flags & 4u)
+        hipLaunchKernelGGL(k_skyview, dim3((P.sw * P.sh + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS, SV);
     if (ev) hipEventRecord(ev[3], s);
     if (flags & 8u)
-        hipLaunchKernelGGL(k_aerial, dim3((P.aw * P.ah * P.ad + 63) / 64), dim3(64), 0, s, P, T, MS, AP);
+        hipLaunchKernelGGL(k_aerial, dim3((P.aw * P.ah * P.ad + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS, AP);
     if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }

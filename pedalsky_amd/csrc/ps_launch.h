@@ -6,12 +6,24 @@
 struct LutParams;
 struct RenderParams;
 
+// LUT chain of n scenes: d_params holds n LutParams (device); `dims` is a
+// host copy of one of them (every scene of a launch has the same LUT sizes);
+// scene s writes T + s*tw*th, MS + s*mw*mh, SV + s*sw*sh, AP + s*aw*ah*ad.
 // flags: bit0 transmittance, bit1 multi-scatter, bit2 sky-view, bit3 AP.
 // ev: optional 5 events recorded around the four passes.
-hipError_t ps_launch_luts(const LutParams& P, uint32_t flags, float4* T, float4* MS, float4* SV, float4* AP,
-                          hipStream_t s, hipEvent_t* ev);
+hipError_t ps_launch_luts(const LutParams* d_params, const LutParams& dims, int n, uint32_t flags, float4* T,
+                          float4* MS, float4* SV, float4* AP, hipStream_t s, hipEvent_t* ev);
 
 hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, hipStream_t s);
+
+// Many scenes in one launch: `tiles` lists (scene << 20 | tile_y << 10 |
+// tile_x) for every wave tile of every scene; a wave tile is tile_w x
+// (64 / tile_w) pixels (tile_w in {4, 8, 16}). persistent != 0: `blocks`
+// one-wave workers pull tiles from the counter `next` (zeroed by this call,
+// with a kernel, so the sequence can be captured in a graph); else one block
+// per tile.
+hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* tiles, int n_tiles, int tile_w,
+                                  int persistent, int blocks, unsigned* next, hipStream_t s);
 
 // Noise: bake the stored volumes (format 0 = UNORM8, 1 = FP16), then fold
 // them into the render layouts.

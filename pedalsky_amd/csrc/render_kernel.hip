@@ -521,14 +521,14 @@ __device__ __forceinline__ bool layer_interval(const LayerC& L, double r_cam, do
     return tb > ta;
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(64) void k_render(const RenderParams* __restrict__ Pp) {
-    __shared__ WaveQueue Qs;
-    const RenderParams& P = *Pp;
+// One wave tile of TW x (64 / TW) pixels: everything from the view ray to
+// the written output pixels. Called by one full wave.
+template <bool COUNT, int TW>
+__device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, int tile_y, WaveQueue& Q) {
+    static_assert(TW == 4 || TW == 8 || TW == 16, "wave tiles are 4x16, 8x8 or 16x4 pixels");
     const int lane = threadIdx.x;
-    WaveQueue& Q = Qs;
-    const int x = blockIdx.x * 8 + (lane & 7);
-    const int y = blockIdx.y * 8 + (lane >> 3);
+    const int x = tile_x * TW + (lane % TW);
+    const int y = tile_y * (64 / TW) + lane / TW;
     // Lanes outside the image stay in the wave (they help shade queued
     // samples) but march nothing and write nothing.
     const bool valid = x < P.W && y < P.H;
@@ -716,9 +716,13 @@ __global__ __launch_bounds__(64) void k_render(const RenderParams* __restrict__ 
                 const uint32_t g = ld_g<uint8_t>(P.type_grid, gy * 128u + gx);
                 if (g != 255u) { ctype = g; nc = ndf_coef(g); }
             }
-            // sample_density (cloud_march.wgsl:432-530)
+            // sample_density (cloud_march.wgsl:432-530). The upper gate has a
+            // 1e-5 tolerance: with blue-noise jitter 255/255 the last sample
+            // lies exactly on the top shell (h = 1 in exact arithmetic), which
+            // the literal `h > 1` test would decide by rounding; only the
+            // cumulonimbus anvil is non-zero there.
             int stage = 0;
-            if (h >= 0.f && h <= 1.f) {
+            if (h >= 0.f && h <= 1.f + 1e-5f) {
                 const float cov = clampf(wr * L.coverage, 0.f, 1.f);
                 if (cov >= 0.01f) {
                     stage = 1;
@@ -871,11 +875,64 @@ __global__ __launch_bounds__(64) void k_render(const RenderParams* __restrict__ 
     }
 }
 
+template <bool COUNT>
+__global__ __launch_bounds__(64) void k_render(const RenderParams* __restrict__ Pp) {
+    __shared__ WaveQueue Qs;
+    render_tile<COUNT, 8>(*Pp, blockIdx.x, blockIdx.y, Qs);
+}
+
+// Tiles of many scenes in one launch. Each entry of `tiles` is
+// scene << 20 | tile_y << 10 | tile_x. Persistent: every one-wave block
+// pulls the next entry from the counter until the list is exhausted (the
+// exit test is wave-uniform and the counter only grows, so every wave
+// leaves). Otherwise block b renders entry b.
+template <int TW, bool PERSISTENT>
+__global__ __launch_bounds__(64) void k_render_tiles(const RenderParams* __restrict__ Ps,
+                                                     const uint32_t* __restrict__ tiles, int n_tiles,
+                                                     unsigned* __restrict__ next) {
+    __shared__ WaveQueue Qs;
+    if (PERSISTENT) {
+        for (;;) {
+            // readfirstlane keeps the tile index, the entry and hence the
+            // scene's parameter block in scalar registers
+            unsigned t = 0;
+            if (threadIdx.x == 0) t = atomicAdd(next, 1u);
+            t = __builtin_amdgcn_readfirstlane(t);
+            if (t >= (unsigned)n_tiles) break;
+            const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[t]);
+            render_tile<false, TW>(Ps[e >> 20], (int)(e & 1023u), (int)((e >> 10) & 1023u), Qs);
+        }
+    } else {
+        const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
+        render_tile<false, TW>(Ps[e >> 20], (int)(e & 1023u), (int)((e >> 10) & 1023u), Qs);
+    }
+}
+
+__global__ void k_zero_counter(unsigned* p) { *p = 0u; }
+
 }  // namespace
 
 hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, hipStream_t s) {
     dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
     if (count) hipLaunchKernelGGL(k_render<true>, grid, block, 0, s, d_params);
     else hipLaunchKernelGGL(k_render<false>, grid, block, 0, s, d_params);
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* tiles, int n_tiles, int tile_w,
+                                  int persistent, int blocks, unsigned* next, hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
+    if (persistent) {
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, next);
+        const dim3 g(blocks), b(64);
+        if (tile_w == 4) hipLaunchKernelGGL((k_render_tiles<4, true>), g, b, 0, s, d_params, tiles, n_tiles, next);
+        else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, true>), g, b, 0, s, d_params, tiles, n_tiles, next);
+        else hipLaunchKernelGGL((k_render_tiles<8, true>), g, b, 0, s, d_params, tiles, n_tiles, next);
+    } else {
+        const dim3 g(n_tiles), b(64);
+        if (tile_w == 4) hipLaunchKernelGGL((k_render_tiles<4, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
+        else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
+        else hipLaunchKernelGGL((k_render_tiles<8, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
+    }
     return hipGetLastError();
 }

@@ -381,3 +381,48 @@ def workload_full_frame(index: int = 0, width: int = 1280, height: int = 720, **
                 noise_format=abi.PS_NOISE_FP16)
     args.update(kw)
     return _workload(f"sky+clouds {width}x{height}, cumulus + cirrus, 128/6 steps, frame {index}", **args)
+
+
+BATCH_RESOLUTIONS = ((256, 144), (640, 360), (1280, 720), (1920, 1080))
+
+
+def workload_batch(n: int = 64, seed: int = 0xC3, steps: int = 128, ground: bool = True) -> list:
+    """`n` uneven scenes for one batched launch, drawn from one PCG32 stream
+    (seed 0xC3): resolution uniform over 256x144 / 640x360 / 1280x720 /
+    1920x1080; 1-3 vertically disjoint layers, each type uniform over the 8
+    with base/top inside the type's altitude band, scene coverage U[0.1, 1]
+    and density_scale U[0.3, 2]; sun zenith U[0, 95] deg (azimuth 180);
+    Rayleigh, Mie scattering, Mie absorption and ozone coefficients each
+    scaled by U[0.5, 1.5]; camera at 500 m with pitch U[-10, 30] and yaw
+    U[0, 360) deg; 128/6 steps, FP16 noise shared by the batch."""
+    rng = Pcg32(seed)
+    u = lambda: float(rng.uniform(1)[0])  # noqa: E731
+    out = []
+    for i in range(n):
+        W, H = BATCH_RESOLUTIONS[min(int(u() * 4), 3)]
+        k = 1 + min(int(u() * 3), 2)
+        layers = []
+        for _ in range(k):
+            t = min(int(u() * 8), 7)
+            lo, hi = TYPE_BANDS_M[t]
+            thick = (0.25 + 0.5 * u()) * (hi - lo)
+            base = lo + u() * (hi - lo - thick)
+            cov, ds = 0.1 + 0.9 * u(), 0.3 + 1.7 * u()
+            cand = synth_layer(t, base, base + thick, cov, density_scale=ds)
+            trial = (abi.CloudLayer * (len(layers) + 1))(*(layers + [cand]))
+            if abi.load().ps_check_layers_overlap(trial, len(layers) + 1, None, None) == 0:
+                layers.append(cand)
+        zen = 95.0 * u()
+        pitch, yaw = -10.0 + 40.0 * u(), 360.0 * u()
+        scale = {key: 0.5 + u() for key in ("rayleigh_scattering", "mie_scattering", "mie_absorption",
+                                             "ozone_absorption")}
+        frame = make_frame(W, H, camera=(0.0, 500.0, 0.0), pitch_deg=pitch, yaw_deg=yaw, sun_zenith_deg=zen)
+        atm = earth_atmosphere(scale=scale)
+        clouds = cloud_params(steps=steps, light_steps=6, n_layers=len(layers))
+        out.append(Workload(
+            name=f"batch scene {i}: {W}x{H}, {len(layers)} layer(s), sun zenith {zen:.1f} deg", frame=frame,
+            atmosphere=atm, clouds=clouds, layers=layers, winds=layer_winds(layers), noise_seed=0,
+            noise_format=abi.PS_NOISE_FP16, lut_dims=lut_dims(32), weather=weather_map(512),
+            overcast=overcast_mask(layers), blue=blue_noise(),
+            depth=ground_depth(frame, atm.planet_radius_m) if ground else None))
+    return out

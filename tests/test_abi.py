@@ -23,6 +23,9 @@ def test_struct_layouts_match_reference():
     assert C.sizeof(abi.CloudLayer) == 48
     for cls in (abi.FrameUniforms, abi.AtmosphereParams, abi.CloudParams, abi.CloudLayer):
         assert C.sizeof(cls) % 16 == 0
+    # batch descriptors (static_assert'ed in batch.cpp)
+    assert C.sizeof(abi.SceneDesc) == 96 and abi.SceneDesc.out.offset == 64
+    assert C.sizeof(abi.Tuning) == 16
 
 
 def test_library_exports_every_header_symbol(lib):
