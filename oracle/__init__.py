@@ -62,6 +62,27 @@ class OrInputs(C.Structure):
     ]
 
 
+class OrDiff(C.Structure):
+    _fields_ = [
+        ("density_scale", C.c_void_p), ("extinction_scale", C.c_void_p), ("coverage", C.c_void_p),
+        ("delta", C.c_void_p), ("n_params", C.c_int32), ("param_kind", C.c_void_p), ("param_index", C.c_void_p),
+    ]
+
+
+# tracked-parameter kinds of the dual VJP (or_diff.param_kind)
+PARAM_DENSITY_SCALE = 0
+PARAM_EXTINCTION_SCALE = 1
+PARAM_COVERAGE = 2
+PARAM_DELTA = 3
+DUAL_N = 16
+DELTA_N = 128
+
+
+def delta_index(i: int, j: int, k: int) -> int:
+    """Flat index of density-perturbation voxel (x = i, h = j, z = k)."""
+    return int(i) + DELTA_N * (int(j) + DELTA_N * int(k))
+
+
 def load():
     global _lib
     if _lib is not None:
@@ -83,6 +104,8 @@ def load():
         "or_render": (None, [P, i, i, i, i, P, P, P, P, i]),
         "orf_render": (None, [P, i, i, i, i, P, P, P, P, i]),
         "or_render_pixels": (None, [P, P, i, P, P, P, i]),
+        "or_render_ext": (None, [P, P, i, i, i, i, P, P, P, P, i]),
+        "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i]),
         "or_noise_base": (None, [C.c_uint32, i, i, P, i]),
         "or_noise_detail": (None, [C.c_uint32, P, i]),
         "or_noise_curl": (None, [C.c_uint32, P, i]),
@@ -251,6 +274,85 @@ class Frame:
         trn = np.zeros((n, 4))
         load().or_render_pixels(C.byref(self.inputs), _p(xy), n, _p(comp), _p(lum), _p(trn), THREADS)
         return comp, lum, trn
+
+
+class Diff:
+    """FP64 differentiation extensions of one render: per-layer
+    density_scale / extinction_scale (kappa) / coverage overrides (None keeps
+    the layer struct's value, kappa 1) and the 128^3 density perturbation
+    (None = 0), stored (z, h, x) like the device volume."""
+
+    def __init__(self, n_layers, density_scale=None, extinction_scale=None, coverage=None, delta=None):
+        def arr(x):
+            return None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        self.n_layers = n_layers
+        self.density_scale = arr(density_scale)
+        self.extinction_scale = arr(extinction_scale)
+        self.coverage = arr(coverage)
+        self.delta = None if delta is None else np.ascontiguousarray(delta, dtype=np.float64).reshape(-1)
+
+    def copy(self) -> "Diff":
+        return Diff(self.n_layers, *(None if a is None else a.copy() for a in
+                                     (self.density_scale, self.extinction_scale, self.coverage, self.delta)))
+
+    def with_layers(self, layers) -> "Diff":
+        """Fill unset per-layer arrays from the layer structs (kappa = 1)."""
+        d = self.copy()
+        if d.density_scale is None:
+            d.density_scale = np.array([L.density_scale for L in layers], dtype=np.float64)
+        if d.coverage is None:
+            d.coverage = np.array([L.coverage for L in layers], dtype=np.float64)
+        if d.extinction_scale is None:
+            d.extinction_scale = np.ones(len(layers), dtype=np.float64)
+        return d
+
+    def perturbed(self, kind: int, index: int, h: float) -> "Diff":
+        d = self.copy()
+        arr = {PARAM_DENSITY_SCALE: d.density_scale, PARAM_EXTINCTION_SCALE: d.extinction_scale,
+               PARAM_COVERAGE: d.coverage, PARAM_DELTA: d.delta}[kind]
+        arr[index] += h
+        return d
+
+    def struct(self, params=()):
+        kinds = np.ascontiguousarray([k for k, _ in params], dtype=np.int32)
+        idx = np.ascontiguousarray([i for _, i in params], dtype=np.int32)
+        self._keep = (kinds, idx)
+        return OrDiff(_p(self.density_scale), _p(self.extinction_scale), _p(self.coverage), _p(self.delta),
+                      len(params), _p(kinds) if len(params) else None, _p(idx) if len(params) else None)
+
+
+def _frame_render_ext(self, diff: Diff, x0, y0, w, h, counters=False):
+    comp = np.zeros((h, w, 4))
+    lum = np.zeros((h, w, 4))
+    trn = np.zeros((h, w, 4))
+    cnt = np.zeros(10, dtype=np.uint64) if counters else None
+    st = diff.struct()
+    load().or_render_ext(C.byref(self.inputs), C.byref(st), x0, y0, w, h, _p(comp), _p(lum), _p(trn), _p(cnt),
+                         THREADS)
+    return comp, lum, trn, cnt
+
+
+def _frame_vjp_clouds(self, diff: Diff, params, x0, y0, w, h, g_comp=None, g_lum=None, g_trn=None):
+    """Exact FP64 vector-Jacobian product (dual instantiation) for a list of
+    (kind, index) parameters; cotangent images are (h, w, 4) or None."""
+    def cot(g):
+        return None if g is None else np.ascontiguousarray(g, dtype=np.float64).reshape(h, w, 4)
+    gc, gl, gt = cot(g_comp), cot(g_lum), cot(g_trn)
+    out = np.zeros(len(params))
+    for s0 in range(0, len(params), DUAL_N):
+        chunk = list(params[s0:s0 + DUAL_N])
+        st = diff.struct(chunk)
+        g = np.zeros(len(chunk))
+        rc = load().or_vjp_clouds(C.byref(self.inputs), C.byref(st), x0, y0, w, h, _p(gc), _p(gl), _p(gt),
+                                  _p(g), THREADS)
+        if rc != 0:
+            raise ValueError("or_vjp_clouds rejected its arguments")
+        out[s0:s0 + len(chunk)] = g
+    return out
+
+
+Frame.render_ext = _frame_render_ext
+Frame.vjp_clouds = _frame_vjp_clouds
 
 
 def render_f32_frame(workload, base, detail, curl, nthreads=0):

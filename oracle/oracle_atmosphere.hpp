@@ -80,10 +80,10 @@ inline void mat4_mul(const R m[16], const R v[4], R out[4]) {
 // ---- atmosphere.wgsl:23-61 density profiles ------------------------------
 template <class R>
 inline V3<R> atmosphere_density(const Atm<R>& a, R h) {
-    R r = std::exp(-h / a.hr);
-    R m = std::exp(-h / a.hm);
+    R r = exp(-h / a.hr);
+    R m = exp(-h / a.hm);
     R half_thickness = a.oz_t * R(0.5);
-    R dz = std::fabs(h - a.oz_c);
+    R dz = fabs(h - a.oz_c);
     R o = std::max(R(0), R(1) - dz / std::max(half_thickness, R(1)));
     return V3<R>(r, m, o);
 }
@@ -111,7 +111,7 @@ template <class R>
 inline R phase_mie(R c, R g) {
     R g2 = g * g;
     R denom = R(1) + g2 - R(2) * g * c;
-    return (R(1) - g2) / (R(4) * R(kPI) * std::pow(std::max(denom, R(1e-4)), R(1.5)));
+    return (R(1) - g2) / (R(4) * R(kPI) * pow(std::max(denom, R(1e-4)), R(1.5)));
 }
 
 // ---- atmosphere.wgsl:95-138 geometry ---------------------------------------
@@ -121,7 +121,7 @@ inline bool ray_sphere_intersect_origin(const V3<R>& ro, const V3<R>& rd, R r, R
     R c = dot(ro, ro) - r * r;
     R disc = b * b - c;
     if (disc < R(0)) return false;
-    R sq = std::sqrt(disc);
+    R sq = sqrt(disc);
     t0 = -b - sq; t1 = -b + sq;
     return true;
 }
@@ -150,8 +150,8 @@ inline V3<R> world_to_atmosphere_pos(const Atm<R>& a, const V3<R>& pw) {
 template <class R>
 inline void transmittance_lut_uv(const Atm<R>& a, const V3<R>& p, const V3<R>& dir, R& u, R& v) {
     R r = length(p);
-    R H = std::sqrt(std::max(a.top * a.top - a.planet_r * a.planet_r, R(0)));
-    R rho = std::sqrt(std::max(r * r - a.planet_r * a.planet_r, R(0)));
+    R H = sqrt(std::max(a.top * a.top - a.planet_r * a.planet_r, R(0)));
+    R rho = sqrt(std::max(r * r - a.planet_r * a.planet_r, R(0)));
     R d = std::max(distance_to_top_atmosphere(a, p, dir), R(0));
     R d_min = a.top - r;
     R d_max = rho + H;
@@ -160,16 +160,16 @@ inline void transmittance_lut_uv(const Atm<R>& a, const V3<R>& p, const V3<R>& d
 }
 template <class R>
 inline void transmittance_lut_uv_to_pos_dir(const Atm<R>& a, R u, R v, V3<R>& p, V3<R>& dir) {
-    R H = std::sqrt(std::max(a.top * a.top - a.planet_r * a.planet_r, R(0)));
+    R H = sqrt(std::max(a.top * a.top - a.planet_r * a.planet_r, R(0)));
     R rho = v * H;
-    R r = std::sqrt(rho * rho + a.planet_r * a.planet_r);
+    R r = sqrt(rho * rho + a.planet_r * a.planet_r);
     R d_min = a.top - r;
     R d_max = rho + H;
     R d = d_min + u * (d_max - d_min);
     R mu = R(1);
     if (d != R(0)) mu = clampr<R>((H * H - rho * rho - d * d) / std::max(R(2) * r * d, R(1e-6)), R(-1), R(1));
     p = V3<R>(R(0), r, R(0));
-    R sin_t = std::sqrt(std::max(R(1) - mu * mu, R(0)));
+    R sin_t = sqrt(std::max(R(1) - mu * mu, R(0)));
     dir = V3<R>(sin_t, mu, R(0));
 }
 
@@ -229,9 +229,9 @@ template <class R>
 inline V3<R> sphere_sample(int i, int n_dirs) {
     R golden = R(0.6180339887498949);
     R z = R(1) - R(2) * (R(i) + R(0.5)) / R(n_dirs);
-    R r = std::sqrt(std::max(R(1) - z * z, R(0)));
+    R r = sqrt(std::max(R(1) - z * z, R(0)));
     R phi = R(2) * R(kPI) * fractr<R>(R(i) * golden);
-    return V3<R>(r * std::cos(phi), r * std::sin(phi), z);
+    return V3<R>(r * cos(phi), r * sin(phi), z);
 }
 
 // ---- multiscatter.comp.wgsl:67-156 -------------------------------------------
@@ -239,7 +239,7 @@ template <class R, class S>
 void bake_multiscatter_texel(const Atm<R>& a, const Tex<R, S>& T, R u, R v, R out[4]) {
     const int N_DIRS = 64, N_STEPS = 20;
     R sun_cos = clampr<R>(u * R(2) - R(1), R(-1), R(1));
-    R sun_sin = std::sqrt(std::max(R(1) - sun_cos * sun_cos, R(0)));
+    R sun_sin = sqrt(std::max(R(1) - sun_cos * sun_cos, R(0)));
     R h_norm = v;
     R r = a.planet_r + h_norm * (a.top - a.planet_r);
     V3<R> p0(R(0), r, R(0));
@@ -266,7 +266,7 @@ void bake_multiscatter_texel(const Atm<R>& a, const Tex<R, S>& T, R u, R v, R ou
             R cos_sun = dot(p, sun_dir) / std::max(r_p, R(1));
             V3<R> sun_vis(R(0));
             R ratio = a.planet_r / r_p;
-            if (cos_sun > -std::sqrt(std::max(R(1) - ratio * ratio, R(0))))
+            if (cos_sun > -sqrt(std::max(R(1) - ratio * ratio, R(0))))
                 sun_vis = sample_transmittance_lut(a, T, p, sun_dir);
             V3<R> in_scatter = sigma_s * sun_vis;
             V3<R> safe = vmax(sigma_t, V3<R>(R(1e-7)));
@@ -307,7 +307,7 @@ void bake_skyview_texel(const Frame<R>& f, const Atm<R>& a, const Tex<R, S>& T, 
         r = a.planet_r + R(1);
     }
     R sin_horizon = clampr<R>(a.planet_r / r, R(0), R(1));
-    R zha = R(kPI) - std::asin(sin_horizon);
+    R zha = R(kPI) - asin(sin_horizon);
     R vza;
     if (v < R(0.5)) {
         R coord = v * R(2);
@@ -318,10 +318,10 @@ void bake_skyview_texel(const Frame<R>& f, const Atm<R>& a, const Tex<R, S>& T, 
         vza = zha + coord * coord * below;
     }
     V3<R> sun_dir = f.sun;
-    R sun_az = std::atan2(sun_dir.x, sun_dir.z);
+    R sun_az = atan2(sun_dir.x, sun_dir.z);
     R azimuth = sun_az + u * R(2) * R(kPI);
-    R cos_v = std::cos(vza), sin_v = std::sin(vza);
-    V3<R> view_dir(sin_v * std::sin(azimuth), cos_v, sin_v * std::cos(azimuth));
+    R cos_v = cos(vza), sin_v = sin(vza);
+    V3<R> view_dir(sin_v * sin(azimuth), cos_v, sin_v * cos(azimuth));
     R cos_theta = dot(view_dir, sun_dir);
     R phase_r = phase_rayleigh(cos_theta);
     R phase_m = phase_mie(cos_theta, a.mie_g);
@@ -390,7 +390,7 @@ void bake_ap_froxel(const Frame<R>& f, const Atm<R>& a, const Tex<R, S>& T, cons
     V3<R> near_p(nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]);
     V3<R> view_dir = normalize(near_p - f.cam);
     R z_norm = (R(gz) + R(0.5)) / R(D);
-    R t_target = R(kApNear) * std::pow(R(kApFar) / R(kApNear), z_norm);
+    R t_target = R(kApNear) * pow(R(kApFar) / R(kApNear), z_norm);
     V3<R> p0 = world_to_atmosphere_pos(a, f.cam);
     R r0 = length(p0);
     R cos_view = dot(p0 / std::max(r0, R(1)), view_dir);

@@ -33,6 +33,22 @@ typedef struct or_inputs {
     const float* depth;          // W*H or null
 } or_inputs;
 
+// Differentiation extensions and parameter overrides in double precision
+// (FP64 finite differences need perturbations finer than the float layer
+// structs hold). Null arrays keep the inputs' values (kappa 1, delta 0).
+// The tracked parameters seed the tangents of the dual instantiation:
+// kind 0 density_scale[index], 1 extinction_scale[index], 2 coverage[index],
+// 3 delta voxel `index` (x + 128 (h + 128 z)).
+typedef struct or_diff {
+    const double* density_scale;     // per layer or null
+    const double* extinction_scale;  // per layer or null
+    const double* coverage;          // per layer or null
+    const double* delta;             // 128^3 or null
+    int32_t n_params;
+    const int32_t* param_kind;
+    const int32_t* param_index;
+} or_diff;
+
 }  // extern "C"
 
 namespace {
@@ -49,9 +65,10 @@ Scene<R> make_scene(const or_inputs* in) {
         if (in->layer_wind_xz) s.layer_wind[i] = V3<R>(in->layer_wind_xz[2 * i], 0, in->layer_wind_xz[2 * i + 1]);
         else s.layer_wind[i] = V3<R>(0);
     }
-    s.T.data = static_cast<const R*>(in->T); s.T.w = in->tw; s.T.h = in->th; s.T.ch = 4;
-    s.SV.data = static_cast<const R*>(in->SV); s.SV.w = in->sw; s.SV.h = in->sh; s.SV.ch = 4;
-    s.AP.data = static_cast<const R*>(in->AP); s.AP.w = in->aw; s.AP.h = in->ah; s.AP.d = in->ad; s.AP.ch = 4;
+    typedef typename LutStore<R>::type L;
+    s.T.data = static_cast<const L*>(in->T); s.T.w = in->tw; s.T.h = in->th; s.T.ch = 4;
+    s.SV.data = static_cast<const L*>(in->SV); s.SV.w = in->sw; s.SV.h = in->sh; s.SV.ch = 4;
+    s.AP.data = static_cast<const L*>(in->AP); s.AP.w = in->aw; s.AP.h = in->ah; s.AP.d = in->ad; s.AP.ch = 4;
     s.base.data = in->base; s.base.w = s.base.h = s.base.d = 128; s.base.ch = 4;
     s.detail.data = in->detail; s.detail.w = s.detail.h = s.detail.d = 32; s.detail.ch = 4;
     s.curl.data = in->curl; s.curl.w = s.curl.h = 128; s.curl.ch = 2;
@@ -60,13 +77,46 @@ Scene<R> make_scene(const or_inputs* in) {
     s.overcast = in->overcast;
     s.blue = in->blue;
     s.depth = in->depth;
+    for (int i = 0; i < PS_MAX_CLOUD_LAYERS; ++i) s.kappa[i] = R(1);
+    s.delta = nullptr;
+    s.delta_track = s.delta_track_slot = nullptr;
+    s.n_delta_track = 0;
     return s;
+}
+
+// Overrides and tangent seeds of `d` on a scene built by make_scene.
+template <class R>
+void apply_diff(Scene<R>& s, const or_diff* d, std::vector<int>& track, std::vector<int>& slot) {
+    if (!d) return;
+    for (int l = 0; l < s.n_layers; ++l) {
+        if (d->density_scale) s.layers[l].density_scale = R(d->density_scale[l]);
+        if (d->extinction_scale) s.kappa[l] = R(d->extinction_scale[l]);
+        if (d->coverage) s.layers[l].coverage = R(d->coverage[l]);
+    }
+    s.delta = d->delta;
+    if constexpr (is_dual<R>::value) {
+        for (int k = 0; k < d->n_params; ++k) {
+            const int kind = d->param_kind[k], idx = d->param_index[k];
+            if (kind == 0) s.layers[idx].density_scale.d[k] = 1.0;
+            else if (kind == 1) s.kappa[idx].d[k] = 1.0;
+            else if (kind == 2) s.layers[idx].coverage.d[k] = 1.0;
+            else {
+                track.push_back(idx);
+                slot.push_back(k);
+            }
+        }
+        s.delta_track = track.data();
+        s.delta_track_slot = slot.data();
+        s.n_delta_track = (int)track.size();
+    }
 }
 
 template <class R>
 void render_region(const or_inputs* in, int x0, int y0, int w, int h, R* comp, R* lum, R* trn,
-                   uint64_t* counters, int nthreads) {
+                   uint64_t* counters, int nthreads, const or_diff* diff = nullptr) {
     Scene<R> s = make_scene<R>(in);
+    std::vector<int> track, slot;
+    apply_diff(s, diff, track, slot);
     const int n = w * h;
     Counters total;
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -225,6 +275,56 @@ void or_render_pixels(const or_inputs* in, const int32_t* xy, int n, double* com
             if (trn) trn[size_t(i) * 4 + k] = t4[k];
         }
     }
+}
+
+// ---- differentiation (extensions + exact derivatives) ------------------------------------
+// The FP64 render with the differentiation extensions and overrides of `d`.
+void or_render_ext(const or_inputs* in, const or_diff* d, int x0, int y0, int w, int h, double* comp,
+                   double* lum, double* trn, uint64_t* counters, int nthreads) {
+    render_region<double>(in, x0, y0, w, h, comp, lum, trn, counters, nthreads, d);
+}
+
+constexpr int kDualN = 16;
+
+// Vector-Jacobian product of the w x h region at (x0, y0): grad[k] = sum over
+// pixels and RGB of g_comp * d comp / d theta_k + g_lum * d lum / d theta_k +
+// g_trn * d trn / d theta_k for the (at most 16) tracked parameters of `d`,
+// computed with the Dual<16> instantiation of the FP64 restatement. The
+// cotangent images are w*h RGBA (alpha ignored), any of them may be null.
+int or_vjp_clouds(const or_inputs* in, const or_diff* d, int x0, int y0, int w, int h, const double* g_comp,
+                  const double* g_lum, const double* g_trn, double* grad, int nthreads) {
+    typedef Dual<kDualN> D;
+    if (!d || d->n_params < 0 || d->n_params > kDualN) return 1;
+    Scene<D> s = make_scene<D>(in);
+    std::vector<int> track, slot;
+    apply_diff(s, d, track, slot);
+    const int n = w * h, np = d->n_params;
+    std::vector<double> total(np, 0.0);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+    {
+        std::vector<double> part(np, 0.0);
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 0; i < n; ++i) {
+            int ix = x0 + i % w, iy = y0 + i / w;
+            D c4[4], l4[4], t4[4];
+            render_pixel(s, ix, iy, c4, l4, t4, (Counters*)nullptr);
+            for (int c = 0; c < 3; ++c) {
+                const size_t o = size_t(i) * 4 + c;
+                for (int k = 0; k < np; ++k) {
+                    double v = 0.0;
+                    if (g_comp) v += g_comp[o] * c4[c].d[k];
+                    if (g_lum) v += g_lum[o] * l4[c].d[k];
+                    if (g_trn) v += g_trn[o] * t4[c].d[k];
+                    part[k] += v;
+                }
+            }
+        }
+#pragma omp critical
+        for (int k = 0; k < np; ++k) total[k] += part[k];
+    }
+    for (int k = 0; k < np; ++k) grad[k] = total[k];
+    return 0;
 }
 
 // ---- noise bakes (decoded float values before storage quantisation) ---------------------

@@ -4,6 +4,7 @@
 // cloud_composite.wgsl), templated on the scalar type.
 #pragma once
 #include "oracle_atmosphere.hpp"
+#include "oracle_dual.hpp"
 
 namespace oracle {
 
@@ -63,7 +64,17 @@ struct Scene {
     Layer<R> layers[PS_MAX_CLOUD_LAYERS];
     V3<R> layer_wind[PS_MAX_CLOUD_LAYERS];  // (u, 0, v) wind m/s per layer (x, z used)
     int n_layers;
-    Tex<R, R> T, SV, AP;                   // LUTs (RGBA)
+    // Forward-identity extensions for differentiation (absent from the
+    // reference; the neutral values reproduce it exactly): a per-layer
+    // extinction multiplier on sigma_t, and an additive 128^3 density
+    // perturbation (see delta_sample).
+    R kappa[PS_MAX_CLOUD_LAYERS];
+    const double* delta;                   // 128^3 or null (zero)
+    // dual instantiation: delta voxels whose value seeds tangent slot k
+    const int* delta_track;
+    const int* delta_track_slot;
+    int n_delta_track;
+    Tex<R, typename LutStore<R>::type> T, SV, AP;  // LUTs (RGBA)
     Tex<R, float> base, detail, curl, weather;
     const uint8_t* type_grid;              // 128x128 or null (all sentinel)
     const uint8_t* overcast;               // 128x128 R8Unorm or null (zero)
@@ -118,7 +129,7 @@ inline Wind<R> layer_wind_cache(const Scene<R>& s, int li) {
     w.off_x = wx * drift; w.off_z = wz * drift;
     w.skew_x = 0; w.skew_z = 0; w.skew_thickness = 0;
     if (s.c.skew != R(0)) {
-        R speed = std::sqrt(wx * wx + wz * wz);
+        R speed = sqrt(wx * wx + wz * wz);
         if (speed >= R(0.1)) {
             w.skew_x = wx / speed; w.skew_z = wz / speed;
             w.skew_thickness = std::max(L.top - L.base, R(1)) * s.c.skew;
@@ -143,10 +154,52 @@ inline uint32_t effective_cloud_type(const Scene<R>& s, R px, R pz, const Layer<
     if (!s.type_grid) return L.type;
     R half = R(16000);
     R u = (px + half) / R(32000), v = (pz + half) / R(32000);
-    int tx_raw = int(std::floor(u * R(128))), ty_raw = int(std::floor(v * R(128)));
+    int tx_raw = int(floor(u * R(128))), ty_raw = int(floor(v * R(128)));
     int tx = ((tx_raw % 128) + 128) % 128, ty = ((ty_raw % 128) + 128) % 128;
     uint32_t t = s.type_grid[ty * 128 + tx];
     return t == 255u ? L.type : t;
+}
+
+// ---- density perturbation (extension; no reference counterpart) ------------------------
+// delta is a 128^3 volume over the weather tile horizontally and the layer's
+// normalised height vertically: texel (i, j, k) covers x = i, h = j, z = k
+// (x fastest), sampled trilinearly at u = wx / weather_scale + 0.5 (Repeat),
+// h (ClampToEdge), v = wz / weather_scale + 0.5 (Repeat), where (wx, wz) is
+// the drift-offset position the weather map is read at. It is added to the
+// shaped cloud value just before the final clamp, in both the primary and
+// the light-march density, past the altitude and coverage gates.
+constexpr int kDeltaN = 128;
+
+template <class R>
+inline R delta_texel(const Scene<R>& s, int idx) {
+    R v = R(s.delta[idx]);
+    if constexpr (is_dual<R>::value) {
+        for (int k = 0; k < s.n_delta_track; ++k)
+            if (s.delta_track[k] == idx) v.d[s.delta_track_slot[k]] = 1.0;
+    }
+    return v;
+}
+
+template <class R>
+inline R delta_sample(const Scene<R>& s, R wx, R wz, R h) {
+    if (!s.delta) return R(0);
+    R wsc = std::max(s.c.weather_scale, R(1));
+    R cx = (wx / wsc + R(0.5)) * R(kDeltaN) - R(0.5);
+    R cy = h * R(kDeltaN) - R(0.5);
+    R cz = (wz / wsc + R(0.5)) * R(kDeltaN) - R(0.5);
+    R fx0 = floor(cx), fy0 = floor(cy), fz0 = floor(cz);
+    R fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
+    int ix = int(fx0), iy = int(fy0), iz = int(fz0);
+    int x0 = wrap_index(ix, kDeltaN), x1 = wrap_index(ix + 1, kDeltaN);
+    int y0 = clamp_index(iy, kDeltaN), y1 = clamp_index(iy + 1, kDeltaN);
+    int z0 = wrap_index(iz, kDeltaN), z1 = wrap_index(iz + 1, kDeltaN);
+    auto at = [&](int x, int y, int z) { return delta_texel(s, (z * kDeltaN + y) * kDeltaN + x); };
+    R a0 = at(x0, y0, z0) + (at(x1, y0, z0) - at(x0, y0, z0)) * fx;
+    R a1 = at(x0, y1, z0) + (at(x1, y1, z0) - at(x0, y1, z0)) * fx;
+    R b0 = at(x0, y0, z1) + (at(x1, y0, z1) - at(x0, y0, z1)) * fx;
+    R b1 = at(x0, y1, z1) + (at(x1, y1, z1) - at(x0, y1, z1)) * fx;
+    R p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
+    return p0 + (p1 - p0) * fz;
 }
 
 // ---- cloud_density_common.wgsl:376-415 ------------------------------------------------
@@ -172,6 +225,7 @@ inline R sample_density_light(const Scene<R>& s, const V3<R>& p, R p_alt, const 
     R cov = eff;
     cloud = remap<R>(cloud, R(1) - cov, 1, 0, 1);
     cloud = cloud * cov;
+    cloud = cloud + delta_sample(s, p.x - w.off_x, p.z - w.off_z, h);
     return clampr<R>(cloud, 0, 1) * L.density_scale;
 }
 
@@ -223,6 +277,7 @@ inline R sample_density(const Scene<R>& s, const V3<R>& p, R p_alt, const Layer<
         R ld = clampr<R>(s.c.detail_strength + edb, 0, 1);
         cloud = remap<R>(cloud, dmod * ld, 1, 0, 1);
     }
+    cloud = cloud + delta_sample(s, p.x - w.off_x, p.z - w.off_z, h);
     return clampr<R>(cloud, 0, 1) * L.density_scale;
 }
 
@@ -303,14 +358,14 @@ template <class R>
 inline R henyey_greenstein(R c, R g) {
     R g2 = g * g;
     R denom = R(1) + g2 - R(2) * g * c;
-    return (R(1) - g2) / (R(4) * R(kPI) * std::pow(std::max(denom, R(1e-4)), R(1.5)));
+    return (R(1) - g2) / (R(4) * R(kPI) * pow(std::max(denom, R(1e-4)), R(1.5)));
 }
 template <class R>
 inline R draine_phase(R c, R g, R alpha) {
     R g2 = g * g;
     R denom = R(1) + g2 - R(2) * g * c;
     R num = (R(1) - g2) * (R(1) + alpha * c * c);
-    R den = R(4) * (R(1) + alpha * (R(1) + R(2) * g2) / R(3)) * R(kPI) * std::pow(std::max(denom, R(1e-4)), R(1.5));
+    R den = R(4) * (R(1) + alpha * (R(1) + R(2) * g2) / R(3)) * R(kPI) * pow(std::max(denom, R(1e-4)), R(1.5));
     return num / std::max(den, R(1e-6));
 }
 
@@ -320,9 +375,9 @@ struct PhaseConsts { R g_hg, g_d, alpha, w_d, ice; };
 
 template <class R>
 inline R ice_halo_lobe(R c, R peak_cos, R width_rad) {
-    R sigma_cos = width_rad * std::sqrt(std::max(R(1) - peak_cos * peak_cos, R(1e-4)));
+    R sigma_cos = width_rad * sqrt(std::max(R(1) - peak_cos * peak_cos, R(1e-4)));
     R dx = (c - peak_cos) / std::max(sigma_cos, R(1e-4));
-    return std::exp(R(-0.5) * dx * dx);
+    return exp(R(-0.5) * dx * dx);
 }
 template <class R>
 inline PhaseConsts<R> cloud_phase_constants(const Scene<R>& s, const Layer<R>& L, R h_norm) {
@@ -333,10 +388,10 @@ inline PhaseConsts<R> cloud_phase_constants(const Scene<R>& s, const Layer<R>& L
     }
     d = clampr<R>(d * s.c.droplet_bias, R(5), R(50));
     PhaseConsts<R> c;
-    c.g_hg = std::exp(R(-0.0990567) / (d - R(1.67154)));
-    c.g_d = std::exp(R(-2.20679) / (d + R(3.91029))) - R(0.428934);
-    c.alpha = std::exp(R(3.62489) - R(8.29288) / (d + R(5.52825)));
-    c.w_d = std::exp(R(-0.599085) / (d - R(0.641583)) - R(0.665888));
+    c.g_hg = exp(R(-0.0990567) / (d - R(1.67154)));
+    c.g_d = exp(R(-2.20679) / (d + R(3.91029))) - R(0.428934);
+    c.alpha = exp(R(3.62489) - R(8.29288) / (d + R(5.52825)));
+    c.w_d = exp(R(-0.599085) / (d - R(0.641583)) - R(0.665888));
     c.ice = smoothstep<R>(R(35), R(50), d);
     return c;
 }
@@ -364,13 +419,13 @@ inline V3<R> sun_transmittance_at(const Scene<R>& s, const V3<R>& p) {
     V3<R> up = pa / r;
     R mu = clampr<R>(dot(up, sun), -1, 1);
     R ratio = s.a.planet_r / r;
-    R sin_h = -std::sqrt(std::max(R(1) - ratio * ratio, R(0)));
+    R sin_h = -sqrt(std::max(R(1) - ratio * ratio, R(0)));
     if (mu <= sin_h) return V3<R>(0);
     R top = s.a.top, Rp = s.a.planet_r;
-    R big_h = std::sqrt(std::max(top * top - Rp * Rp, R(0)));
-    R rho = std::sqrt(std::max(r * r - Rp * Rp, R(0)));
+    R big_h = sqrt(std::max(top * top - Rp * Rp, R(0)));
+    R rho = sqrt(std::max(r * r - Rp * Rp, R(0)));
     R disc = r * r * (mu * mu - R(1)) + top * top;
-    R d = std::max(-r * mu + std::sqrt(std::max(disc, R(0))), R(0));
+    R d = std::max(-r * mu + sqrt(std::max(disc, R(0))), R(0));
     R d_min = top - r, d_max = rho + big_h;
     R xm = clampr<R>((d - d_min) / std::max(d_max - d_min, R(1)), 0, 1);
     R xr = rho / std::max(big_h, R(1));
@@ -383,21 +438,21 @@ inline V3<R> sun_transmittance_at(const Scene<R>& s, const V3<R>& p) {
 template <class R>
 inline void skyview_uv(const Atm<R>& a, const V3<R>& view_dir, const V3<R>& sun, R cam_r, R& u, R& v) {
     R cos_v = clampr<R>(view_dir.y, -1, 1);
-    R vza = std::acos(cos_v);
+    R vza = acos(cos_v);
     R sin_h = clampr<R>(a.planet_r / cam_r, 0, 1);
-    R zha = R(kPI) - std::asin(sin_h);
+    R zha = R(kPI) - asin(sin_h);
     if (vza < zha) {
-        R coord = std::sqrt(clampr<R>(vza / std::max(zha, R(1e-6)), 0, 1));
+        R coord = sqrt(clampr<R>(vza / std::max(zha, R(1e-6)), 0, 1));
         v = coord * R(0.5);
     } else {
         R below = std::max(R(kPI) - zha, R(1e-6));
-        R coord = std::sqrt(clampr<R>((vza - zha) / below, 0, 1));
+        R coord = sqrt(clampr<R>((vza - zha) / below, 0, 1));
         v = R(0.5) + coord * R(0.5);
     }
-    R view_az = std::atan2(view_dir.x, view_dir.z);
-    R sun_az = std::atan2(sun.x, sun.z);
+    R view_az = atan2(view_dir.x, view_dir.z);
+    R sun_az = atan2(sun.x, sun.z);
     R du = (view_az - sun_az) / (R(2) * R(kPI));
-    u = du - std::floor(du);
+    u = du - floor(du);
 }
 
 template <class R>
@@ -420,7 +475,7 @@ inline bool ray_sphere_intersect(const V3<R>& ro, const V3<R>& rd, const V3<R>& 
     R cc = dot(oc, oc) - radius * radius;
     R disc = b * b - cc;
     if (disc < R(0)) return false;
-    R sq = std::sqrt(disc);
+    R sq = sqrt(disc);
     t0 = -b - sq; t1 = -b + sq;
     return true;
 }
@@ -471,12 +526,12 @@ inline R sample_overcast_at_view(const Scene<R>& s, const V3<R>& vd) {
         sample2d(m, u, v, false, o);
         mask = o[0] / R(255);
     }
-    return R(1) - std::exp(R(-10) * mask);
+    return R(1) - exp(R(-10) * mask);
 }
 template <class R>
 inline V3<R> overcast_diffuse_luminance(const Scene<R>& s) {
     R hf = smoothstep<R>(R(-0.09), R(0.05), s.f.sun.y);
-    R saf = std::sqrt(clampr<R>(s.f.sun.y, R(0.05), R(1))) * hf;
+    R saf = sqrt(clampr<R>(s.f.sun.y, R(0.05), R(1))) * hf;
     R irr = s.f.toa * saf * R(0.25);
     return V3<R>(irr);
 }
@@ -501,12 +556,12 @@ inline V3<R> sky_pass_pixel(const Scene<R>& s, R ndc_x, R ndc_y) {
         R cb = sample_overcast_at_view(s, vd);
         lum = mixv(clear, overcast_diffuse_luminance(s), cb);
     }
-    R cos_disk = std::cos(s.f.sun_ang);
+    R cos_disk = cos(s.f.sun_ang);
     R cvs = dot(vd, sun);
     if (cvs > cos_disk) {
-        R theta = std::acos(clampr<R>(cvs, -1, 1));
+        R theta = acos(clampr<R>(cvs, -1, 1));
         R tn = theta / std::max(s.f.sun_ang, R(1e-6));
-        R limb = R(1) - R(0.6) * (R(1) - std::sqrt(std::max(R(1) - tn * tn, R(0))));
+        R limb = R(1) - R(0.6) * (R(1) - sqrt(std::max(R(1) - tn * tn, R(0))));
         R omega = R(kPI) * s.f.sun_ang * s.f.sun_ang;
         V3<R> ld = s.f.E * limb / std::max(omega, R(1e-6));
         V3<R> svis = sample_transmittance_lut(s.a, s.T, cam_atm, sun);
@@ -539,7 +594,7 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
 
     R depth_t_max = R(1e30);
     if (s.depth) {
-        R dn = s.depth[size_t(iy) * size_t(s.f.vw) + ix];
+        R dn = s.depth[size_t(iy) * size_t(value_of(s.f.vw)) + ix];
         if (dn > R(0)) {
             R dv[4] = {ndc_x, ndc_y, dn, R(1)}, wh[4];
             mat4_mul(s.f.inv_vp, dv, wh);
@@ -581,7 +636,7 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
         const Hit& lh = hits[i];
         if (lh.hit == 0) continue;
         const Layer<R>& L = s.layers[lh.idx];
-        V3<R> sigma_t = sigma_t_base * chromatic_mie_modulation(L.droplet);
+        V3<R> sigma_t = sigma_t_base * chromatic_mie_modulation(L.droplet) * s.kappa[lh.idx];
         R segment = lh.t1 - lh.t0;
         if (segment <= R(0)) continue;
         if (cnt) cnt->layer_marches++;
@@ -626,7 +681,7 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
                 R lr = s.f.light_radius;
                 if (lr > R(0)) {
                     R dx = p.x - s.f.light_origin.x, dz = p.z - s.f.light_origin.z;
-                    R d = std::sqrt(dx * dx + dz * dz);
+                    R d = sqrt(dx * dx + dz * dz);
                     R fo = clampr<R>(R(1) - d / lr, 0, 1);
                     R wgt = fo * fo * (R(3) - R(2) * fo);
                     lightning = s.f.light_rgb * wgt;
@@ -654,7 +709,7 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
     if (opacity > R(1e-4)) {
         if (cnt) cnt->ap_taps++;
         R d_safe = std::max(t_cloud, R(50));
-        R ap_z = clampr<R>(std::log(d_safe / R(50)) / std::log(R(100000) / R(50)), 0, 1);
+        R ap_z = clampr<R>(log(d_safe / R(50)) / log(R(100000) / R(50)), 0, 1);
         R ap[4];
         sample3d(s.AP, fx / s.f.vw, fy / s.f.vh, ap_z, false, ap);
         lo = lum * ap[3] + V3<R>(ap[0], ap[1], ap[2]) * opacity;

@@ -18,6 +18,21 @@
 
 namespace oracle {
 
+// Math calls are unqualified so that the forward-mode dual numbers of
+// oracle_dual.hpp (the oracle's exact derivatives) find their own overloads;
+// for float and double these are the std:: functions.
+using std::exp;
+using std::log;
+using std::sqrt;
+using std::pow;
+using std::sin;
+using std::cos;
+using std::acos;
+using std::asin;
+using std::atan2;
+using std::floor;
+using std::fabs;
+
 template <class R>
 struct V3 {
     R x, y, z;
@@ -34,9 +49,9 @@ struct V3 {
 };
 template <class R> inline V3<R> operator*(R s, const V3<R>& v) { return v * s; }
 template <class R> inline R dot(const V3<R>& a, const V3<R>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <class R> inline R length(const V3<R>& a) { return std::sqrt(dot(a, a)); }
+template <class R> inline R length(const V3<R>& a) { return sqrt(dot(a, a)); }
 template <class R> inline V3<R> normalize(const V3<R>& a) { return a / length(a); }
-template <class R> inline V3<R> vexp(const V3<R>& a) { return V3<R>(std::exp(a.x), std::exp(a.y), std::exp(a.z)); }
+template <class R> inline V3<R> vexp(const V3<R>& a) { return V3<R>(exp(a.x), exp(a.y), exp(a.z)); }
 template <class R> inline V3<R> vmax(const V3<R>& a, const V3<R>& b) {
     return V3<R>(std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z));
 }
@@ -49,7 +64,7 @@ template <class R> inline R smoothstep(R e0, R e1, R x) {
 }
 template <class R> inline R mixr(R a, R b, R t) { return a + (b - a) * t; }
 template <class R> inline V3<R> mixv(const V3<R>& a, const V3<R>& b, R t) { return a + (b - a) * t; }
-template <class R> inline R fractr(R x) { return x - std::floor(x); }
+template <class R> inline R fractr(R x) { return x - floor(x); }
 
 // Texel storage is a flat array of `ch` channels per texel, x fastest, then
 // y, then z (the memory order of wgpu texture uploads).
@@ -69,7 +84,7 @@ inline int clamp_index(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); 
 template <class R, class S>
 inline void sample2d(const Tex<R, S>& t, R u, R v, bool repeat, R out[4]) {
     R cx = u * R(t.w) - R(0.5), cy = v * R(t.h) - R(0.5);
-    R fx0 = std::floor(cx), fy0 = std::floor(cy);
+    R fx0 = floor(cx), fy0 = floor(cy);
     R fx = cx - fx0, fy = cy - fy0;
     int ix = int(fx0), iy = int(fy0);
     int x0, x1, y0, y1;
@@ -93,7 +108,7 @@ inline void sample2d(const Tex<R, S>& t, R u, R v, bool repeat, R out[4]) {
 template <class R, class S>
 inline void sample3d(const Tex<R, S>& t, R u, R v, R w, bool repeat, R out[4]) {
     R cx = u * R(t.w) - R(0.5), cy = v * R(t.h) - R(0.5), cz = w * R(t.d) - R(0.5);
-    R fx0 = std::floor(cx), fy0 = std::floor(cy), fz0 = std::floor(cz);
+    R fx0 = floor(cx), fy0 = floor(cy), fz0 = floor(cz);
     R fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = int(fx0), iy = int(fy0), iz = int(fz0);
     int x0, x1, y0, y1, z0, z1;

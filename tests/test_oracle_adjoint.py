@@ -1,0 +1,187 @@
+"""Pins of the oracle's derivatives (CPU only).
+
+The oracle differentiates its FP64 restatement with forward-mode dual
+numbers (oracle/oracle_dual.hpp). These tests pin that derivative against
+  * FP64 central finite differences of the double instantiation, taken only
+    where the discrete decisions of the march (coverage gate, dense-sample
+    threshold, early exit: the work counters) agree at theta +- h, with h
+    reduced until they do (the loss is piecewise smooth);
+  * closed forms on a constant-density column, which also pin where the two
+    differentiation extensions (per-layer extinction scale, additive density
+    perturbation) enter the march;
+and check that the neutral extensions reproduce the reference restatement
+bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, ndimage
+
+import oracle
+from pedalsky_amd import scene
+
+DS, KAPPA, COV, DELTA = (oracle.PARAM_DENSITY_SCALE, oracle.PARAM_EXTINCTION_SCALE, oracle.PARAM_COVERAGE,
+                         oracle.PARAM_DELTA)
+
+
+def _smooth_noise(seed=1):
+    """Smooth random volumes in [0, 1] standing in for the baked noise (the
+    derivative checks need structure, not the reference's exact texels)."""
+    rng = np.random.default_rng(seed)
+
+    def field(shape, sigma):
+        a = ndimage.gaussian_filter(rng.random(shape), sigma, mode="wrap")
+        return (a - a.min()) / (a.max() - a.min())
+    base = np.stack([field((128, 128, 128), 3.0 - 0.5 * c) for c in range(4)], axis=-1).astype(np.float32)
+    det = np.stack([field((32, 32, 32), 1.5) for _ in range(4)], axis=-1).astype(np.float32)
+    curl = field((128, 128, 2), 4.0).astype(np.float32)
+    return base, det, curl
+
+
+def fd_derivative(loss, diff, kind, index, h0=1e-4, h_min=1e-8):
+    """FP64 central difference of loss(diff) in one parameter, with h shrunk
+    (by 10x, down to h_min) until the march makes the same discrete
+    decisions at theta - h, theta and theta + h. Returns (derivative, h)."""
+    _, c0 = loss(diff)
+    h = h0
+    while True:
+        lp, cp = loss(diff.perturbed(kind, index, h))
+        lm, cm = loss(diff.perturbed(kind, index, -h))
+        if (np.array_equal(cp, c0) and np.array_equal(cm, c0)) or h <= h_min:
+            return (lp - lm) / (2 * h), h
+        h /= 10
+
+
+def _voxel_at(frame, px, py, y_world, layer_base, layer_top, planet_r=6_360_000.0):
+    """Density-perturbation voxel (i, j, k) containing the point where pixel
+    (px, py)'s ray first reaches altitude y_world above the planet sphere."""
+    cam, d = scene.pixel_rays(frame)
+    ray = d[py, px]
+    oc = cam + np.array([0.0, planet_r, 0.0])
+    b = float(oc @ ray)
+    r = planet_r + y_world
+    c = (np.linalg.norm(oc) - r) * (np.linalg.norm(oc) + r)
+    t = -b + math.sqrt(b * b - c)
+    x, z = cam[0] + t * ray[0], cam[2] + t * ray[2]
+    h = (y_world - layer_base) / (layer_top - layer_base)
+    i = int(math.floor((x / 32000.0 + 0.5) * 128 - 0.5)) % 128
+    j = int(math.floor(h * 128 - 0.5))
+    k = int(math.floor((z / 32000.0 + 0.5) * 128 - 0.5)) % 128
+    return i, j, k
+
+
+@pytest.fixture(scope="module")
+def cloudy_frame():
+    w = scene.workload_full_frame(0, width=160, height=90)
+    base, det, curl = _smooth_noise()
+    fr = oracle.Frame(w, base, det, curl)
+    rng = np.random.default_rng(0xC4)
+    delta = rng.normal(0.0, 0.05, (128, 128, 128))
+    return w, fr, oracle.Diff(2, delta=delta).with_layers(w.layers)
+
+
+def test_extensions_are_neutral(cloudy_frame):
+    """kappa = 1 and a zero perturbation volume reproduce the reference
+    restatement exactly."""
+    w, fr, _ = cloudy_frame
+    ref, rl, rt, _ = fr.render_region(30, 10, 32, 24)
+    neutral = oracle.Diff(2, delta=np.zeros((128, 128, 128))).with_layers(w.layers)
+    got, gl, gt, _ = fr.render_ext(neutral, 30, 10, 32, 24)
+    assert np.array_equal(got, ref) and np.array_equal(gl, rl) and np.array_equal(gt, rt)
+
+
+def test_dual_matches_central_differences(cloudy_frame):
+    """d(sum g * outputs)/d theta from the dual instantiation equals the FP64
+    central difference for every parameter kind, including perturbation
+    voxels on a cloudy pixel's path through both layers."""
+    w, fr, diff = cloudy_frame
+    x0, y0, cw, ch = 40, 30, 24, 16
+    rng = np.random.default_rng(7)
+    g_comp, g_lum, g_trn = (rng.normal(size=(ch, cw, 4)) for _ in range(3))
+    g_trn *= 1e4  # transmittance is O(1), radiance O(1e4)
+    # perturbation voxels that the crop's most opaque pixel depends on:
+    # candidates along its ray through the cumulus layer, screened with a
+    # one-pixel dual pass
+    _, _, trn, _ = fr.render_ext(diff, x0, y0, cw, ch)
+    py, px = np.unravel_index(np.argmin(trn[..., 1]), trn.shape[:2])
+    cand = []
+    for q in range(100):
+        i, j, k = _voxel_at(w.frame, x0 + px, y0 + py, 1502.0 + 8.0 * q, 1500.0, 2300.0)
+        for a, b in ((0, 0), (1, 0), (0, 1), (1, 1)):
+            v = oracle.delta_index((i + a) % 128, j, (k + b) % 128)
+            if v not in cand:
+                cand.append(v)
+    g1 = np.zeros((1, 1, 4))
+    g1[..., :3] = 1.0
+    screen = fr.vjp_clouds(diff, [(DELTA, v) for v in cand], x0 + px, y0 + py, 1, 1, g_trn=g1)
+    vox = [v for v, gv in zip(cand, screen) if gv != 0.0][:10]
+    assert len(vox) == 10, "the pixel's ray should cross ten influential voxels"
+    params = [(DS, 0), (DS, 1), (KAPPA, 0), (KAPPA, 1), (COV, 0), (COV, 1)] + [(DELTA, v) for v in vox]
+    dual = fr.vjp_clouds(diff, params, x0, y0, cw, ch, g_comp=g_comp, g_lum=g_lum, g_trn=g_trn)
+
+    def loss(d):
+        c, lu, t, cnt = fr.render_ext(d, x0, y0, cw, ch, counters=True)
+        v = (g_comp[..., :3] * c[..., :3]).sum() + (g_lum[..., :3] * lu[..., :3]).sum()
+        return float(v + (g_trn[..., :3] * t[..., :3]).sum()), cnt
+
+    nonzero = 0
+    for (kind, idx), gd in zip(params, dual):
+        fd, h = fd_derivative(loss, diff, kind, idx)
+        scale = max(abs(fd), 1e-6 * np.abs(dual).max())
+        assert abs(gd - fd) <= 1e-5 * scale, (kind, idx, gd, fd, h)
+        nonzero += abs(gd) > 1e-9 * np.abs(dual).max()
+    assert nonzero >= 10, "too few parameters influence the crop"
+
+
+def _column(density_scale=0.05, kappa=1.0, delta_value=None):
+    """Straight-up view through a cumulus layer with constant noise (base R
+    = 1, coverage 1, no erosion): density = density_scale * clamp(NDF(h) +
+    delta, 0, 1)."""
+    w = scene.workload_small(width=16, height=16)
+    w.frame = scene.make_frame(16, 16, pitch_deg=89.0, yaw_deg=0.0, fov_deg=1.0, sun_zenith_deg=40.0)
+    w.layers = [scene.synth_layer(0, 1500.0, 2300.0, 1.0, density_scale=density_scale)]
+    w.layers[0].coverage = 1.0
+    w.clouds = scene.cloud_params(steps=4096, detail_strength=0.0, skew=0.0, n_layers=1)
+    w.depth = None
+    base = np.zeros((128, 128, 128, 4), dtype=np.float32)
+    base[..., 0] = 1.0
+    det = np.zeros((32, 32, 32, 4), dtype=np.float32)
+    curl = np.full((128, 128, 2), 0.5, dtype=np.float32)
+    wm = np.zeros((8, 8, 2), dtype=np.float16)
+    wm[..., 0] = 1.0
+    w.weather = wm.view(np.uint16)
+    fr = oracle.Frame(w, base, det, curl)
+    delta = None if delta_value is None else np.full((128, 128, 128), delta_value)
+    diff = oracle.Diff(1, density_scale=[density_scale], extinction_scale=[kappa], coverage=[1.0], delta=delta)
+    return w, fr, diff
+
+
+def _ndf_cu(h):
+    s = lambda e0, e1, x: np.clip((x - e0) / (e1 - e0), 0, 1) ** 2 * (3 - 2 * np.clip((x - e0) / (e1 - e0), 0, 1))  # noqa: E731
+    return s(0, 0.07, h) * s(1, 0.2, h)
+
+
+@pytest.mark.parametrize("delta_value", [None, 0.05])
+def test_column_transmittance_derivatives_closed_form(delta_value):
+    """T = exp(-kappa sigma_t ds L int clamp(NDF + delta) dh) through the
+    column, so dT/dkappa = -sigma_t ds L I T / 1 and dT/dds = -kappa sigma_t
+    L I T; with a constant delta the integrand is clamp(NDF + delta, 0, 1),
+    which pins where the perturbation enters (after the shape, before the
+    clamp)."""
+    ds, kappa = 0.05, 1.3
+    w, fr, diff = _column(ds, kappa, delta_value)
+    c = 0.0 if delta_value is None else delta_value
+    integral, _ = integrate.quad(lambda h: min(max(_ndf_cu(h) + c, 0.0), 1.0), 0, 1, points=[0.07, 0.2],
+                                 epsrel=1e-12, limit=200)
+    path = 800.0 / math.sin(math.radians(89.0))
+    g = np.zeros((2, 2, 4))
+    for ch, sig in enumerate(w.clouds.sigma_s[:3]):
+        g[...] = 0.0
+        g[..., ch] = 1.0
+        _, _, trn, _ = fr.render_ext(diff, 7, 7, 2, 2)
+        t_exp = math.exp(-kappa * sig * ds * path * integral)
+        assert trn[..., ch] == pytest.approx(t_exp, rel=2e-3)
+        dk, dds = fr.vjp_clouds(diff, [(KAPPA, 0), (DS, 0)], 7, 7, 2, 2, g_trn=g)
+        assert dk / 4 == pytest.approx(-sig * ds * path * integral * t_exp, rel=3e-3)
+        assert dds / 4 == pytest.approx(-kappa * sig * path * integral * t_exp, rel=3e-3)
