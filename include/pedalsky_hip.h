@@ -359,6 +359,54 @@ void ps_batch_destroy(ps_batch* batch);
 ps_status ps_render_batch(ps_ctx* ctx, const ps_scene_desc* scenes, int32_t n, const ps_tuning* tuning);
 
 /* ------------------------------------------------------------------------ */
+/* Differentiation. No reference counterpart: the reference only renders
+ * forward. These entry points differentiate the cloud march of ps_render
+ * (cloud_march.wgsl:731-1054 with sample_density :432-530, march_to_light
+ * :543-666, the shading :886-993 and the AP composite :1016-1048, plus the
+ * composite of cloud_composite.wgsl:47-57).
+ *
+ * Two forward-identity extensions make the differentiated parameters
+ * complete; their neutral values reproduce the reference exactly:
+ *   - a per-layer extinction scale kappa_l multiplying the layer's sigma_t
+ *     (Beer-Powder, the multi-scatter octaves and the step extinction);
+ *   - an additive density perturbation volume: 128^3 FP32, x fastest, then
+ *     the layer-normalised height h, then z; x and z span the weather tile
+ *     (texel coordinate wx * 128 / weather_scale + 63.5, Repeat, with (wx,
+ *     wz) the drift-offset position the weather map is read at), h is
+ *     ClampToEdge (128 h - 0.5). Its trilinear value is added to the shaped
+ *     cloud value before the final clamp, in the primary and the light-march
+ *     density, past the altitude and coverage gates.                      */
+/* ------------------------------------------------------------------------ */
+/* `extinction_scale`: n host floats (layers >= n keep 1), NULL with n = 0
+ * for all 1. `density_delta`: device 128^3 floats owned by the caller and
+ * read by every later render (its contents may change between renders), or
+ * NULL for none. */
+ps_status ps_set_cloud_extensions(ps_ctx* ctx, const float* extinction_scale, int32_t n,
+                                  const float* density_delta);
+/* Transmittance checkpoint interval of ps_render_train, in primary steps
+ * (1..32, default 16). */
+ps_status ps_set_checkpoint_interval(ps_ctx* ctx, int32_t steps);
+/* ps_render, and also store what ps_backward needs: the cloud transmittance
+ * at the first step of every checkpoint chunk of every layer segment, and
+ * per-pixel forward totals. The backward recomputes each chunk from its
+ * checkpoint. */
+ps_status ps_render_train(ps_ctx* ctx, const float* depth, const ps_image_out* out);
+/* Gradient outputs of ps_backward; both are ADDED to (the caller clears
+ * them). */
+typedef struct ps_cloud_grads {
+    float* layers;          /* device, 24 floats: dL/d density_scale[8], dL/d extinction_scale[8],
+                               dL/d coverage[8] (layer order of ps_set_clouds), or NULL */
+    float* density_delta;   /* device 128^3 floats: dL/d density_delta, or NULL */
+} ps_cloud_grads;
+/* Backward of the last ps_render_train: the cotangents are device W*H RGBA
+ * FP32 images (alpha ignored) of dL/d composite, dL/d cloud_luminance and
+ * dL/d cloud_transmittance, each NULL for zero. The context's LUTs, inputs
+ * and perturbation volume must be those of the training forward.
+ * Perturbation-volume gradients need light_steps <= 32 (or the cone march). */
+ps_status ps_backward(ps_ctx* ctx, const float* g_composite, const float* g_luminance,
+                      const float* g_transmittance, const ps_cloud_grads* grads);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side synthesis helpers (CPU, no device work).                       */
 /* ------------------------------------------------------------------------ */
 /* AtmosphereParams::default (weather.rs:60-84), CloudParamsGpu::default

@@ -199,6 +199,12 @@ def check_layouts() -> None:
                 raise AssertionError(f"{cls.__name__}.{field}: offset {got} != {off}")
 
 
+class CloudGrads(C.Structure):
+    """ps_cloud_grads: device gradient accumulators."""
+
+    _fields_ = [("layers", C.c_void_p), ("density_delta", C.c_void_p)]
+
+
 # (name, restype, argtypes) of every exported entry point in include/pedalsky_hip.h
 _P = C.c_void_p
 _SIGNATURES = [
@@ -239,6 +245,10 @@ _SIGNATURES = [
     ("ps_batch_time", C.c_int32, [_P, C.c_int32, C.POINTER(C.c_float)]),
     ("ps_batch_destroy", None, [_P]),
     ("ps_render_batch", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32, C.POINTER(Tuning)]),
+    ("ps_set_cloud_extensions", C.c_int32, [_P, C.POINTER(C.c_float), C.c_int32, _P]),
+    ("ps_set_checkpoint_interval", C.c_int32, [_P, C.c_int32]),
+    ("ps_render_train", C.c_int32, [_P, _P, C.POINTER(ImageOut)]),
+    ("ps_backward", C.c_int32, [_P, _P, _P, _P, C.POINTER(CloudGrads)]),
     ("ps_default_atmosphere", None, [C.POINTER(AtmosphereParams)]),
     ("ps_default_cloud_params", None, [C.POINTER(CloudParams)]),
     ("ps_remap_coverage", C.c_float, [C.c_float]),

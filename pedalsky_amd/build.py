@@ -20,8 +20,8 @@ LIB = LIB_DIR / "libpedalsky_hip.so"
 OBJ_DIR = PKG / "_build"
 INCLUDE = PKG.parent / "include"
 
-SOURCES = ["lut_kernels.hip", "render_kernel.hip", "noise_kernels.hip", "capi.cpp", "batch.cpp",
-           "host_synthesis.cpp"]
+SOURCES = ["lut_kernels.hip", "render_kernel.hip", "adjoint_kernel.hip", "noise_kernels.hip", "capi.cpp",
+           "batch.cpp", "host_synthesis.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PEDALSKY_OFFLOAD_ARCH", "gfx950")
@@ -33,9 +33,11 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 # The march runs FP32 divides and square roots in the loop body; their
 # correctly-rounded expansions are ~10 instructions each, the hardware
-# approximations (<= 2.5 ulp) one or two. The LUT kernels are FP64 and the
-# noise bakes keep the WGSL's IEEE FP32 semantics.
-PER_FILE = {"render_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+# approximations (<= 2.5 ulp) one or two. The adjoint recomputes the march
+# and must round exactly as the forward does, so it shares the flag. The LUT
+# kernels are FP64 and the noise bakes keep the WGSL's IEEE FP32 semantics.
+PER_FILE = {"render_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+            "adjoint_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def _compile(src: str) -> Path:

@@ -1,0 +1,575 @@
+// Adjoint of the fused sky + cloud render (render_kernel.hip) for gfx950:
+// gradients of a loss L(composite, cloud luminance, cloud transmittance)
+// with respect to each layer's density_scale, extinction scale and
+// coverage, and to the 128^3 density-perturbation volume.
+//
+// The forward pass being differentiated is cloud_march.wgsl:731-1054 (march,
+// light march, Beer-Powder, multi-octave phase, energy-conserving step, AP
+// composite) plus the dual-source composite of cloud_composite.wgsl:47-57.
+//
+// Design
+//  * The training forward (k_render<RF_TRAIN>) stores, per pixel, the
+//    transmittance at the first step of every checkpoint chunk (ck_len
+//    primary steps, 16 by default) and the forward totals (pre-AP
+//    luminance, final transmittance, the depth weights, the number of steps
+//    executed).
+//  * One lane per pixel, one wave per 8x8 tile, as in the forward. The
+//    adjoint walks a pixel's chunks from the last executed one back to the
+//    first. For each chunk it recomputes the chunk forward from its
+//    checkpoint (density, light march, shading), which gives every term of
+//    the gradient that does not depend on later samples, and writes a
+//    6-float record per dense sample; it then sweeps the chunk's records in
+//    reverse for the transmittance adjoint.
+//  * Transmittance adjoint: with T_i the transmittance in front of dense
+//    sample i, e_i its step transmittance and v_i = T_i (.) s_i (.) g_i the
+//    sample's luminance times its luminance cotangent, the quantity
+//    A_i = Tdot_i (.) T_i obeys A_i = A_{i+1} + v_i from A_N = Tdot_N (.) T_N,
+//    and dL/drho_i through e_i is -step * sum_c sigma_t,c A_{i+1,c}. The
+//    recurrence never divides by a transmittance, so deep, nearly opaque
+//    samples cost no precision.
+//  * Parameter gradients of the layer being marched are lane registers,
+//    added to per-wave LDS sums after every chunk and to the global
+//    accumulators once per wave. Perturbation-volume gradients are scattered
+//    with global float atomics (8 trilinear corners per sample).
+#include "ps_device.h"
+#include "ps_launch.h"
+
+#include "ps_march.h"
+
+namespace {
+
+constexpr int kRecFields = 6;
+enum { REC_V0 = 0, REC_V1, REC_V2, REC_C, REC_GRHO, REC_Q };
+
+// (1 - (1 + x) e^-x) / x^2, the x-derivative factor of (1 - e^-x) / x:
+// d/dx[(1 - e^-x)/x] = -psi(x). Series below x = 1/2 (the direct form
+// cancels there), the closed form above.
+__device__ __forceinline__ float psi_fn(float x) {
+    if (x < 0.5f) {
+        // sum_{m >= 2} (-1)^m (m - 1) / m! x^(m-2), through m = 10
+        float p = 9.f / 3628800.f;
+        p = p * x - 8.f / 362880.f;
+        p = p * x + 7.f / 40320.f;
+        p = p * x - 6.f / 5040.f;
+        p = p * x + 5.f / 720.f;
+        p = p * x - 4.f / 120.f;
+        p = p * x + 3.f / 24.f;
+        p = p * x - 2.f / 6.f;
+        p = p * x + 0.5f;
+        return p;
+    }
+    const float e = __expf(-x);
+    return (1.f - (1.f + x) * e) / (x * x);
+}
+
+// Scatter v over the eight trilinear corners of a perturbation-volume tap.
+__device__ __forceinline__ void delta_scatter(float* g, const DeltaTap& d, float v) {
+    const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
+    atomicAdd(g + d.z0 + d.y0 + d.x0, v * wz0 * wy0 * wx0);
+    atomicAdd(g + d.z0 + d.y0 + d.x1, v * wz0 * wy0 * wx1);
+    atomicAdd(g + d.z0 + d.y1 + d.x0, v * wz0 * wy1 * wx0);
+    atomicAdd(g + d.z0 + d.y1 + d.x1, v * wz0 * wy1 * wx1);
+    atomicAdd(g + d.z1 + d.y0 + d.x0, v * wz1 * wy0 * wx0);
+    atomicAdd(g + d.z1 + d.y0 + d.x1, v * wz1 * wy0 * wx1);
+    atomicAdd(g + d.z1 + d.y1 + d.x0, v * wz1 * wy1 * wx0);
+    atomicAdd(g + d.z1 + d.y1 + d.x1, v * wz1 * wy1 * wx1);
+}
+
+// Geometry of the march toward the sun (cloud_march.wgsl:543-578 straight,
+// 599-666 cone) with the arithmetic of march_light, so sample positions and
+// altitudes match the forward bit for bit.
+struct LightWalk {
+    float dist, k, r0c, step;
+    int n;
+};
+
+__device__ __forceinline__ LightWalk light_walk(const RenderParams& P, const LayerF& L, float px, float py, float pz,
+                                                float alt) {
+    const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
+    LightWalk w;
+    float cos_sun = fmaxf(sy, 0.05f);
+    w.dist = fmaxf((L.top - alt) / cos_sun, 1.f);
+    float r0 = P.R + alt;
+    float qy = py + P.R;
+    w.r0c = fmaxf(r0, 1.f);
+    float cv = (px * sx + qy * sy + pz * sz) * frcp(w.r0c);
+    w.k = 2.f * r0 * cv;
+    if (P.cone) {
+        w.step = w.dist / 5.f;
+        w.n = 6;
+    } else {
+        w.step = w.dist / (float)P.light_steps;
+        w.n = P.light_steps;
+    }
+    return w;
+}
+
+// Light sample i: position, altitude and quadrature weight.
+__device__ __forceinline__ void light_sample(const RenderParams& P, const LightWalk& w, int i, float px, float py,
+                                             float pz, float alt, float& qx, float& qyy, float& qz, float& a,
+                                             float& wt) {
+    const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
+    if (P.cone) {
+        const float off[6][3] = {{0.38051305f, 0.92453449f, -0.02111345f}, {-0.50625799f, -0.03590792f, -0.86163418f},
+                                 {-0.32509218f, -0.94557439f, 0.01428793f}, {0.09026238f, -0.27376545f, 0.95755165f},
+                                 {0.28128598f, 0.42443639f, -0.86065785f}, {-0.16852403f, 0.14748697f, 0.97460106f}};
+        float t, scale;
+        if (i < 5) { t = (i + 0.5f) * w.step; scale = w.step * (i + 1.f) / 5.f; wt = w.step; }
+        else { t = w.dist * 3.f; scale = w.step * 3.f; wt = w.step * 3.f; }
+        qx = px + sx * t + off[i][0] * scale;
+        qyy = py + sy * t + off[i][1] * scale;
+        qz = pz + sz * t + off[i][2] * scale;
+        float yy = qyy + P.R;
+        float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+        a = alt + t * (w.k + t) * frcp(w.r0c + rr);
+        return;
+    }
+    float t = (i + 0.5f) * w.step;
+    qx = px + sx * t; qyy = py + sy * t; qz = pz + sz * t;
+    float yy = qyy + P.R;
+    float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+    a = alt + t * (w.k + t) * frcp(w.r0c + rr);
+    wt = w.step;
+}
+
+// Light march with its parameter partials: returns the optical depth and
+// d od / d density_scale, d od / d coverage; bit i of `lin` is set when
+// light sample i sits inside the clamp's linear range (0 < x < 1), where the
+// perturbation volume receives d od / d delta = density_scale * weight.
+template <bool DELTA>
+__device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& L, const NdfC& nc, float px, float py,
+                                           float pz, float alt, float wr, uint32_t ctype, float& dod_ds,
+                                           float& dod_cov, uint32_t& lin) {
+    const LightWalk w = light_walk(P, L, px, py, pz, alt);
+    const float wc = wr * L.coverage;
+    const float cov = clampf(wc, 0.f, 1.f);
+    const float dcov = (wc > 0.f && wc < 1.f) ? wr : 0.f;
+    float od = 0.f;
+    dod_ds = 0.f;
+    dod_cov = 0.f;
+    lin = 0u;
+    for (int i = 0; i < w.n; ++i) {
+        float qx, qyy, qz, a, wt;
+        light_sample(P, w, i, px, py, pz, alt, qx, qyy, qz, a, wt);
+        const float h = (a - L.base) * L.inv_thick;
+        float d = 0.f;
+        if (!(h < 0.f || h > 1.f) && cov >= 0.01f) {
+            float ax, ay, az, cx, cy, cz;
+            advect(P, L, qx, qyy, qz, h, ax, ay, az, cx, cy, cz);
+            float dcx = 0.f, dcy = 0.f, dcz = 0.f;
+            if constexpr (DELTA) delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
+            const float x = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, -1.f, dcx, dcy, dcz);
+            const float c = clampf(x, 0.f, 1.f);
+            d = c * L.density_scale;
+            dod_ds += c * wt;
+            if (x > 0.f && x < 1.f) {
+                dod_cov += L.density_scale * dcov * wt;
+                lin |= 1u << i;
+            }
+        }
+        od += d * wt;
+    }
+    return od;
+}
+
+// Scatter g (= dL/d od * density_scale) over the light samples in `lin`.
+__device__ __forceinline__ void light_delta_scatter(const RenderParams& P, const LayerF& L, float px, float py,
+                                                    float pz, float alt, uint32_t lin, float g) {
+    const LightWalk w = light_walk(P, L, px, py, pz, alt);
+    while (lin) {
+        const int i = __builtin_ctz(lin);
+        lin &= lin - 1u;
+        float qx, qyy, qz, a, wt;
+        light_sample(P, w, i, px, py, pz, alt, qx, qyy, qz, a, wt);
+        const float h = (a - L.base) * L.inv_thick;
+        float dcx, dcy, dcz;
+        delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
+        delta_scatter(P.g_delta, delta_tap(dcx, dcy, dcz), g * wt);
+    }
+}
+
+template <bool DELTA>
+__device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x, int tile_y, float* wacc) {
+    const int lane = threadIdx.x;
+    const int x = tile_x * 8 + (lane & 7);
+    const int y = tile_y * 8 + (lane >> 3);
+    const bool valid = x < P.W && y < P.H;
+    const int xc = min(x, P.W - 1), yc = min(y, P.H - 1);
+    const size_t tile = (size_t)tile_y * P.n_tiles_x + tile_x;
+    if (lane < 3 * PS_MAX_LAYERS) wacc[lane] = 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+
+    const PixelRay ray = pixel_ray(P, xc, yc);
+    const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
+    const float3 sky = sky_pass(P, ray);
+    const float jitter = pixel_jitter(P, xc, yc);
+    const double depth_t_max = pixel_depth_tmax(P, ray, xc, yc);
+    int nseg = 0;
+    const uint32_t order = valid ? layer_order(P, ray, depth_t_max, nseg) : 0u;
+    (void)nseg;
+    const float lw[3] = {0.2126f, 0.7152f, 0.0722f};
+
+    // ---- forward totals and output cotangents ---------------------------------------
+    const float* ps = P.px_state + tile * (PX_FIELDS * 64) + lane;
+    const float lum_pre[3] = {ps[PX_LR * 64], ps[PX_LG * 64], ps[PX_LB * 64]};
+    const float Tf[3] = {ps[PX_TR * 64], ps[PX_TG * 64], ps[PX_TB * 64]};
+    const float tw = ps[PX_TW * 64], twn = ps[PX_TWN * 64];
+    const int n_exec = valid ? (int)ps[PX_NEXEC * 64] : 0;
+    const size_t o = ((size_t)yc * P.W + xc) * 4;
+    float g_lo[3] = {0.f, 0.f, 0.f}, g_T[3] = {0.f, 0.f, 0.f};
+    const float skyv[3] = {sky.x, sky.y, sky.z};
+    for (int c = 0; c < 3; ++c) {
+        if (P.g_comp) {
+            const float g = P.g_comp[o + c];
+            g_lo[c] += g;
+            g_T[c] += g * skyv[c];
+        }
+        if (P.g_lum) g_lo[c] += P.g_lum[o + c];
+        if (P.g_trn) g_T[c] += P.g_trn[o + c];
+    }
+
+    // ---- adjoint of the AP composite (cloud_march.wgsl:1016-1048) --------------------
+    float G[3] = {g_lo[0], g_lo[1], g_lo[2]};
+    float g_tw = 0.f, g_twn = 0.f;
+    const float opacity = 1.f - (Tf[0] * lw[0] + Tf[1] * lw[1] + Tf[2] * lw[2]);
+    if (valid && opacity > 1e-4f) {
+        const float tc = tw / fmaxf(twn, 1e-6f);
+        const float dd = fmaxf(tc, 50.f);
+        const float apz = clampf(__logf(dd / 50.f) * (float)(1.0 / 7.6009024595420822), 0.f, 1.f);
+        const float u = (float)ray.fx * P.inv_W, v = (float)ray.fy * P.inv_H;
+        const float cx = u * P.aw - 0.5f, cy = v * P.ah - 0.5f, cz = apz * P.ad - 0.5f;
+        const float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
+        const float wx = cx - fx0, wy = cy - fy0, wz = cz - fz0;
+        const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+        const int x0 = clampi(ix, P.aw), x1 = clampi(ix + 1, P.aw), y0 = clampi(iy, P.ah), y1 = clampi(iy + 1, P.ah),
+                  z0 = clampi(iz, P.ad), z1 = clampi(iz + 1, P.ad);
+        const int sxy = P.aw * P.ah;
+        const float4 c000 = P.AP[z0 * sxy + y0 * P.aw + x0], c100 = P.AP[z0 * sxy + y0 * P.aw + x1];
+        const float4 c010 = P.AP[z0 * sxy + y1 * P.aw + x0], c110 = P.AP[z0 * sxy + y1 * P.aw + x1];
+        const float4 c001 = P.AP[z1 * sxy + y0 * P.aw + x0], c101 = P.AP[z1 * sxy + y0 * P.aw + x1];
+        const float4 c011 = P.AP[z1 * sxy + y1 * P.aw + x0], c111 = P.AP[z1 * sxy + y1 * P.aw + x1];
+        float ap[4], dap[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float v000 = ((const float*)&c000)[k], v100 = ((const float*)&c100)[k];
+            const float v010 = ((const float*)&c010)[k], v110 = ((const float*)&c110)[k];
+            const float v001 = ((const float*)&c001)[k], v101 = ((const float*)&c101)[k];
+            const float v011 = ((const float*)&c011)[k], v111 = ((const float*)&c111)[k];
+            const float a0 = v000 + (v100 - v000) * wx, a1 = v010 + (v110 - v010) * wx;
+            const float b0 = v001 + (v101 - v001) * wx, b1 = v011 + (v111 - v011) * wx;
+            const float p0 = a0 + (a1 - a0) * wy, p1 = b0 + (b1 - b0) * wy;
+            ap[k] = p0 + (p1 - p0) * wz;
+            dap[k] = (p1 - p0) * (float)P.ad;  // d/d apz (0 where both slices clamp to one)
+        }
+        float g_ap_a = 0.f, g_op = 0.f, g_apz_rgb = 0.f;
+        for (int c = 0; c < 3; ++c) {
+            g_ap_a += g_lo[c] * lum_pre[c];
+            g_op += g_lo[c] * ap[c];
+            g_apz_rgb += g_lo[c] * dap[c];
+            G[c] = g_lo[c] * ap[3];
+        }
+        for (int c = 0; c < 3; ++c) g_T[c] -= lw[c] * g_op;
+        const float g_apz = g_ap_a * dap[3] + opacity * g_apz_rgb;
+        const bool inside = tc > 50.f && apz > 0.f && apz < 1.f;
+        const float g_tc = inside ? g_apz * (float)(1.0 / 7.6009024595420822) / tc : 0.f;
+        const float twn_ = fmaxf(twn, 1e-6f);
+        g_tw = g_tc / twn_;
+        g_twn = twn > 1e-6f ? -g_tc * tc / twn_ : 0.f;
+    }
+    float A[3] = {g_T[0] * Tf[0], g_T[1] * Tf[1], g_T[2] * Tf[2]};
+
+    // ---- chunks, last to first ----------------------------------------------------------
+    const float r0c = fmaxf((float)P.r0, 1.f);
+    const float kcam = (float)(2.0 * P.r0 * ray.b / fmax(P.r0, 1.0));
+    const float h0f = (float)P.h0;
+    const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
+    const float wscale = P.inv_weather_scale, nscale = P.inv_base_scale * 128.f;
+    const int steps = P.steps, cps = P.ck_per_seg, ckl = P.ck_len;
+    int chunk = -1;
+    if (n_exec > 0) {
+        const int g_last = n_exec - 1;
+        chunk = (g_last / steps) * cps + (g_last % steps) / ckl;
+    }
+    int cur_seg = -1, li = 0;
+    float tstart = 0.f, step = 0.f;
+    LayerF L;
+    const bool want_delta = DELTA && P.g_delta != nullptr;
+    float* rec = P.rec + tile * (size_t)(ckl * kRecFields * 64) + lane;
+
+    while (__ballot(chunk >= 0) != 0ull) {
+        if (chunk >= 0) {
+            const int seg = chunk / cps, sub = chunk % cps;
+            if (seg != cur_seg) {
+                cur_seg = seg;
+                li = (int)((order >> (3 * seg)) & 7u);
+                const LayerC& LC = P.layers[li];
+                L = LC.f;
+                double ta, tb;
+                layer_interval(LC, P.r0, ray.b, depth_t_max, ta, tb);
+                const double stepd = (tb - ta) / P.steps;
+                tstart = (float)(ta + jitter * stepd);
+                step = (float)stepd;
+            }
+            const LayerC& LC = P.layers[li];
+            const int s0 = sub * ckl;
+            const int s1 = min(min(steps, s0 + ckl), n_exec - seg * steps);
+            const float* ckp = P.ck + (tile * P.ck_max + chunk) * 192 + lane;
+            float T[3] = {ckp[0], ckp[64], ckp[128]};
+            float gds = 0.f, gk = 0.f, gcov = 0.f;
+            uint32_t dmask = 0u;
+            const float cosv = 1.f - ray.omcf;
+
+            // chunk forward: recompute from the checkpoint
+            for (int s = s0; s < s1; ++s) {
+                const float t = tstart + (float)s * step;
+                const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
+                const float qy = py + P.R;
+                const float rr = fsqrt(px * px + qy * qy + pz * pz);
+                const float alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+                const float h = (alt - L.base) * L.inv_thick;
+                const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
+                float wfx, wfy;
+                const uint32_t woff = weather_tap(P, wx * wscale + 0.5f, wz * wscale + 0.5f, wfx, wfy);
+                const float ax = wx - L.skew_x * h, ay = py, az = wz - L.skew_z * h;
+                const float cx = ax * nscale - 0.5f, cy = ay * nscale - 0.5f, cz = az * nscale - 0.5f;
+                const uint32_t roff = ((((int)floorf(cz) >> 2) & 31) << 10) | ((((int)floorf(cy) >> 2) & 31) << 5) |
+                                      (((int)floorf(cx) >> 2) & 31);
+                const uint2 wq = ld_g<uint2>(P.weather, woff);
+                const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
+                float dcx = 0.f, dcy = 0.f, dcz = 0.f, dm = 0.f;
+                if constexpr (DELTA) {
+                    delta_coords(P, wx, wz, h, dcx, dcy, dcz);
+                    dm = delta_bound(P, dcx, dcy, dcz);
+                }
+                const float wr = weather_blend(wq, wfx, wfy);
+                uint32_t ctype = L.type;
+                NdfC nc = layer_ndf(L);
+                if (P.use_type_grid) {
+                    float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
+                    const uint32_t gx = (uint32_t)(int)floorf(gu * 128.f) & 127u,
+                                   gy = (uint32_t)(int)floorf(gv * 128.f) & 127u;
+                    const uint32_t g = ld_g<uint8_t>(P.type_grid, gy * 128u + gx);
+                    if (g != 255u) { ctype = g; nc = ndf_coef(g); }
+                }
+                // sample_density with its partials (cloud_march.wgsl:432-530)
+                float density = 0.f, cval = 0.f, q = 0.f;
+                if (h >= 0.f && h <= 1.f + 1e-5f) {
+                    const float wc = wr * L.coverage;
+                    const float cov = clampf(wc, 0.f, 1.f);
+                    if (cov >= 0.01f) {
+                        const float ndfv = ndf_eval(nc, ctype, h, L.anvil);
+                        const float thr = DELTA ? (1.f - cov - dm) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
+                        const float rm = (float)rmb * (1.f / 255.f);
+                        float cloud = -1.f, dsh = 1.f;
+                        if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
+                            const float ds_ = P.inv_detail_scale;
+                            const float2 cu = curl_fetch(P.curl, ax * ds_ * 128.f - 0.5f, az * ds_ * 128.f - 0.5f);
+                            const float2 b = base_fetch(P.base_rl, cx, cy, cz);
+                            const float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
+                            const float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
+                            cloud = base_cloud * ndfv - (1.f - cov);
+                            if (cloud > 0.f) {
+                                const float ddx = ax + cu.x * P.curl_off_scale, ddz = az + cu.y * P.curl_off_scale;
+                                const float hf = detail_fetch(P.detail_hf, ddx * ds_ * 32.f - 0.5f,
+                                                              ay * ds_ * 32.f - 0.5f, ddz * ds_ * 32.f - 0.5f);
+                                const float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
+                                const float e = dmod * L.ld;
+                                dsh = frcp(fmaxf(1.f - e, 1e-5f));
+                                cloud = (cloud - e) * dsh;
+                            }
+                            if constexpr (DELTA) cloud += delta_fetch(P.delta, delta_tap(dcx, dcy, dcz));
+                        }
+                        cval = clampf(cloud, 0.f, 1.f);
+                        density = cval * L.density_scale;
+                        if (cloud > 0.f && cloud < 1.f && wc > 0.f && wc < 1.f) q = L.density_scale * dsh * wr;
+                    }
+                }
+                if (!(density > 1e-3f)) continue;
+
+                // dense sample: light march and shading with their adjoints
+                float dod_ds, dod_cov;
+                uint32_t lin;
+                const float od = light_adj<DELTA>(P, L, nc, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
+                const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
+                float st[3] = {0.f, 0.f, 0.f};
+                {
+                    const float hh = fmaxf(alt, 1.f);
+                    const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / (P.R + hh), -1.f, 1.f);
+                    float tu, tv;
+                    if (trans_uv_from_h(P, hh, mu, tu, tv)) {
+                        const float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
+                        st[0] = tt.x; st[1] = tt.y; st[2] = tt.z;
+                    }
+                }
+                float g_hg = LC.g_hg, omg0 = LC.omg_hg, g_d = LC.g_d, alpha = LC.alpha, w_d = LC.w_d, ice = LC.ice;
+                if (LC.is_cb) {
+                    const float mt = smoothstepf(0.60f, 0.85f, hn);
+                    float d = LC.droplet + (50.f - LC.droplet) * mt;
+                    d = clampf(d * P.droplet_bias, 5.f, 50.f);
+                    const float xg = -0.0990567f / (d - 1.67154f);
+                    g_hg = __expf(xg);
+                    omg0 = -expm1f(xg);
+                    g_d = __expf(-2.20679f / (d + 3.91029f)) - 0.428934f;
+                    alpha = __expf(3.62489f - 8.29288f / (d + 5.52825f));
+                    w_d = __expf(-0.599085f / (d - 0.641583f) - 0.665888f);
+                    ice = smoothstepf(35.f, 50.f, d);
+                }
+                float S1[3] = {0.f, 0.f, 0.f}, D1[3] = {0.f, 0.f, 0.f}, e0[3];
+                for (int c = 0; c < 3; ++c) e0[c] = __expf(-L.sigma_t[c] * od);
+                for (int n = 0; n < P.octaves; ++n) {
+                    const float ph = P.a_pow[n] * phase_octave_f(ray.omcf, cosv, P.c_pow[n], g_hg, omg0, g_d, alpha,
+                                                                 w_d, ice);
+                    const float bn = P.b_pow[n];
+                    for (int c = 0; c < 3; ++c) {
+                        const float en = n == 0 ? e0[c] : __expf(-L.sigma_t[c] * od * bn);
+                        S1[c] += ph * en;
+                        D1[c] += ph * bn * en;
+                    }
+                }
+                float fbm = 1.f;
+                if (P.forward_bias > 0.f) fbm = 1.f + P.forward_bias * smoothstepf(0.f, 1.f, fmaxf(cosv, 0.f));
+                float amb[3];
+                {
+                    const float rho = sqrtf(fmaxf(alt * (2.f * P.R + alt), 0.f));
+                    const float zha = (float)PS_PI - atan2f(P.R, rho);
+                    float u, v;
+                    skyview_uv_f(P, px, qy, pz, zha, u, v);
+                    const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
+                    const float g = P.ambient * (0.3f + 0.7f * hn);
+                    amb[0] = sv.x * P.E[0] * g; amb[1] = sv.y * P.E[1] * g; amb[2] = sv.z * P.E[2] * g;
+                }
+                if (P.light_radius > 0.f) {
+                    const float lx = px - P.light_x, lz = pz - P.light_z;
+                    const float fo = clampf(1.f - sqrtf(lx * lx + lz * lz) / P.light_radius, 0.f, 1.f);
+                    const float wgt = fo * fo * (3.f - 2.f * fo);
+                    for (int c = 0; c < 3; ++c) amb[c] += P.light_rgb[c] * wgt;
+                }
+                const float pw = P.powder;
+                float g_rho = 0.f, g_od = 0.f, v[3], e_step[3];
+                for (int c = 0; c < 3; ++c) {
+                    const float sig = L.sigma_t[c];
+                    const float en = e0[c] + (2.f * e0[c] * (1.f - e0[c] * e0[c]) - e0[c]) * pw;
+                    const float den = -e0[c] * ((1.f - pw) + 2.f * pw * (1.f - 3.f * e0[c] * e0[c]));
+                    const float es = P.E[c] * st[c];
+                    const float sun = S1[c] * es * fbm, dsun = -D1[c] * es * fbm;
+                    const float Qc = sun * en + amb[c];
+                    const float dQ = dsun * en + sun * den;  // d Q / d (sigma_t od)
+                    const float in_c = P.sigma_s[c] * density * Qc;
+                    const float rs = density * sig;
+                    const bool lin_t = rs > 1e-6f;
+                    const float ls = fmaxf(rs, 1e-6f);
+                    const float xx = ls * step;
+                    const float ome = one_minus_exp_neg(xx);
+                    const float es_t = __expf(-xx);
+                    e_step[c] = es_t;
+                    const float s_int = in_c * ome / ls;
+                    const float g_sl = G[c] + lw[c] * (g_tw * t + g_twn);
+                    v[c] = T[c] * s_int * g_sl;
+                    const float g_s = T[c] * g_sl;
+                    const float K = P.sigma_s[c] * density * ome / ls;
+                    const float g_Q = g_s * K;
+                    const float dsdr = lin_t ? P.sigma_s[c] * Qc * step * es_t : P.sigma_s[c] * Qc * ome / ls;
+                    g_rho += g_s * dsdr;
+                    const float dsdsig = lin_t ? -P.sigma_s[c] * Qc * density * step * (density * step * psi_fn(xx)) : 0.f;
+                    const float g_sig = g_Q * od * dQ + g_s * dsdsig;
+                    g_od += g_Q * sig * dQ;
+                    gk += g_sig * LC.sigma_t0[c];
+                }
+                gds += g_od * dod_ds;
+                gcov += g_od * dod_cov;
+                if (want_delta && lin) light_delta_scatter(P, L, px, py, pz, alt, lin, g_od * L.density_scale);
+                const int slot = s - s0;
+                float* rp = rec + (size_t)slot * (kRecFields * 64);
+                rp[REC_V0 * 64] = v[0]; rp[REC_V1 * 64] = v[1]; rp[REC_V2 * 64] = v[2];
+                rp[REC_C * 64] = cval; rp[REC_GRHO * 64] = g_rho; rp[REC_Q * 64] = q;
+                dmask |= 1u << slot;
+                T[0] *= e_step[0]; T[1] *= e_step[1]; T[2] *= e_step[2];
+            }
+
+            // reverse sweep over the chunk's dense samples: transmittance adjoint
+            while (dmask) {
+                const int slot = 31 - __builtin_clz(dmask);
+                dmask &= ~(1u << slot);
+                const float* rp = rec + (size_t)slot * (kRecFields * 64);
+                const float v0 = rp[REC_V0 * 64], v1 = rp[REC_V1 * 64], v2 = rp[REC_V2 * 64];
+                const float cval = rp[REC_C * 64], g_rho_s = rp[REC_GRHO * 64], q = rp[REC_Q * 64];
+                const float rho = cval * L.density_scale;
+                float g_rho = g_rho_s;
+                for (int c = 0; c < 3; ++c) {
+                    const float sig = L.sigma_t[c];
+                    if (rho * sig > 1e-6f) {
+                        g_rho -= step * sig * A[c];
+                        gk -= rho * step * A[c] * LC.sigma_t0[c];
+                    }
+                }
+                gds += g_rho * cval;
+                gcov += g_rho * q;
+                if (want_delta && cval < 1.f) {
+                    const int s = s0 + slot;
+                    const float t = tstart + (float)s * step;
+                    const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
+                    const float qy = py + P.R;
+                    const float rr = fsqrt(px * px + qy * qy + pz * pz);
+                    const float alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+                    const float h = (alt - L.base) * L.inv_thick;
+                    float dcx, dcy, dcz;
+                    delta_coords(P, px - L.wind_off_x, pz - L.wind_off_z, h, dcx, dcy, dcz);
+                    delta_scatter(P.g_delta, delta_tap(dcx, dcy, dcz), g_rho * L.density_scale);
+                }
+                A[0] += v0; A[1] += v1; A[2] += v2;
+            }
+            atomicAdd(&wacc[li], gds);
+            atomicAdd(&wacc[PS_MAX_LAYERS + li], gk);
+            atomicAdd(&wacc[2 * PS_MAX_LAYERS + li], gcov);
+            --chunk;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 3 * PS_MAX_LAYERS) {
+        const float v = wacc[lane];
+        if (v != 0.f) atomicAdd(P.g_layers + lane, v);
+    }
+}
+
+template <bool DELTA>
+__global__ __launch_bounds__(64) void k_render_backward(const RenderParams* __restrict__ Pp) {
+    __shared__ float wacc[3 * PS_MAX_LAYERS];
+    backward_tile<DELTA>(*Pp, blockIdx.x, blockIdx.y, wacc);
+}
+
+// Per-brick maxima of the perturbation volume over each 4^3 brick and its +1
+// halo (Repeat in x and z, ClampToEdge in height): every trilinear tap whose
+// lower corner lies in the brick reads only texels inside this range.
+__global__ void k_delta_bmax(const float* __restrict__ d, float* __restrict__ bmax) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= 32 * 32 * 32) return;
+    const int bx = b & 31, by = (b >> 5) & 31, bz = b >> 10;
+    float m = -3.0e38f;
+    for (int k = 0; k < 5; ++k) {
+        const int z = (bz * 4 + k) & 127;
+        for (int j = 0; j < 5; ++j) {
+            const int yy = min(by * 4 + j, 127);
+            for (int i = 0; i < 5; ++i) {
+                const int xx = (bx * 4 + i) & 127;
+                m = fmaxf(m, d[(z * 128 + yy) * 128 + xx]);
+            }
+        }
+    }
+    bmax[b] = m;
+}
+
+}  // namespace
+
+hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, hipStream_t s) {
+    dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
+    if (delta) hipLaunchKernelGGL(k_render_backward<true>, grid, block, 0, s, d_params);
+    else hipLaunchKernelGGL(k_render_backward<false>, grid, block, 0, s, d_params);
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_delta_bmax(const float* delta, float* bmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_delta_bmax, dim3(128), dim3(256), 0, s, delta, bmax);
+    return hipGetLastError();
+}

@@ -202,7 +202,11 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
             double st = 0.25 * t * t;
             mod[0] = 1.0 - 0.5 * st; mod[2] = 1.0 + st;
         }
-        for (int k = 0; k < 3; ++k) o.sigma_t[k] = (float)(((double)cp.sigma_s[k] + cp.sigma_a[k]) * mod[k]);
+        for (int k = 0; k < 3; ++k) {
+            const double st0 = ((double)cp.sigma_s[k] + cp.sigma_a[k]) * mod[k];
+            oc.sigma_t0[k] = (float)st0;
+            o.sigma_t[k] = (float)(st0 * (double)sc.kappa[l]);
+        }
         double wx = sc.wind[2 * l], wz = sc.wind[2 * l + 1];
         o.wind_off_x = (float)(wx * drift);
         o.wind_off_z = (float)(wz * drift);
@@ -254,6 +258,10 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
         P.out_f16 = out->format == PS_PIXEL_F16 ? 1 : 0;
     }
     P.counters = c->d_counters;
+    P.delta = c->delta;
+    P.delta_bmax = c->delta_bmax;
+    P.delta_su = (float)(128.0 / std::max((double)cp.weather_scale_m, 1.0));
+    P.n_tiles_x = (P.W + 7) / 8;
     return PS_OK;
 }
 
@@ -297,16 +305,38 @@ ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
     return PS_OK;
 }
 
+// Brick maxima of the caller's perturbation volume (it may have changed
+// since the last render: recomputed on every render that uses it).
+ps_status prepare_delta(ps_ctx* c) {
+    if (!c->delta) return PS_OK;
+    ps_status s;
+    if ((s = dev_alloc(&c->delta_bmax, sizeof(float) * 32 * 32 * 32, "perturbation brick maxima"))) return s;
+    PS_HIP(ps_launch_delta_bmax(c->delta, c->delta_bmax, c->stream));
+    return PS_OK;
+}
+
 ps_status launch_render(ps_ctx* c, const float* depth, const ps_image_out* out, bool count) {
     ps_status s;
     if ((s = check_ready(c, true))) return s;
     if (!out && !count) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render: out is NULL");
+    if ((s = prepare_delta(c))) return s;
     RenderParams next;
     if ((s = fill_render_params(c, c->sc, c->luts, depth, out, next))) return s;
     RenderParams* dP = nullptr;
     if ((s = c->render_ring.upload(next, c->stream, &dP))) return s;
-    PS_HIP(ps_launch_render(dP, next.W, next.H, count, c->stream));
+    PS_HIP(ps_launch_render(dP, next.W, next.H, count, c->delta != nullptr, false, c->stream));
     return c->render_ring.mark_used(c->stream);
+}
+
+// Grow a context-owned device buffer to at least `floats` floats.
+ps_status grow(float** p, size_t* cap, size_t floats, const char* what) {
+    if (*cap >= floats && *p) return PS_OK;
+    dev_free(*p);
+    *cap = 0;
+    ps_status s;
+    if ((s = dev_alloc(p, floats * sizeof(float), what))) return s;
+    *cap = floats;
+    return PS_OK;
 }
 
 }  // namespace
@@ -378,6 +408,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
+    dev_free(c->delta_bmax); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec);
     c->render_ring.release();
     c->lut_ring.release();
     for (hipEvent_t e : c->events) hipEventDestroy(e);
@@ -685,6 +716,86 @@ ps_status ps_count_work(ps_ctx* c, const float* depth, ps_work_counters* out) {
     out->eroded_steps = h[4]; out->dense_samples = h[5]; out->light_steps = h[6]; out->light_covered = h[7];
     out->octaves = h[8]; out->ap_taps = h[9];
     return PS_OK;
+}
+
+// ---- differentiation -----------------------------------------------------------------
+ps_status ps_set_cloud_extensions(ps_ctx* c, const float* extinction_scale, int32_t n, const float* density_delta) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_cloud_extensions: NULL context");
+    if (n < 0 || n > PS_MAX_CLOUD_LAYERS || (n > 0 && !extinction_scale))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_cloud_extensions: need 0 <= n <= 8 extinction scales");
+    for (int l = 0; l < PS_MAX_CLOUD_LAYERS; ++l) {
+        const float k = l < n ? extinction_scale[l] : 1.f;
+        if (!(k >= 0.f) || !std::isfinite(k))
+            return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_cloud_extensions: extinction scales must be finite and >= 0");
+        c->sc.kappa[l] = k;
+    }
+    c->delta = density_delta;
+    c->train_valid = false;
+    return PS_OK;
+}
+
+ps_status ps_set_checkpoint_interval(ps_ctx* c, int32_t steps) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_checkpoint_interval: NULL context");
+    if (steps < 1 || steps > 32)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_checkpoint_interval: the interval must be in [1, 32] steps");
+    c->ck_len = steps;
+    c->train_valid = false;
+    return PS_OK;
+}
+
+ps_status ps_render_train(ps_ctx* c, const float* depth, const ps_image_out* out) {
+    if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render_train: NULL argument");
+    ps_status s;
+    if ((s = check_ready(c, true))) return s;
+    if ((s = prepare_delta(c))) return s;
+    RenderParams P;
+    if ((s = fill_render_params(c, c->sc, c->luts, depth, out, P))) return s;
+    const size_t tiles = (size_t)((P.W + 7) / 8) * ((P.H + 7) / 8);
+    P.ck_len = c->ck_len;
+    P.ck_per_seg = (P.steps + c->ck_len - 1) / c->ck_len;
+    P.ck_max = std::max(P.n_layers, 1) * P.ck_per_seg;
+    if ((s = grow(&c->ck, &c->ck_cap, tiles * P.ck_max * 192, "transmittance checkpoints"))) return s;
+    if ((s = grow(&c->px_state, &c->px_cap, tiles * PX_FIELDS * 64, "forward totals"))) return s;
+    P.ck = c->ck;
+    P.px_state = c->px_state;
+    RenderParams* dP = nullptr;
+    if ((s = c->render_ring.upload(P, c->stream, &dP))) return s;
+    PS_HIP(ps_launch_render(dP, P.W, P.H, false, c->delta != nullptr, true, c->stream));
+    if ((s = c->render_ring.mark_used(c->stream))) return s;
+    c->train = P;
+    c->train_valid = true;
+    return PS_OK;
+}
+
+ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_luminance, const float* g_transmittance,
+                      const ps_cloud_grads* grads) {
+    if (!c || !grads) return fail(PS_ERR_INVALID_ARGUMENT, "ps_backward: NULL argument");
+    if (!c->train_valid)
+        return fail(PS_ERR_NOT_READY, "ps_backward: no training forward to differentiate (call ps_render_train; "
+                                      "changing the extensions or the checkpoint interval invalidates it)");
+    if (!grads->layers && !grads->density_delta)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_backward: no gradient output requested");
+    RenderParams P = c->train;
+    if (grads->density_delta && !P.cone && P.light_steps > 32)
+        return fail(PS_ERR_UNSUPPORTED, "ps_backward: perturbation-volume gradients support at most 32 light steps");
+    const size_t tiles = (size_t)((P.W + 7) / 8) * ((P.H + 7) / 8);
+    ps_status s;
+    if ((s = grow(&c->rec, &c->rec_cap, tiles * (size_t)P.ck_len * 6 * 64, "backward records"))) return s;
+    if (!grads->layers) {
+        if ((s = dev_alloc(&c->d_counters, 16 * sizeof(unsigned long long), "counters"))) return s;
+    }
+    P.g_comp = g_composite;
+    P.g_lum = g_luminance;
+    P.g_trn = g_transmittance;
+    // without a caller buffer for the per-layer sums, the (ignored) sums go
+    // to the context's counter scratch
+    P.g_layers = grads->layers ? grads->layers : (float*)c->d_counters;
+    P.g_delta = grads->density_delta;
+    P.rec = c->rec;
+    RenderParams* dP = nullptr;
+    if ((s = c->render_ring.upload(P, c->stream, &dP))) return s;
+    PS_HIP(ps_launch_render_backward(dP, P.W, P.H, P.delta != nullptr, c->stream));
+    return c->render_ring.mark_used(c->stream);
 }
 
 }  // extern "C"

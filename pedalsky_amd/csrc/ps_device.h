@@ -56,6 +56,7 @@ struct LayerF {
 
 struct LayerC {
     LayerF f;
+    float sigma_t0[3];                // sigma_t before the per-layer extinction scale (adjoint)
     uint32_t is_cb;                   // layer type 7: phase depends on height
     float droplet;                    // raw layer droplet diameter (um)
     // phase constants for non-Cb layers (cloud_march.wgsl:256-272), folded in
@@ -122,7 +123,37 @@ struct RenderParams {
     int out_f16;
     int _pad;
     unsigned long long* counters;     // 10 x u64 when counting
+    // ---- differentiation extensions (no reference counterpart; the
+    // neutral values, no volume and unit extinction scales, are the
+    // reference) ----
+    // Additive density perturbation: 128^3 floats, x fastest, then the
+    // layer-normalised height, then z; x and z span the weather tile
+    // (Repeat), height is ClampToEdge.
+    const float* delta;
+    const float* delta_bmax;          // 32^3: max of delta over each 4^3 brick and its +1 halo
+    float delta_su;                   // 128 / weather_scale: delta texels per metre
+    // ---- training forward (checkpoints) and backward ----
+    // Per wave tile (tiles row-major, 64 lanes): transmittance checkpoints
+    // [tile][chunk][rgb][lane] at the first step of every ck_len-step chunk
+    // of every layer segment, and the per-pixel forward totals
+    // [tile][PX_FIELDS][lane].
+    float* ck;
+    float* px_state;
+    int ck_len;                       // checkpoint interval (primary steps), 1..32
+    int ck_per_seg;                   // ceil(steps / ck_len)
+    int ck_max;                       // chunk slots per pixel (n_layers * ck_per_seg)
+    int n_tiles_x;
+    // cotangents of the three outputs (W*H RGBA FP32) or null
+    const float* g_comp;
+    const float* g_lum;
+    const float* g_trn;
+    float* g_layers;                  // 3 x 8 accumulated: density_scale, extinction_scale, coverage
+    float* g_delta;                   // 128^3 accumulated, or null
+    float* rec;                       // backward records [tile][slot][REC_FIELDS][lane]
 };
+
+// Per-pixel forward totals stored by the training forward.
+enum { PX_LR = 0, PX_LG, PX_LB, PX_TR, PX_TG, PX_TB, PX_TW, PX_TWN, PX_NEXEC, PX_FIELDS };
 
 // ---------------------------------------------------------------------------
 // Small vector helpers.

@@ -49,6 +49,8 @@ struct SceneIn {
     ps_cloud_layer layers[PS_MAX_CLOUD_LAYERS]{};
     float wind[2 * PS_MAX_CLOUD_LAYERS]{};
     int n_layers = 0;
+    // differentiation extension: per-layer extinction scale (1 = reference)
+    float kappa[PS_MAX_CLOUD_LAYERS] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
 };
 
 // One scene's LUT set (device).
@@ -141,6 +143,20 @@ struct ps_ctx {
     // in-loop timing (ps_timing_begin / ps_timing_end)
     bool timing = false;
     int timing_max = 0, timing_n = 0;
+
+    // differentiation: caller-owned perturbation volume and its brick maxima
+    const float* delta = nullptr;
+    float* delta_bmax = nullptr;
+    // training forward state read by ps_backward
+    int ck_len = 16;
+    float* ck = nullptr;
+    size_t ck_cap = 0;            // floats
+    float* px_state = nullptr;
+    size_t px_cap = 0;
+    float* rec = nullptr;
+    size_t rec_cap = 0;
+    RenderParams train{};         // parameter block of the last training forward
+    bool train_valid = false;
 };
 
 namespace psi {

@@ -14,7 +14,16 @@ struct RenderParams;
 hipError_t ps_launch_luts(const LutParams* d_params, const LutParams& dims, int n, uint32_t flags, float4* T,
                           float4* MS, float4* SV, float4* AP, hipStream_t s, hipEvent_t* ev);
 
-hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, hipStream_t s);
+// Single-scene render: `delta` enables the density-perturbation volume,
+// `train` also stores the checkpoints and totals the backward reads.
+hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, bool delta, bool train,
+                            hipStream_t s);
+// Backward of a training render (the same parameter block, with cotangents
+// and gradient outputs set).
+hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, hipStream_t s);
+// Per-brick maxima of the 128^3 density-perturbation volume (32^3 bricks of
+// 4^3 texels plus a +1 halo: Repeat in x and z, ClampToEdge in height).
+hipError_t ps_launch_delta_bmax(const float* delta, float* bmax, hipStream_t s);
 
 // Many scenes in one launch: `tiles` lists (scene << 20 | tile_y << 10 |
 // tile_x) for every wave tile of every scene; a wave tile is tile_w x

@@ -183,6 +183,41 @@ class Renderer:
         check(self.lib.ps_count_work(self.ctx, self.depth_ptr, C.byref(c)))
         return c
 
+    # differentiation ----------------------------------------------------------
+    def set_cloud_extensions(self, extinction_scale=None, density_delta=None) -> None:
+        """Per-layer extinction scales (host sequence, None = all 1) and the
+        128^3 density-perturbation volume (a contiguous float32 device
+        tensor the library reads on every render, or None)."""
+        k = None
+        n = 0
+        if extinction_scale is not None:
+            vals = [float(v) for v in extinction_scale]
+            n = len(vals)
+            k = (C.c_float * max(n, 1))(*vals)
+        ptr = None
+        if density_delta is not None:
+            if density_delta.dtype != _torch().float32 or not density_delta.is_contiguous() or \
+                    density_delta.numel() != 128 ** 3 or not density_delta.is_cuda:
+                raise ValueError("density_delta must be a contiguous float32 CUDA tensor of 128^3 elements")
+            ptr = density_delta.data_ptr()
+        check(self.lib.ps_set_cloud_extensions(self.ctx, k, n, ptr))
+
+    def set_checkpoint_interval(self, steps: int) -> None:
+        check(self.lib.ps_set_checkpoint_interval(self.ctx, steps))
+
+    def render_train(self, out: abi.ImageOut) -> None:
+        check(self.lib.ps_render_train(self.ctx, self.depth_ptr, C.byref(out)))
+
+    def backward(self, g_composite=None, g_luminance=None, g_transmittance=None, layer_grads=None,
+                 delta_grad=None) -> None:
+        """Accumulate gradients of the last render_train into the device
+        buffers `layer_grads` (24 floats) and `delta_grad` (128^3)."""
+        def ptr(t):
+            return None if t is None else t.data_ptr()
+        g = abi.CloudGrads(ptr(layer_grads), ptr(delta_grad))
+        check(self.lib.ps_backward(self.ctx, ptr(g_composite), ptr(g_luminance), ptr(g_transmittance),
+                                   C.byref(g)))
+
     def render_numpy(self, which=("composite", "cloud_luminance", "cloud_transmittance"),
                      fmt: int = abi.PS_PIXEL_F32, bake: bool = True) -> dict:
         """Bake the LUTs, render, and return the outputs as numpy arrays."""

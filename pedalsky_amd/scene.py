@@ -426,3 +426,40 @@ def workload_batch(n: int = 64, seed: int = 0xC3, steps: int = 128, ground: bool
             overcast=overcast_mask(layers), blue=blue_noise(),
             depth=ground_depth(frame, atm.planet_radius_m) if ground else None))
     return out
+
+
+# ---------------------------------------------------------------------------
+# differentiation workload
+# ---------------------------------------------------------------------------
+def workload_adjoint(index: int = 0, width: int = 640, height: int = 360) -> Workload:
+    """The cloud-differentiation workload: the full-frame two-layer stack
+    (cumulus 1500-2300 m + cirrus 8000-9000 m), 128/6 steps, FP16 noise,
+    sun zenith 65 deg, camera of sweep frame `index`, at 640x360."""
+    w = workload_full_frame(index, width=width, height=height)
+    w.name = f"cloud adjoint {width}x{height}, cumulus + cirrus, 128/6 steps, frame {index}"
+    return w
+
+
+@dataclasses.dataclass
+class AdjointTarget:
+    """Parameters of the target render of the differentiation workload."""
+    density_scale: np.ndarray      # per layer
+    extinction_scale: np.ndarray   # per layer
+    coverage: np.ndarray           # per layer
+    delta: np.ndarray              # (128, 128, 128) float32, (z, h, x)
+
+
+def adjoint_target(layers, seed: int = 0xC4) -> AdjointTarget:
+    """Perturbed parameters the target image is rendered with: per layer
+    density_scale x U[0.7, 1.3], extinction scale U[0.7, 1.3], coverage +
+    U[-0.15, 0.15] (clamped to [0, 1]) from PCG32 `seed`; the density
+    perturbation volume is i.i.d. N(0, 0.05^2) from numpy's PCG64 with the
+    same seed."""
+    rng = Pcg32(seed)
+    n = len(layers)
+    u = rng.uniform(3 * n).reshape(3, n)
+    ds = np.array([L.density_scale for L in layers]) * (0.7 + 0.6 * u[0])
+    kap = 0.7 + 0.6 * u[1]
+    cov = np.clip(np.array([L.coverage for L in layers]) + (-0.15 + 0.3 * u[2]), 0.0, 1.0)
+    delta = np.random.default_rng(seed).normal(0.0, 0.05, (128, 128, 128)).astype(np.float32)
+    return AdjointTarget(ds.astype(np.float32), kap.astype(np.float32), cov.astype(np.float32), delta)

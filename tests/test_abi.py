@@ -26,6 +26,8 @@ def test_struct_layouts_match_reference():
     # batch descriptors (static_assert'ed in batch.cpp)
     assert C.sizeof(abi.SceneDesc) == 96 and abi.SceneDesc.out.offset == 64
     assert C.sizeof(abi.Tuning) == 16
+    # differentiation outputs: two device pointers
+    assert C.sizeof(abi.CloudGrads) == 16 and abi.CloudGrads.density_delta.offset == 8
 
 
 def test_library_exports_every_header_symbol(lib):

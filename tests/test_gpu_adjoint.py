@@ -1,0 +1,329 @@
+"""GPU tests of the differentiable cloud march (ps_render_train / ps_backward),
+through the C ABI and the torch.autograd binding.
+
+References:
+  * forward with the differentiation extensions: the FP64 oracle's render
+    with the same extinction scales and perturbation volume (bar: p99.9
+    relative radiance error <= 1e-3, median <= 1e-5);
+  * gradients: the oracle's exact FP64 derivatives (dual numbers, themselves
+    pinned by FP64 central differences on the CPU), and FP64 central finite
+    differences of the oracle's loss with h = 1e-4, shrunk where the march's
+    discrete decisions (gates, dense threshold, early exit) would change
+    inside [theta - h, theta + h]. Bar: relative error <= 1e-3 against the
+    finite difference, with the relative scale floored at 1e-3 of the
+    largest gradient of the same parameter kind.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from pedalsky_amd import abi, scene
+from pedalsky_amd.renderer import Renderer
+
+pytestmark = pytest.mark.gpu
+
+DS, KAPPA, COV, DELTA = (oracle.PARAM_DENSITY_SCALE, oracle.PARAM_EXTINCTION_SCALE, oracle.PARAM_COVERAGE,
+                         oracle.PARAM_DELTA)
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU (no CPU fallback exists)"
+    r = Renderer(0)
+    yield r
+    r.close()
+
+
+def _load(r, w):
+    r.load_workload(w)
+    r.bake_luts()
+    r.synchronize()
+    fmt = w.noise_format
+    base = oracle.decode(r.read_noise(abi.PS_NOISE_BASE, fmt), fmt)
+    det = oracle.decode(r.read_noise(abi.PS_NOISE_DETAIL, fmt), fmt)
+    curl = oracle.decode(r.read_noise(abi.PS_NOISE_CURL, fmt), fmt)
+    return oracle.Frame(w, base, det, curl)
+
+
+def _render(r, train=False):
+    W, H = r.workload.width, r.workload.height
+    bufs = r.alloc_outputs(W, H, abi.PS_PIXEL_F32, ("composite", "cloud_luminance", "cloud_transmittance"))
+    out = r.image_out(bufs, abi.PS_PIXEL_F32)
+    if train:
+        r.render_train(out)
+    else:
+        r.render(out)
+    r.synchronize()
+    return bufs
+
+
+def _set_layers(r, w, ds, cov):
+    layers = []
+    for L, d, c in zip(w.layers, ds, cov):
+        L2 = abi.CloudLayer.from_buffer_copy(L)
+        L2.density_scale = float(d)
+        L2.coverage = float(c)
+        layers.append(L2)
+    r.set_clouds(w.clouds, layers, w.winds)
+
+
+def _delta_tensor(delta):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(delta, dtype=np.float32).reshape(-1)).to("cuda:0")
+    torch.cuda.synchronize()
+    return t
+
+
+def _grads(r, g_comp=None, g_lum=None, g_trn=None, want_delta=True):
+    import torch
+    lg = torch.zeros(24, dtype=torch.float32, device="cuda:0")
+    dg = torch.zeros(128 ** 3, dtype=torch.float32, device="cuda:0") if want_delta else None
+    torch.cuda.synchronize()
+    r.backward(g_comp, g_lum, g_trn, lg, dg)
+    r.synchronize()
+    return lg.cpu().numpy().astype(np.float64), None if dg is None else dg.cpu().numpy().astype(np.float64)
+
+
+def test_training_forward_matches_render(renderer):
+    """ps_render_train writes exactly what ps_render writes."""
+    for w in (scene.workload_small(), scene.workload_adjoint(width=320, height=180)):
+        _log(f"case: training forward vs render, {w.name}")
+        _load(renderer, w)
+        renderer.set_cloud_extensions(None, None)
+        a = _render(renderer)
+        b = _render(renderer, train=True)
+        for k in a:
+            assert np.array_equal(a[k].cpu().numpy(), b[k].cpu().numpy()), k
+
+
+def test_extensions_forward_matches_oracle(renderer):
+    """Extinction scales and a perturbation volume: GPU forward vs the FP64
+    oracle's render with the same extensions."""
+    w = scene.workload_adjoint(width=320, height=180)
+    _log("case: extension forward parity, 320x180 two-layer")
+    fr = _load(renderer, w)
+    tgt = scene.adjoint_target(w.layers)
+    d_dev = _delta_tensor(tgt.delta)
+    _set_layers(renderer, w, tgt.density_scale, tgt.coverage)
+    renderer.set_cloud_extensions(tgt.extinction_scale.tolist(), d_dev)
+    out = _render(renderer)
+    renderer.set_cloud_extensions(None, None)
+    diff = oracle.Diff(2, density_scale=tgt.density_scale, extinction_scale=tgt.extinction_scale,
+                       coverage=tgt.coverage, delta=tgt.delta)
+    comp, lum, trn, _ = fr.render_ext(diff, 0, 0, w.width, w.height)
+    for name, ref in (("composite", comp), ("cloud_luminance", lum), ("cloud_transmittance", trn)):
+        e = oracle.relative_error(out[name].cpu().numpy(), ref)
+        _log(f"  {name}: p99.9 {np.percentile(e, 99.9):.3e} median {np.median(e):.3e}")
+        assert np.percentile(e, 99.9) <= 1e-3 and np.median(e) <= 1e-5, name
+    assert (trn[..., 1] < 0.9).mean() > 0.05, "the workload should be cloudy"
+
+
+def test_neutral_extensions_match_plain_render(renderer):
+    """A zero perturbation volume and unit extinction scales reproduce the
+    plain render (the perturbation path only adds an exact zero)."""
+    w = scene.workload_adjoint(width=320, height=180)
+    _log("case: neutral extensions vs plain render")
+    _load(renderer, w)
+    renderer.set_cloud_extensions(None, None)
+    plain = _render(renderer)
+    z = _delta_tensor(np.zeros((128, 128, 128), np.float32))
+    renderer.set_cloud_extensions([1.0, 1.0], z)
+    ext = _render(renderer)
+    renderer.set_cloud_extensions(None, None)
+    for k in plain:
+        a, b = plain[k].cpu().numpy(), ext[k].cpu().numpy()
+        e = oracle.relative_error(b, a)
+        assert e.max() <= 1e-5, (k, e.max())
+
+
+def _problem(r, width=640, height=360):
+    """The differentiation workload: target rendered with the perturbed
+    parameters; the current point is the unperturbed stack with a small
+    random perturbation volume (so delta gradients are exercised away from
+    zero)."""
+    import torch
+    w = scene.workload_adjoint(width=width, height=height)
+    fr = _load(r, w)
+    tgt = scene.adjoint_target(w.layers)
+    _set_layers(r, w, tgt.density_scale, tgt.coverage)
+    r.set_cloud_extensions(tgt.extinction_scale.tolist(), _delta_tensor(tgt.delta))
+    target = _render(r)["composite"].clone()
+    ds0 = np.array([L.density_scale for L in w.layers], np.float32)
+    cov0 = np.array([L.coverage for L in w.layers], np.float32)
+    kap0 = np.ones(len(w.layers), np.float32)
+    delta0 = (np.random.default_rng(0xC40).normal(0.0, 0.02, (128, 128, 128))).astype(np.float32)
+    d0 = _delta_tensor(delta0)
+    _set_layers(r, w, ds0, cov0)
+    r.set_cloud_extensions(kap0.tolist(), d0)
+    torch.cuda.synchronize()
+    return w, fr, target, (ds0, kap0, cov0, delta0, d0)
+
+
+def _params_for(lg, dg, n_layers, n_delta, rng, floor=1e-3):
+    params = [(k, l) for k in (DS, KAPPA, COV) for l in range(n_layers)]
+    mag = np.abs(dg)
+    cand = np.nonzero(mag >= floor * mag.max())[0]
+    pick = rng.choice(cand, size=min(n_delta, cand.size), replace=False)
+    return params + [(DELTA, int(v)) for v in sorted(pick)]
+
+
+def _gpu_value(lg, dg, kind, idx):
+    if kind == DELTA:
+        return dg[idx]
+    return lg[{DS: 0, KAPPA: 8, COV: 16}[kind] + idx]
+
+
+def _check(params, got, ref, what):
+    got, ref = np.asarray(got), np.asarray(ref)
+    worst = 0.0
+    for kind in (DS, KAPPA, COV, DELTA):
+        sel = [i for i, (k, _) in enumerate(params) if k == kind]
+        if not sel:
+            continue
+        floor = 1e-3 * np.abs(ref[sel]).max()
+        for i in sel:
+            rel = abs(got[i] - ref[i]) / max(abs(ref[i]), floor)
+            worst = max(worst, rel)
+            assert rel <= 1e-3, (what, params[i], got[i], ref[i], rel)
+    _log(f"  {what}: {len(params)} parameters, max relative error {worst:.3e}")
+
+
+def test_gradients_match_oracle_dual(renderer):
+    """GPU gradients of a windowed L2 loss vs the oracle's exact FP64
+    derivatives (dual numbers) on the per-layer parameters and 10 influential
+    perturbation voxels."""
+    import torch
+    w, fr, target, (ds0, kap0, cov0, delta0, d0) = _problem(renderer)
+    _log("case: backward vs oracle dual numbers, 640x360 two-layer, 96x64 loss window")
+    out = _render(renderer, train=True)
+    # the cloudiest 96x64 window on a 32-pixel grid
+    cl = (out["cloud_transmittance"][..., 1] < 0.9).float().cpu().numpy()
+    cw, ch = 96, 64
+    x0, y0 = max(((x, y) for x in range(0, 640 - cw + 1, 32) for y in range(0, 360 - ch + 1, 32)),
+                 key=lambda p: cl[p[1]:p[1] + ch, p[0]:p[0] + cw].mean())
+    comp = out["composite"]
+    g = torch.zeros_like(comp)
+    g[y0:y0 + ch, x0:x0 + cw, :3] = 2.0 * (comp - target)[y0:y0 + ch, x0:x0 + cw, :3]
+    lg, dg = _grads(renderer, g_comp=g.contiguous())
+    diff = oracle.Diff(2, density_scale=ds0, extinction_scale=kap0, coverage=cov0, delta=delta0)
+    c_or, _, t_or, _ = fr.render_ext(diff, x0, y0, cw, ch)
+    assert (t_or[..., 1] < 0.9).mean() > 0.05, "the loss window should be cloudy"
+    gw = np.zeros((ch, cw, 4))
+    gw[..., :3] = 2.0 * (c_or[..., :3] - target[y0:y0 + ch, x0:x0 + cw, :3].cpu().numpy())
+    params = _params_for(lg, dg, 2, 10, np.random.default_rng(1))
+    ref = fr.vjp_clouds(diff, params, x0, y0, cw, ch, g_comp=gw)
+    got = [_gpu_value(lg, dg, k, i) for k, i in params]
+    _check(params, got, ref, "GPU vs FP64 dual")
+
+
+def test_gradients_match_fp64_finite_differences(renderer):
+    """The differentiation workload's check: GPU gradients of the L2 loss
+    against a target rendered with perturbed density_scale, extinction,
+    coverage and perturbation volume, vs FP64 central finite differences
+    (h = 1e-4) of the oracle's loss on 32 random parameters (the 6 per-layer
+    parameters and 26 perturbation voxels drawn among those the loss
+    depends on). The loss window is the central 320x180 of the 640x360
+    frame, so each finite difference costs two quarter-frame FP64 renders."""
+    import torch
+    w, fr, target, (ds0, kap0, cov0, delta0, d0) = _problem(renderer)
+    _log("case: backward vs FP64 finite differences, 640x360, 32 parameters")
+    x0, y0, cw, ch = 160, 90, 320, 180
+    out = _render(renderer, train=True)
+    comp = out["composite"]
+    g = torch.zeros_like(comp)
+    g[y0:y0 + ch, x0:x0 + cw, :3] = 2.0 * (comp - target)[y0:y0 + ch, x0:x0 + cw, :3]
+    lg, dg = _grads(renderer, g_comp=g.contiguous())
+    tgt_win = target[y0:y0 + ch, x0:x0 + cw, :3].cpu().numpy().astype(np.float64)
+    diff = oracle.Diff(2, density_scale=ds0, extinction_scale=kap0, coverage=cov0, delta=delta0)
+
+    def loss(d):
+        c, _, _, cnt = fr.render_ext(d, x0, y0, cw, ch, counters=True)
+        return float(((c[..., :3] - tgt_win) ** 2).sum()), cnt
+
+    params = _params_for(lg, dg, 2, 26, np.random.default_rng(0xC4F))
+    assert len(params) == 32
+    fd = []
+    _, c0 = loss(diff)
+    for kind, idx in params:
+        h = 1e-4
+        while True:
+            lp, cp = loss(diff.perturbed(kind, idx, h))
+            lm, cm = loss(diff.perturbed(kind, idx, -h))
+            if (np.array_equal(cp, c0) and np.array_equal(cm, c0)) or h < 1e-9:
+                break
+            h /= 10
+        fd.append((lp - lm) / (2 * h))
+    got = [_gpu_value(lg, dg, k, i) for k, i in params]
+    _check(params, got, fd, "GPU vs FP64 finite differences")
+
+
+def test_checkpoint_interval_does_not_change_gradients(renderer):
+    """Checkpoints only bound the recomputation: intervals of 4, 16 and 32
+    steps give the same gradients (to FP32 summation order)."""
+    import torch
+    w, fr, target, _ = _problem(renderer, 320, 180)
+    _log("case: checkpoint intervals 4 / 16 / 32")
+    res = []
+    for k in (4, 16, 32):
+        renderer.set_checkpoint_interval(k)
+        out = _render(renderer, train=True)
+        g = (2.0 * (out["composite"] - target)).contiguous()
+        g[..., 3] = 0.0
+        res.append(_grads(renderer, g_comp=g))
+    renderer.set_checkpoint_interval(16)
+    for lg, dg in res[1:]:
+        assert np.allclose(lg, res[0][0], rtol=1e-4, atol=1e-6 * np.abs(res[0][0]).max())
+        assert np.allclose(dg, res[0][1], rtol=1e-3, atol=1e-5 * np.abs(res[0][1]).max())
+    renderer.set_cloud_extensions(None, None)
+
+
+def test_autograd_function_matches_abi(renderer):
+    """The torch.autograd binding returns the gradients ps_backward gives
+    for the same loss, and its composite equals ps_render's."""
+    import torch
+    from pedalsky_amd import autograd as ag
+    w, fr, target, (ds0, kap0, cov0, delta0, d0) = _problem(renderer, 320, 180)
+    _log("case: torch.autograd binding")
+    out = _render(renderer, train=True)
+    g = (2.0 * (out["composite"] - target)).contiguous()
+    g[..., 3] = 0.0
+    lg, dg = _grads(renderer, g_comp=g)
+    ds = torch.tensor(ds0, requires_grad=True)
+    kap = torch.tensor(kap0, requires_grad=True)
+    cov = torch.tensor(cov0, requires_grad=True)
+    delta = d0.clone().requires_grad_(True)
+    comp, lum, trn = ag.render(renderer, w.clouds, w.layers, w.winds, ds, kap, cov, delta)
+    assert torch.equal(comp, out["composite"])
+    loss = ((comp[..., :3] - target[..., :3]) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    assert np.allclose(ds.grad.numpy(), lg[:2], rtol=1e-5)
+    assert np.allclose(kap.grad.numpy(), lg[8:10], rtol=1e-5)
+    assert np.allclose(cov.grad.numpy(), lg[16:18], rtol=1e-5)
+    assert np.allclose(delta.grad.cpu().numpy(), dg, rtol=1e-4, atol=1e-6 * np.abs(dg).max())
+    renderer.set_cloud_extensions(None, None)
+
+
+def test_backward_error_paths(renderer):
+    lib = renderer.lib
+    g = abi.CloudGrads(None, None)
+    w = scene.workload_small()
+    _log("case: backward error paths")
+    _load(renderer, w)
+    renderer.set_cloud_extensions(None, None)  # invalidates any training forward
+    assert lib.ps_backward(renderer.ctx, None, None, None, C.byref(g)) == abi.PS_ERR_NOT_READY
+    assert lib.ps_set_checkpoint_interval(renderer.ctx, 0) == abi.PS_ERR_INVALID_ARGUMENT
+    assert lib.ps_set_checkpoint_interval(renderer.ctx, 33) == abi.PS_ERR_INVALID_ARGUMENT
+    _render(renderer, train=True)
+    assert lib.ps_backward(renderer.ctx, None, None, None, C.byref(g)) == abi.PS_ERR_INVALID_ARGUMENT
+    bad = (C.c_float * 1)(-1.0)
+    assert lib.ps_set_cloud_extensions(renderer.ctx, bad, 1, None) == abi.PS_ERR_INVALID_ARGUMENT
