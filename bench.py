@@ -74,6 +74,93 @@ def bench_batch(r, n_scenes: int) -> dict:
             "note": "every scene bakes its own LUT set; noise and weather map shared"}
 
 
+def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
+    """Forward + adjoint of the two-layer march: per step the LUT chain, the
+    training forward (transmittance checkpoints every 16 steps), the L2
+    cotangent against a target rendered with perturbed density_scale,
+    extinction scale, coverage and perturbation volume, and the backward
+    (per-layer gradients and the 128^3 perturbation-volume gradient).
+    Timed beside the plain forward frame at the same size."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from pedalsky_amd import abi, scene
+    w = scene.workload_adjoint(0, width=W, height=H)
+    r.load_workload(w)
+    tgt = scene.adjoint_target(w.layers)
+    dev = "cuda:0"
+
+    def set_params(ds, cov, kap, delta):
+        ls = []
+        for L, d, c in zip(w.layers, ds, cov):
+            L2 = abi.CloudLayer.from_buffer_copy(L)
+            L2.density_scale, L2.coverage = float(d), float(c)
+            ls.append(L2)
+        r.set_clouds(w.clouds, ls, w.winds)
+        r.set_cloud_extensions([float(k) for k in kap], delta)
+
+    d_tgt = torch.from_numpy(tgt.delta.reshape(-1)).to(dev)
+    set_params(tgt.density_scale, tgt.coverage, tgt.extinction_scale, d_tgt)
+    bufs = r.alloc_outputs(W, H, abi.PS_PIXEL_F32, ("composite",))
+    out = r.image_out(bufs, abi.PS_PIXEL_F32)
+    r.frame(out)
+    r.synchronize()
+    target = bufs["composite"].clone()
+    d0 = torch.zeros(128 ** 3, dtype=torch.float32, device=dev)
+    set_params([L.density_scale for L in w.layers], [L.coverage for L in w.layers], [1.0] * len(w.layers), d0)
+    lg = torch.zeros(24, dtype=torch.float32, device=dev)
+    dg = torch.zeros(128 ** 3, dtype=torch.float32, device=dev)
+    g = torch.zeros_like(target)
+    lib = r.lib
+    grads = abi.CloudGrads(lg.data_ptr(), dg.data_ptr())
+    depth = C.c_void_p(r.depth_ptr)
+    # one stream for the library and torch's cotangent kernels: no host
+    # synchronisation inside a step
+    r.synchronize()
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()  # default priority
+    r.set_stream(stream.cuda_stream)
+
+    def fwd_adj():
+        abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
+        abi.check(lib.ps_render_train(r.ctx, depth, C.byref(out)))
+        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)
+        g[..., 3] = 0.0
+        abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads)))
+
+    def fwd():
+        abi.check(lib.ps_frame(r.ctx, abi.PS_LUT_ALL, depth, C.byref(out)))
+
+    res = {}
+    for name, fn, ext in (("fwd", fwd, False), ("fwd_delta", fwd, True), ("fwd_adjoint", fwd_adj, True)):
+        if ext:
+            r.set_cloud_extensions([1.0] * len(w.layers), d0)
+        else:
+            r.set_cloud_extensions(None, None)
+        with torch.cuda.stream(stream):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps * 1e3
+    r.set_stream(None)
+    r.set_cloud_extensions(None, None)
+    mp = W * H / 1e6
+    return {"workload": f"{W}x{H} cumulus + cirrus, 128/6 steps, perturbation volume + extinction scales, "
+                        f"L2 loss vs a perturbed target, checkpoints every 16 steps",
+            "steps": steps, "ms_fwd": round(res["fwd"], 4), "ms_fwd_delta": round(res["fwd_delta"], 4),
+            "ms_fwd_adjoint": round(res["fwd_adjoint"], 4),
+            "value": round(mp / (res["fwd_adjoint"] * 1e-3), 3), "unit": "Mpix/s",
+            "fwd_mpix_s": round(mp / (res["fwd"] * 1e-3), 3),
+            "fwd_adjoint_over_fwd": round(res["fwd_adjoint"] / res["fwd"], 3),
+            "note": "fwd = plain frame (LUT chain + render); fwd_delta = the same with the perturbation "
+                    "volume and extinction scales; fwd_adjoint = LUT chain + training forward (checkpoints) + "
+                    "L2 cotangent + ps_backward (layer and volume gradients), all on one stream"}
+
+
 def measured_traffic():
     """HBM bytes per k_render launch from the committed counter summary
     (FETCH_SIZE x its copy-calibrated factor + WRITE_SIZE, rocprofv3 --pmc
@@ -94,6 +181,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=16)
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
@@ -218,6 +307,11 @@ def main() -> int:
                      "hbm_GBps": round(hbm_bytes / render_s / 1e9, 1)},
         "work_per_frame": {k: v / len(used) for k, v in cnt_sum.items()},
     }
+    if not args.no_adjoint:
+        log("forward + adjoint")
+        result["fwd_adjoint"] = bench_adjoint(r, W, H, args.adjoint_steps, 3)
+        result["fwd_adjoint_640x360"] = bench_adjoint(r, 640, 360, args.adjoint_steps, 3)
+        r.load_workload(w0)
     if not args.no_batch:
         result["batched"] = bench_batch(r, args.batch_scenes)
     if not args.no_cpu_baseline:

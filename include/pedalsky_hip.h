@@ -407,6 +407,33 @@ ps_status ps_backward(ps_ctx* ctx, const float* g_composite, const float* g_lumi
                       const float* g_transmittance, const ps_cloud_grads* grads);
 
 /* ------------------------------------------------------------------------ */
+/* Atmosphere calibration. No reference counterpart: the reference bakes the
+ * LUT chain forward only (AtmosphereSubsystem::dispatch_pass,
+ * crates/ps-atmosphere/src/lib.rs:494-584). The image differentiated here is
+ * the sky + aerial-perspective image of the context's frame: the sky pass
+ * (sky.wgsl:159-220, no cloud layers) where the depth buffer is far (0), and
+ * where it holds a surface, that surface's radiance `ground_radiance`
+ * attenuated and in-scattered by the AP froxels at its distance
+ * (L = ground_radiance * ap.a + ap.rgb, the AP application of
+ * pbr.wgsl:653-655 with the ground BRDF replaced by a constant). Frame, LUT
+ * sizes and the non-differentiated atmosphere fields come from the context.
+ * Coefficient sets are 14 floats: rayleigh_scattering[3], mie_scattering[3],
+ * mie_absorption[3], ozone_absorption[3], rayleigh_scale_height_m,
+ * mie_scale_height_m.                                                      */
+/* ------------------------------------------------------------------------ */
+/* The image for the context's atmosphere: W*H RGBA FP32 into device `out`.
+ * Bakes its own LUT set (the context's LUTs are not touched). */
+ps_status ps_sky_ap_image(ps_ctx* ctx, const float* depth, float ground_radiance, float* out);
+/* For n coefficient sets (device n*14 floats): bake each set's LUT chain and
+ * its tangents, render the image, and write to device `out` (n*16 floats)
+ * per set the gradient of sum over pixels and RGB of (image - target)^2 with
+ * respect to the 14 coefficients (out[16 s + 0..13]) and that loss
+ * (out[16 s + 14]). `target`: device W*H RGBA FP32 (alpha ignored).
+ * `image` (device W*H RGBA, or NULL) receives set 0's image. */
+ps_status ps_atmosphere_grad(ps_ctx* ctx, int32_t n, const float* coeffs, const float* depth,
+                             float ground_radiance, const float* target, float* out, float* image);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side synthesis helpers (CPU, no device work).                       */
 /* ------------------------------------------------------------------------ */
 /* AtmosphereParams::default (weather.rs:60-84), CloudParamsGpu::default

@@ -106,6 +106,7 @@ def load():
         "or_render_pixels": (None, [P, P, i, P, P, P, i]),
         "or_render_ext": (None, [P, P, i, i, i, i, P, P, P, P, i]),
         "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i]),
+        "or_sky_ap_image": (None, [P, P, i, i, i, i, i, i, i, i, i, d, i, i, i, i, P, i]),
         "or_noise_base": (None, [C.c_uint32, i, i, P, i]),
         "or_noise_detail": (None, [C.c_uint32, P, i]),
         "or_noise_curl": (None, [C.c_uint32, P, i]),
@@ -348,6 +349,35 @@ def _frame_vjp_clouds(self, diff: Diff, params, x0, y0, w, h, g_comp=None, g_lum
         if rc != 0:
             raise ValueError("or_vjp_clouds rejected its arguments")
         out[s0:s0 + len(chunk)] = g
+    return out
+
+
+def atmosphere_coeffs(atm) -> np.ndarray:
+    """The 14 differentiated atmosphere coefficients of an AtmosphereParams
+    in double: rayleigh_scattering[3], mie_scattering[3], mie_absorption[3],
+    ozone_absorption[3], rayleigh_scale_height_m, mie_scale_height_m."""
+    return np.array(list(atm.rayleigh_scattering[:3]) + list(atm.mie_scattering[:3]) +
+                    list(atm.mie_absorption[:3]) + list(atm.ozone_absorption[:3]) +
+                    [atm.rayleigh_scale_height_m, atm.mie_scale_height_m], dtype=np.float64)
+
+
+def sky_ap_image(workload, coeffs=None, ground_radiance=1000.0, region=None, depth="workload"):
+    """FP64 sky + aerial-perspective image (LUT chain baked with `coeffs`):
+    (h, w, 3). The frame, atmosphere and depth buffer come from `workload`."""
+    dims = workload.lut_dims
+    W, H = workload.width, workload.height
+    x0, y0, w, h = region if region is not None else (0, 0, W, H)
+    d = workload.depth if isinstance(depth, str) else depth
+    dd = None if d is None else np.ascontiguousarray(d, dtype=np.float32)
+    inp = OrInputs()
+    inp.frame = _ref(workload.frame)
+    inp.atm = _ref(workload.atmosphere)
+    inp.depth = _p(dd)
+    k = None if coeffs is None else np.ascontiguousarray(coeffs, dtype=np.float64)
+    out = np.zeros((h, w, 3))
+    load().or_sky_ap_image(C.byref(inp), _p(k), dims.transmittance_w, dims.transmittance_h, dims.multiscatter_w,
+                           dims.multiscatter_h, dims.skyview_w, dims.skyview_h, dims.ap_w, dims.ap_h, dims.ap_d,
+                           float(ground_radiance), x0, y0, w, h, _p(out), THREADS)
     return out
 
 

@@ -327,6 +327,69 @@ int or_vjp_clouds(const or_inputs* in, const or_diff* d, int x0, int y0, int w, 
     return 0;
 }
 
+// ---- atmosphere differentiation: sky + AP image with FP64 coefficients ----------------------
+// coeffs (14 doubles, or null = the struct's values): rayleigh_scattering[3],
+// mie_scattering[3], mie_absorption[3], ozone_absorption[3],
+// rayleigh_scale_height_m, mie_scale_height_m.
+static void apply_coeffs(Atm<double>& a, const double* k) {
+    if (!k) return;
+    a.ray_s = V3<double>(k[0], k[1], k[2]);
+    a.mie_s = V3<double>(k[3], k[4], k[5]);
+    a.mie_a = V3<double>(k[6], k[7], k[8]);
+    a.ozone = V3<double>(k[9], k[10], k[11]);
+    a.hr = k[12];
+    a.hm = k[13];
+}
+
+// Bakes the four LUTs in FP64 with the coefficients, then renders the w x h
+// region at (x0, y0) of the sky + AP image (sky_ap_pixel) into out (w*h*3).
+// Only frame, atm and depth of `in` are read. luts_out (optional, may be
+// null): T, MS, SV, AP concatenated as RGBA doubles.
+void or_sky_ap_image(const or_inputs* in, const double* coeffs, int tw, int th, int mw, int mh, int sw, int sh,
+                     int aw, int ah, int ad, double ground_radiance, int x0, int y0, int w, int h, double* out,
+                     int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    Atm<double> a = atm_from<double>(*in->atm);
+    apply_coeffs(a, coeffs);
+    Frame<double> f = frame_from<double>(*in->frame);
+    std::vector<double> T(size_t(tw) * th * 4), MS(size_t(mw) * mh * 4), SV(size_t(sw) * sh * 4),
+        AP(size_t(aw) * ah * ad * 4);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < tw * th; ++i) {
+        int x = i % tw, y = i / tw;
+        V3<double> p, dir;
+        transmittance_lut_uv_to_pos_dir(a, (x + 0.5) / tw, (y + 0.5) / th, p, dir);
+        V3<double> t = vexp(-integrate_optical_depth(a, p, dir, 40));
+        double* o = &T[size_t(i) * 4];
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = 1.0;
+    }
+    Tex<double, double> tt; tt.data = T.data(); tt.w = tw; tt.h = th; tt.ch = 4;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < mw * mh; ++i)
+        bake_multiscatter_texel(a, tt, (i % mw + 0.5) / mw, (i / mw + 0.5) / mh, &MS[size_t(i) * 4]);
+    Tex<double, double> mt; mt.data = MS.data(); mt.w = mw; mt.h = mh; mt.ch = 4;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < sw * sh; ++i)
+        bake_skyview_texel(f, a, tt, mt, (i % sw + 0.5) / sw, (i / sw + 0.5) / sh, &SV[size_t(i) * 4]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < aw * ah * ad; ++i)
+        bake_ap_froxel(f, a, tt, mt, i % aw, (i / aw) % ah, i / (aw * ah), aw, ah, ad, &AP[size_t(i) * 4]);
+    Scene<double> s{};
+    s.f = f;
+    s.a = a;
+    s.depth = in->depth;
+    s.T.data = T.data(); s.T.w = tw; s.T.h = th;
+    s.SV.data = SV.data(); s.SV.w = sw; s.SV.h = sh;
+    s.AP.data = AP.data(); s.AP.w = aw; s.AP.h = ah; s.AP.d = ad;
+    s.n_layers = 0;
+    s.overcast = nullptr;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < w * h; ++i) {
+        V3<double> v = sky_ap_pixel(s, x0 + i % w, y0 + i / w, ground_radiance);
+        out[size_t(i) * 3] = v.x; out[size_t(i) * 3 + 1] = v.y; out[size_t(i) * 3 + 2] = v.z;
+    }
+}
+
 // ---- noise bakes (decoded float values before storage quantisation) ---------------------
 void or_noise_base(uint32_t seed, int z0, int z1, double* out, int nthreads) {
     if (nthreads > 0) omp_set_num_threads(nthreads);

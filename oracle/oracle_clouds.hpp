@@ -718,6 +718,35 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
     out_trans = trans;
 }
 
+// Sky + aerial-perspective image (the atmosphere calibration image; no
+// reference counterpart as a pass): the sky pass (sky.wgsl:159-220) where the
+// view ray meets no opaque surface, and where the depth buffer holds one,
+// that surface's radiance attenuated and in-scattered by the AP froxels the
+// way pbr.wgsl:653-655 applies them (L = L_ground * ap.a + ap.rgb, sampled at
+// the surface distance with the mapping of atmosphere_lut_sampling.wgsl:33-38).
+// The ground BRDF is out of scope: the surface radiance is a constant.
+template <class R>
+inline V3<R> sky_ap_pixel(const Scene<R>& s, int ix, int iy, R ground_radiance) {
+    R fx = R(ix) + R(0.5), fy = R(iy) + R(0.5);
+    R ndc_x = (fx / s.f.vw) * R(2) - R(1), ndc_y = R(1) - (fy / s.f.vh) * R(2);
+    if (s.depth) {
+        R dn = s.depth[size_t(iy) * size_t(value_of(s.f.vw)) + ix];
+        if (dn > R(0)) {
+            R dv[4] = {ndc_x, ndc_y, dn, R(1)}, wh[4];
+            mat4_mul(s.f.inv_vp, dv, wh);
+            V3<R> wp(wh[0] / wh[3], wh[1] / wh[3], wh[2] / wh[3]);
+            R d = length(wp - s.f.cam);
+            R d_safe = std::max(d, R(50));
+            R ap_z = clampr<R>(log(d_safe / R(50)) / log(R(100000) / R(50)), 0, 1);
+            R ap[4];
+            sample3d(s.AP, fx / s.f.vw, fy / s.f.vh, ap_z, false, ap);
+            return V3<R>(ground_radiance * ap[3] + ap[0], ground_radiance * ap[3] + ap[1],
+                         ground_radiance * ap[3] + ap[2]);
+        }
+    }
+    return sky_pass_pixel(s, ndc_x, ndc_y);
+}
+
 // One output pixel: sky pass, cloud march, dual-source composite
 // (cloud_composite.wgsl:47-57 with blend dst = src0 + dst * src1).
 template <class R>

@@ -45,6 +45,7 @@ class CloudRender(torch.autograd.Function):
         renderer.render_train(renderer.image_out(bufs, abi.PS_PIXEL_F32))
         ctx.renderer = renderer
         ctx.n = n
+        ctx.devices = (density_scale.device, extinction_scale.device, coverage.device)
         ctx.has_delta = density_delta is not None
         ctx.delta_shape = None if density_delta is None else density_delta.shape
         renderer.synchronize()
@@ -65,7 +66,8 @@ class CloudRender(torch.autograd.Function):
         r.backward(gc, gl, gt, layer_grads, delta_grad)
         r.synchronize()
         n = ctx.n
-        g_ds, g_k, g_cov = layer_grads[:n], layer_grads[8:8 + n], layer_grads[16:16 + n]
+        dd, dk, dc = ctx.devices
+        g_ds, g_k, g_cov = layer_grads[:n].to(dd), layer_grads[8:8 + n].to(dk), layer_grads[16:16 + n].to(dc)
         gd = None if delta_grad is None else delta_grad.view(ctx.delta_shape)
         return None, None, None, None, g_ds, g_k, g_cov, gd
 

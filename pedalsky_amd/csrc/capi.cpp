@@ -409,6 +409,10 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
     dev_free(c->delta_bmax); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec);
+    dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
+    dev_free(c->aTT); dev_free(c->aMST); dev_free(c->aSVT); dev_free(c->aAPT);
+    if (c->atmo_params_h) hipHostFree(c->atmo_params_h);
+    if (c->atmo_ev) hipEventDestroy(c->atmo_ev);
     c->render_ring.release();
     c->lut_ring.release();
     for (hipEvent_t e : c->events) hipEventDestroy(e);
@@ -796,6 +800,134 @@ ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_lumina
     if ((s = c->render_ring.upload(P, c->stream, &dP))) return s;
     PS_HIP(ps_launch_render_backward(dP, P.W, P.H, P.delta != nullptr, c->stream));
     return c->render_ring.mark_used(c->stream);
+}
+
+// ---- atmosphere differentiation ------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// Scratch LUTs, tangent LUTs and parameter blocks for n sets at the context's
+// LUT sizes.
+ps_status ensure_atmo(ps_ctx* c, int n) {
+    const ps_lut_dims& d = c->dims;
+    if (n <= c->atmo_cap && std::memcmp(&d, &c->atmo_dims, sizeof(d)) == 0) return PS_OK;
+    if (c->atmo_ev) PS_HIP(hipEventSynchronize(c->atmo_ev));
+    dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
+    dev_free(c->aTT); dev_free(c->aMST); dev_free(c->aSVT); dev_free(c->aAPT);
+    if (c->atmo_params_h) hipHostFree(c->atmo_params_h);
+    c->atmo_params_h = nullptr;
+    c->atmo_cap = 0;
+    c->atmo_uploaded = 0;
+    const size_t nt = (size_t)d.transmittance_w * d.transmittance_h, nm = (size_t)d.multiscatter_w * d.multiscatter_h,
+                 ns = (size_t)d.skyview_w * d.skyview_h, na = (size_t)d.ap_w * d.ap_h * d.ap_d;
+    ps_status s;
+    if ((s = dev_alloc(&c->atmo_params, sizeof(LutParams) * n, "atmosphere parameter blocks"))) return s;
+    if ((s = dev_alloc(&c->aT, sizeof(float4) * nt * n, "transmittance LUTs"))) return s;
+    if ((s = dev_alloc(&c->aMS, sizeof(float4) * nm * n, "multi-scatter LUTs"))) return s;
+    if ((s = dev_alloc(&c->aSV, sizeof(float4) * ns * n, "sky-view LUTs"))) return s;
+    if ((s = dev_alloc(&c->aAP, sizeof(float4) * na * n, "aerial-perspective LUTs"))) return s;
+    if ((s = dev_alloc(&c->aTT, sizeof(float) * 18 * nt * n, "transmittance tangents"))) return s;
+    if ((s = dev_alloc(&c->aMST, sizeof(float) * 18 * nm * n, "multi-scatter tangents"))) return s;
+    if ((s = dev_alloc(&c->aSVT, sizeof(float) * 18 * ns * n, "sky-view tangents"))) return s;
+    if ((s = dev_alloc(&c->aAPT, sizeof(float) * 36 * na * n, "aerial-perspective tangents"))) return s;
+    PS_HIP(hipHostMalloc((void**)&c->atmo_params_h, sizeof(LutParams) * n, hipHostMallocDefault));
+    if (!c->atmo_ev) PS_HIP(hipEventCreateWithFlags(&c->atmo_ev, hipEventDisableTiming));
+    c->atmo_cap = n;
+    c->atmo_dims = d;
+    return PS_OK;
+}
+
+// The frame geometry of the sky + AP image (the quantities fill_render_params
+// folds for the sky pass).
+void sky_geometry(const psi::SceneIn& sc, SkyApLaunch& L) {
+    const ps_frame_uniforms& f = sc.frame;
+    const ps_atmosphere_params& a = sc.atm;
+    L.W = (int)f.viewport_size[0];
+    L.H = (int)f.viewport_size[1];
+    for (int i = 0; i < 16; ++i) L.inv_vp[i] = f.inv_view_proj[i];
+    const double R = a.planet_radius_m, top = a.atmosphere_top_m;
+    for (int i = 0; i < 3; ++i) {
+        L.cam[i] = f.camera_position_world[i];
+        L.sun_d[i] = f.sun_direction[i];
+        L.sun[i] = f.sun_direction[i];
+        L.E[i] = f.sun_illuminance[i];
+    }
+    L.sun_ang = f.sun_direction[3];
+    L.cos_disk = std::cos((double)f.sun_direction[3]);
+    L.R = (float)R;
+    L.top = (float)top;
+    L.big_h = (float)std::sqrt(std::max(top * top - R * R, 0.0));
+    L.top_minus_R = (float)(top - R);
+    const double oy = L.cam[1] + R;
+    const double r_cam = std::sqrt(L.cam[0] * L.cam[0] + oy * oy + L.cam[2] * L.cam[2]);
+    const double h_cam = (L.cam[0] * L.cam[0] + L.cam[2] * L.cam[2] + L.cam[1] * (L.cam[1] + 2.0 * R)) / (r_cam + R);
+    L.cam_alt = (float)h_cam;
+    const double cam_r = std::max(r_cam, R + 1.0);
+    L.cam_zha = (float)(PS_PI - std::asin(std::min(std::max(R / cam_r, 0.0), 1.0)));
+    L.sun_az = (float)std::atan2(L.sun_d[0], L.sun_d[2]);
+}
+
+// Bake the n value LUT sets (coefficients from `coeffs`, or the context's
+// atmosphere when null), optionally their tangents, and run the image pass.
+ps_status sky_ap_run(ps_ctx* c, int n, const float* coeffs, const float* depth, float ground, const float* target,
+                     float* out, float* image, bool grad) {
+    ps_status s;
+    if ((s = check_ready(c, false))) return s;
+    const float vw = c->sc.frame.viewport_size[0], vh = c->sc.frame.viewport_size[1];
+    if (!(vw >= 1.f && vh >= 1.f && vw <= 32768.f && vh <= 32768.f))
+        return fail(PS_ERR_INVALID_ARGUMENT, "frame.viewport_size must hold the image size");
+    if ((s = ensure_atmo(c, n))) return s;
+    LutParams base;
+    fill_lut_params(c->sc, c->dims, base);
+    // The device blocks must equal `base` except for the 14 coefficients,
+    // which ps_launch_set_coeffs rewrites when `coeffs` is given. Restage
+    // when the base changed, when more blocks are needed, or when a call
+    // without coefficients follows a coefficient overwrite.
+    const bool staged = c->atmo_uploaded >= n && std::memcmp(&base, c->atmo_params_h, sizeof(base)) == 0;
+    if (!staged || (!coeffs && c->atmo_coeffs_dirty)) {
+        PS_HIP(hipEventSynchronize(c->atmo_ev));
+        for (int i = 0; i < n; ++i) c->atmo_params_h[i] = base;
+        PS_HIP(hipMemcpyAsync(c->atmo_params, c->atmo_params_h, sizeof(LutParams) * n, hipMemcpyHostToDevice,
+                              c->stream));
+        PS_HIP(hipEventRecord(c->atmo_ev, c->stream));
+        c->atmo_uploaded = n;
+        c->atmo_coeffs_dirty = false;
+    }
+    if (coeffs) {
+        PS_HIP(ps_launch_set_coeffs(c->atmo_params, coeffs, n, c->stream));
+        c->atmo_coeffs_dirty = true;
+    }
+    PS_HIP(ps_launch_luts(c->atmo_params, base, n, 15u, c->aT, c->aMS, c->aSV, c->aAP, c->stream, nullptr));
+    if (grad)
+        PS_HIP(ps_launch_lut_tangents(c->atmo_params, base, n, c->aT, c->aMS, c->aTT, c->aMST, c->aSVT, c->aAPT,
+                                      c->stream));
+    SkyApLaunch L{};
+    sky_geometry(c->sc, L);
+    L.lp = c->atmo_params;
+    L.T = c->aT; L.SV = c->aSV; L.AP = c->aAP;
+    L.TT = c->aTT; L.SVT = c->aSVT; L.APT = c->aAPT;
+    L.depth = depth; L.target = target; L.out = out; L.image = image;
+    L.ground = ground;
+    L.want_grad = grad ? 1 : 0;
+    PS_HIP(ps_launch_sky_ap_grad(L, n, c->stream));
+    return PS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ps_status ps_sky_ap_image(ps_ctx* c, const float* depth, float ground_radiance, float* out) {
+    if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_sky_ap_image: NULL argument");
+    return sky_ap_run(c, 1, nullptr, depth, ground_radiance, nullptr, nullptr, out, false);
+}
+
+ps_status ps_atmosphere_grad(ps_ctx* c, int32_t n, const float* coeffs, const float* depth, float ground_radiance,
+                             const float* target, float* out, float* image) {
+    if (!c || !coeffs || !target || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_atmosphere_grad: NULL argument");
+    if (n < 1 || n > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "ps_atmosphere_grad: n must be in [1, 4096]");
+    return sky_ap_run(c, n, coeffs, depth, ground_radiance, target, out, image, true);
 }
 
 }  // extern "C"

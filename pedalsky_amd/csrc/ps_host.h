@@ -157,6 +157,17 @@ struct ps_ctx {
     size_t rec_cap = 0;
     RenderParams train{};         // parameter block of the last training forward
     bool train_valid = false;
+
+    // atmosphere differentiation: LUT and tangent-LUT sets for n parameter sets
+    int atmo_cap = 0;
+    ps_lut_dims atmo_dims{};
+    LutParams* atmo_params = nullptr;     // device, atmo_cap blocks
+    LutParams* atmo_params_h = nullptr;   // pinned staging
+    int atmo_uploaded = 0;                // blocks of atmo_params staged from atmo_params_h
+    bool atmo_coeffs_dirty = false;       // coefficients rewritten on the device since staging
+    hipEvent_t atmo_ev = nullptr;         // last staging copy
+    float4 *aT = nullptr, *aMS = nullptr, *aSV = nullptr, *aAP = nullptr;
+    float *aTT = nullptr, *aMST = nullptr, *aSVT = nullptr, *aAPT = nullptr;
 };
 
 namespace psi {

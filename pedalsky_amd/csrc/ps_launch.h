@@ -42,3 +42,32 @@ hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail
 // Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
 // (Repeat wrap) of the coverage plane, still binary16.
 hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s);
+
+// ---- atmosphere differentiation (atmo_adjoint.hip) ----------------------------
+// Overwrite the 14 differentiated coefficients of n LUT parameter blocks from
+// a device array (n x 14: beta_R[3], beta_Ms[3], beta_Ma[3], beta_O[3], H_R,
+// H_M).
+hipError_t ps_launch_set_coeffs(LutParams* d_params, const float* coeffs, int n, hipStream_t s);
+// Tangent LUTs of n sets (value LUTs T and MS of the same sets must be baked):
+// per set [channel][6 tangents][texel] for T, MS, SV and [channel][12][froxel]
+// for AP (in-scatter tangents, then transmittance tangents).
+hipError_t ps_launch_lut_tangents(const LutParams* d_params, const LutParams& dims, int n, const float4* T,
+                                  const float4* MS, float* TT, float* MST, float* SVT, float* APT, hipStream_t s);
+
+// One frame's sky + AP image over n parameter sets: per set, 14 gradients and
+// the L2 loss against `target` into out[set * 16 + 0..14].
+struct SkyApLaunch {
+    const LutParams* lp;
+    const float4 *T, *SV, *AP;
+    const float *TT, *SVT, *APT;
+    const float* depth;
+    const float* target;
+    float* out;
+    float* image;                     // optional image of set 0
+    int W, H;
+    int want_grad;                    // 0: image only (target, tangents and out unused)
+    float ground;
+    double inv_vp[16], cam[3], sun_d[3], cos_disk, sun_ang;
+    float sun[3], E[3], cam_zha, sun_az, cam_alt, R, top, big_h, top_minus_R;
+};
+hipError_t ps_launch_sky_ap_grad(const SkyApLaunch& L, int n, hipStream_t s);

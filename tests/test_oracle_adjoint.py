@@ -185,3 +185,44 @@ def test_column_transmittance_derivatives_closed_form(delta_value):
         dk, dds = fr.vjp_clouds(diff, [(KAPPA, 0), (DS, 0)], 7, 7, 2, 2, g_trn=g)
         assert dk / 4 == pytest.approx(-sig * ds * path * integral * t_exp, rel=3e-3)
         assert dds / 4 == pytest.approx(-kappa * sig * path * integral * t_exp, rel=3e-3)
+
+
+# ---------------------------------------------------------------------------
+# sky + aerial-perspective image (atmosphere calibration)
+# ---------------------------------------------------------------------------
+def test_sky_ap_image_transparent_atmosphere():
+    """Without scattering or absorption the AP froxels hold no in-scatter and
+    unit transmittance: ground pixels show exactly the surface radiance, sky
+    pixels are black outside the sun disk."""
+    w = scene.workload_full_frame(0, width=160, height=90)
+    k = oracle.atmosphere_coeffs(w.atmosphere)
+    k[:12] = 0.0
+    img = oracle.sky_ap_image(w, k, ground_radiance=1234.5)
+    ground = w.depth > 0
+    assert 0.2 < ground.mean() < 0.8
+    assert np.allclose(img[ground], 1234.5, rtol=1e-12)
+    # sky pixels clearly above the horizon (the sky-view texels next to the
+    # horizon blend in the ground bounce below it); the sun (zenith 65 deg)
+    # is outside this view
+    _, d = scene.pixel_rays(w.frame)
+    high = ~ground & (d[..., 1] > np.sin(np.radians(3.0)))
+    assert high.mean() > 0.05
+    assert np.abs(img[high]).max() == 0.0
+
+
+def test_sky_ap_image_is_affine_in_ground_radiance():
+    """L = L_ground * a + ap.rgb: doubling the surface radiance adds
+    L_ground * a on ground pixels, with 0 < a <= 1 decreasing with distance,
+    and leaves sky pixels unchanged."""
+    w = scene.workload_full_frame(0, width=160, height=90)
+    a1 = oracle.sky_ap_image(w, ground_radiance=1000.0)
+    a2 = oracle.sky_ap_image(w, ground_radiance=2000.0)
+    ground = w.depth > 0
+    assert np.array_equal(a1[~ground], a2[~ground])
+    alpha = (a2 - a1)[ground][:, 0] / 1000.0
+    assert np.allclose((a2 - a1)[ground], alpha[:, None] * 1000.0, rtol=1e-9)
+    assert np.all(alpha > 0.0) and np.all(alpha <= 1.0 + 1e-12)
+    dist = scene.ground_distance(w.frame)[ground]
+    order = np.argsort(dist)
+    # transmittance falls with distance (allow the trilinear froxel interpolation's small ripples)
+    assert np.corrcoef(dist[order], alpha[order])[0, 1] < -0.5
