@@ -426,12 +426,14 @@ ps_status ps_backward(ps_ctx* ctx, const float* g_composite, const float* g_lumi
 ps_status ps_sky_ap_image(ps_ctx* ctx, const float* depth, float ground_radiance, float* out);
 /* For n coefficient sets (device n*14 floats): bake each set's LUT chain and
  * its tangents, render the image, and write to device `out` (n*16 floats)
- * per set the gradient of sum over pixels and RGB of (image - target)^2 with
- * respect to the 14 coefficients (out[16 s + 0..13]) and that loss
- * (out[16 s + 14]). `target`: device W*H RGBA FP32 (alpha ignored).
- * `image` (device W*H RGBA, or NULL) receives set 0's image. */
+ * per set the loss sum over pixels p and RGB c of w_p (image - target)^2
+ * (out[16 s + 14]) and its gradient with respect to the 14 coefficients
+ * (out[16 s + 0..13]). `target`: device W*H RGBA FP32 (alpha ignored);
+ * `weight`: device W*H FP32 per-pixel weights w_p, or NULL for 1. `image`
+ * (device W*H RGBA, or NULL) receives set 0's image. */
 ps_status ps_atmosphere_grad(ps_ctx* ctx, int32_t n, const float* coeffs, const float* depth,
-                             float ground_radiance, const float* target, float* out, float* image);
+                             float ground_radiance, const float* target, const float* weight, float* out,
+                             float* image);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side synthesis helpers (CPU, no device work).                       */

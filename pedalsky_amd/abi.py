@@ -250,7 +250,7 @@ _SIGNATURES = [
     ("ps_render_train", C.c_int32, [_P, _P, C.POINTER(ImageOut)]),
     ("ps_backward", C.c_int32, [_P, _P, _P, _P, C.POINTER(CloudGrads)]),
     ("ps_sky_ap_image", C.c_int32, [_P, _P, C.c_float, _P]),
-    ("ps_atmosphere_grad", C.c_int32, [_P, C.c_int32, _P, _P, C.c_float, _P, _P, _P]),
+    ("ps_atmosphere_grad", C.c_int32, [_P, C.c_int32, _P, _P, C.c_float, _P, _P, _P, _P]),
     ("ps_default_atmosphere", None, [C.POINTER(AtmosphereParams)]),
     ("ps_default_cloud_params", None, [C.POINTER(CloudParams)]),
     ("ps_remap_coverage", C.c_float, [C.c_float]),

@@ -110,24 +110,38 @@ class SkyApProblem:
         self.r.set_atmosphere(self.w.atmosphere)
         return out
 
-    def grad(self, coeffs, target, out=None, image=None, sync=True):
+    def grad(self, coeffs, target, out=None, image=None, sync=True, weight=None):
         """coeffs: (n, 14) float32 CUDA tensor; returns (n, 16) float32 CUDA:
-        14 gradients, the loss, padding."""
+        14 gradients, the loss, padding. `weight`: optional (H, W) float32
+        CUDA per-pixel loss weights."""
         torch = self.torch
         n = coeffs.shape[0]
         if out is None:
             out = torch.empty((n, 16), dtype=torch.float32, device=self.dev)
         abi.check(self.r.lib.ps_atmosphere_grad(self.r.ctx, n, coeffs.data_ptr(), self.r.depth_ptr, self.ground,
-                                                target.data_ptr(), out.data_ptr(),
-                                                None if image is None else image.data_ptr()))
+                                                target.data_ptr(), None if weight is None else weight.data_ptr(),
+                                                out.data_ptr(), None if image is None else image.data_ptr()))
         if sync:
             self.r.synchronize()
         return out
 
 
-def calibrate(problem: SkyApProblem, target, init_mult: np.ndarray, iters: int = 200, lr: float = 1e-2):
-    """Adam on the log group multipliers of every set toward `target`.
-    Returns (final multipliers (n, 6), loss history (iters, n))."""
+def relative_weights(target):
+    """Per-pixel weights that make the L2 loss a relative one:
+    1 / (mean RGB of the target)^2 (floored at 1e-3 of the frame mean)."""
+    m = target[..., :3].mean(dim=-1)
+    return (1.0 / torch_clamp_min(m, 1e-3 * float(m.mean())) ** 2).contiguous()
+
+
+def torch_clamp_min(x, lo):
+    return x.clamp_min(lo)
+
+
+def calibrate(problem: SkyApProblem, target, init_mult: np.ndarray, iters: int = 200, lr: float = 1e-2,
+              weight=None):
+    """Adam on the log group multipliers of every set toward `target`
+    (optionally per-pixel weighted L2). Returns (final multipliers (n, 6),
+    loss history (iters, n))."""
     torch = problem.torch
     dev = problem.dev
     base = torch.tensor(coeffs_of(problem.w.atmosphere), dtype=torch.float32, device=dev)
@@ -147,7 +161,7 @@ def calibrate(problem: SkyApProblem, target, init_mult: np.ndarray, iters: int =
                 with torch.no_grad():
                     mult = torch.exp(s)[:, gmap]            # (n, 14)
                     coeffs = (base * mult).contiguous()
-                problem.grad(coeffs, target, out=out, sync=False)
+                problem.grad(coeffs, target, out=out, sync=False, weight=weight)
                 with torch.no_grad():
                     g14 = out[:, :14] * coeffs              # d loss / d log coeff
                     g6 = torch.zeros_like(s).index_add_(1, gmap, g14)

@@ -188,17 +188,186 @@ __device__ __forceinline__ void light_delta_scatter(const RenderParams& P, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// Deferred dense-sample adjoint shading. The light march, the shading and
+// their adjoints need only the sample, the transmittance in front of it and
+// the owning pixel's output seeds (not the later samples), so dense samples
+// are queued per wave as in the forward and shaded 64 at a time by the whole
+// wave; each result goes to the owner's record slot. Only the transmittance
+// adjoint needs the reverse order; the owner runs it after the chunk's queue
+// has drained.
+// ---------------------------------------------------------------------------
+constexpr int kBQCap = 128;  // >= 64 + 63: a full flush plus one step's arrivals
+enum { BQ_PX = 0, BQ_PY, BQ_PZ, BQ_ALT, BQ_WR, BQ_DENS, BQ_TR, BQ_TG, BQ_TB, BQ_T, BQ_STEP, BQ_META, BQ_FIELDS };
+enum { SD_G0 = 0, SD_G1, SD_G2, SD_GTW, SD_GTWN, SD_OMC, SD_FIELDS };
+
+struct BackQueue {
+    float f[BQ_FIELDS][kBQCap];
+    float seed[SD_FIELDS][64];        // per owner lane: luminance cotangent, depth-weight cotangents, 1 - cos
+    float wacc[3 * PS_MAX_LAYERS];    // per-layer gradient sums of the wave
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Shade queue entry j with its adjoint (cloud_march.wgsl:886-993): writes
+// the owner's record (v, dL/drho through the shading) and adds the
+// light-march and extinction contributions to the wave's layer sums.
 template <bool DELTA>
-__device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x, int tile_y, float* wacc) {
+__device__ __forceinline__ void shade_adj(const RenderParams& P, BackQueue& Q, int j, float* rec_tile,
+                                          bool want_delta) {
+    const float px = Q.f[BQ_PX][j], py = Q.f[BQ_PY][j], pz = Q.f[BQ_PZ][j];
+    const float alt = Q.f[BQ_ALT][j], wr = Q.f[BQ_WR][j], density = Q.f[BQ_DENS][j];
+    const float T[3] = {Q.f[BQ_TR][j], Q.f[BQ_TG][j], Q.f[BQ_TB][j]};
+    const float t = Q.f[BQ_T][j], step = Q.f[BQ_STEP][j];
+    const uint32_t meta = __float_as_uint(Q.f[BQ_META][j]);
+    const int owner = (int)(meta & 63u), li = (int)((meta >> 6) & 7u), slot = (int)((meta >> 12) & 31u);
+    const uint32_t ctype = (meta >> 9) & 7u;
+    const NdfC nc = ndf_coef(ctype);
+    const LayerC& LC = P.layers[li];
+    const LayerF L = LC.f;
+    const float G[3] = {Q.seed[SD_G0][owner], Q.seed[SD_G1][owner], Q.seed[SD_G2][owner]};
+    const float g_tw = Q.seed[SD_GTW][owner], g_twn = Q.seed[SD_GTWN][owner];
+    const float omcf = Q.seed[SD_OMC][owner];
+    const float cosv = 1.f - omcf;
+    const float lw[3] = {0.2126f, 0.7152f, 0.0722f};
+    const float qy = py + P.R;
+    float gds = 0.f, gcov = 0.f, gk = 0.f;
+    // dense sample: light march and shading with their adjoints
+    float dod_ds, dod_cov;
+    uint32_t lin;
+    const float od = light_adj<DELTA>(P, L, nc, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
+    const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
+    float st[3] = {0.f, 0.f, 0.f};
+    {
+        const float hh = fmaxf(alt, 1.f);
+        const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / (P.R + hh), -1.f, 1.f);
+        float tu, tv;
+        if (trans_uv_from_h(P, hh, mu, tu, tv)) {
+            const float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
+            st[0] = tt.x; st[1] = tt.y; st[2] = tt.z;
+        }
+    }
+    float g_hg = LC.g_hg, omg0 = LC.omg_hg, g_d = LC.g_d, alpha = LC.alpha, w_d = LC.w_d, ice = LC.ice;
+    if (LC.is_cb) {
+        const float mt = smoothstepf(0.60f, 0.85f, hn);
+        float d = LC.droplet + (50.f - LC.droplet) * mt;
+        d = clampf(d * P.droplet_bias, 5.f, 50.f);
+        const float xg = -0.0990567f / (d - 1.67154f);
+        g_hg = __expf(xg);
+        omg0 = -expm1f(xg);
+        g_d = __expf(-2.20679f / (d + 3.91029f)) - 0.428934f;
+        alpha = __expf(3.62489f - 8.29288f / (d + 5.52825f));
+        w_d = __expf(-0.599085f / (d - 0.641583f) - 0.665888f);
+        ice = smoothstepf(35.f, 50.f, d);
+    }
+    float S1[3] = {0.f, 0.f, 0.f}, D1[3] = {0.f, 0.f, 0.f}, e0[3];
+    for (int c = 0; c < 3; ++c) e0[c] = __expf(-L.sigma_t[c] * od);
+    for (int n = 0; n < P.octaves; ++n) {
+        const float ph = P.a_pow[n] * phase_octave_f(omcf, cosv, P.c_pow[n], g_hg, omg0, g_d, alpha,
+                                                     w_d, ice);
+        const float bn = P.b_pow[n];
+        for (int c = 0; c < 3; ++c) {
+            const float en = n == 0 ? e0[c] : __expf(-L.sigma_t[c] * od * bn);
+            S1[c] += ph * en;
+            D1[c] += ph * bn * en;
+        }
+    }
+    float fbm = 1.f;
+    if (P.forward_bias > 0.f) fbm = 1.f + P.forward_bias * smoothstepf(0.f, 1.f, fmaxf(cosv, 0.f));
+    float amb[3];
+    {
+        const float rho = sqrtf(fmaxf(alt * (2.f * P.R + alt), 0.f));
+        const float zha = (float)PS_PI - atan2f(P.R, rho);
+        float u, v;
+        skyview_uv_f(P, px, qy, pz, zha, u, v);
+        const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
+        const float g = P.ambient * (0.3f + 0.7f * hn);
+        amb[0] = sv.x * P.E[0] * g; amb[1] = sv.y * P.E[1] * g; amb[2] = sv.z * P.E[2] * g;
+    }
+    if (P.light_radius > 0.f) {
+        const float lx = px - P.light_x, lz = pz - P.light_z;
+        const float fo = clampf(1.f - sqrtf(lx * lx + lz * lz) / P.light_radius, 0.f, 1.f);
+        const float wgt = fo * fo * (3.f - 2.f * fo);
+        for (int c = 0; c < 3; ++c) amb[c] += P.light_rgb[c] * wgt;
+    }
+    const float pw = P.powder;
+    float g_rho = 0.f, g_od = 0.f, v[3], e_step[3];
+    for (int c = 0; c < 3; ++c) {
+        const float sig = L.sigma_t[c];
+        const float en = e0[c] + (2.f * e0[c] * (1.f - e0[c] * e0[c]) - e0[c]) * pw;
+        const float den = -e0[c] * ((1.f - pw) + 2.f * pw * (1.f - 3.f * e0[c] * e0[c]));
+        const float es = P.E[c] * st[c];
+        const float sun = S1[c] * es * fbm, dsun = -D1[c] * es * fbm;
+        const float Qc = sun * en + amb[c];
+        const float dQ = dsun * en + sun * den;  // d Q / d (sigma_t od)
+        const float in_c = P.sigma_s[c] * density * Qc;
+        const float rs = density * sig;
+        const bool lin_t = rs > 1e-6f;
+        const float ls = fmaxf(rs, 1e-6f);
+        const float xx = ls * step;
+        const float ome = one_minus_exp_neg(xx);
+        const float es_t = __expf(-xx);
+        e_step[c] = es_t;
+        const float s_int = in_c * ome / ls;
+        const float g_sl = G[c] + lw[c] * (g_tw * t + g_twn);
+        v[c] = T[c] * s_int * g_sl;
+        const float g_s = T[c] * g_sl;
+        const float K = P.sigma_s[c] * density * ome / ls;
+        const float g_Q = g_s * K;
+        const float dsdr = lin_t ? P.sigma_s[c] * Qc * step * es_t : P.sigma_s[c] * Qc * ome / ls;
+        g_rho += g_s * dsdr;
+        const float dsdsig = lin_t ? -P.sigma_s[c] * Qc * density * step * (density * step * psi_fn(xx)) : 0.f;
+        const float g_sig = g_Q * od * dQ + g_s * dsdsig;
+        g_od += g_Q * sig * dQ;
+        gk += g_sig * LC.sigma_t0[c];
+    }
+    gds += g_od * dod_ds;
+    gcov += g_od * dod_cov;
+    if (want_delta && lin) light_delta_scatter(P, L, px, py, pz, alt, lin, g_od * L.density_scale);
+    float* rp = rec_tile + (size_t)slot * (kRecFields * 64) + owner;
+    rp[REC_V0 * 64] = v[0]; rp[REC_V1 * 64] = v[1]; rp[REC_V2 * 64] = v[2];
+    rp[REC_GRHO * 64] = g_rho;
+    atomicAdd(&Q.wacc[li], gds);
+    atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
+    atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
+}
+
+// Shade the first n (<= 64) queued samples with the whole wave, then move
+// the entries [64, qn) to the front. Called with all 64 lanes active.
+template <bool DELTA>
+__device__ __forceinline__ void flush_adj(const RenderParams& P, BackQueue& Q, int lane, int n, int qn,
+                                          float* rec_tile, bool want_delta) {
+    wave_sync_lds();
+    if (lane < n) shade_adj<DELTA>(P, Q, lane, rec_tile, want_delta);
+    wave_sync_lds();
+    const int rest = qn - n;
+    if (rest > 0) {
+        float v[BQ_FIELDS];
+        if (lane < rest) {
+#pragma unroll
+            for (int k = 0; k < BQ_FIELDS; ++k) v[k] = Q.f[k][64 + lane];
+        }
+        wave_sync_lds();
+        if (lane < rest) {
+#pragma unroll
+            for (int k = 0; k < BQ_FIELDS; ++k) Q.f[k][lane] = v[k];
+        }
+        wave_sync_lds();
+    }
+}
+
+template <bool DELTA>
+__device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x, int tile_y, BackQueue& Q) {
     const int lane = threadIdx.x;
     const int x = tile_x * 8 + (lane & 7);
     const int y = tile_y * 8 + (lane >> 3);
     const bool valid = x < P.W && y < P.H;
     const int xc = min(x, P.W - 1), yc = min(y, P.H - 1);
     const size_t tile = (size_t)tile_y * P.n_tiles_x + tile_x;
-    if (lane < 3 * PS_MAX_LAYERS) wacc[lane] = 0.f;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    if (lane < 3 * PS_MAX_LAYERS) Q.wacc[lane] = 0.f;
 
     const PixelRay ray = pixel_ray(P, xc, yc);
     const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
@@ -279,6 +448,10 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     }
     float A[3] = {g_T[0] * Tf[0], g_T[1] * Tf[1], g_T[2] * Tf[2]};
 
+    Q.seed[SD_G0][lane] = G[0]; Q.seed[SD_G1][lane] = G[1]; Q.seed[SD_G2][lane] = G[2];
+    Q.seed[SD_GTW][lane] = g_tw; Q.seed[SD_GTWN][lane] = g_twn; Q.seed[SD_OMC][lane] = ray.omcf;
+    wave_sync_lds();
+
     // ---- chunks, last to first ----------------------------------------------------------
     const float r0c = fmaxf((float)P.r0, 1.f);
     const float kcam = (float)(2.0 * P.r0 * ray.b / fmax(P.r0, 1.0));
@@ -294,11 +467,19 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     int cur_seg = -1, li = 0;
     float tstart = 0.f, step = 0.f;
     LayerF L;
-    const bool want_delta = DELTA && P.g_delta != nullptr;
-    float* rec = P.rec + tile * (size_t)(ckl * kRecFields * 64) + lane;
+    // perturbation-volume gradients need no volume: at delta = 0 they are
+    // the gradients with respect to an additive perturbation
+    const bool want_delta = P.g_delta != nullptr;
+    float* rec_tile = P.rec + tile * (size_t)(ckl * kRecFields * 64);
+    float* rec = rec_tile + lane;
+    int qn = 0;  // queued samples (wave-uniform)
 
     while (__ballot(chunk >= 0) != 0ull) {
-        if (chunk >= 0) {
+        const bool act = chunk >= 0;
+        int s0 = 0, s1 = 0;
+        float T[3] = {1.f, 1.f, 1.f};
+        uint32_t dmask = 0u;
+        if (act) {
             const int seg = chunk / cps, sub = chunk % cps;
             if (seg != cur_seg) {
                 cur_seg = seg;
@@ -311,22 +492,23 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 tstart = (float)(ta + jitter * stepd);
                 step = (float)stepd;
             }
-            const LayerC& LC = P.layers[li];
-            const int s0 = sub * ckl;
-            const int s1 = min(min(steps, s0 + ckl), n_exec - seg * steps);
+            s0 = sub * ckl;
+            s1 = min(min(steps, s0 + ckl), n_exec - seg * steps);
             const float* ckp = P.ck + (tile * P.ck_max + chunk) * 192 + lane;
-            float T[3] = {ckp[0], ckp[64], ckp[128]};
-            float gds = 0.f, gk = 0.f, gcov = 0.f;
-            uint32_t dmask = 0u;
-            const float cosv = 1.f - ray.omcf;
-
-            // chunk forward: recompute from the checkpoint
-            for (int s = s0; s < s1; ++s) {
-                const float t = tstart + (float)s * step;
-                const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
+            T[0] = ckp[0]; T[1] = ckp[64]; T[2] = ckp[128];
+        }
+        // chunk forward, recomputed from the checkpoint; dense samples queued
+        for (int k = 0; k < ckl; ++k) {
+            const int s = s0 + k;
+            bool dense = false;
+            float t = 0.f, px = 0.f, py = 0.f, pz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
+            uint32_t ctype = 0u;
+            if (act && s < s1) {
+                t = tstart + (float)s * step;
+                px = camx + dx * t; py = camy + dy * t; pz = camz + dz * t;
                 const float qy = py + P.R;
                 const float rr = fsqrt(px * px + qy * qy + pz * pz);
-                const float alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+                alt = h0f + t * (kcam + t) * frcp(r0c + rr);
                 const float h = (alt - L.base) * L.inv_thick;
                 const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
                 float wfx, wfy;
@@ -342,8 +524,8 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     delta_coords(P, wx, wz, h, dcx, dcy, dcz);
                     dm = delta_bound(P, dcx, dcy, dcz);
                 }
-                const float wr = weather_blend(wq, wfx, wfy);
-                uint32_t ctype = L.type;
+                wr = weather_blend(wq, wfx, wfy);
+                ctype = L.type;
                 NdfC nc = layer_ndf(L);
                 if (P.use_type_grid) {
                     float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
@@ -353,7 +535,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     if (g != 255u) { ctype = g; nc = ndf_coef(g); }
                 }
                 // sample_density with its partials (cloud_march.wgsl:432-530)
-                float density = 0.f, cval = 0.f, q = 0.f;
+                float cval = 0.f, q = 0.f;
                 if (h >= 0.f && h <= 1.f + 1e-5f) {
                     const float wc = wr * L.coverage;
                     const float cov = clampf(wc, 0.f, 1.f);
@@ -385,108 +567,47 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                         if (cloud > 0.f && cloud < 1.f && wc > 0.f && wc < 1.f) q = L.density_scale * dsh * wr;
                     }
                 }
-                if (!(density > 1e-3f)) continue;
-
-                // dense sample: light march and shading with their adjoints
-                float dod_ds, dod_cov;
-                uint32_t lin;
-                const float od = light_adj<DELTA>(P, L, nc, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
-                const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
-                float st[3] = {0.f, 0.f, 0.f};
-                {
-                    const float hh = fmaxf(alt, 1.f);
-                    const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / (P.R + hh), -1.f, 1.f);
-                    float tu, tv;
-                    if (trans_uv_from_h(P, hh, mu, tu, tv)) {
-                        const float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
-                        st[0] = tt.x; st[1] = tt.y; st[2] = tt.z;
-                    }
+                dense = density > 1e-3f;
+                if (dense) {
+                    float* rp = rec + (size_t)k * (kRecFields * 64);
+                    rp[REC_C * 64] = cval;
+                    rp[REC_Q * 64] = q;
+                    dmask |= 1u << k;
                 }
-                float g_hg = LC.g_hg, omg0 = LC.omg_hg, g_d = LC.g_d, alpha = LC.alpha, w_d = LC.w_d, ice = LC.ice;
-                if (LC.is_cb) {
-                    const float mt = smoothstepf(0.60f, 0.85f, hn);
-                    float d = LC.droplet + (50.f - LC.droplet) * mt;
-                    d = clampf(d * P.droplet_bias, 5.f, 50.f);
-                    const float xg = -0.0990567f / (d - 1.67154f);
-                    g_hg = __expf(xg);
-                    omg0 = -expm1f(xg);
-                    g_d = __expf(-2.20679f / (d + 3.91029f)) - 0.428934f;
-                    alpha = __expf(3.62489f - 8.29288f / (d + 5.52825f));
-                    w_d = __expf(-0.599085f / (d - 0.641583f) - 0.665888f);
-                    ice = smoothstepf(35.f, 50.f, d);
-                }
-                float S1[3] = {0.f, 0.f, 0.f}, D1[3] = {0.f, 0.f, 0.f}, e0[3];
-                for (int c = 0; c < 3; ++c) e0[c] = __expf(-L.sigma_t[c] * od);
-                for (int n = 0; n < P.octaves; ++n) {
-                    const float ph = P.a_pow[n] * phase_octave_f(ray.omcf, cosv, P.c_pow[n], g_hg, omg0, g_d, alpha,
-                                                                 w_d, ice);
-                    const float bn = P.b_pow[n];
-                    for (int c = 0; c < 3; ++c) {
-                        const float en = n == 0 ? e0[c] : __expf(-L.sigma_t[c] * od * bn);
-                        S1[c] += ph * en;
-                        D1[c] += ph * bn * en;
-                    }
-                }
-                float fbm = 1.f;
-                if (P.forward_bias > 0.f) fbm = 1.f + P.forward_bias * smoothstepf(0.f, 1.f, fmaxf(cosv, 0.f));
-                float amb[3];
-                {
-                    const float rho = sqrtf(fmaxf(alt * (2.f * P.R + alt), 0.f));
-                    const float zha = (float)PS_PI - atan2f(P.R, rho);
-                    float u, v;
-                    skyview_uv_f(P, px, qy, pz, zha, u, v);
-                    const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
-                    const float g = P.ambient * (0.3f + 0.7f * hn);
-                    amb[0] = sv.x * P.E[0] * g; amb[1] = sv.y * P.E[1] * g; amb[2] = sv.z * P.E[2] * g;
-                }
-                if (P.light_radius > 0.f) {
-                    const float lx = px - P.light_x, lz = pz - P.light_z;
-                    const float fo = clampf(1.f - sqrtf(lx * lx + lz * lz) / P.light_radius, 0.f, 1.f);
-                    const float wgt = fo * fo * (3.f - 2.f * fo);
-                    for (int c = 0; c < 3; ++c) amb[c] += P.light_rgb[c] * wgt;
-                }
-                const float pw = P.powder;
-                float g_rho = 0.f, g_od = 0.f, v[3], e_step[3];
-                for (int c = 0; c < 3; ++c) {
-                    const float sig = L.sigma_t[c];
-                    const float en = e0[c] + (2.f * e0[c] * (1.f - e0[c] * e0[c]) - e0[c]) * pw;
-                    const float den = -e0[c] * ((1.f - pw) + 2.f * pw * (1.f - 3.f * e0[c] * e0[c]));
-                    const float es = P.E[c] * st[c];
-                    const float sun = S1[c] * es * fbm, dsun = -D1[c] * es * fbm;
-                    const float Qc = sun * en + amb[c];
-                    const float dQ = dsun * en + sun * den;  // d Q / d (sigma_t od)
-                    const float in_c = P.sigma_s[c] * density * Qc;
-                    const float rs = density * sig;
-                    const bool lin_t = rs > 1e-6f;
-                    const float ls = fmaxf(rs, 1e-6f);
-                    const float xx = ls * step;
-                    const float ome = one_minus_exp_neg(xx);
-                    const float es_t = __expf(-xx);
-                    e_step[c] = es_t;
-                    const float s_int = in_c * ome / ls;
-                    const float g_sl = G[c] + lw[c] * (g_tw * t + g_twn);
-                    v[c] = T[c] * s_int * g_sl;
-                    const float g_s = T[c] * g_sl;
-                    const float K = P.sigma_s[c] * density * ome / ls;
-                    const float g_Q = g_s * K;
-                    const float dsdr = lin_t ? P.sigma_s[c] * Qc * step * es_t : P.sigma_s[c] * Qc * ome / ls;
-                    g_rho += g_s * dsdr;
-                    const float dsdsig = lin_t ? -P.sigma_s[c] * Qc * density * step * (density * step * psi_fn(xx)) : 0.f;
-                    const float g_sig = g_Q * od * dQ + g_s * dsdsig;
-                    g_od += g_Q * sig * dQ;
-                    gk += g_sig * LC.sigma_t0[c];
-                }
-                gds += g_od * dod_ds;
-                gcov += g_od * dod_cov;
-                if (want_delta && lin) light_delta_scatter(P, L, px, py, pz, alt, lin, g_od * L.density_scale);
-                const int slot = s - s0;
-                float* rp = rec + (size_t)slot * (kRecFields * 64);
-                rp[REC_V0 * 64] = v[0]; rp[REC_V1 * 64] = v[1]; rp[REC_V2 * 64] = v[2];
-                rp[REC_C * 64] = cval; rp[REC_GRHO * 64] = g_rho; rp[REC_Q * 64] = q;
-                dmask |= 1u << slot;
-                T[0] *= e_step[0]; T[1] *= e_step[1]; T[2] *= e_step[2];
             }
-
+            const unsigned long long bal = __ballot(dense);
+            if (bal != 0ull) {
+                if (dense) {
+                    const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    Q.f[BQ_PX][pos] = px; Q.f[BQ_PY][pos] = py; Q.f[BQ_PZ][pos] = pz;
+                    Q.f[BQ_ALT][pos] = alt; Q.f[BQ_WR][pos] = wr; Q.f[BQ_DENS][pos] = density;
+                    Q.f[BQ_TR][pos] = T[0]; Q.f[BQ_TG][pos] = T[1]; Q.f[BQ_TB][pos] = T[2];
+                    Q.f[BQ_T][pos] = t; Q.f[BQ_STEP][pos] = step;
+                    Q.f[BQ_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
+                                                        ((uint32_t)k << 12));
+                    // step transmittance (the forward's update)
+                    T[0] *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * step);
+                    T[1] *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * step);
+                    T[2] *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * step);
+                }
+                qn += __popcll(bal);
+            }
+            if (qn >= 64) {
+                flush_adj<DELTA>(P, Q, lane, 64, qn, rec_tile, want_delta);
+                qn -= 64;
+            }
+        }
+        if (qn > 0) {
+            flush_adj<DELTA>(P, Q, lane, qn, qn, rec_tile, want_delta);
+            qn = 0;
+        }
+        // the records written by other lanes are visible after the fence
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (act) {
+            const LayerC& LC = P.layers[li];
+            float gds = 0.f, gk = 0.f, gcov = 0.f;
             // reverse sweep over the chunk's dense samples: transmittance adjoint
             while (dmask) {
                 const int slot = 31 - __builtin_clz(dmask);
@@ -519,24 +640,23 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 }
                 A[0] += v0; A[1] += v1; A[2] += v2;
             }
-            atomicAdd(&wacc[li], gds);
-            atomicAdd(&wacc[PS_MAX_LAYERS + li], gk);
-            atomicAdd(&wacc[2 * PS_MAX_LAYERS + li], gcov);
+            atomicAdd(&Q.wacc[li], gds);
+            atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
+            atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
             --chunk;
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync_lds();
     if (lane < 3 * PS_MAX_LAYERS) {
-        const float v = wacc[lane];
+        const float v = Q.wacc[lane];
         if (v != 0.f) atomicAdd(P.g_layers + lane, v);
     }
 }
 
 template <bool DELTA>
 __global__ __launch_bounds__(64) void k_render_backward(const RenderParams* __restrict__ Pp) {
-    __shared__ float wacc[3 * PS_MAX_LAYERS];
-    backward_tile<DELTA>(*Pp, blockIdx.x, blockIdx.y, wacc);
+    __shared__ BackQueue Qs;
+    backward_tile<DELTA>(*Pp, blockIdx.x, blockIdx.y, Qs);
 }
 
 // Per-brick maxima of the perturbation volume over each 4^3 brick and its +1

@@ -488,6 +488,7 @@ struct SkyApArgs {
     const float *TT, *SVT, *APT;      // tangent LUTs, set-major
     const float* depth;               // W*H reverse-Z NDC or null
     const float* target;              // W*H RGBA
+    const float* weight;              // W*H per-pixel loss weights, or null (1)
     float* out;                       // per set 16 floats: 14 gradients, loss, unused
     float* image;                     // optional W*H RGBA image of set 0 (or null)
     int W, H;
@@ -560,6 +561,7 @@ __global__ __launch_bounds__(256) void k_sky_ap_grad(SkyApArgs A) {
         const double ndx = fx / A.W * 2.0 - 1.0, ndy = 1.0 - fy / A.H * 2.0;
         const double* m = A.inv_vp;
         const float4 tg = A.want_grad ? ((const float4*)A.target)[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float wgt = A.weight ? A.weight[pix] : 1.f;
         float dn = A.depth ? A.depth[pix] : 0.f;
         if (dn > 0.f) {
             // surface distance (cloud_march.wgsl:764-775 unprojection, FP64)
@@ -603,16 +605,16 @@ __global__ __launch_bounds__(256) void k_sky_ap_grad(SkyApArgs A) {
             if (A.image && set == 0) ((float4*)A.image)[pix] = make_float4(L[0], L[1], L[2], 1.f);
             if (!A.want_grad) continue;
             const float r[3] = {L[0] - tg.x, L[1] - tg.y, L[2] - tg.z};
-            loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            loss += wgt * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
             // dL_c = d ap_c + L_ground * sum_k lw_k dT_k: channel c's
             // parameters reach the in-scatter of channel c and the shared alpha
-            const float gsum = 2.f * (r[0] + r[1] + r[2]) * A.ground;
+            const float gsum = 2.f * wgt * (r[0] + r[1] + r[2]) * A.ground;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const float wq = (q & 1 ? wx : 1.f - wx) * ((q >> 1) & 1 ? wy : 1.f - wy) * (q >> 2 ? wz : 1.f - wz);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float gl = wq * 2.f * r[c], gt = wq * gsum * lw[c];
+                    const float gl = wq * 2.f * wgt * r[c], gt = wq * gsum * lw[c];
 #pragma unroll
                     for (int j = 0; j < NT; ++j)
                         G[c][j] += gl * APT[((size_t)c * 2 * NT + j) * na + id8[q]] +
@@ -670,8 +672,8 @@ __global__ __launch_bounds__(256) void k_sky_ap_grad(SkyApArgs A) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float rc = L[c] - ((const float*)&tg)[c];
-                loss += rc * rc;
-                gr[c] = pos[c] ? 2.f * rc : 0.f;
+                loss += wgt * rc * rc;
+                gr[c] = pos[c] ? 2.f * wgt * rc : 0.f;
             }
             const Tap4 tp = tap4(P.sw, P.sh, u, v);
 #pragma unroll
@@ -751,7 +753,7 @@ hipError_t ps_launch_lut_tangents(const LutParams* d_params, const LutParams& P,
 hipError_t ps_launch_sky_ap_grad(const SkyApLaunch& L, int n, hipStream_t s) {
     SkyApArgs A;
     A.lp = L.lp; A.T = L.T; A.SV = L.SV; A.AP = L.AP; A.TT = L.TT; A.SVT = L.SVT; A.APT = L.APT;
-    A.depth = L.depth; A.target = L.target; A.out = L.out; A.image = L.image;
+    A.depth = L.depth; A.target = L.target; A.weight = L.weight; A.out = L.out; A.image = L.image;
     A.W = L.W; A.H = L.H; A.ground = L.ground; A.want_grad = L.want_grad;
     for (int i = 0; i < 16; ++i) A.inv_vp[i] = L.inv_vp[i];
     for (int i = 0; i < 3; ++i) {

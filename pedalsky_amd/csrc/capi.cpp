@@ -871,7 +871,7 @@ void sky_geometry(const psi::SceneIn& sc, SkyApLaunch& L) {
 // Bake the n value LUT sets (coefficients from `coeffs`, or the context's
 // atmosphere when null), optionally their tangents, and run the image pass.
 ps_status sky_ap_run(ps_ctx* c, int n, const float* coeffs, const float* depth, float ground, const float* target,
-                     float* out, float* image, bool grad) {
+                     const float* weight, float* out, float* image, bool grad) {
     ps_status s;
     if ((s = check_ready(c, false))) return s;
     const float vw = c->sc.frame.viewport_size[0], vh = c->sc.frame.viewport_size[1];
@@ -907,7 +907,7 @@ ps_status sky_ap_run(ps_ctx* c, int n, const float* coeffs, const float* depth, 
     L.lp = c->atmo_params;
     L.T = c->aT; L.SV = c->aSV; L.AP = c->aAP;
     L.TT = c->aTT; L.SVT = c->aSVT; L.APT = c->aAPT;
-    L.depth = depth; L.target = target; L.out = out; L.image = image;
+    L.depth = depth; L.target = target; L.weight = weight; L.out = out; L.image = image;
     L.ground = ground;
     L.want_grad = grad ? 1 : 0;
     PS_HIP(ps_launch_sky_ap_grad(L, n, c->stream));
@@ -920,14 +920,14 @@ extern "C" {
 
 ps_status ps_sky_ap_image(ps_ctx* c, const float* depth, float ground_radiance, float* out) {
     if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_sky_ap_image: NULL argument");
-    return sky_ap_run(c, 1, nullptr, depth, ground_radiance, nullptr, nullptr, out, false);
+    return sky_ap_run(c, 1, nullptr, depth, ground_radiance, nullptr, nullptr, nullptr, out, false);
 }
 
 ps_status ps_atmosphere_grad(ps_ctx* c, int32_t n, const float* coeffs, const float* depth, float ground_radiance,
-                             const float* target, float* out, float* image) {
+                             const float* target, const float* weight, float* out, float* image) {
     if (!c || !coeffs || !target || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_atmosphere_grad: NULL argument");
     if (n < 1 || n > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "ps_atmosphere_grad: n must be in [1, 4096]");
-    return sky_ap_run(c, n, coeffs, depth, ground_radiance, target, out, image, true);
+    return sky_ap_run(c, n, coeffs, depth, ground_radiance, target, weight, out, image, true);
 }
 
 }  // extern "C"

@@ -62,6 +62,7 @@ struct SkyApLaunch {
     const float *TT, *SVT, *APT;
     const float* depth;
     const float* target;
+    const float* weight;              // per-pixel loss weights or null
     float* out;
     float* image;                     // optional image of set 0
     int W, H;

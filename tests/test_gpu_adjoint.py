@@ -193,17 +193,20 @@ def _gpu_value(lg, dg, kind, idx):
 
 def _check(params, got, ref, what):
     got, ref = np.asarray(got), np.asarray(ref)
-    worst = 0.0
+    worst, bad = 0.0, []
     for kind in (DS, KAPPA, COV, DELTA):
         sel = [i for i, (k, _) in enumerate(params) if k == kind]
         if not sel:
             continue
         floor = 1e-3 * np.abs(ref[sel]).max()
-        for i in sel:
-            rel = abs(got[i] - ref[i]) / max(abs(ref[i]), floor)
+        rels = [abs(got[i] - ref[i]) / max(abs(ref[i]), floor) for i in sel]
+        _log(f"  {what}: kind {kind}: max relative error {max(rels):.3e} over {len(sel)}")
+        for i, rel in zip(sel, rels):
             worst = max(worst, rel)
-            assert rel <= 1e-3, (what, params[i], got[i], ref[i], rel)
+            if rel > 1e-3:
+                bad.append((params[i], got[i], ref[i], rel))
     _log(f"  {what}: {len(params)} parameters, max relative error {worst:.3e}")
+    assert not bad, bad
 
 
 def test_gradients_match_oracle_dual(renderer):
@@ -212,11 +215,11 @@ def test_gradients_match_oracle_dual(renderer):
     perturbation voxels."""
     import torch
     w, fr, target, (ds0, kap0, cov0, delta0, d0) = _problem(renderer)
-    _log("case: backward vs oracle dual numbers, 640x360 two-layer, 96x64 loss window")
+    _log("case: backward vs oracle dual numbers, 640x360 two-layer, 192x128 loss window")
     out = _render(renderer, train=True)
-    # the cloudiest 96x64 window on a 32-pixel grid
+    # the cloudiest 192x128 window on a 32-pixel grid
     cl = (out["cloud_transmittance"][..., 1] < 0.9).float().cpu().numpy()
-    cw, ch = 96, 64
+    cw, ch = 192, 128
     x0, y0 = max(((x, y) for x in range(0, 640 - cw + 1, 32) for y in range(0, 360 - ch + 1, 32)),
                  key=lambda p: cl[p[1]:p[1] + ch, p[0]:p[0] + cw].mean())
     comp = out["composite"]
