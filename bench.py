@@ -128,11 +128,21 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
         g[..., 3] = 0.0
         abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads)))
 
+    grads_layers = abi.CloudGrads(lg.data_ptr(), None)
+
+    def fwd_adj_layers():
+        abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
+        abi.check(lib.ps_render_train(r.ctx, depth, C.byref(out)))
+        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)
+        g[..., 3] = 0.0
+        abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads_layers)))
+
     def fwd():
         abi.check(lib.ps_frame(r.ctx, abi.PS_LUT_ALL, depth, C.byref(out)))
 
     res = {}
-    for name, fn, ext in (("fwd", fwd, False), ("fwd_delta", fwd, True), ("fwd_adjoint", fwd_adj, True)):
+    for name, fn, ext in (("fwd", fwd, False), ("fwd_delta", fwd, True), ("fwd_adjoint", fwd_adj, True),
+                          ("fwd_adjoint_layers", fwd_adj_layers, True)):
         if ext:
             r.set_cloud_extensions([1.0] * len(w.layers), d0)
         else:
@@ -153,12 +163,46 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
                         f"L2 loss vs a perturbed target, checkpoints every 16 steps",
             "steps": steps, "ms_fwd": round(res["fwd"], 4), "ms_fwd_delta": round(res["fwd_delta"], 4),
             "ms_fwd_adjoint": round(res["fwd_adjoint"], 4),
+            "ms_fwd_adjoint_layer_grads_only": round(res["fwd_adjoint_layers"], 4),
             "value": round(mp / (res["fwd_adjoint"] * 1e-3), 3), "unit": "Mpix/s",
             "fwd_mpix_s": round(mp / (res["fwd"] * 1e-3), 3),
             "fwd_adjoint_over_fwd": round(res["fwd_adjoint"] / res["fwd"], 3),
             "note": "fwd = plain frame (LUT chain + render); fwd_delta = the same with the perturbation "
                     "volume and extinction scales; fwd_adjoint = LUT chain + training forward (checkpoints) + "
                     "L2 cotangent + ps_backward (layer and volume gradients), all on one stream"}
+
+
+def bench_calibration(r, n_sets: int, iters: int) -> dict:
+    """Atmosphere inverse calibration: `n_sets` seeded starting points
+    (group multipliers U[0.5, 1.5], PCG32 seed 0xC5) fitted by Adam (lr 1e-2
+    on the log multipliers) to the 1280x720 sky + AP image of a hidden set
+    (seed 0xC5A) with the relative L2 loss; every iteration bakes all sets'
+    LUT chains with tangents and evaluates all sets' images and gradients in
+    one call."""
+    import numpy as np
+    import torch
+    from pedalsky_amd import calibration as cal
+    su = cal.calibration_setup(n_sets=n_sets)
+    prob = cal.SkyApProblem(r, su.workload, su.ground_radiance)
+    hidden = cal.atmosphere_with(su.workload.atmosphere, su.base * cal.expand_multipliers(su.hidden))
+    target = prob.image(hidden)
+    w = cal.relative_weights(target)
+    cal.calibrate(prob, target, su.init[:2], iters=2, weight=w)  # warm-up (allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    final, hist = cal.calibrate(prob, target, su.init, iters=iters, weight=w)
+    dt = time.perf_counter() - t0
+    err = np.abs(final / su.hidden[None, :] - 1.0)
+    drop = hist[-1] / hist[0]
+    mpix = n_sets * su.workload.width * su.workload.height * iters / 1e6
+    return {"workload": f"{n_sets} parameter sets x {iters} Adam iterations, 1280x720 sky + AP image, "
+                        f"relative L2 vs a hidden set; per iteration: LUT chains + tangent LUTs + image gradients "
+                        f"of all sets",
+            "seconds": round(dt, 3), "ms_per_iteration": round(dt / iters * 1e3, 3),
+            "value": round(mpix / dt, 3), "unit": "Mpix/s (image gradient evaluations)",
+            "loss_ratio_median": float(np.median(drop)),
+            "recovery_error_median": float(np.median(err)), "recovery_error_p90": float(np.percentile(err, 90)),
+            "recovery_error_median_per_group": {k: float(v) for k, v in zip(cal.GROUP_NAMES, np.median(err, axis=0))}}
 
 
 def measured_traffic():
@@ -183,6 +227,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--adjoint-steps", type=int, default=16)
+    ap.add_argument("--no-calibration", action="store_true", help="skip the atmosphere calibration run")
+    ap.add_argument("--calibration-sets", type=int, default=256)
+    ap.add_argument("--calibration-iters", type=int, default=200)
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
@@ -311,6 +358,10 @@ def main() -> int:
         log("forward + adjoint")
         result["fwd_adjoint"] = bench_adjoint(r, W, H, args.adjoint_steps, 3)
         result["fwd_adjoint_640x360"] = bench_adjoint(r, 640, 360, args.adjoint_steps, 3)
+        r.load_workload(w0)
+    if not args.no_calibration:
+        log("atmosphere calibration")
+        result["atmosphere_calibration"] = bench_calibration(r, args.calibration_sets, args.calibration_iters)
         r.load_workload(w0)
     if not args.no_batch:
         result["batched"] = bench_batch(r, args.batch_scenes)

@@ -128,13 +128,11 @@ class SkyApProblem:
 
 def relative_weights(target):
     """Per-pixel weights that make the L2 loss a relative one:
-    1 / (mean RGB of the target)^2 (floored at 1e-3 of the frame mean)."""
+    1 / (mean RGB of the target)^2, the mean floored at 1e-3 of the frame
+    mean. Without them the few brightest pixels (around the sun) dominate
+    the loss and the parameters they are insensitive to drift."""
     m = target[..., :3].mean(dim=-1)
-    return (1.0 / torch_clamp_min(m, 1e-3 * float(m.mean())) ** 2).contiguous()
-
-
-def torch_clamp_min(x, lo):
-    return x.clamp_min(lo)
+    return (1.0 / m.clamp_min(1e-3 * float(m.mean())) ** 2).contiguous()
 
 
 def calibrate(problem: SkyApProblem, target, init_mult: np.ndarray, iters: int = 200, lr: float = 1e-2,

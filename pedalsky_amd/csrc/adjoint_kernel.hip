@@ -62,17 +62,54 @@ __device__ __forceinline__ float psi_fn(float x) {
     return (1.f - (1.f + x) * e) / (x * x);
 }
 
-// Scatter v over the eight trilinear corners of a perturbation-volume tap.
-__device__ __forceinline__ void delta_scatter(float* g, const DeltaTap& d, float v) {
+// Sum over the wave (all 64 lanes active).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Scatter v times the eight trilinear weights of each lane's tap (lanes with
+// `has`) into the FP64 gradient accumulator. Neighbouring samples of a wave
+// mostly fall in the same 250 m voxel, and same-address atomics serialise,
+// so lanes whose taps share all eight corners are first summed across the
+// wave: up to kPeel groups (one atomic per corner per group); lanes left
+// after that add their own. Called with all 64 lanes active.
+constexpr int kPeel = 4;
+
+__device__ __forceinline__ void delta_scatter_wave(double* __restrict__ g, bool has, const DeltaTap& d, float v) {
+    const int lane = threadIdx.x;
+    // the corner set is determined by the lower corner and whether the
+    // height axis is clamped (y1 == y0)
+    const uint32_t key = has ? ((d.z0 + d.y0 + d.x0) | (d.y1 == d.y0 ? 0x80000000u : 0u)) : 0xffffffffu;
     const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
-    atomicAdd(g + d.z0 + d.y0 + d.x0, v * wz0 * wy0 * wx0);
-    atomicAdd(g + d.z0 + d.y0 + d.x1, v * wz0 * wy0 * wx1);
-    atomicAdd(g + d.z0 + d.y1 + d.x0, v * wz0 * wy1 * wx0);
-    atomicAdd(g + d.z0 + d.y1 + d.x1, v * wz0 * wy1 * wx1);
-    atomicAdd(g + d.z1 + d.y0 + d.x0, v * wz1 * wy0 * wx0);
-    atomicAdd(g + d.z1 + d.y0 + d.x1, v * wz1 * wy0 * wx1);
-    atomicAdd(g + d.z1 + d.y1 + d.x0, v * wz1 * wy1 * wx0);
-    atomicAdd(g + d.z1 + d.y1 + d.x1, v * wz1 * wy1 * wx1);
+    const float w[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
+                        v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
+    const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
+                             d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
+    unsigned long long act = __ballot(has);
+    for (int it = 0; it < kPeel && act != 0ull; ++it) {
+        const int leader = __builtin_ctzll(act);
+        const uint32_t k = __builtin_amdgcn_readlane(key, leader);
+        const bool m = key == k;
+        const unsigned long long mm = __ballot(m);
+        if (__popcll(mm) > 1) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float s = wave_sum(m ? w[c] : 0.f);
+                if (lane == leader) atomicAdd(g + off[c], (double)s);
+            }
+        } else if (lane == leader) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
+        }
+        has = has && !m;
+        act &= ~mm;
+    }
+    if (has) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
+    }
 }
 
 // Geometry of the march toward the sun (cloud_march.wgsl:543-578 straight,
@@ -172,19 +209,29 @@ __device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& 
     return od;
 }
 
-// Scatter g (= dL/d od * density_scale) over the light samples in `lin`.
-__device__ __forceinline__ void light_delta_scatter(const RenderParams& P, const LayerF& L, float px, float py,
-                                                    float pz, float alt, uint32_t lin, float g) {
+// Scatter g (= dL/d od * density_scale) over each lane's light samples in
+// `lin` (lanes with has). Called with all 64 lanes active; light sample i of
+// every lane is scattered together.
+__device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, const LayerF& L, bool has, float px,
+                                                         float py, float pz, float alt, uint32_t lin, float g) {
     const LightWalk w = light_walk(P, L, px, py, pz, alt);
-    while (lin) {
-        const int i = __builtin_ctz(lin);
-        lin &= lin - 1u;
-        float qx, qyy, qz, a, wt;
-        light_sample(P, w, i, px, py, pz, alt, qx, qyy, qz, a, wt);
-        const float h = (a - L.base) * L.inv_thick;
-        float dcx, dcy, dcz;
-        delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
-        delta_scatter(P.g_delta, delta_tap(dcx, dcy, dcz), g * wt);
+    const int n = P.cone ? 6 : P.light_steps;
+    unsigned long long any = __ballot(has && lin != 0u);
+    if (any == 0ull) return;
+    for (int i = 0; i < n; ++i) {
+        const bool hi = has && ((lin >> i) & 1u);
+        if (__ballot(hi) == 0ull) continue;
+        DeltaTap d{};
+        float wt = 0.f;
+        if (hi) {
+            float qx, qyy, qz, a;
+            light_sample(P, w, i, px, py, pz, alt, qx, qyy, qz, a, wt);
+            const float h = (a - L.base) * L.inv_thick;
+            float dcx, dcy, dcz;
+            delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
+            d = delta_tap(dcx, dcy, dcz);
+        }
+        delta_scatter_wave(P.g_delta64, hi, d, g * wt);
     }
 }
 
@@ -215,9 +262,15 @@ __device__ __forceinline__ void wave_sync_lds() {
 // Shade queue entry j with its adjoint (cloud_march.wgsl:886-993): writes
 // the owner's record (v, dL/drho through the shading) and adds the
 // light-march and extinction contributions to the wave's layer sums.
+// Light-march scatter work a shaded sample leaves for the wave.
+struct LightScatter {
+    float px, py, pz, alt, g;
+    uint32_t lin;
+    int li;
+};
+
 template <bool DELTA>
-__device__ __forceinline__ void shade_adj(const RenderParams& P, BackQueue& Q, int j, float* rec_tile,
-                                          bool want_delta) {
+__device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQueue& Q, int j, float* rec_tile) {
     const float px = Q.f[BQ_PX][j], py = Q.f[BQ_PY][j], pz = Q.f[BQ_PZ][j];
     const float alt = Q.f[BQ_ALT][j], wr = Q.f[BQ_WR][j], density = Q.f[BQ_DENS][j];
     const float T[3] = {Q.f[BQ_TR][j], Q.f[BQ_TG][j], Q.f[BQ_TB][j]};
@@ -326,13 +379,16 @@ __device__ __forceinline__ void shade_adj(const RenderParams& P, BackQueue& Q, i
     }
     gds += g_od * dod_ds;
     gcov += g_od * dod_cov;
-    if (want_delta && lin) light_delta_scatter(P, L, px, py, pz, alt, lin, g_od * L.density_scale);
     float* rp = rec_tile + (size_t)slot * (kRecFields * 64) + owner;
     rp[REC_V0 * 64] = v[0]; rp[REC_V1 * 64] = v[1]; rp[REC_V2 * 64] = v[2];
     rp[REC_GRHO * 64] = g_rho;
     atomicAdd(&Q.wacc[li], gds);
     atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
     atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
+    LightScatter ls;
+    ls.px = px; ls.py = py; ls.pz = pz; ls.alt = alt; ls.lin = lin; ls.li = li;
+    ls.g = g_od * L.density_scale;
+    return ls;
 }
 
 // Shade the first n (<= 64) queued samples with the whole wave, then move
@@ -341,7 +397,9 @@ template <bool DELTA>
 __device__ __forceinline__ void flush_adj(const RenderParams& P, BackQueue& Q, int lane, int n, int qn,
                                           float* rec_tile, bool want_delta) {
     wave_sync_lds();
-    if (lane < n) shade_adj<DELTA>(P, Q, lane, rec_tile, want_delta);
+    LightScatter ls{};
+    if (lane < n) ls = shade_adj<DELTA>(P, Q, lane, rec_tile);
+    if (want_delta) light_delta_scatter_wave(P, P.layers[ls.li].f, lane < n, ls.px, ls.py, ls.pz, ls.alt, ls.lin, ls.g);
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
@@ -605,45 +663,57 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
         // the records written by other lanes are visible after the fence
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        if (act) {
+        {
+            // reverse sweep over the chunk's dense samples (transmittance
+            // adjoint), slot by slot for the whole wave so the perturbation
+            // scatter can combine lanes
             const LayerC& LC = P.layers[li];
             float gds = 0.f, gk = 0.f, gcov = 0.f;
-            // reverse sweep over the chunk's dense samples: transmittance adjoint
-            while (dmask) {
-                const int slot = 31 - __builtin_clz(dmask);
-                dmask &= ~(1u << slot);
-                const float* rp = rec + (size_t)slot * (kRecFields * 64);
-                const float v0 = rp[REC_V0 * 64], v1 = rp[REC_V1 * 64], v2 = rp[REC_V2 * 64];
-                const float cval = rp[REC_C * 64], g_rho_s = rp[REC_GRHO * 64], q = rp[REC_Q * 64];
-                const float rho = cval * L.density_scale;
-                float g_rho = g_rho_s;
-                for (int c = 0; c < 3; ++c) {
-                    const float sig = L.sigma_t[c];
-                    if (rho * sig > 1e-6f) {
-                        g_rho -= step * sig * A[c];
-                        gk -= rho * step * A[c] * LC.sigma_t0[c];
+            for (int slot = ckl - 1; slot >= 0; --slot) {
+                const bool mine = act && ((dmask >> slot) & 1u);
+                if (__ballot(mine) == 0ull) continue;
+                float g_rho = 0.f, cval = 0.f;
+                DeltaTap dtap{};
+                bool scat = false;
+                if (mine) {
+                    const float* rp = rec + (size_t)slot * (kRecFields * 64);
+                    const float v0 = rp[REC_V0 * 64], v1 = rp[REC_V1 * 64], v2 = rp[REC_V2 * 64];
+                    cval = rp[REC_C * 64];
+                    const float g_rho_s = rp[REC_GRHO * 64], q = rp[REC_Q * 64];
+                    const float rho = cval * L.density_scale;
+                    g_rho = g_rho_s;
+                    for (int c = 0; c < 3; ++c) {
+                        const float sig = L.sigma_t[c];
+                        if (rho * sig > 1e-6f) {
+                            g_rho -= step * sig * A[c];
+                            gk -= rho * step * A[c] * LC.sigma_t0[c];
+                        }
+                    }
+                    gds += g_rho * cval;
+                    gcov += g_rho * q;
+                    A[0] += v0; A[1] += v1; A[2] += v2;
+                    if (want_delta && cval < 1.f) {
+                        const int s = s0 + slot;
+                        const float t = tstart + (float)s * step;
+                        const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
+                        const float qy = py + P.R;
+                        const float rr = fsqrt(px * px + qy * qy + pz * pz);
+                        const float alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+                        const float h = (alt - L.base) * L.inv_thick;
+                        float dcx, dcy, dcz;
+                        delta_coords(P, px - L.wind_off_x, pz - L.wind_off_z, h, dcx, dcy, dcz);
+                        dtap = delta_tap(dcx, dcy, dcz);
+                        scat = true;
                     }
                 }
-                gds += g_rho * cval;
-                gcov += g_rho * q;
-                if (want_delta && cval < 1.f) {
-                    const int s = s0 + slot;
-                    const float t = tstart + (float)s * step;
-                    const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
-                    const float qy = py + P.R;
-                    const float rr = fsqrt(px * px + qy * qy + pz * pz);
-                    const float alt = h0f + t * (kcam + t) * frcp(r0c + rr);
-                    const float h = (alt - L.base) * L.inv_thick;
-                    float dcx, dcy, dcz;
-                    delta_coords(P, px - L.wind_off_x, pz - L.wind_off_z, h, dcx, dcy, dcz);
-                    delta_scatter(P.g_delta, delta_tap(dcx, dcy, dcz), g_rho * L.density_scale);
-                }
-                A[0] += v0; A[1] += v1; A[2] += v2;
+                if (want_delta) delta_scatter_wave(P.g_delta64, scat, dtap, g_rho * L.density_scale);
             }
-            atomicAdd(&Q.wacc[li], gds);
-            atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
-            atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
-            --chunk;
+            if (act) {
+                atomicAdd(&Q.wacc[li], gds);
+                atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
+                atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
+                --chunk;
+            }
         }
     }
     wave_sync_lds();
@@ -680,12 +750,26 @@ __global__ void k_delta_bmax(const float* __restrict__ d, float* __restrict__ bm
     bmax[b] = m;
 }
 
+__global__ void k_zero_f64(double* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_add_f64_to_f32(float* __restrict__ dst, const double* __restrict__ src, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += (float)src[i];
+}
+
 }  // namespace
 
-hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, hipStream_t s) {
+hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, float* g_delta,
+                                     double* g_delta64, hipStream_t s) {
+    const int nv = 128 * 128 * 128;
+    if (g_delta) hipLaunchKernelGGL(k_zero_f64, dim3(nv / 256), dim3(256), 0, s, g_delta64, nv);
     dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
     if (delta) hipLaunchKernelGGL(k_render_backward<true>, grid, block, 0, s, d_params);
     else hipLaunchKernelGGL(k_render_backward<false>, grid, block, 0, s, d_params);
+    if (g_delta) hipLaunchKernelGGL(k_add_f64_to_f32, dim3(nv / 256), dim3(256), 0, s, g_delta, g_delta64, nv);
     return hipGetLastError();
 }
 

@@ -408,7 +408,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
-    dev_free(c->delta_bmax); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec);
+    dev_free(c->delta_bmax); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->g_delta64);
     dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
     dev_free(c->aTT); dev_free(c->aMST); dev_free(c->aSVT); dev_free(c->aAPT);
     if (c->atmo_params_h) hipHostFree(c->atmo_params_h);
@@ -795,10 +795,14 @@ ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_lumina
     // to the context's counter scratch
     P.g_layers = grads->layers ? grads->layers : (float*)c->d_counters;
     P.g_delta = grads->density_delta;
+    if (P.g_delta) {
+        if ((s = dev_alloc(&c->g_delta64, sizeof(double) * 128 * 128 * 128, "FP64 volume gradient"))) return s;
+    }
+    P.g_delta64 = c->g_delta64;
     P.rec = c->rec;
     RenderParams* dP = nullptr;
     if ((s = c->render_ring.upload(P, c->stream, &dP))) return s;
-    PS_HIP(ps_launch_render_backward(dP, P.W, P.H, P.delta != nullptr, c->stream));
+    PS_HIP(ps_launch_render_backward(dP, P.W, P.H, P.delta != nullptr, P.g_delta, c->g_delta64, c->stream));
     return c->render_ring.mark_used(c->stream);
 }
 

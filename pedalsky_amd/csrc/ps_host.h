@@ -155,6 +155,7 @@ struct ps_ctx {
     size_t px_cap = 0;
     float* rec = nullptr;
     size_t rec_cap = 0;
+    double* g_delta64 = nullptr;  // FP64 accumulator of the perturbation-volume gradient
     RenderParams train{};         // parameter block of the last training forward
     bool train_valid = false;
 

@@ -20,7 +20,10 @@ hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool cou
                             hipStream_t s);
 // Backward of a training render (the same parameter block, with cotangents
 // and gradient outputs set).
-hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, hipStream_t s);
+// With g_delta non-null the perturbation-volume gradient is accumulated in
+// FP64 (g_delta64, zeroed here) and then added into g_delta.
+hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, float* g_delta,
+                                     double* g_delta64, hipStream_t s);
 // Per-brick maxima of the 128^3 density-perturbation volume (32^3 bricks of
 // 4^3 texels plus a +1 halo: Repeat in x and z, ClampToEdge in height).
 hipError_t ps_launch_delta_bmax(const float* delta, float* bmax, hipStream_t s);

@@ -7,11 +7,11 @@ References:
     relative radiance error <= 1e-3, median <= 1e-5);
   * gradients: the oracle's exact FP64 derivatives (dual numbers, themselves
     pinned by FP64 central differences on the CPU), and FP64 central finite
-    differences of the oracle's loss with h = 1e-4, shrunk where the march's
-    discrete decisions (gates, dense threshold, early exit) would change
-    inside [theta - h, theta + h]. Bar: relative error <= 1e-3 against the
-    finite difference, with the relative scale floored at 1e-3 of the
-    largest gradient of the same parameter kind.
+    differences of the oracle's loss with h = 1e-4, refined (1e-5, ...) until
+    two successive steps agree, which excludes steps across the march's
+    discrete decisions (gates, dense threshold, early exit). Bar: relative
+    error <= 1e-3 against the finite difference, with the relative scale
+    floored at 1e-3 of the largest gradient of the same parameter kind.
 """
 import ctypes as C
 import sys
@@ -263,17 +263,29 @@ def test_gradients_match_fp64_finite_differences(renderer):
 
     params = _params_for(lg, dg, 2, 26, np.random.default_rng(0xC4F))
     assert len(params) == 32
+    # Central differences at h = 1e-4, 1e-5, ...: the loss is piecewise
+    # smooth (gates, dense threshold, early exit), and a decision that flips
+    # inside [theta - h, theta + h] shows up as two successive steps that
+    # disagree. The first h whose difference agrees with the next smaller
+    # step's to 1e-4 is used.
     fd = []
-    _, c0 = loss(diff)
     for kind, idx in params:
-        h = 1e-4
-        while True:
-            lp, cp = loss(diff.perturbed(kind, idx, h))
-            lm, cm = loss(diff.perturbed(kind, idx, -h))
-            if (np.array_equal(cp, c0) and np.array_equal(cm, c0)) or h < 1e-9:
+        prev, h, d, used = None, 1e-4, None, None
+        while h >= 1e-8:
+            lp, _ = loss(diff.perturbed(kind, idx, h))
+            lm, _ = loss(diff.perturbed(kind, idx, -h))
+            cur = (lp - lm) / (2 * h)
+            if prev is not None and abs(cur - prev) <= 1e-4 * max(abs(cur), 1e-30):
+                d, used = prev, h * 10
                 break
+            prev = cur
             h /= 10
-        fd.append((lp - lm) / (2 * h))
+        if d is None:
+            d, used = prev, h * 10
+            _log(f"  FD {kind, idx}: no two steps agreed down to h = {used:g}")
+        elif used != 1e-4:
+            _log(f"  FD {kind, idx}: h = {used:g}")
+        fd.append(d)
     got = [_gpu_value(lg, dg, k, i) for k, i in params]
     _check(params, got, fd, "GPU vs FP64 finite differences")
 
