@@ -105,7 +105,7 @@ def load():
         "orf_render": (None, [P, i, i, i, i, P, P, P, P, i]),
         "or_render_pixels": (None, [P, P, i, P, P, P, i]),
         "or_render_ext": (None, [P, P, i, i, i, i, P, P, P, P, i]),
-        "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i]),
+        "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i, d, P]),
         "or_sky_ap_image": (None, [P, P, i, i, i, i, i, i, i, i, i, d, i, i, i, i, P, i]),
         "or_noise_base": (None, [C.c_uint32, i, i, P, i]),
         "or_noise_detail": (None, [C.c_uint32, P, i]),
@@ -333,23 +333,32 @@ def _frame_render_ext(self, diff: Diff, x0, y0, w, h, counters=False):
     return comp, lum, trn, cnt
 
 
-def _frame_vjp_clouds(self, diff: Diff, params, x0, y0, w, h, g_comp=None, g_lum=None, g_trn=None):
+def _frame_vjp_clouds(self, diff: Diff, params, x0, y0, w, h, g_comp=None, g_lum=None, g_trn=None,
+                      near_tol=0.0):
     """Exact FP64 vector-Jacobian product (dual instantiation) for a list of
-    (kind, index) parameters; cotangent images are (h, w, 4) or None."""
+    (kind, index) parameters; cotangent images are (h, w, 4) or None.
+
+    With near_tol > 0 also returns a boolean per parameter: whether a primary
+    sample whose density lies within near_tol of the 1e-3 dense threshold
+    depends on it (an FP32 march may decide that sample the other way, which
+    changes this parameter's derivative by that sample's whole term)."""
     def cot(g):
         return None if g is None else np.ascontiguousarray(g, dtype=np.float64).reshape(h, w, 4)
     gc, gl, gt = cot(g_comp), cot(g_lum), cot(g_trn)
     out = np.zeros(len(params))
+    near = np.zeros(len(params), dtype=bool)
     for s0 in range(0, len(params), DUAL_N):
         chunk = list(params[s0:s0 + DUAL_N])
         st = diff.struct(chunk)
         g = np.zeros(len(chunk))
+        mask = C.c_uint32(0)
         rc = load().or_vjp_clouds(C.byref(self.inputs), C.byref(st), x0, y0, w, h, _p(gc), _p(gl), _p(gt),
-                                  _p(g), THREADS)
+                                  _p(g), THREADS, float(near_tol), C.byref(mask))
         if rc != 0:
             raise ValueError("or_vjp_clouds rejected its arguments")
         out[s0:s0 + len(chunk)] = g
-    return out
+        near[s0:s0 + len(chunk)] = [(mask.value >> k) & 1 for k in range(len(chunk))]
+    return (out, near) if near_tol > 0 else out
 
 
 def atmosphere_coeffs(atm) -> np.ndarray:

@@ -291,8 +291,11 @@ constexpr int kDualN = 16;
 // g_trn * d trn / d theta_k for the (at most 16) tracked parameters of `d`,
 // computed with the Dual<16> instantiation of the FP64 restatement. The
 // cotangent images are w*h RGBA (alpha ignored), any of them may be null.
+// near_tol > 0: *near_mask receives the tangent slots a primary sample
+// within near_tol of the dense threshold depends on (see Counters).
 int or_vjp_clouds(const or_inputs* in, const or_diff* d, int x0, int y0, int w, int h, const double* g_comp,
-                  const double* g_lum, const double* g_trn, double* grad, int nthreads) {
+                  const double* g_lum, const double* g_trn, double* grad, int nthreads, double near_tol,
+                  uint32_t* near_mask) {
     typedef Dual<kDualN> D;
     if (!d || d->n_params < 0 || d->n_params > kDualN) return 1;
     Scene<D> s = make_scene<D>(in);
@@ -300,15 +303,18 @@ int or_vjp_clouds(const or_inputs* in, const or_diff* d, int x0, int y0, int w, 
     apply_diff(s, d, track, slot);
     const int n = w * h, np = d->n_params;
     std::vector<double> total(np, 0.0);
+    uint32_t mask = 0u;
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel
     {
         std::vector<double> part(np, 0.0);
+        Counters cnt;
+        cnt.near_tol = near_tol;
 #pragma omp for schedule(dynamic, 4)
         for (int i = 0; i < n; ++i) {
             int ix = x0 + i % w, iy = y0 + i / w;
             D c4[4], l4[4], t4[4];
-            render_pixel(s, ix, iy, c4, l4, t4, (Counters*)nullptr);
+            render_pixel(s, ix, iy, c4, l4, t4, near_tol > 0.0 ? &cnt : (Counters*)nullptr);
             for (int c = 0; c < 3; ++c) {
                 const size_t o = size_t(i) * 4 + c;
                 for (int k = 0; k < np; ++k) {
@@ -321,9 +327,13 @@ int or_vjp_clouds(const or_inputs* in, const or_diff* d, int x0, int y0, int w, 
             }
         }
 #pragma omp critical
-        for (int k = 0; k < np; ++k) total[k] += part[k];
+        {
+            for (int k = 0; k < np; ++k) total[k] += part[k];
+            mask |= cnt.near_mask;
+        }
     }
     for (int k = 0; k < np; ++k) grad[k] = total[k];
+    if (near_mask) *near_mask = mask;
     return 0;
 }
 

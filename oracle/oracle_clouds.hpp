@@ -85,6 +85,11 @@ struct Scene {
 struct Counters {
     uint64_t pixels = 0, layer_marches = 0, primary_steps = 0, covered_steps = 0, eroded_steps = 0;
     uint64_t dense_samples = 0, light_steps = 0, light_covered = 0, octaves = 0, ap_taps = 0;
+    // dual instantiation only: tangent slots whose derivative a primary
+    // sample within near_tol of the dense threshold depends on (an FP32
+    // evaluation may take the other side of that decision)
+    double near_tol = 0.0;
+    uint32_t near_mask = 0u;
 };
 
 // ---- math.wgsl:7-9 -----------------------------------------------------------------
@@ -656,6 +661,11 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
             sample2d(s.weather, wx / wsc + R(0.5), wz / wsc + R(0.5), true, wsample);
             uint32_t ctype = effective_cloud_type(s, wx, wz, L);
             R density = sample_density(s, p, alt, L, wsample[0], wind, ctype, cnt);
+            if constexpr (is_dual<R>::value) {
+                if (cnt && std::fabs(density.v - 1e-3) < cnt->near_tol)
+                    for (int k = 0; k < R::n; ++k)
+                        if (density.d[k] != 0.0) cnt->near_mask |= 1u << k;
+            }
             if (density > R(1e-3)) {
                 if (cnt) cnt->dense_samples++;
                 R od = march_to_light(s, p, alt, sun, L, wsample[0], wind, ctype, cnt);

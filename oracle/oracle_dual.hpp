@@ -16,6 +16,7 @@ namespace oracle {
 
 template <int N>
 struct Dual {
+    static constexpr int n = N;
     double v;
     double d[N];
 

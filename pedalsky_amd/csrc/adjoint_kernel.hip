@@ -69,13 +69,59 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Cross-lane moves within rows of 16 lanes (DPP, all lanes active).
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+constexpr int kDppQuadXor1 = 0xB1;     // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadXor2 = 0x4E;     // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // lane i <- lane 7 - i within 8
+constexpr int kDppRowRor8 = 0x128;     // lane i <- lane i ^ 8 within 16
+
+// a + b where (a, b) are exchanged so that, for lanes in the lower half
+// (of the wave for W = 32, of each row pair for W = 16), the result is
+// a_l + a_partner and for the upper half b_l + b_partner (the partner is the
+// lane W away). One v_permlane*_swap per pair.
+template <int W>
+__device__ __forceinline__ float swap_add(float a, float b) {
+    if constexpr (W == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+}
+
+// Transposed sum of eight values over the wave: lane l returns the sum over
+// all 64 lanes of w[l >> 3]. Halving exchanges (lanes 32 apart keep corners
+// 0-3 / 4-7, then 16 apart, then 8 apart) followed by a sum over the eight
+// lanes that share bits 5..3: 10 cross-lane moves for the eight sums where
+// eight independent butterfly reductions take 48.
+__device__ __forceinline__ float wave_sum8_transposed(const float (&w)[8]) {
+    const int lane = threadIdx.x;
+    float a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = swap_add<32>(w[k], w[4 + k]);  // corner 4*b5 + k
+    float b[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) b[k] = swap_add<16>(a[k], a[2 + k]);  // corner 4*b5 + 2*b4 + k
+    const bool h3 = (lane & 8) != 0;
+    float c = (h3 ? b[1] : b[0]) + dpp<kDppRowRor8>(h3 ? b[0] : b[1]);  // corner l >> 3
+    c += dpp<kDppHalfMirror>(c);
+    c += dpp<kDppQuadXor2>(c);
+    c += dpp<kDppQuadXor1>(c);
+    return c;
+}
+
 // Scatter v times the eight trilinear weights of each lane's tap (lanes with
 // `has`) into the FP64 gradient accumulator. Neighbouring samples of a wave
-// mostly fall in the same 250 m voxel, and same-address atomics serialise,
-// so lanes whose taps share all eight corners are first summed across the
-// wave: up to kPeel groups (one atomic per corner per group); lanes left
-// after that add their own. Called with all 64 lanes active.
-constexpr int kPeel = 4;
+// mostly fall in the same voxel cell, and same-address atomics serialise, so
+// lanes whose taps share all eight corners are first summed across the wave
+// (up to kPeel groups, one atomic instruction of eight lanes per group);
+// lanes left after that add their own. Called with all 64 lanes active.
+constexpr int kPeel = 8;
 
 __device__ __forceinline__ void delta_scatter_wave(double* __restrict__ g, bool has, const DeltaTap& d, float v) {
     const int lane = threadIdx.x;
@@ -85,8 +131,8 @@ __device__ __forceinline__ void delta_scatter_wave(double* __restrict__ g, bool 
     const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
     const float w[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
                         v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
-    const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
-                             d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
+    // this lane's corner when it adds a group's sum (lane >> 3 = 4 z + 2 y + x)
+    const int cz = (lane >> 5) & 1, cy = (lane >> 4) & 1, cx = (lane >> 3) & 1;
     unsigned long long act = __ballot(has);
     for (int it = 0; it < kPeel && act != 0ull; ++it) {
         const int leader = __builtin_ctzll(act);
@@ -94,12 +140,19 @@ __device__ __forceinline__ void delta_scatter_wave(double* __restrict__ g, bool 
         const bool m = key == k;
         const unsigned long long mm = __ballot(m);
         if (__popcll(mm) > 1) {
+            float wm[8];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const float s = wave_sum(m ? w[c] : 0.f);
-                if (lane == leader) atomicAdd(g + off[c], (double)s);
-            }
+            for (int c = 0; c < 8; ++c) wm[c] = m ? w[c] : 0.f;
+            const float sum = wave_sum8_transposed(wm);
+            // the group's corner offsets (the leader's), selected per lane
+            const uint32_t z0 = __builtin_amdgcn_readlane(d.z0, leader), z1 = __builtin_amdgcn_readlane(d.z1, leader);
+            const uint32_t y0 = __builtin_amdgcn_readlane(d.y0, leader), y1 = __builtin_amdgcn_readlane(d.y1, leader);
+            const uint32_t x0 = __builtin_amdgcn_readlane(d.x0, leader), x1 = __builtin_amdgcn_readlane(d.x1, leader);
+            const uint32_t o = (cz ? z1 : z0) + (cy ? y1 : y0) + (cx ? x1 : x0);
+            if ((lane & 7) == 0) atomicAdd(g + o, (double)sum);
         } else if (lane == leader) {
+            const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
+                                     d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
 #pragma unroll
             for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
         }
@@ -107,6 +160,8 @@ __device__ __forceinline__ void delta_scatter_wave(double* __restrict__ g, bool 
         act &= ~mm;
     }
     if (has) {
+        const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
+                                 d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
 #pragma unroll
         for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
     }
@@ -516,6 +571,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     const float h0f = (float)P.h0;
     const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
     const float wscale = P.inv_weather_scale, nscale = P.inv_base_scale * 128.f;
+    const float gmax = DELTA ? delta_gmax(P) : 0.f;
     const int steps = P.steps, cps = P.ck_per_seg, ckl = P.ck_len;
     int chunk = -1;
     if (n_exec > 0) {
@@ -577,11 +633,8 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                                       (((int)floorf(cx) >> 2) & 31);
                 const uint2 wq = ld_g<uint2>(P.weather, woff);
                 const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
-                float dcx = 0.f, dcy = 0.f, dcz = 0.f, dm = 0.f;
-                if constexpr (DELTA) {
-                    delta_coords(P, wx, wz, h, dcx, dcy, dcz);
-                    dm = delta_bound(P, dcx, dcy, dcz);
-                }
+                float dcx = 0.f, dcy = 0.f, dcz = 0.f;
+                if constexpr (DELTA) delta_coords(P, wx, wz, h, dcx, dcy, dcz);
                 wr = weather_blend(wq, wfx, wfy);
                 ctype = L.type;
                 NdfC nc = layer_ndf(L);
@@ -599,7 +652,12 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     const float cov = clampf(wc, 0.f, 1.f);
                     if (cov >= 0.01f) {
                         const float ndfv = ndf_eval(nc, ctype, h, L.anvil);
-                        const float thr = DELTA ? (1.f - cov - dm) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
+                        float thr = DELTA ? (1.f - cov - gmax) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
+                        float2 db = make_float2(0.f, 0.f);
+                        if (DELTA && ndfv > thr) {
+                            db = delta_bound(P, dcx, dcy, dcz);
+                            thr = (1.f - cov - db.x) - 1e-5f;
+                        }
                         const float rm = (float)rmb * (1.f / 255.f);
                         float cloud = -1.f, dsh = 1.f;
                         if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
@@ -618,7 +676,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                                 dsh = frcp(fmaxf(1.f - e, 1e-5f));
                                 cloud = (cloud - e) * dsh;
                             }
-                            if constexpr (DELTA) cloud += delta_fetch(P.delta, delta_tap(dcx, dcy, dcz));
+                            if constexpr (DELTA) cloud += delta_value(P, db, dcx, dcy, dcz);
                         }
                         cval = clampf(cloud, 0.f, 1.f);
                         density = cval * L.density_scale;
@@ -723,31 +781,66 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     }
 }
 
+// Three waves per SIMD: the register budget is 168 VGPRs (the kernel fits
+// without spilling).
 template <bool DELTA>
-__global__ __launch_bounds__(64) void k_render_backward(const RenderParams* __restrict__ Pp) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_render_backward(
+    const RenderParams* __restrict__ Pp) {
     __shared__ BackQueue Qs;
     backward_tile<DELTA>(*Pp, blockIdx.x, blockIdx.y, Qs);
 }
 
-// Per-brick maxima of the perturbation volume over each 4^3 brick and its +1
-// halo (Repeat in x and z, ClampToEdge in height): every trilinear tap whose
-// lower corner lies in the brick reads only texels inside this range.
-__global__ void k_delta_bmax(const float* __restrict__ d, float* __restrict__ bmax) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= 32 * 32 * 32) return;
+// Bounds (max, min) of the perturbation volume over each 4^3 brick and its
+// +1 halo (Repeat in x and z, ClampToEdge in height): every trilinear tap
+// whose lower corner lies in the brick reads only texels inside this range.
+// Each block of 256 bricks also writes its own bounds after the volume
+// entry (kDeltaBricks + 1 + block) for k_delta_global.
+constexpr int kDeltaBricks = 32 * 32 * 32;
+constexpr int kDeltaBoundBlocks = kDeltaBricks / 256;
+
+__global__ __launch_bounds__(256) void k_delta_bounds(const float* __restrict__ d, float2* __restrict__ bnd) {
+    __shared__ float smx[256], smn[256];
+    const int t = threadIdx.x;
+    const int b = blockIdx.x * 256 + t;
     const int bx = b & 31, by = (b >> 5) & 31, bz = b >> 10;
-    float m = -3.0e38f;
+    float mx = -3.0e38f, mn = 3.0e38f;
     for (int k = 0; k < 5; ++k) {
         const int z = (bz * 4 + k) & 127;
         for (int j = 0; j < 5; ++j) {
             const int yy = min(by * 4 + j, 127);
             for (int i = 0; i < 5; ++i) {
                 const int xx = (bx * 4 + i) & 127;
-                m = fmaxf(m, d[(z * 128 + yy) * 128 + xx]);
+                const float v = d[(z * 128 + yy) * 128 + xx];
+                mx = fmaxf(mx, v);
+                mn = fminf(mn, v);
             }
         }
     }
-    bmax[b] = m;
+    bnd[b] = make_float2(mx, mn);
+    smx[t] = mx;
+    smn[t] = mn;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            smx[t] = fmaxf(smx[t], smx[t + o]);
+            smn[t] = fminf(smn[t], smn[t + o]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) bnd[kDeltaBricks + 1 + blockIdx.x] = make_float2(smx[0], smn[0]);
+}
+
+// Volume bounds from the per-block bounds, written after the brick entries.
+__global__ __launch_bounds__(64) void k_delta_global(float2* __restrict__ bnd) {
+    const int t = threadIdx.x;
+    const float2 a = bnd[kDeltaBricks + 1 + t], c = bnd[kDeltaBricks + 1 + 64 + t];
+    float mx = fmaxf(a.x, c.x), mn = fminf(a.y, c.y);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        mn = fminf(mn, __shfl_xor(mn, o, 64));
+    }
+    if (t == 0) bnd[kDeltaBricks] = make_float2(mx, mn);
 }
 
 __global__ void k_zero_f64(double* __restrict__ p, int n) {
@@ -773,7 +866,9 @@ hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H,
     return hipGetLastError();
 }
 
-hipError_t ps_launch_delta_bmax(const float* delta, float* bmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_delta_bmax, dim3(128), dim3(256), 0, s, delta, bmax);
+hipError_t ps_launch_delta_bounds(const float* delta, float2* bnd, hipStream_t s) {
+    static_assert(kDeltaBoundBlocks == 128, "k_delta_global reduces 128 block bounds with one wave");
+    hipLaunchKernelGGL(k_delta_bounds, dim3(kDeltaBoundBlocks), dim3(256), 0, s, delta, bnd);
+    hipLaunchKernelGGL(k_delta_global, dim3(1), dim3(64), 0, s, bnd);
     return hipGetLastError();
 }

@@ -259,7 +259,7 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
     }
     P.counters = c->d_counters;
     P.delta = c->delta;
-    P.delta_bmax = c->delta_bmax;
+    P.delta_bnd = c->delta_bnd;
     P.delta_su = (float)(128.0 / std::max((double)cp.weather_scale_m, 1.0));
     P.n_tiles_x = (P.W + 7) / 8;
     return PS_OK;
@@ -305,13 +305,13 @@ ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
     return PS_OK;
 }
 
-// Brick maxima of the caller's perturbation volume (it may have changed
-// since the last render: recomputed on every render that uses it).
+// Brick and volume bounds of the caller's perturbation volume (it may have
+// changed since the last render: recomputed on every render that uses it).
 ps_status prepare_delta(ps_ctx* c) {
     if (!c->delta) return PS_OK;
     ps_status s;
-    if ((s = dev_alloc(&c->delta_bmax, sizeof(float) * 32 * 32 * 32, "perturbation brick maxima"))) return s;
-    PS_HIP(ps_launch_delta_bmax(c->delta, c->delta_bmax, c->stream));
+    if ((s = dev_alloc(&c->delta_bnd, sizeof(float2) * (32 * 32 * 32 + 1 + 128), "perturbation brick bounds"))) return s;
+    PS_HIP(ps_launch_delta_bounds(c->delta, c->delta_bnd, c->stream));
     return PS_OK;
 }
 
@@ -408,7 +408,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
-    dev_free(c->delta_bmax); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->g_delta64);
+    dev_free(c->delta_bnd); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->g_delta64);
     dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
     dev_free(c->aTT); dev_free(c->aMST); dev_free(c->aSVT); dev_free(c->aAPT);
     if (c->atmo_params_h) hipHostFree(c->atmo_params_h);

@@ -130,7 +130,9 @@ struct RenderParams {
     // layer-normalised height, then z; x and z span the weather tile
     // (Repeat), height is ClampToEdge.
     const float* delta;
-    const float* delta_bmax;          // 32^3: max of delta over each 4^3 brick and its +1 halo
+    // (max, min) of delta over each 4^3 brick and its +1 halo (32^3
+    // float2), then over the whole volume (1), then 128 per-block partials
+    const float2* delta_bnd;
     float delta_su;                   // 128 / weather_scale: delta texels per metre
     // ---- training forward (checkpoints) and backward ----
     // Per wave tile (tiles row-major, 64 lanes): transmittance checkpoints

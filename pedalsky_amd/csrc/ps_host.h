@@ -144,9 +144,9 @@ struct ps_ctx {
     bool timing = false;
     int timing_max = 0, timing_n = 0;
 
-    // differentiation: caller-owned perturbation volume and its brick maxima
+    // differentiation: caller-owned perturbation volume and its brick bounds
     const float* delta = nullptr;
-    float* delta_bmax = nullptr;
+    float2* delta_bnd = nullptr;
     // training forward state read by ps_backward
     int ck_len = 16;
     float* ck = nullptr;

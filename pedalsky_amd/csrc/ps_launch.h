@@ -26,7 +26,7 @@ hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H,
                                      double* g_delta64, hipStream_t s);
 // Per-brick maxima of the 128^3 density-perturbation volume (32^3 bricks of
 // 4^3 texels plus a +1 halo: Repeat in x and z, ClampToEdge in height).
-hipError_t ps_launch_delta_bmax(const float* delta, float* bmax, hipStream_t s);
+hipError_t ps_launch_delta_bounds(const float* delta, float2* bnd, hipStream_t s);
 
 // Many scenes in one launch: `tiles` lists (scene << 20 | tile_y << 10 |
 // tile_x) for every wave tile of every scene; a wave tile is tile_w x

@@ -179,12 +179,19 @@ __device__ __forceinline__ DeltaTap delta_tap(float cx, float cy, float cz) {
     return d;
 }
 
-// Upper bound of the perturbation around a tap: the max over the 4^3 brick
+// Bounds (max, min) of the perturbation around a tap: over the 4^3 brick
 // holding its lower corner and the brick's +1 halo (all eight corners).
-__device__ __forceinline__ float delta_bound(const RenderParams& P, float cx, float cy, float cz) {
+__device__ __forceinline__ float2 delta_bound(const RenderParams& P, float cx, float cy, float cz) {
     const uint32_t bx = ((uint32_t)(int)floorf(cx) & 127u) >> 2, bz = ((uint32_t)(int)floorf(cz) & 127u) >> 2;
     const uint32_t by = (uint32_t)clampi((int)floorf(cy), 128) >> 2;
-    return ld_g<float>(P.delta_bmax, ((bz << 10) | (by << 5) | bx) * 4u);
+    return ld_g<float2>(P.delta_bnd, ((bz << 10) | (by << 5) | bx) * 8u);
+}
+
+// Maximum of the perturbation over the whole volume. Written before the
+// launch and constant during it: read through the constant address space,
+// which makes it one scalar load the compiler may hoist.
+__device__ __forceinline__ float delta_gmax(const RenderParams& P) {
+    return ((const __attribute__((address_space(4))) float2*)(P.delta_bnd))[32 * 32 * 32].x;
 }
 
 __device__ __forceinline__ float delta_fetch(const float* __restrict__ v, const DeltaTap& d) {
@@ -196,6 +203,13 @@ __device__ __forceinline__ float delta_fetch(const float* __restrict__ v, const 
     const float b0 = c001 + (c101 - c001) * d.fx, b1 = c011 + (c111 - c011) * d.fx;
     const float p0 = a0 + (a1 - a0) * d.fy, p1 = b0 + (b1 - b0) * d.fy;
     return p0 + (p1 - p0) * d.fz;
+}
+
+// Perturbation at a tap whose brick bounds are b: a brick whose eight-corner
+// range is one value needs no fetch (the trilinear blend of equal corners is
+// that value exactly).
+__device__ __forceinline__ float delta_value(const RenderParams& P, float2 b, float cx, float cy, float cz) {
+    return b.x == b.y ? b.x : delta_fetch(P.delta, delta_tap(cx, cy, cz));
 }
 
 __device__ __forceinline__ void delta_coords(const RenderParams& P, float wx, float wz, float h, float& cx, float& cy,
@@ -210,10 +224,19 @@ __device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, f
                                        float ndfv, float cov, float rm, float dcx = 0.f, float dcy = 0.f,
                                        float dcz = 0.f) {
     // with the perturbation volume the bound is shape + max(delta) <= 0
-    // (an absolute 1e-5 margin: every term is O(1))
+    // (an absolute 1e-5 margin: every term is O(1)), with the volume's
+    // maximum first and then the brick's (no smaller, so the first test
+    // skips a subset of what the second would)
     float thr;
-    if constexpr (DELTA) thr = (1.f - cov - delta_bound(P, dcx, dcy, dcz)) - 1e-5f;
-    else thr = (1.f - cov) * (1.f - 1e-5f);
+    float2 db = make_float2(0.f, 0.f);
+    if constexpr (DELTA) {
+        thr = (1.f - cov - delta_gmax(P)) - 1e-5f;
+        if (ndfv <= thr) return -1.f;
+        db = delta_bound(P, dcx, dcy, dcz);
+        thr = (1.f - cov - db.x) - 1e-5f;
+    } else {
+        thr = (1.f - cov) * (1.f - 1e-5f);
+    }
     if (ndfv <= thr) return -1.f;
     if (rm < 0.f) {
         const uint32_t bx = ((int)floorf(cx) >> 2) & 31, by = ((int)floorf(cy) >> 2) & 31,
@@ -225,7 +248,7 @@ __device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, f
     float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
     float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
     float v = base_cloud * ndfv - (1.f - cov);
-    if constexpr (DELTA) v += delta_fetch(P.delta, delta_tap(dcx, dcy, dcz));
+    if constexpr (DELTA) v += delta_value(P, db, dcx, dcy, dcz);
     return v;
 }
 

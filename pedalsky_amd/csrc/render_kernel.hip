@@ -224,6 +224,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     const float h0f = (float)P.h0;
     const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
     const float wscale = P.inv_weather_scale, nscale = P.inv_base_scale * 128.f;
+    const float gmax = DELTA ? delta_gmax(P) : 0.f;
     float tr = 1.f, tg = 1.f, tbb = 1.f;
     int cur = 0, s = 0, li = 0;
     float tstart = 0.f, step = 0.f;
@@ -274,11 +275,8 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                                   (((int)floorf(cx) >> 2) & 31);
             const uint2 wq = ld_g<uint2>(P.weather, woff);
             const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
-            float dcx = 0.f, dcy = 0.f, dcz = 0.f, dm = 0.f;
-            if constexpr (DELTA) {
-                delta_coords(P, wx, wz, h, dcx, dcy, dcz);
-                dm = delta_bound(P, dcx, dcy, dcz);
-            }
+            float dcx = 0.f, dcy = 0.f, dcz = 0.f;
+            if constexpr (DELTA) delta_coords(P, wx, wz, h, dcx, dcy, dcz);
             wr = weather_blend(wq, wfx, wfy);
             ctype = L.type;
             NdfC nc = layer_ndf(L);
@@ -299,8 +297,14 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                 if (cov >= 0.01f) {
                     stage = 1;
                     const float ndfv = ndf_eval(nc, ctype, h, L.anvil);
-                    // with the perturbation volume, skip only when shape + max(delta) <= 0
-                    const float thr = DELTA ? (1.f - cov - dm) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
+                    // with the perturbation volume, skip only when shape + max(delta) <= 0:
+                    // the volume's maximum first, then the brick's (see shape())
+                    float thr = DELTA ? (1.f - cov - gmax) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
+                    float2 db = make_float2(0.f, 0.f);
+                    if (DELTA && ndfv > thr) {
+                        db = delta_bound(P, dcx, dcy, dcz);
+                        thr = (1.f - cov - db.x) - 1e-5f;
+                    }
                     const float rm = (float)rmb * (1.f / 255.f);
                     float cloud = -1.f;
                     if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
@@ -319,7 +323,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                             const float e = dmod * L.ld;
                             cloud = (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
                         }
-                        if constexpr (DELTA) cloud += delta_fetch(P.delta, delta_tap(dcx, dcy, dcz));
+                        if constexpr (DELTA) cloud += delta_value(P, db, dcx, dcy, dcz);
                     }
                     density = clampf(cloud, 0.f, 1.f) * L.density_scale;
                 }

@@ -191,22 +191,42 @@ def _gpu_value(lg, dg, kind, idx):
     return lg[{DS: 0, KAPPA: 8, COV: 16}[kind] + idx]
 
 
-def _check(params, got, ref, what):
+# A primary sample whose FP64 density lies within this of the 1e-3 dense
+# threshold may be decided the other way by the FP32 march (its density
+# carries ~1e-6 absolute rounding error); a voxel whose derivative depends on
+# such a sample can differ by that sample's whole term.
+NEAR_TOL = 1e-5
+
+
+def _check(params, got, ref, what, skip=None):
+    """Relative error <= 1e-3 per parameter, the scale floored at 1e-3 of the
+    largest reference of the same kind. Perturbation voxels in `skip` (their
+    derivative has a term from a sample at the dense threshold, see
+    Frame.vjp_clouds) are reported but not held to the bar."""
     got, ref = np.asarray(got), np.asarray(ref)
+    skip = np.zeros(len(params), bool) if skip is None else np.asarray(skip, bool)
     worst, bad = 0.0, []
     for kind in (DS, KAPPA, COV, DELTA):
         sel = [i for i, (k, _) in enumerate(params) if k == kind]
         if not sel:
             continue
         floor = 1e-3 * np.abs(ref[sel]).max()
-        rels = [abs(got[i] - ref[i]) / max(abs(ref[i]), floor) for i in sel]
-        _log(f"  {what}: kind {kind}: max relative error {max(rels):.3e} over {len(sel)}")
-        for i, rel in zip(sel, rels):
-            worst = max(worst, rel)
-            if rel > 1e-3:
-                bad.append((params[i], got[i], ref[i], rel))
+        rels = {i: abs(got[i] - ref[i]) / max(abs(ref[i]), floor) for i in sel}
+        held = [i for i in sel if not (kind == DELTA and skip[i])]
+        for i in sel:
+            if i not in held:
+                _log(f"  {what}: {params[i]} at a dense-threshold sample, not held: gpu {got[i]:.6e} "
+                     f"ref {ref[i]:.6e} rel {rels[i]:.3e}")
+        _log(f"  {what}: kind {kind}: max relative error {max(rels[i] for i in held):.3e} over {len(held)}")
+        for i in held:
+            worst = max(worst, rels[i])
+            if rels[i] > 1e-3:
+                bad.append((params[i], got[i], ref[i], rels[i]))
     _log(f"  {what}: {len(params)} parameters, max relative error {worst:.3e}")
     assert not bad, bad
+    is_delta = np.array([k == DELTA for k, _ in params])
+    n_delta, n_skip = int(is_delta.sum()), int((is_delta & skip).sum())
+    assert n_delta - n_skip >= (7 * n_delta) // 10, "too few perturbation voxels held to the bar"
 
 
 def test_gradients_match_oracle_dual(renderer):
@@ -232,9 +252,10 @@ def test_gradients_match_oracle_dual(renderer):
     # the same cotangent image on both sides: only the Jacobians are compared
     gw = g[y0:y0 + ch, x0:x0 + cw].cpu().numpy().astype(np.float64)
     params = _params_for(lg, dg, 2, 10, np.random.default_rng(1))
-    ref = fr.vjp_clouds(diff, params, x0, y0, cw, ch, g_comp=gw)
+    _log(f"  window ({x0}, {y0}); voxels {[i for k, i in params if k == DELTA]}")
+    ref, near = fr.vjp_clouds(diff, params, x0, y0, cw, ch, g_comp=gw, near_tol=NEAR_TOL)
     got = [_gpu_value(lg, dg, k, i) for k, i in params]
-    _check(params, got, ref, "GPU vs FP64 dual")
+    _check(params, got, ref, "GPU vs FP64 dual", skip=near)
 
 
 def test_gradients_match_fp64_finite_differences(renderer):
@@ -287,7 +308,12 @@ def test_gradients_match_fp64_finite_differences(renderer):
             _log(f"  FD {kind, idx}: h = {used:g}")
         fd.append(d)
     got = [_gpu_value(lg, dg, k, i) for k, i in params]
-    _check(params, got, fd, "GPU vs FP64 finite differences")
+    # which voxels have a term from a sample at the dense threshold (the
+    # finite differences see the FP64 side of that decision)
+    g_or = np.zeros((ch, cw, 4))
+    g_or[..., :3] = 2.0 * (comp[y0:y0 + ch, x0:x0 + cw, :3].cpu().numpy().astype(np.float64) - tgt_win)
+    _, near = fr.vjp_clouds(diff, params, x0, y0, cw, ch, g_comp=g_or, near_tol=NEAR_TOL)
+    _check(params, got, fd, "GPU vs FP64 finite differences", skip=near)
 
 
 def test_checkpoint_interval_does_not_change_gradients(renderer):
