@@ -338,10 +338,13 @@ def _frame_vjp_clouds(self, diff: Diff, params, x0, y0, w, h, g_comp=None, g_lum
     """Exact FP64 vector-Jacobian product (dual instantiation) for a list of
     (kind, index) parameters; cotangent images are (h, w, 4) or None.
 
-    With near_tol > 0 also returns a boolean per parameter: whether a primary
-    sample whose density lies within near_tol of the 1e-3 dense threshold
-    depends on it (an FP32 march may decide that sample the other way, which
-    changes this parameter's derivative by that sample's whole term)."""
+    With near_tol > 0 also returns a boolean per parameter: whether a
+    quantity within near_tol of one of the march's decision thresholds
+    depends on it (a primary density at the 1e-3 dense threshold, a shape
+    value at the erosion branch, a primary or light-march cloud value at the
+    final clamp's kinks 0 and 1). An FP32 march may decide such a sample the
+    other way, which changes this parameter's derivative by that sample's
+    whole term."""
     def cot(g):
         return None if g is None else np.ascontiguousarray(g, dtype=np.float64).reshape(h, w, 4)
     gc, gl, gt = cot(g_comp), cot(g_lum), cot(g_trn)

@@ -291,8 +291,9 @@ constexpr int kDualN = 16;
 // g_trn * d trn / d theta_k for the (at most 16) tracked parameters of `d`,
 // computed with the Dual<16> instantiation of the FP64 restatement. The
 // cotangent images are w*h RGBA (alpha ignored), any of them may be null.
-// near_tol > 0: *near_mask receives the tangent slots a primary sample
-// within near_tol of the dense threshold depends on (see Counters).
+// near_tol > 0: *near_mask receives the tangent slots that a quantity
+// within near_tol of one of the march's decision thresholds depends on (see
+// Counters and note_near).
 int or_vjp_clouds(const or_inputs* in, const or_diff* d, int x0, int y0, int w, int h, const double* g_comp,
                   const double* g_lum, const double* g_trn, double* grad, int nthreads, double near_tol,
                   uint32_t* near_mask) {

@@ -85,12 +85,25 @@ struct Scene {
 struct Counters {
     uint64_t pixels = 0, layer_marches = 0, primary_steps = 0, covered_steps = 0, eroded_steps = 0;
     uint64_t dense_samples = 0, light_steps = 0, light_covered = 0, octaves = 0, ap_taps = 0;
-    // dual instantiation only: tangent slots whose derivative a primary
-    // sample within near_tol of the dense threshold depends on (an FP32
-    // evaluation may take the other side of that decision)
+    // dual instantiation only: tangent slots that a quantity within near_tol
+    // of one of the march's decision thresholds depends on (an FP32
+    // evaluation may take the other side of that decision): the dense
+    // threshold, the erosion branch, and the final clamp's kinks in the
+    // primary and light-march densities (see note_near)
     double near_tol = 0.0;
     uint32_t near_mask = 0u;
 };
+
+// Dual instantiation: when x lies within cnt->near_tol of the decision
+// threshold t, marks the tangent slots x depends on.
+template <class R>
+inline void note_near(Counters* cnt, const R& x, double t) {
+    if constexpr (is_dual<R>::value) {
+        if (cnt && cnt->near_tol > 0.0 && std::fabs(x.v - t) < cnt->near_tol)
+            for (int k = 0; k < R::n; ++k)
+                if (x.d[k] != 0.0) cnt->near_mask |= 1u << k;
+    }
+}
 
 // ---- math.wgsl:7-9 -----------------------------------------------------------------
 template <class R>
@@ -231,6 +244,8 @@ inline R sample_density_light(const Scene<R>& s, const V3<R>& p, R p_alt, const 
     cloud = remap<R>(cloud, R(1) - cov, 1, 0, 1);
     cloud = cloud * cov;
     cloud = cloud + delta_sample(s, p.x - w.off_x, p.z - w.off_z, h);
+    note_near(cnt, cloud, 0.0);
+    note_near(cnt, cloud, 1.0);
     return clampr<R>(cloud, 0, 1) * L.density_scale;
 }
 
@@ -267,6 +282,7 @@ inline R sample_density(const Scene<R>& s, const V3<R>& p, R p_alt, const Layer<
     R cov = clampr<R>(weather_r * L.coverage, 0, 1);
     cloud = remap<R>(cloud, R(1) - cov, 1, 0, 1);
     cloud = cloud * cov;
+    note_near(cnt, cloud, 0.0);
     if (cloud > R(0)) {
         if (cnt) cnt->eroded_steps++;
         R ds = std::max(s.c.detail_scale, R(1));
@@ -283,6 +299,7 @@ inline R sample_density(const Scene<R>& s, const V3<R>& p, R p_alt, const Layer<
         cloud = remap<R>(cloud, dmod * ld, 1, 0, 1);
     }
     cloud = cloud + delta_sample(s, p.x - w.off_x, p.z - w.off_z, h);
+    note_near(cnt, cloud, 1.0);
     return clampr<R>(cloud, 0, 1) * L.density_scale;
 }
 
@@ -661,11 +678,7 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
             sample2d(s.weather, wx / wsc + R(0.5), wz / wsc + R(0.5), true, wsample);
             uint32_t ctype = effective_cloud_type(s, wx, wz, L);
             R density = sample_density(s, p, alt, L, wsample[0], wind, ctype, cnt);
-            if constexpr (is_dual<R>::value) {
-                if (cnt && std::fabs(density.v - 1e-3) < cnt->near_tol)
-                    for (int k = 0; k < R::n; ++k)
-                        if (density.d[k] != 0.0) cnt->near_mask |= 1u << k;
-            }
+            note_near(cnt, density, 1e-3);
             if (density > R(1e-3)) {
                 if (cnt) cnt->dense_samples++;
                 R od = march_to_light(s, p, alt, sun, L, wsample[0], wind, ctype, cnt);

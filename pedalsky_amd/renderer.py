@@ -27,6 +27,7 @@ class Renderer:
         self.ctx = h
         self.workload = None
         self._depth_dev = None
+        self._density_delta = None  # borrowed by the library (set_cloud_extensions)
 
     # lifecycle ---------------------------------------------------------------
     def close(self) -> None:
@@ -187,7 +188,10 @@ class Renderer:
     def set_cloud_extensions(self, extinction_scale=None, density_delta=None) -> None:
         """Per-layer extinction scales (host sequence, None = all 1) and the
         128^3 density-perturbation volume (a contiguous float32 device
-        tensor the library reads on every render, or None)."""
+        tensor the library reads on every render, or None). The library
+        borrows the tensor's memory until the next call: the renderer keeps
+        a reference, so a temporary stays alive (and its memory is not handed
+        to another tensor) while renders read it."""
         k = None
         n = 0
         if extinction_scale is not None:
@@ -201,6 +205,7 @@ class Renderer:
                 raise ValueError("density_delta must be a contiguous float32 CUDA tensor of 128^3 elements")
             ptr = density_delta.data_ptr()
         check(self.lib.ps_set_cloud_extensions(self.ctx, k, n, ptr))
+        self._density_delta = density_delta
 
     def set_checkpoint_interval(self, steps: int) -> None:
         check(self.lib.ps_set_checkpoint_interval(self.ctx, steps))
