@@ -221,16 +221,14 @@ def _gpu_value(lg, dg, kind, idx):
 NEAR_TOL = 1e-5
 
 
-def _check(params, got, ref, what, skip=None):
-    """Per-layer parameters: relative error <= 1e-3 each. Perturbation
-    voxels: relative error <= 1e-3 for at least 90 % of them and <= 5e-3 for
-    every one (the scale floored at 1e-3 of the largest reference of the
-    kind). The FP32 march's own per-pixel error has a tail (the cloud
-    luminance reaches 2e-3 relative at the 99.9th percentile), and a voxel
-    whose gradient comes from few pixels inherits it; over 116 voxels of two
-    windows the errors against the exact FP64 derivatives have a median of
-    ~1e-4 and reach 6.5e-4. Voxels in `skip` (a term from a sample at a
-    decision threshold, see Frame.vjp_clouds) are reported, not held."""
+def _check(params, got, ref, what, skip=None, fd=False):
+    """Relative error <= 1e-3 per parameter (the scale floored at 1e-3 of the
+    largest reference of the same kind). Against finite differences (fd),
+    whose own error against the exact derivatives reaches ~1e-3 for a few
+    voxels (decisions flipping inside [theta - h, theta + h] at every step
+    tried), perturbation voxels are held to <= 1e-3 for at least 90 % of
+    them and <= 5e-3 for every one. Voxels in `skip` (a term from a sample at
+    a decision threshold, see Frame.vjp_clouds) are reported, not held."""
     got, ref = np.asarray(got), np.asarray(ref)
     skip = np.zeros(len(params), bool) if skip is None else np.asarray(skip, bool)
     worst, bad = 0.0, []
@@ -251,7 +249,7 @@ def _check(params, got, ref, what, skip=None):
         _log(f"  {what}: kind {kind}: relative error median {np.median(hr):.3e} p90 {np.percentile(hr, 90):.3e} "
              f"max {hr.max():.3e} over {len(held)}")
         worst = max(worst, hr.max())
-        if kind != DELTA:
+        if kind != DELTA or not fd:
             bad += [(params[i], got[i], ref[i], rels[i]) for i in held if rels[i] > 1e-3]
         else:
             if np.percentile(hr, 90) > 1e-3:
@@ -353,7 +351,7 @@ def test_gradients_match_fp64_finite_differences(renderer):
     _log(f"  FD vs FP64 dual: median {np.median(fd_err):.3e} max {fd_err.max():.3e}")
     gpu_err = np.abs(np.asarray(got) - dual) / np.maximum(np.abs(dual), 1e-3 * np.abs(dual[6:]).max())
     _log(f"  GPU vs FP64 dual on these parameters: median {np.median(gpu_err):.3e} max {gpu_err.max():.3e}")
-    _check(params, got, fd, "GPU vs FP64 finite differences", skip=near)
+    _check(params, got, fd, "GPU vs FP64 finite differences", skip=near, fd=True)
 
 
 def test_checkpoint_interval_does_not_change_gradients(renderer):
