@@ -230,6 +230,9 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     float tstart = 0.f, step = 0.f;
     LayerF L;
     bool active = nseg > 0;
+    // training forward: steps until the next checkpoint and its chunk index
+    // (counters instead of a division by the runtime interval every step)
+    int ck_left = 0, ck_c = 0;
     // enter the march's next layer interval (the cur-th in entry order)
     auto enter = [&](int k) {
         li = (int)((order >> (3 * k)) & 7u);
@@ -241,6 +244,8 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         tstart = (float)(ta + jitter * stepd);
         step = (float)stepd;
         s = 0;
+        ck_left = 0;
+        ck_c = k * P.ck_per_seg;
     };
     if (active) enter(0);
     int qn = 0;  // queued samples (wave-uniform)
@@ -251,11 +256,15 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         if (active) {
             if constexpr (TRAIN) {
                 // transmittance at the first step of every checkpoint chunk
-                if (s % P.ck_len == 0) {
-                    const int c = cur * P.ck_per_seg + s / P.ck_len;
-                    float* ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + c) * 192 + lane;
+                // (steps 0, ck_len, 2 ck_len, ... of segment cur: chunk
+                // cur * ck_per_seg + s / ck_len)
+                if (ck_left == 0) {
+                    float* ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + ck_c) * 192 + lane;
                     ckp[0] = tr; ckp[64] = tg; ckp[128] = tbb;
+                    ++ck_c;
+                    ck_left = P.ck_len;
                 }
+                --ck_left;
             }
             cnt.primary++;
             t = tstart + (float)s * step;
