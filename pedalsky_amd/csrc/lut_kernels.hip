@@ -106,12 +106,58 @@ __device__ __forceinline__ void store4(float4* out, int idx, D3 v, double w) {
     out[idx] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)w);
 }
 
+// ---------------------------------------------------------------------------
+// Lane-parallel integration. A LUT bake is a few hundred waves of long
+// serial FP64 loops (exp, sqrt, divides): one wave per SIMD at most, so the
+// loop latency is exposed. Each texel's march is therefore split into SEG
+// contiguous step ranges on SEG neighbouring lanes: optical depths add, and
+// an in-scatter integral combines segment results (L_k, T_k), each computed
+// from T = 1 at its segment start, as L = sum_k (prod_{j<k} T_j) L_k with an
+// exclusive product scan over the segment lanes.
+// ---------------------------------------------------------------------------
+template <int SEG>
+__device__ __forceinline__ double seg_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < SEG; o <<= 1) v += __shfl_xor(v, o, SEG);
+    return v;
+}
+template <int SEG>
+__device__ __forceinline__ D3 seg_sum3(D3 v) { return d3(seg_sum<SEG>(v.x), seg_sum<SEG>(v.y), seg_sum<SEG>(v.z)); }
+
+// Inclusive product scan over the SEG lanes of a group (lane `seg` gets
+// prod_{j <= seg} v_j).
+template <int SEG>
+__device__ __forceinline__ D3 seg_scan_mul3(D3 v, int seg) {
+#pragma unroll
+    for (int d = 1; d < SEG; d <<= 1) {
+        const D3 u = d3(__shfl_up(v.x, d, SEG), __shfl_up(v.y, d, SEG), __shfl_up(v.z, d, SEG));
+        if (seg >= d) v = mul(v, u);
+    }
+    return v;
+}
+
+// Combine segment results: returns sum_k (prod_{j<k} T_j) L_k and the total
+// transmittance prod_k T_k on every lane of the group.
+template <int SEG>
+__device__ __forceinline__ void seg_combine(D3 lum, D3 tr, int seg, D3& lum_all, D3& tr_all) {
+    const D3 inc = seg_scan_mul3<SEG>(tr, seg);
+    D3 exc = d3(__shfl_up(inc.x, 1, SEG), __shfl_up(inc.y, 1, SEG), __shfl_up(inc.z, 1, SEG));
+    if (seg == 0) exc = d3(1, 1, 1);
+    lum_all = seg_sum3<SEG>(mul(exc, lum));
+    tr_all = d3(__shfl(inc.x, SEG - 1, SEG), __shfl(inc.y, SEG - 1, SEG), __shfl(inc.z, SEG - 1, SEG));
+}
+
 // ---- transmittance.comp.wgsl:18-31 with atmosphere.wgsl:162-214 ---------------
+// 40 trapezoid steps on kTSeg lanes (steps k0 + 1 .. k0 + 40 / kTSeg each,
+// the lane re-evaluating the extinction at its first node).
+constexpr int kTSeg = 8;
 __device__ __forceinline__ void transmittance_body(const LutParams& P, float4* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.tw * P.th) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / kTSeg, seg = gid % kTSeg;
+    const bool valid = i < P.tw * P.th;
+    const int ic = valid ? i : 0;
     const AtmD& a = P.a;
-    int x = i % P.tw, y = i / P.tw;
+    int x = ic % P.tw, y = ic / P.tw;
     double u = (x + 0.5) / P.tw, v = (y + 0.5) / P.th;
     double H = sqrt(fmax(a.top * a.top - a.R * a.R, 0.0));
     double rho = v * H;
@@ -121,38 +167,52 @@ __device__ __forceinline__ void transmittance_body(const LutParams& P, float4* _
     double mu = 1.0;
     if (d != 0.0) mu = clampd((H * H - rho * rho - d * d) / fmax(2.0 * r * d, 1e-6), -1.0, 1.0);
     D3 p = d3(0, r, 0), dir = d3(sqrt(fmax(1.0 - mu * mu, 0.0)), mu, 0);
-    const int N = 40;
+    constexpr int N = 40, NS = N / kTSeg;
+    static_assert(N % kTSeg == 0, "whole steps per segment lane");
     double t_max = dist_boundary(a, p, dir);
     D3 od = d3(0, 0, 0);
-    if (t_max > 0.0) {
+    if (valid && t_max > 0.0) {
         double dt = t_max / N;
         double r0 = len(p);
         double cv = dot(scl(p, 1.0 / fmax(r0, 1.0)), dir);
         double h0 = r0 - a.R;
+        const int k0 = seg * NS;
+        double h = h0;
+        if (k0 > 0) {
+            const double t = k0 * dt;
+            const D3 pi = add(p, scl(dir, t));
+            h = h0 + (2.0 * t * r0 * cv + t * t) / (fmax(r0, 1.0) + len(pi));
+        }
         double dr, dm, dz;
-        density(a, h0, dr, dm, dz);
+        density(a, h, dr, dm, dz);
         D3 prev = extinction(a, dr, dm, dz);
-        for (int k = 1; k <= N; ++k) {
+        for (int k = k0 + 1; k <= k0 + NS; ++k) {
             double t = k * dt;
             D3 pi = add(p, scl(dir, t));
             double num = 2.0 * t * r0 * cv + t * t;
-            double h = h0 + num / (fmax(r0, 1.0) + len(pi));
-            density(a, h, dr, dm, dz);
+            double hk = h0 + num / (fmax(r0, 1.0) + len(pi));
+            density(a, hk, dr, dm, dz);
             D3 cur = extinction(a, dr, dm, dz);
             od = add(od, scl(add(prev, cur), 0.5 * dt));
             prev = cur;
         }
     }
-    store4(out, i, vexp_neg(od), 1.0);
+    od = seg_sum3<kTSeg>(od);
+    if (valid && seg == 0) store4(out, i, vexp_neg(od), 1.0);
 }
 
-// ---- multiscatter.comp.wgsl:58-156: one wave64 per texel, one lane per
-// Fibonacci direction, directions summed by a wave reduction. ---------------------
+// ---- multiscatter.comp.wgsl:58-156: one 256-thread block per texel; wave w
+// marches steps [5w, 5w + 5) of the 64 Fibonacci directions (one per lane);
+// the four segments are combined through LDS, the directions by a wave
+// reduction. ------------------------------------------------------------------------
+constexpr int kMsSeg = 4;
 __device__ __forceinline__ void multiscatter_body(const LutParams& P, const float4* __restrict__ T,
                                                   float4* __restrict__ out) {
+    __shared__ double part[kMsSeg][9][64];
     int texel = blockIdx.x;
     if (texel >= P.mw * P.mh) return;
     const AtmD& a = P.a;
+    const int di = threadIdx.x & 63, seg = threadIdx.x >> 6;  // blockDim.x == 64 * kMsSeg
     if (P.ms_disabled) {
         if (threadIdx.x == 0) out[texel] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
@@ -164,20 +224,19 @@ __device__ __forceinline__ void multiscatter_body(const LutParams& P, const floa
     double r = a.R + v * (a.top - a.R);
     D3 p0 = d3(0, r, 0), sun = d3(sun_sin, sun_cos, 0);
     double h0 = r - a.R;
-    const int N_DIRS = 64, N_STEPS = 20;
-    int di = threadIdx.x;  // blockDim.x == 64
+    constexpr int N_DIRS = 64, N_STEPS = 20, NS = N_STEPS / kMsSeg;
+    static_assert(N_STEPS % kMsSeg == 0, "whole steps per segment");
     double z = 1.0 - 2.0 * (di + 0.5) / N_DIRS;
     double rr = sqrt(fmax(1.0 - z * z, 0.0));
     double fr = di * 0.6180339887498949;
     double phi = 2.0 * PS_PI * (fr - floor(fr));
     D3 vd = d3(rr * cos(phi), rr * sin(phi), z);
-    D3 l_dir = d3(0, 0, 0), f_dir = d3(0, 0, 0);
+    D3 l_dir = d3(0, 0, 0), f_dir = d3(0, 0, 0), tr = d3(1, 1, 1);
     double t_max = dist_boundary(a, p0, vd);
     if (t_max > 0.0) {
         double dt = t_max / N_STEPS;
         double cv = dot(scl(p0, 1.0 / fmax(r, 1.0)), vd);
-        D3 tr = d3(1, 1, 1);
-        for (int s = 0; s < N_STEPS; ++s) {
+        for (int s = seg * NS; s < seg * NS + NS; ++s) {
             double t = (s + 0.5) * dt;
             D3 p = add(p0, scl(vd, t));
             double num = 2.0 * t * r * cv + t * t;
@@ -202,8 +261,20 @@ __device__ __forceinline__ void multiscatter_body(const LutParams& P, const floa
             tr = mul(tr, trs);
         }
     }
-    // wave64 reduction of the six sums
-    double v6[6] = {l_dir.x, l_dir.y, l_dir.z, f_dir.x, f_dir.y, f_dir.z};
+    const double vals[9] = {l_dir.x, l_dir.y, l_dir.z, f_dir.x, f_dir.y, f_dir.z, tr.x, tr.y, tr.z};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) part[seg][k][di] = vals[k];
+    __syncthreads();
+    if (seg != 0) return;
+    // segments in march order: sums weighted by the transmittance in front
+    D3 L = d3(0, 0, 0), F = d3(0, 0, 0), Tp = d3(1, 1, 1);
+    for (int k = 0; k < kMsSeg; ++k) {
+        L = add(L, mul(Tp, d3(part[k][0][di], part[k][1][di], part[k][2][di])));
+        F = add(F, mul(Tp, d3(part[k][3][di], part[k][4][di], part[k][5][di])));
+        Tp = mul(Tp, d3(part[k][6][di], part[k][7][di], part[k][8][di]));
+    }
+    // wave64 reduction of the six sums over the directions
+    double v6[6] = {L.x, L.y, L.z, F.x, F.y, F.z};
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         double s = v6[k];
@@ -214,31 +285,37 @@ __device__ __forceinline__ void multiscatter_body(const LutParams& P, const floa
     if (threadIdx.x == 0) {
         double norm = (4.0 * PS_PI) / N_DIRS * (1.0 / (4.0 * PS_PI));
         D3 L2 = d3(v6[0] * norm, v6[1] * norm, v6[2] * norm);
-        D3 F = d3(clampd(v6[3] * norm, 0.0, 0.95), clampd(v6[4] * norm, 0.0, 0.95), clampd(v6[5] * norm, 0.0, 0.95));
-        store4(out, texel, d3(L2.x / (1.0 - F.x), L2.y / (1.0 - F.y), L2.z / (1.0 - F.z)), 1.0);
+        D3 Fc = d3(clampd(v6[3] * norm, 0.0, 0.95), clampd(v6[4] * norm, 0.0, 0.95), clampd(v6[5] * norm, 0.0, 0.95));
+        store4(out, texel, d3(L2.x / (1.0 - Fc.x), L2.y / (1.0 - Fc.y), L2.z / (1.0 - Fc.z)), 1.0);
     }
 }
 
-// One in-scatter march shared by the sky-view and aerial-perspective bakes
-// (skyview.comp.wgsl:112-147, aerialperspective.comp.wgsl:259-293).
+// One in-scatter march segment shared by the sky-view and aerial-perspective
+// bakes (skyview.comp.wgsl:112-147, aerialperspective.comp.wgsl:259-293):
+// steps [s0, s1) from transmittance 1. AP: stops at the first step below the
+// ground (`broke`), as the reference's loop does.
 template <bool AP>
-__device__ __forceinline__ void inscatter_march(const LutParams& P, const float4* __restrict__ T,
-                                                const float4* __restrict__ MS, D3 p, double r, D3 vd,
-                                                double dt, int n_steps, double phase_r, double phase_m,
-                                                D3& lum, D3& tr) {
+__device__ __forceinline__ void inscatter_segment(const LutParams& P, const float4* __restrict__ T,
+                                                  const float4* __restrict__ MS, D3 p, double r, D3 vd,
+                                                  double dt, int s0, int s1, double phase_r, double phase_m,
+                                                  D3& lum, D3& tr, bool& broke) {
     const AtmD& a = P.a;
     D3 sun = ld3(P.f.sun);
     double cv = dot(scl(p, 1.0 / fmax(r, 1.0)), vd);
     double h0 = r - a.R;
     lum = d3(0, 0, 0);
     tr = d3(1, 1, 1);
-    for (int s = 0; s < n_steps; ++s) {
+    broke = false;
+    for (int s = s0; s < s1; ++s) {
         double t = (s + 0.5) * dt;
         D3 pi = add(p, scl(vd, t));
         double num = 2.0 * t * r * cv + t * t;
         double r_delta = num / (fmax(r, 1.0) + len(pi));
         double h = h0 + r_delta;
-        if (AP && h < 0.0) break;
+        if (AP && h < 0.0) {
+            broke = true;
+            break;
+        }
         double dr, dm, dz;
         density(a, h, dr, dm, dz);
         D3 st = extinction(a, dr, dm, dz);
@@ -260,13 +337,16 @@ __device__ __forceinline__ void inscatter_march(const LutParams& P, const float4
     }
 }
 
-// ---- skyview.comp.wgsl:46-174 -----------------------------------------------------
+// ---- skyview.comp.wgsl:46-174: 32 steps on kSvSeg lanes per texel ----------------
+constexpr int kSvSeg = 8;
 __device__ __forceinline__ void skyview_body(const LutParams& P, const float4* __restrict__ T,
                                              const float4* __restrict__ MS, float4* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.sw * P.sh) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / kSvSeg, seg = gid % kSvSeg;
+    const bool valid = i < P.sw * P.sh;
+    const int ic = valid ? i : 0;
     const AtmD& a = P.a;
-    int x = i % P.sw, y = i / P.sw;
+    int x = ic % P.sw, y = ic / P.sw;
     double u = (x + 0.5) / P.sw, v = (y + 0.5) / P.sh;
     D3 p = add(ld3(P.f.cam), d3(0, a.R, 0));
     double r = len(p);
@@ -291,12 +371,19 @@ __device__ __forceinline__ void skyview_body(const LutParams& P, const float4* _
     double ct = dot(vd, sun);
     double pr = phase_rayleigh(ct), pm = phase_mie(ct, a.mie_g);
     double t_max = dist_boundary(a, p, vd);
+    constexpr int N = 32, NS = N / kSvSeg;
+    static_assert(N % kSvSeg == 0, "whole steps per segment lane");
+    D3 lum_k = d3(0, 0, 0), tr_k = d3(1, 1, 1);
+    bool broke;
+    if (t_max > 0.0)
+        inscatter_segment<false>(P, T, MS, p, r, vd, t_max / N, seg * NS, seg * NS + NS, pr, pm, lum_k, tr_k, broke);
+    D3 lum, tr;
+    seg_combine<kSvSeg>(lum_k, tr_k, seg, lum, tr);
+    if (!valid || seg != 0) return;
     if (t_max <= 0.0) {
         out[i] = make_float4(0.f, 0.f, 0.f, 1.f);
         return;
     }
-    D3 lum, tr;
-    inscatter_march<false>(P, T, MS, p, r, vd, t_max / 32.0, 32, pr, pm, lum, tr);
     double tp0, tp1;
     if (sphere_hit(p, vd, a.R, tp0, tp1) && tp0 > 0.0) {
         D3 pg = add(p, scl(vd, tp0));
@@ -309,13 +396,16 @@ __device__ __forceinline__ void skyview_body(const LutParams& P, const float4* _
     store4(out, i, lum, 1.0);
 }
 
-// ---- aerialperspective.comp.wgsl:41-127 ----------------------------------------------
+// ---- aerialperspective.comp.wgsl:41-127: 16 steps on kApSeg lanes per froxel ------
+constexpr int kApSeg = 4;
 __device__ __forceinline__ void aerial_body(const LutParams& P, const float4* __restrict__ T,
                                             const float4* __restrict__ MS, float4* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.aw * P.ah * P.ad) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / kApSeg, seg = gid % kApSeg;
+    const bool valid = i < P.aw * P.ah * P.ad;
+    const int ic = valid ? i : 0;
     const AtmD& a = P.a;
-    int x = i % P.aw, y = (i / P.aw) % P.ah, z = i / (P.aw * P.ah);
+    int x = ic % P.aw, y = (ic / P.aw) % P.ah, z = ic / (P.aw * P.ah);
     double ndx = (x + 0.5) / P.aw * 2.0 - 1.0, ndy = (y + 0.5) / P.ah * 2.0 - 1.0;
     const double* m = P.f.inv_vp;
     double hv[4] = {ndx, -ndy, 1.0, 1.0}, nh[4];
@@ -331,8 +421,23 @@ __device__ __forceinline__ void aerial_body(const LutParams& P, const float4* __
     D3 sun = ld3(P.f.sun);
     double ct = dot(vd, sun);
     double pr = phase_rayleigh(ct), pm = phase_mie(ct, a.mie_g);
+    constexpr int N = 16, NS = N / kApSeg;
+    static_assert(N % kApSeg == 0, "whole steps per segment lane");
+    D3 lum_k, tr_k;
+    bool broke;
+    inscatter_segment<true>(P, T, MS, p0, r0, vd, t_target / N, seg * NS, seg * NS + NS, pr, pm, lum_k, tr_k, broke);
+    // the march ends at the first segment that reached the ground: later
+    // segments contribute nothing
+    int first = broke ? seg : kApSeg;
+#pragma unroll
+    for (int o = 1; o < kApSeg; o <<= 1) first = min(first, __shfl_xor(first, o, kApSeg));
+    if (seg > first) {
+        lum_k = d3(0, 0, 0);
+        tr_k = d3(1, 1, 1);
+    }
     D3 lum, tr;
-    inscatter_march<true>(P, T, MS, p0, r0, vd, t_target / 16.0, 16, pr, pm, lum, tr);
+    seg_combine<kApSeg>(lum_k, tr_k, seg, lum, tr);
+    if (!valid || seg != 0) return;
     double t_lum = tr.x * 0.2126 + tr.y * 0.7152 + tr.z * 0.0722;
     store4(out, i, lum, t_lum);
 }
@@ -344,7 +449,7 @@ __global__ __launch_bounds__(64) void k_transmittance(const LutParams* __restric
     transmittance_body(P, T + (size_t)blockIdx.y * P.tw * P.th);
 }
 
-__global__ __launch_bounds__(64) void k_multiscatter(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
+__global__ __launch_bounds__(256) void k_multiscatter(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
                                                      float4* __restrict__ MS) {
     const LutParams& P = Ps[blockIdx.y];
     multiscatter_body(P, T + (size_t)blockIdx.y * P.tw * P.th, MS + (size_t)blockIdx.y * P.mw * P.mh);
@@ -372,15 +477,17 @@ hipError_t ps_launch_luts(const LutParams* d_params, const LutParams& P, int n, 
     const unsigned ns = (unsigned)n;
     if (ev) hipEventRecord(ev[0], s);
     if (flags & 1u)
-        hipLaunchKernelGGL(k_transmittance, dim3((P.tw * P.th + 63) / 64, ns), dim3(64), 0, s, d_params, T);
+        hipLaunchKernelGGL(k_transmittance, dim3((P.tw * P.th * kTSeg + 63) / 64, ns), dim3(64), 0, s, d_params, T);
     if (ev) hipEventRecord(ev[1], s);
-    if (flags & 2u) hipLaunchKernelGGL(k_multiscatter, dim3(P.mw * P.mh, ns), dim3(64), 0, s, d_params, T, MS);
+    if (flags & 2u)
+        hipLaunchKernelGGL(k_multiscatter, dim3(P.mw * P.mh, ns), dim3(64 * kMsSeg), 0, s, d_params, T, MS);
     if (ev) hipEventRecord(ev[2], s);
     if (flags & 4u)
-        hipLaunchKernelGGL(k_skyview, dim3((P.sw * P.sh + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS, SV);
+        hipLaunchKernelGGL(k_skyview, dim3((P.sw * P.sh * kSvSeg + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS, SV);
     if (ev) hipEventRecord(ev[3], s);
     if (flags & 8u)
-        hipLaunchKernelGGL(k_aerial, dim3((P.aw * P.ah * P.ad + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS, AP);
+        hipLaunchKernelGGL(k_aerial, dim3((P.aw * P.ah * P.ad * kApSeg + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS,
+                           AP);
     if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }
