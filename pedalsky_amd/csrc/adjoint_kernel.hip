@@ -29,8 +29,9 @@
 //    samples cost no precision.
 //  * Parameter gradients of the layer being marched are lane registers,
 //    added to per-wave LDS sums after every chunk and to the global
-//    accumulators once per wave. Perturbation-volume gradients are scattered
-//    with global float atomics (8 trilinear corners per sample).
+//    accumulators once per wave. Perturbation-volume gradients (8 trilinear
+//    corners per sample) are accumulated per texel column in LDS and added
+//    to an FP64 accumulator (delta_scatter_columns).
 #include "ps_device.h"
 #include "ps_launch.h"
 
@@ -62,106 +63,76 @@ __device__ __forceinline__ float psi_fn(float x) {
     return (1.f - (1.f + x) * e) / (x * x);
 }
 
-// Sum over the wave (all 64 lanes active).
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Cross-lane moves within rows of 16 lanes (DPP, all lanes active).
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-constexpr int kDppQuadXor1 = 0xB1;     // quad_perm [1, 0, 3, 2]
-constexpr int kDppQuadXor2 = 0x4E;     // quad_perm [2, 3, 0, 1]
-constexpr int kDppHalfMirror = 0x141;  // lane i <- lane 7 - i within 8
-constexpr int kDppRowRor8 = 0x128;     // lane i <- lane i ^ 8 within 16
-
-// a + b where (a, b) are exchanged so that, for lanes in the lower half
-// (of the wave for W = 32, of each row pair for W = 16), the result is
-// a_l + a_partner and for the upper half b_l + b_partner (the partner is the
-// lane W away). One v_permlane*_swap per pair.
-template <int W>
-__device__ __forceinline__ float swap_add(float a, float b) {
-    if constexpr (W == 32) {
-        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
-        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-    } else {
-        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
-        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-    }
-}
-
-// Transposed sum of eight values over the wave: lane l returns the sum over
-// all 64 lanes of w[l >> 3]. Halving exchanges (lanes 32 apart keep corners
-// 0-3 / 4-7, then 16 apart, then 8 apart) followed by a sum over the eight
-// lanes that share bits 5..3: 10 cross-lane moves for the eight sums where
-// eight independent butterfly reductions take 48.
-__device__ __forceinline__ float wave_sum8_transposed(const float (&w)[8]) {
-    const int lane = threadIdx.x;
-    float a[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) a[k] = swap_add<32>(w[k], w[4 + k]);  // corner 4*b5 + k
-    float b[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) b[k] = swap_add<16>(a[k], a[2 + k]);  // corner 4*b5 + 2*b4 + k
-    const bool h3 = (lane & 8) != 0;
-    float c = (h3 ? b[1] : b[0]) + dpp<kDppRowRor8>(h3 ? b[0] : b[1]);  // corner l >> 3
-    c += dpp<kDppHalfMirror>(c);
-    c += dpp<kDppQuadXor2>(c);
-    c += dpp<kDppQuadXor1>(c);
-    return c;
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Scatter v times the eight trilinear weights of each lane's tap (lanes with
-// `has`) into the FP64 gradient accumulator. Neighbouring samples of a wave
-// mostly fall in the same voxel cell, and same-address atomics serialise, so
-// lanes whose taps share all eight corners are first summed across the wave
-// (up to kPeel groups, one atomic instruction of eight lanes per group);
-// lanes left after that add their own. Called with all 64 lanes active.
-constexpr int kPeel = 8;
+// `has`) into the FP64 gradient accumulator. Same-address atomics within one
+// instruction serialise and a wave's taps share texels heavily, so the taps
+// are first accumulated by texel column. A wave's taps crowd into few (x, z)
+// texel columns but spread over several height rows (a row is 1/128 of the
+// layer, a few metres), so a round takes the first remaining lane's column
+// and every lane of it whose two rows fall in a kColRows window centred on
+// the leader's; their weights go into a per-wave LDS block of 4 (x, z)
+// corners x kColRows rows with LDS float atomics, and the block is added to
+// the accumulator (one FP64 atomic per non-zero texel) and cleared. Lanes
+// left after kColRounds rounds add their own eight weights. `colacc` is the
+// wave's block, zero on entry and on exit. Called with all 64 lanes active.
+constexpr int kColRows = 9;
+constexpr int kColSlots = 4 * kColRows;
+constexpr int kColRounds = 4;
 
-__device__ __forceinline__ void delta_scatter_wave(double* __restrict__ g, bool has, const DeltaTap& d, float v) {
+__device__ __forceinline__ void delta_scatter_columns(double* __restrict__ g, float* colacc, bool has,
+                                                      const DeltaTap& d, float v) {
     const int lane = threadIdx.x;
-    // the corner set is determined by the lower corner and whether the
-    // height axis is clamped (y1 == y0)
-    const uint32_t key = has ? ((d.z0 + d.y0 + d.x0) | (d.y1 == d.y0 ? 0x80000000u : 0u)) : 0xffffffffu;
+    const uint32_t y0r = d.y0 >> 7, y1r = d.y1 >> 7;
+    const uint32_t col = d.z0 | d.x0;
     const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
-    const float w[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
-                        v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
-    // this lane's corner when it adds a group's sum (lane >> 3 = 4 z + 2 y + x)
-    const int cz = (lane >> 5) & 1, cy = (lane >> 4) & 1, cx = (lane >> 3) & 1;
     unsigned long long act = __ballot(has);
-    for (int it = 0; it < kPeel && act != 0ull; ++it) {
+    for (int it = 0; it < kColRounds && act != 0ull; ++it) {
         const int leader = __builtin_ctzll(act);
-        const uint32_t k = __builtin_amdgcn_readlane(key, leader);
-        const bool m = key == k;
-        const unsigned long long mm = __ballot(m);
-        if (__popcll(mm) > 1) {
-            float wm[8];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) wm[c] = m ? w[c] : 0.f;
-            const float sum = wave_sum8_transposed(wm);
-            // the group's corner offsets (the leader's), selected per lane
-            const uint32_t z0 = __builtin_amdgcn_readlane(d.z0, leader), z1 = __builtin_amdgcn_readlane(d.z1, leader);
-            const uint32_t y0 = __builtin_amdgcn_readlane(d.y0, leader), y1 = __builtin_amdgcn_readlane(d.y1, leader);
-            const uint32_t x0 = __builtin_amdgcn_readlane(d.x0, leader), x1 = __builtin_amdgcn_readlane(d.x1, leader);
-            const uint32_t o = (cz ? z1 : z0) + (cy ? y1 : y0) + (cx ? x1 : x0);
-            if ((lane & 7) == 0) atomicAdd(g + o, (double)sum);
-        } else if (lane == leader) {
-            const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
-                                     d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
-#pragma unroll
-            for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
+        const uint32_t kcol = __builtin_amdgcn_readlane(col, leader);
+        const uint32_t ly = __builtin_amdgcn_readlane(y0r, leader);
+        const uint32_t base = ly >= (kColRows - 1) / 2 ? ly - (kColRows - 1) / 2 : 0u;
+        // the leader always qualifies: its rows are base + 4 and base + 5 at most
+        const bool m = has && col == kcol && y0r >= base && y1r - base < (uint32_t)kColRows;
+        if (m) {
+            // slot = (2 * z corner + x corner) * kColRows + row - base
+            const uint32_t r0 = y0r - base, r1 = y1r - base;
+            atomicAdd(colacc + 0 * kColRows + r0, v * wz0 * wy0 * wx0);
+            atomicAdd(colacc + 1 * kColRows + r0, v * wz0 * wy0 * wx1);
+            atomicAdd(colacc + 0 * kColRows + r1, v * wz0 * wy1 * wx0);
+            atomicAdd(colacc + 1 * kColRows + r1, v * wz0 * wy1 * wx1);
+            atomicAdd(colacc + 2 * kColRows + r0, v * wz1 * wy0 * wx0);
+            atomicAdd(colacc + 3 * kColRows + r0, v * wz1 * wy0 * wx1);
+            atomicAdd(colacc + 2 * kColRows + r1, v * wz1 * wy1 * wx0);
+            atomicAdd(colacc + 3 * kColRows + r1, v * wz1 * wy1 * wx1);
         }
+        wave_sync_lds();
+        // the column's corner offsets (the leader's)
+        const uint32_t x0 = __builtin_amdgcn_readlane(d.x0, leader), x1 = __builtin_amdgcn_readlane(d.x1, leader);
+        const uint32_t z0 = __builtin_amdgcn_readlane(d.z0, leader), z1 = __builtin_amdgcn_readlane(d.z1, leader);
+        static_assert(kColSlots <= 64, "one flush pass");
+        if (lane < kColSlots) {
+            const float sum = colacc[lane];
+            if (sum != 0.f) {
+                const int xz = lane / kColRows, r = lane - xz * kColRows;
+                const uint32_t o = ((xz & 2) ? z1 : z0) + ((base + (uint32_t)r) << 7) + ((xz & 1) ? x1 : x0);
+                atomicAdd(g + o, (double)sum);
+                colacc[lane] = 0.f;
+            }
+        }
+        wave_sync_lds();
         has = has && !m;
-        act &= ~mm;
+        act = __ballot(has);
     }
     if (has) {
         const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
                                  d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
+        const float w[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
+                            v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
 #pragma unroll
         for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
     }
@@ -267,8 +238,9 @@ __device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& 
 // Scatter g (= dL/d od * density_scale) over each lane's light samples in
 // `lin` (lanes with has). Called with all 64 lanes active; light sample i of
 // every lane is scattered together.
-__device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, const LayerF& L, bool has, float px,
-                                                         float py, float pz, float alt, uint32_t lin, float g) {
+__device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, float* colacc, const LayerF& L,
+                                                         bool has, float px, float py, float pz, float alt,
+                                                         uint32_t lin, float g) {
     const LightWalk w = light_walk(P, L, px, py, pz, alt);
     const int n = P.cone ? 6 : P.light_steps;
     unsigned long long any = __ballot(has && lin != 0u);
@@ -286,7 +258,7 @@ __device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, 
             delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
             d = delta_tap(dcx, dcy, dcz);
         }
-        delta_scatter_wave(P.g_delta64, hi, d, g * wt);
+        delta_scatter_columns(P.g_delta64, colacc, hi, d, g * wt);
     }
 }
 
@@ -307,12 +279,8 @@ struct BackQueue {
     float f[BQ_FIELDS][kBQCap];
     float seed[SD_FIELDS][64];        // per owner lane: luminance cotangent, depth-weight cotangents, 1 - cos
     float wacc[3 * PS_MAX_LAYERS];    // per-layer gradient sums of the wave
+    float colacc[kColSlots];          // perturbation-gradient column block (zero between uses)
 };
-
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Shade queue entry j with its adjoint (cloud_march.wgsl:886-993): writes
 // the owner's record (v, dL/drho through the shading) and adds the
@@ -454,7 +422,9 @@ __device__ __forceinline__ void flush_adj(const RenderParams& P, BackQueue& Q, i
     wave_sync_lds();
     LightScatter ls{};
     if (lane < n) ls = shade_adj<DELTA>(P, Q, lane, rec_tile);
-    if (want_delta) light_delta_scatter_wave(P, P.layers[ls.li].f, lane < n, ls.px, ls.py, ls.pz, ls.alt, ls.lin, ls.g);
+    if (want_delta)
+        light_delta_scatter_wave(P, Q.colacc, P.layers[ls.li].f, lane < n, ls.px, ls.py, ls.pz, ls.alt, ls.lin,
+                                 ls.g);
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
@@ -481,6 +451,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     const int xc = min(x, P.W - 1), yc = min(y, P.H - 1);
     const size_t tile = (size_t)tile_y * P.n_tiles_x + tile_x;
     if (lane < 3 * PS_MAX_LAYERS) Q.wacc[lane] = 0.f;
+    if (lane < kColSlots) Q.colacc[lane] = 0.f;
 
     const PixelRay ray = pixel_ray(P, xc, yc);
     const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
@@ -764,7 +735,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                         scat = true;
                     }
                 }
-                if (want_delta) delta_scatter_wave(P.g_delta64, scat, dtap, g_rho * L.density_scale);
+                if (want_delta) delta_scatter_columns(P.g_delta64, Q.colacc, scat, dtap, g_rho * L.density_scale);
             }
             if (act) {
                 atomicAdd(&Q.wacc[li], gds);
