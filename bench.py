@@ -124,8 +124,7 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
     def fwd_adj():
         abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
         abi.check(lib.ps_render_train(r.ctx, depth, C.byref(out)))
-        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)
-        g[..., 3] = 0.0
+        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)  # the backward reads RGB only
         abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads)))
 
     grads_layers = abi.CloudGrads(lg.data_ptr(), None)
@@ -133,8 +132,7 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
     def fwd_adj_layers():
         abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
         abi.check(lib.ps_render_train(r.ctx, depth, C.byref(out)))
-        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)
-        g[..., 3] = 0.0
+        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)  # the backward reads RGB only
         abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads_layers)))
 
     def fwd():

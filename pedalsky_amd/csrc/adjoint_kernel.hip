@@ -819,9 +819,13 @@ __global__ void k_zero_f64(double* __restrict__ p, int n) {
     if (i < n) p[i] = 0.0;
 }
 
-__global__ void k_add_f64_to_f32(float* __restrict__ dst, const double* __restrict__ src, int n) {
+// dst += src, and src back to zero for the next backward
+__global__ void k_flush_f64_to_f32(float* __restrict__ dst, double* __restrict__ src, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] += (float)src[i];
+    if (i < n) {
+        dst[i] += (float)src[i];
+        src[i] = 0.0;
+    }
 }
 
 }  // namespace
@@ -829,11 +833,15 @@ __global__ void k_add_f64_to_f32(float* __restrict__ dst, const double* __restri
 hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, float* g_delta,
                                      double* g_delta64, hipStream_t s) {
     const int nv = 128 * 128 * 128;
-    if (g_delta) hipLaunchKernelGGL(k_zero_f64, dim3(nv / 256), dim3(256), 0, s, g_delta64, nv);
     dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
     if (delta) hipLaunchKernelGGL(k_render_backward<true>, grid, block, 0, s, d_params);
     else hipLaunchKernelGGL(k_render_backward<false>, grid, block, 0, s, d_params);
-    if (g_delta) hipLaunchKernelGGL(k_add_f64_to_f32, dim3(nv / 256), dim3(256), 0, s, g_delta, g_delta64, nv);
+    if (g_delta) hipLaunchKernelGGL(k_flush_f64_to_f32, dim3(nv / 256), dim3(256), 0, s, g_delta, g_delta64, nv);
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_zero_f64(double* p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_f64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -795,8 +795,10 @@ ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_lumina
     // to the context's counter scratch
     P.g_layers = grads->layers ? grads->layers : (float*)c->d_counters;
     P.g_delta = grads->density_delta;
-    if (P.g_delta) {
+    if (P.g_delta && !c->g_delta64) {
+        // the backward leaves the accumulator at zero: cleared once here
         if ((s = dev_alloc(&c->g_delta64, sizeof(double) * 128 * 128 * 128, "FP64 volume gradient"))) return s;
+        PS_HIP(ps_launch_zero_f64(c->g_delta64, 128 * 128 * 128, c->stream));
     }
     P.g_delta64 = c->g_delta64;
     P.rec = c->rec;

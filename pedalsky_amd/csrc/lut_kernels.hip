@@ -449,7 +449,7 @@ __global__ __launch_bounds__(64) void k_transmittance(const LutParams* __restric
     transmittance_body(P, T + (size_t)blockIdx.y * P.tw * P.th);
 }
 
-__global__ __launch_bounds__(256) void k_multiscatter(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
+__global__ __launch_bounds__(64 * kMsSeg) void k_multiscatter(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
                                                      float4* __restrict__ MS) {
     const LutParams& P = Ps[blockIdx.y];
     multiscatter_body(P, T + (size_t)blockIdx.y * P.tw * P.th, MS + (size_t)blockIdx.y * P.mw * P.mh);

@@ -21,9 +21,12 @@ hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool cou
 // Backward of a training render (the same parameter block, with cotangents
 // and gradient outputs set).
 // With g_delta non-null the perturbation-volume gradient is accumulated in
-// FP64 (g_delta64, zeroed here) and then added into g_delta.
+// FP64 (g_delta64: zero on entry, zero again on return) and then added into
+// g_delta.
 hipError_t ps_launch_render_backward(const RenderParams* d_params, int W, int H, bool delta, float* g_delta,
                                      double* g_delta64, hipStream_t s);
+// p[0, n) = 0 (the FP64 gradient accumulator, once after allocation).
+hipError_t ps_launch_zero_f64(double* p, int n, hipStream_t s);
 // Per-brick maxima of the 128^3 density-perturbation volume (32^3 bricks of
 // 4^3 texels plus a +1 halo: Repeat in x and z, ClampToEdge in height).
 hipError_t ps_launch_delta_bounds(const float* delta, float2* bnd, hipStream_t s);
