@@ -483,9 +483,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 // scene << 20 | tile_y << 10 | tile_x. Persistent: every one-wave block
 // pulls the next entry from the counter until the list is exhausted (the
 // exit test is wave-uniform and the counter only grows, so every wave
-// leaves). Otherwise block b renders entry b.
+// leaves). Otherwise block b renders entry b. Four waves per SIMD, as
+// k_render (8 % faster on the 64-scene batch than three).
 template <int TW, bool PERSISTENT>
-__global__ __launch_bounds__(64) void k_render_tiles(const RenderParams* __restrict__ Ps,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
                                                      const uint32_t* __restrict__ tiles, int n_tiles,
                                                      unsigned* __restrict__ next) {
     __shared__ WaveQueue Qs;
