@@ -46,6 +46,7 @@ class CloudRender(torch.autograd.Function):
         ctx.renderer = renderer
         ctx.n = n
         ctx.devices = (density_scale.device, extinction_scale.device, coverage.device)
+        ctx.dtypes = (density_scale.dtype, extinction_scale.dtype, coverage.dtype)
         ctx.has_delta = density_delta is not None
         ctx.delta_shape = None if density_delta is None else density_delta.shape
         renderer.synchronize()
@@ -66,8 +67,10 @@ class CloudRender(torch.autograd.Function):
         r.backward(gc, gl, gt, layer_grads, delta_grad)
         r.synchronize()
         n = ctx.n
-        dd, dk, dc = ctx.devices
-        g_ds, g_k, g_cov = layer_grads[:n].to(dd), layer_grads[8:8 + n].to(dk), layer_grads[16:16 + n].to(dc)
+        (dd, dk, dc), (td, tk, tc) = ctx.devices, ctx.dtypes
+        g_ds = layer_grads[:n].to(device=dd, dtype=td)
+        g_k = layer_grads[8:8 + n].to(device=dk, dtype=tk)
+        g_cov = layer_grads[16:16 + n].to(device=dc, dtype=tc)
         gd = None if delta_grad is None else delta_grad.view(ctx.delta_shape)
         return None, None, None, None, g_ds, g_k, g_cov, gd
 

@@ -462,6 +462,43 @@ def test_autograd_function_matches_abi(renderer):
     renderer.set_cloud_extensions(None, None)
 
 
+def test_autograd_gradcheck(renderer):
+    """torch.autograd.gradcheck (fast mode: a random projection of the
+    Jacobian, numerical by central differences of the FP32 render, eps 1e-3)
+    of density_scale and the extinction scale through CloudRender, for a
+    scalar of the composite and the transmittance. The render is FP32 and
+    piecewise smooth, so the tolerance is 1e-2 relative. Coverage is not
+    gradchecked: the reference's density jumps with it (a shape crossing
+    zero switches the erosion branch, and with the perturbation volume added
+    the clamped result jumps by up to the perturbation), so a 1e-3 central
+    difference of the FP32 render is dominated by those jumps (it reads 10 %
+    below the derivative); the coverage derivative is checked against the
+    oracle's exact FP64 derivatives instead."""
+    import torch
+    from pedalsky_amd import autograd as ag
+    w = scene.workload_adjoint(width=320, height=180)
+    _log("case: torch.autograd.gradcheck, per-layer parameters, 320x180")
+    _load(renderer, w)
+    d0 = _delta_tensor(np.random.default_rng(9).normal(0.0, 0.02, (128, 128, 128)))
+    rng = np.random.default_rng(11)
+    wc = torch.from_numpy(rng.uniform(0.5, 1.5, (180, 320, 3))).to("cuda:0")
+    wt = torch.from_numpy(1e4 * rng.uniform(0.5, 1.5, (180, 320, 3))).to("cuda:0")
+
+    cov = torch.tensor([L.coverage for L in w.layers], dtype=torch.float64, device="cuda:0")
+
+    def f(ds, kap):
+        comp, _, trn = ag.render(renderer, w.clouds, w.layers, w.winds, ds, kap, cov, d0)
+        return (comp[..., :3].double() * wc).sum() + (trn[..., :3].double() * wt).sum()
+
+    args = tuple(torch.tensor(v, dtype=torch.float64, device="cuda:0", requires_grad=True) for v in
+                 ([L.density_scale for L in w.layers], [1.0, 1.0]))
+    # the per-layer sums are accumulated with float atomics, so a repeated
+    # backward agrees to ~1e-7 relative, not bit for bit
+    scale = max(float(g.abs().max()) for g in torch.autograd.grad(f(*args), args))
+    assert torch.autograd.gradcheck(f, args, eps=1e-3, atol=0.0, rtol=1e-2, fast_mode=True, nondet_tol=1e-5 * scale)
+    renderer.set_cloud_extensions(None, None)
+
+
 def test_backward_error_paths(renderer):
     lib = renderer.lib
     g = abi.CloudGrads(None, None)
