@@ -132,15 +132,17 @@ __device__ __forceinline__ TD trans_tap(const LutParams& P, const float4* T, con
 
 // (1 - trs) / safe-extinction step integral with tangents:
 // s = ins * (1 - e^{-st dt}) / max(st, 1e-7)
+// (one FP64 reciprocal per call: the tangents multiply by it)
 __device__ __forceinline__ TD step_integral(const TD& ins, const TD& st, double dt, const TD& trs) {
     const double safe = fmax(st.v, 1e-7);
+    const double inv = 1.0 / safe;
     const double om = 1.0 - trs.v;
     TD s = td(ins.v * om / safe);
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
         const double dom = -trs.d[j];
         const double dsafe = st.v > 1e-7 ? st.d[j] : 0.0;
-        s.d[j] = (ins.d[j] * om + ins.v * dom) / safe - s.v * dsafe / safe;
+        s.d[j] = ((ins.d[j] * om + ins.v * dom) - s.v * dsafe) * inv;
     }
     return s;
 }
