@@ -226,6 +226,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--adjoint-steps", type=int, default=16)
     ap.add_argument("--no-calibration", action="store_true", help="skip the atmosphere calibration run")
+    ap.add_argument("--no-half-res", action="store_true", help="skip the half-resolution cloud path measurement")
     ap.add_argument("--calibration-sets", type=int, default=256)
     ap.add_argument("--calibration-iters", type=int, default=200)
     ap.add_argument("--batch-scenes", type=int, default=64)
@@ -352,6 +353,23 @@ def main() -> int:
                      "hbm_GBps": round(hbm_bytes / render_s / 1e9, 1)},
         "work_per_frame": {k: v / len(used) for k, v in cnt_sum.items()},
     }
+    if not args.no_half_res:
+        log("half-resolution cloud path")
+        r.set_half_res(True)
+        for k in range(args.warmup):
+            frame(k)
+        r.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            frame(k)
+        r.synchronize()
+        hs = (time.perf_counter() - t0) / args.steps
+        r.set_half_res(False)
+        result["half_res"] = {"ms_per_frame": round(hs * 1e3, 4), "mpix_s": round(W * H / hs / 1e6, 3),
+                              "speedup_vs_full": round(ms_per_step / (hs * 1e3), 3),
+                              "rt": f"{max(1, W // 2)}x{max(1, H // 2)}",
+                              "workload": "the same frames: cloud march into the RT, Catmull-Rom composite "
+                                          "with the sky pass at full resolution"}
     if not args.no_adjoint:
         log("forward + adjoint")
         result["fwd_adjoint"] = bench_adjoint(r, W, H, args.adjoint_steps, 3)

@@ -300,6 +300,16 @@ ps_status ps_timing_begin(ps_ctx* ctx, int32_t max_frames);
 ps_status ps_timing_end(ps_ctx* ctx, ps_frame_timing* timing);
 /* Re-run the render with event counting on; fills the algorithmic counts. */
 ps_status ps_count_work(ps_ctx* ctx, const float* depth, ps_work_counters* out);
+/* Half-resolution cloud march (`clouds.half_res_render`,
+ * crates/ps-clouds/src/lib.rs:124-127 and 748-760; the march patch of
+ * crates/ps-clouds/src/pipeline.rs:277-319; cloud_composite_halfres.wgsl):
+ * with `enabled` != 0, ps_render and ps_frame march the clouds into a
+ * (floor(W/2) x floor(H/2)) RT and composite at full resolution with a
+ * Catmull-Rom upsample of the RT's luminance and transmittance (the
+ * cloud_luminance / cloud_transmittance outputs are those upsampled
+ * values). Off by default. ps_count_work, ps_render_train and batches run
+ * at full resolution only (PS_ERR_UNSUPPORTED while it is on). */
+ps_status ps_set_half_res(ps_ctx* ctx, int32_t enabled);
 
 /* ------------------------------------------------------------------------ */
 /* Many scenes per launch. No reference counterpart: the reference renders

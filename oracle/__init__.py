@@ -105,6 +105,7 @@ def load():
         "orf_render": (None, [P, i, i, i, i, P, P, P, P, i]),
         "or_render_pixels": (None, [P, P, i, P, P, P, i]),
         "or_render_ext": (None, [P, P, i, i, i, i, P, P, P, P, i]),
+        "or_render_halfres": (None, [P, i, i, i, i, P, P, P, i]),
         "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i, d, P]),
         "or_sky_ap_image": (None, [P, P, i, i, i, i, i, i, i, i, i, d, i, i, i, i, P, i]),
         "or_noise_base": (None, [C.c_uint32, i, i, P, i]),
@@ -266,6 +267,16 @@ class Frame:
         cnt = np.zeros(10, dtype=np.uint64) if counters else None
         load().or_render(C.byref(self.inputs), x0, y0, w, h, _p(comp), _p(lum), _p(trn), _p(cnt), THREADS)
         return comp, lum, trn, cnt
+
+    def render_halfres(self, x0, y0, w, h):
+        """The half-resolution path (cloud RT of floor(W/2) x floor(H/2),
+        Catmull-Rom upsample in the full-resolution composite) over a
+        region: (comp, lum, trn), each (h, w, 4)."""
+        comp = np.zeros((h, w, 4))
+        lum = np.zeros((h, w, 4))
+        trn = np.zeros((h, w, 4))
+        load().or_render_halfres(C.byref(self.inputs), x0, y0, w, h, _p(comp), _p(lum), _p(trn), THREADS)
+        return comp, lum, trn
 
     def render_pixels(self, xy: np.ndarray):
         xy = np.ascontiguousarray(xy, dtype=np.int32)

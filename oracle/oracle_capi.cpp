@@ -60,6 +60,8 @@ Scene<R> make_scene(const or_inputs* in) {
     s.a = atm_from<R>(*in->atm);
     s.c = clouds_from<R>(*in->clouds);
     s.n_layers = std::min<int>(in->n_layers, PS_MAX_CLOUD_LAYERS);
+    s.rt_w = 0;
+    s.rt_h = 0;
     for (int i = 0; i < s.n_layers; ++i) {
         s.layers[i] = layer_from<R>(in->layers[i]);
         if (in->layer_wind_xz) s.layer_wind[i] = V3<R>(in->layer_wind_xz[2 * i], 0, in->layer_wind_xz[2 * i + 1]);
@@ -260,6 +262,45 @@ void orf_render(const or_inputs* in, int x0, int y0, int w, int h, float* comp, 
                 uint64_t* counters, int nthreads) {
     render_region<float>(in, x0, y0, w, h, comp, lum, trn, counters, nthreads);
 }
+// The half-resolution path (crates/ps-clouds/src/lib.rs:748-760 RT size,
+// pipeline.rs:277-319 march mapping, cloud_composite_halfres.wgsl): marches
+// the RT texels the w x h region's Catmull-Rom taps read, then composites
+// the region at full resolution. Outputs w*h*4 (any may be null).
+void or_render_halfres(const or_inputs* in, int x0, int y0, int w, int h, double* comp, double* lum, double* trn,
+                       int nthreads) {
+    Scene<double> s = make_scene<double>(in);
+    const double vw = s.f.vw, vh = s.f.vh;
+    s.rt_w = std::max(1, int(float(vw) * 0.5f));
+    s.rt_h = std::max(1, int(float(vh) * 0.5f));
+    // RT texels read by the region: floor((x + 0.5) / vw * rt_w - 0.5) - 1 .. + 2
+    auto lo = [](double p) { return std::max(int(std::floor(p - 0.5)) - 1, 0); };
+    auto hi = [](double p, int n) { return std::min(int(std::floor(p - 0.5)) + 2, n - 1); };
+    const int tx0 = lo((x0 + 0.5) / vw * s.rt_w), tx1 = hi((x0 + w - 0.5) / vw * s.rt_w, s.rt_w);
+    const int ty0 = lo((y0 + 0.5) / vh * s.rt_h), ty1 = hi((y0 + h - 0.5) / vh * s.rt_h, s.rt_h);
+    std::vector<double> rl(size_t(s.rt_w) * s.rt_h * 4, 0.0), rt(size_t(s.rt_w) * s.rt_h * 4, 0.0);
+    const int tw = tx1 - tx0 + 1, th = ty1 - ty0 + 1;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < tw * th; ++i) {
+        const int ix = tx0 + i % tw, iy = ty0 + i / tw;
+        V3<double> cl, ct;
+        cloud_march_pixel(s, ix, iy, cl, ct, (Counters*)nullptr);
+        const size_t o = (size_t(iy) * s.rt_w + ix) * 4;
+        rl[o] = cl.x; rl[o + 1] = cl.y; rl[o + 2] = cl.z; rl[o + 3] = 1.0;
+        rt[o] = ct.x; rt[o + 1] = ct.y; rt[o + 2] = ct.z; rt[o + 3] = 1.0;
+    }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < w * h; ++i) {
+        double c4[4], l4[4], t4[4];
+        render_pixel_half(s, x0 + i % w, y0 + i / w, rl.data(), rt.data(), c4, l4, t4);
+        for (int k = 0; k < 4; ++k) {
+            if (comp) comp[size_t(i) * 4 + k] = c4[k];
+            if (lum) lum[size_t(i) * 4 + k] = l4[k];
+            if (trn) trn[size_t(i) * 4 + k] = t4[k];
+        }
+    }
+}
+
 // Renders an arbitrary set of n pixels given as (x, y) pairs; outputs n*4.
 void or_render_pixels(const or_inputs* in, const int32_t* xy, int n, double* comp, double* lum, double* trn,
                       int nthreads) {

@@ -64,6 +64,11 @@ struct Scene {
     Layer<R> layers[PS_MAX_CLOUD_LAYERS];
     V3<R> layer_wind[PS_MAX_CLOUD_LAYERS];  // (u, 0, v) wind m/s per layer (x, z used)
     int n_layers;
+    // Half-resolution march (crates/ps-clouds/src/pipeline.rs:277-319): the
+    // cloud RT size, 0 for the full-resolution march. An RT pixel's ray goes
+    // through the viewport coordinates (x + 0.5) * vw / rt_w, its depth is
+    // read at (int)(x * vw / rt_w), and its jitter at (x, y).
+    int rt_w, rt_h;
     // Forward-identity extensions for differentiation (absent from the
     // reference; the neutral values reproduce it exactly): a per-layer
     // extinction multiplier on sigma_t, and an additive 128^3 density
@@ -595,7 +600,10 @@ inline V3<R> sky_pass_pixel(const Scene<R>& s, R ndc_x, R ndc_y) {
 // Cloud march fragment (cloud_march.wgsl:731-1054) for pixel (ix, iy).
 template <class R>
 inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum, V3<R>& out_trans, Counters* cnt) {
-    R fx = R(ix) + R(0.5), fy = R(iy) + R(0.5);
+    // viewport / RT scale (1 at full resolution)
+    const double scx = s.rt_w > 0 ? value_of(s.f.vw) / s.rt_w : 1.0;
+    const double scy = s.rt_h > 0 ? value_of(s.f.vh) / s.rt_h : 1.0;
+    R fx = (R(ix) + R(0.5)) * R(scx), fy = (R(iy) + R(0.5)) * R(scy);
     R ndc_x = (fx / s.f.vw) * R(2) - R(1), ndc_y = R(1) - (fy / s.f.vh) * R(2);
     R hv[4] = {ndc_x, ndc_y, R(1), R(1)}, nh[4];
     mat4_mul(s.f.inv_vp, hv, nh);
@@ -616,7 +624,8 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
 
     R depth_t_max = R(1e30);
     if (s.depth) {
-        R dn = s.depth[size_t(iy) * size_t(value_of(s.f.vw)) + ix];
+        const int dx = int(ix * scx), dy = int(iy * scy);
+        R dn = s.depth[size_t(dy) * size_t(value_of(s.f.vw)) + dx];
         if (dn > R(0)) {
             R dv[4] = {ndc_x, ndc_y, dn, R(1)}, wh[4];
             mat4_mul(s.f.inv_vp, dv, wh);
@@ -784,6 +793,52 @@ inline void render_pixel(const Scene<R>& s, int ix, int iy, R comp[4], R lum[4],
     comp[0] = out.x; comp[1] = out.y; comp[2] = out.z; comp[3] = R(1);
     lum[0] = cl.x; lum[1] = cl.y; lum[2] = cl.z; lum[3] = R(1);
     trn[0] = ct.x; trn[1] = ct.y; trn[2] = ct.z; trn[3] = R(1);
+}
+
+// ---- cloud_composite_halfres.wgsl:56-126 ----------------------------------------------
+// The Catmull-Rom (B = 0, C = 0.5) reconstruction of a w x h RT at uv,
+// ClampToEdge: the 16-tap kernel the reference's 9 offset bilinear taps
+// reproduce. `rt` holds w*h RGBA texels.
+inline void catmull_rom_rt(const double* rt, int w, int h, double u, double v, double out[4]) {
+    auto weights = [](double f, double wt[4]) {
+        wt[0] = f * (-0.5 + f * (1.0 - 0.5 * f));
+        wt[1] = 1.0 + f * f * (-2.5 + 1.5 * f);
+        wt[2] = f * (0.5 + f * (2.0 - 1.5 * f));
+        wt[3] = f * f * (-0.5 + 0.5 * f);
+    };
+    const double sx = u * w, sy = v * h;
+    const double bx = std::floor(sx - 0.5), by = std::floor(sy - 0.5);
+    double wx[4], wy[4];
+    weights(sx - (bx + 0.5), wx);
+    weights(sy - (by + 0.5), wy);
+    for (int k = 0; k < 4; ++k) out[k] = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        const int yy = std::min(std::max(int(by) - 1 + j, 0), h - 1);
+        for (int i = 0; i < 4; ++i) {
+            const int xx = std::min(std::max(int(bx) - 1 + i, 0), w - 1);
+            for (int k = 0; k < 4; ++k) out[k] += rt[(size_t(yy) * w + xx) * 4 + k] * wx[i] * wy[j];
+        }
+    }
+}
+
+// One output pixel of the half-resolution path: the full-resolution sky pass
+// and the upsampled RT (luminance clamped >= 0, transmittance to [0, 1]),
+// composited as render_pixel does.
+inline void render_pixel_half(const Scene<double>& s, int ix, int iy, const double* rt_lum, const double* rt_trn,
+                              double comp[4], double lum[4], double trn[4]) {
+    double fx = ix + 0.5, fy = iy + 0.5;
+    double ndc_x = (fx / s.f.vw) * 2.0 - 1.0, ndc_y = 1.0 - (fy / s.f.vh) * 2.0;
+    V3<double> sky = sky_pass_pixel(s, ndc_x, ndc_y);
+    double l[4], t[4];
+    catmull_rom_rt(rt_lum, s.rt_w, s.rt_h, fx / s.f.vw, fy / s.f.vh, l);
+    catmull_rom_rt(rt_trn, s.rt_w, s.rt_h, fx / s.f.vw, fy / s.f.vh, t);
+    for (int k = 0; k < 3; ++k) {
+        l[k] = std::max(l[k], 0.0);
+        t[k] = std::min(std::max(t[k], 0.0), 1.0);
+    }
+    comp[0] = l[0] + sky.x * t[0]; comp[1] = l[1] + sky.y * t[1]; comp[2] = l[2] + sky.z * t[2]; comp[3] = 1.0;
+    lum[0] = l[0]; lum[1] = l[1]; lum[2] = l[2]; lum[3] = 1.0;
+    trn[0] = t[0]; trn[1] = t[1]; trn[2] = t[2]; trn[3] = 1.0;
 }
 
 }  // namespace oracle

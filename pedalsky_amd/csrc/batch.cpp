@@ -217,6 +217,7 @@ extern "C" {
 ps_status ps_batch_create(ps_ctx* c, const ps_scene_desc* scenes, int32_t n, ps_batch** out) {
     if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_create: NULL argument");
     *out = nullptr;
+    if (c->half_res) return fail(PS_ERR_UNSUPPORTED, "ps_batch_create: batches render at full resolution");
     ps_batch* b = new ps_batch();
     b->c = c;
     auto bail = [&](ps_status s) {

@@ -114,6 +114,8 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
     if (!(vw >= 1.f && vh >= 1.f && vw <= 32768.f && vh <= 32768.f) || vw != std::floor(vw) || vh != std::floor(vh))
         return fail(PS_ERR_INVALID_ARGUMENT, "frame.viewport_size must hold integer width/height in [1, 32768]");
     P.W = (int)vw; P.H = (int)vh;
+    P.full_W = P.W; P.full_H = P.H;
+    P.rt_sx = 1.0; P.rt_sy = 1.0;
     P.inv_W = 1.f / vw; P.inv_H = 1.f / vh;
     for (int i = 0; i < 16; ++i) P.inv_vp[i] = f.inv_view_proj[i];
     const double R = a.planet_radius_m, top = a.atmosphere_top_m;
@@ -315,16 +317,40 @@ ps_status prepare_delta(ps_ctx* c) {
     return PS_OK;
 }
 
+ps_status grow(float** p, size_t* cap, size_t floats, const char* what);
+
 ps_status launch_render(ps_ctx* c, const float* depth, const ps_image_out* out, bool count) {
     ps_status s;
     if ((s = check_ready(c, true))) return s;
     if (!out && !count) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render: out is NULL");
+    if (count && c->half_res) return fail(PS_ERR_UNSUPPORTED, "ps_count_work: counts the full-resolution march only");
     if ((s = prepare_delta(c))) return s;
     RenderParams next;
     if ((s = fill_render_params(c, c->sc, c->luts, depth, out, next))) return s;
     RenderParams* dP = nullptr;
+    if (!c->half_res) {
+        if ((s = c->render_ring.upload(next, c->stream, &dP))) return s;
+        PS_HIP(ps_launch_render(dP, next.W, next.H, count, c->delta != nullptr, false, c->stream));
+        return c->render_ring.mark_used(c->stream);
+    }
+    // half resolution: the RT size as the reference computes it (W * 0.5 in
+    // FP32, truncated, at least 1), the march into the RT, then the
+    // full-resolution composite
+    const int rw = std::max(1, (int)((float)next.W * 0.5f)), rh = std::max(1, (int)((float)next.H * 0.5f));
+    if ((s = grow(&c->rt, &c->rt_cap, (size_t)rw * rh * 8, "half-resolution cloud RT"))) return s;
+    float4* rt_lum = (float4*)c->rt;
+    float4* rt_trn = rt_lum + (size_t)rw * rh;
+    RenderParams march = next;
+    march.W = rw; march.H = rh;
+    march.rt_sx = (double)next.W / rw; march.rt_sy = (double)next.H / rh;
+    march.n_tiles_x = (rw + 7) / 8;
+    march.out_comp = nullptr; march.out_lum = rt_lum; march.out_trans = rt_trn; march.out_f16 = 0;
+    if ((s = c->render_ring.upload(march, c->stream, &dP))) return s;
+    PS_HIP(ps_launch_render(dP, rw, rh, false, c->delta != nullptr, false, c->stream, true));
+    if ((s = c->render_ring.mark_used(c->stream))) return s;
+    next.rt_lum = rt_lum; next.rt_trn = rt_trn; next.rt_W = rw; next.rt_H = rh;
     if ((s = c->render_ring.upload(next, c->stream, &dP))) return s;
-    PS_HIP(ps_launch_render(dP, next.W, next.H, count, c->delta != nullptr, false, c->stream));
+    PS_HIP(ps_launch_composite_half(dP, next.W, next.H, c->stream));
     return c->render_ring.mark_used(c->stream);
 }
 
@@ -408,7 +434,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
-    dev_free(c->delta_bnd); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->g_delta64);
+    dev_free(c->delta_bnd); dev_free(c->rt); dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->g_delta64);
     dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
     dev_free(c->aTT); dev_free(c->aMST); dev_free(c->aSVT); dev_free(c->aAPT);
     if (c->atmo_params_h) hipHostFree(c->atmo_params_h);
@@ -747,8 +773,15 @@ ps_status ps_set_checkpoint_interval(ps_ctx* c, int32_t steps) {
     return PS_OK;
 }
 
+ps_status ps_set_half_res(ps_ctx* c, int32_t enabled) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_half_res: NULL context");
+    c->half_res = enabled != 0;
+    return PS_OK;
+}
+
 ps_status ps_render_train(ps_ctx* c, const float* depth, const ps_image_out* out) {
     if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_render_train: NULL argument");
+    if (c->half_res) return fail(PS_ERR_UNSUPPORTED, "ps_render_train: the training forward runs at full resolution");
     ps_status s;
     if ((s = check_ready(c, true))) return s;
     if ((s = prepare_delta(c))) return s;

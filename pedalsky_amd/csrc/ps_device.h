@@ -116,13 +116,26 @@ struct RenderParams {
     const uint8_t* type_grid;         // 128^2
     const uint8_t* overcast;          // 128^2
     const uint8_t* blue;              // 64^2
-    const float* depth;               // W*H reverse-Z NDC or null
+    const float* depth;               // full_W*full_H reverse-Z NDC or null
     void* out_comp;
     void* out_lum;
     void* out_trans;
     int out_f16;
     int _pad;
     unsigned long long* counters;     // 10 x u64 when counting
+    // ---- half-resolution cloud march (crates/ps-clouds/src/lib.rs:748-760,
+    // pipeline.rs:277-319) ----
+    // The march covers W x H pixels (the cloud RT); its view rays pass
+    // through the viewport coordinates (x + 0.5) * rt_sx, (y + 0.5) * rt_sy
+    // of the full_W x full_H viewport, and the depth buffer (full size) is
+    // read at (int)(x * rt_sx), (int)(y * rt_sy). Full resolution: W =
+    // full_W, rt_sx = rt_sy = 1. The composite of the half-resolution path
+    // reads the RT: rt_lum, rt_trn (rt_W x rt_H float4).
+    int full_W, full_H;
+    double rt_sx, rt_sy;
+    const float4* rt_lum;
+    const float4* rt_trn;
+    int rt_W, rt_H;
     // ---- differentiation extensions (no reference counterpart; the
     // neutral values, no volume and unit extinction scales, are the
     // reference) ----

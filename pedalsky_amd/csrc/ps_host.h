@@ -158,6 +158,10 @@ struct ps_ctx {
     double* g_delta64 = nullptr;  // FP64 accumulator of the perturbation-volume gradient
     RenderParams train{};         // parameter block of the last training forward
     bool train_valid = false;
+    // half-resolution cloud march: RT luminance then transmittance (float4)
+    bool half_res = false;
+    float* rt = nullptr;
+    size_t rt_cap = 0;            // floats
 
     // atmosphere differentiation: LUT and tangent-LUT sets for n parameter sets
     int atmo_cap = 0;

@@ -450,8 +450,9 @@ struct PixelRay {
 // FP64 view ray (math.wgsl:23-33 / sky.wgsl:166-168).
 __device__ __forceinline__ PixelRay pixel_ray(const RenderParams& P, int xc, int yc) {
     PixelRay r;
-    r.fx = xc + 0.5; r.fy = yc + 0.5;
-    r.ndx = r.fx / P.W * 2.0 - 1.0; r.ndy = 1.0 - r.fy / P.H * 2.0;
+    // viewport coordinates of the pixel centre (x2 for a half-resolution RT)
+    r.fx = (xc + 0.5) * P.rt_sx; r.fy = (yc + 0.5) * P.rt_sy;
+    r.ndx = r.fx / P.full_W * 2.0 - 1.0; r.ndy = 1.0 - r.fy / P.full_H * 2.0;
     const double* m = P.inv_vp;
     const double ndx = r.ndx, ndy = r.ndy;
     double nh0 = m[0] * ndx + m[4] * ndy + m[8] + m[12];
@@ -548,7 +549,9 @@ __device__ __forceinline__ double pixel_depth_tmax(const RenderParams& P, const 
     if (P.has_depth) {
         const double* m = P.inv_vp;
         const double ndx = ray.ndx, ndy = ray.ndy;
-        float dn = P.depth[(size_t)yc * P.W + xc];
+        // the full-resolution depth at the RT pixel's scaled coordinates
+        const int dx = (int)(xc * P.rt_sx), dy = (int)(yc * P.rt_sy);
+        float dn = P.depth[(size_t)dy * P.full_W + dx];
         if (dn > 0.f) {
             double w0 = m[0] * ndx + m[4] * ndy + m[8] * dn + m[12];
             double w1 = m[1] * ndx + m[5] * ndy + m[9] * dn + m[13];

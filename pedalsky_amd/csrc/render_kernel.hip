@@ -31,7 +31,9 @@ namespace {
 // render_tile variants: event counting, the density-perturbation volume, and
 // the training forward (transmittance checkpoints + per-pixel totals for the
 // backward).
-enum { RF_COUNT = 1, RF_DELTA = 2, RF_TRAIN = 4 };
+// RF_HALF: the cloud RT of the half-resolution path (no sky pass: the
+// full-resolution composite adds it).
+enum { RF_COUNT = 1, RF_DELTA = 2, RF_TRAIN = 4, RF_HALF = 8 };
 
 // ---------------------------------------------------------------------------
 // Deferred dense-sample shading.
@@ -193,6 +195,7 @@ template <int FLAGS, int TW>
 __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, int tile_y, WaveQueue& Q) {
     static_assert(TW == 4 || TW == 8 || TW == 16, "wave tiles are 4x16, 8x8 or 16x4 pixels");
     constexpr bool COUNT = (FLAGS & RF_COUNT) != 0, DELTA = (FLAGS & RF_DELTA) != 0, TRAIN = (FLAGS & RF_TRAIN) != 0;
+    constexpr bool HALF = (FLAGS & RF_HALF) != 0;
     static_assert(!TRAIN || TW == 8, "the training forward uses 8x8 tiles");
     const int lane = threadIdx.x;
     const int x = tile_x * TW + (lane % TW);
@@ -210,7 +213,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
     const double fx = ray.fx, fy = ray.fy;
     const float omcf = ray.omcf;
-    const float3 sky = sky_pass(P, ray);
+    const float3 sky = HALF ? make_float3(0.f, 0.f, 0.f) : sky_pass(P, ray);
     const float jitter = pixel_jitter(P, xc, yc);
     const double depth_t_max = pixel_depth_tmax(P, ray, xc, yc);
     const double r_cam = P.r0;
@@ -509,13 +512,90 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 
 __global__ void k_zero_counter(unsigned* p) { *p = 0u; }
 
+// ---- cloud_composite_halfres.wgsl:47-126: Catmull-Rom upsample of the cloud RT -----
+// The reference samples the RT with 9 bilinear taps at offsets that make the
+// hardware filter reproduce the 16-tap Catmull-Rom kernel (B = 0, C = 0.5);
+// here the 16 taps are read directly (ClampToEdge), which is the same
+// kernel without the filter-weight quantisation.
+__device__ __forceinline__ void catmull_rom_w(float f, float w[4]) {
+    w[0] = f * (-0.5f + f * (1.f - 0.5f * f));
+    w[1] = 1.f + f * f * (-2.5f + 1.5f * f);
+    w[2] = f * (0.5f + f * (2.f - 1.5f * f));
+    w[3] = f * f * (-0.5f + 0.5f * f);
+}
+
+__device__ __forceinline__ void catmull_rom2(const float4* __restrict__ a, const float4* __restrict__ b, int w, int h,
+                                             float u, float v, float4& ra, float4& rb) {
+    const float sx = u * (float)w, sy = v * (float)h;
+    const float bx = floorf(sx - 0.5f), by = floorf(sy - 0.5f);
+    float wx[4], wy[4];
+    catmull_rom_w(sx - (bx + 0.5f), wx);
+    catmull_rom_w(sy - (by + 0.5f), wy);
+    const int ix = (int)bx, iy = (int)by;
+    ra = make_float4(0.f, 0.f, 0.f, 0.f);
+    rb = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int yy = clampi(iy - 1 + j, h);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int xx = clampi(ix - 1 + i, w);
+            const float k = wx[i] * wy[j];
+            const float4 p = a[yy * w + xx], q = b[yy * w + xx];
+            ra.x += p.x * k; ra.y += p.y * k; ra.z += p.z * k;
+            rb.x += q.x * k; rb.y += q.y * k; rb.z += q.z * k;
+        }
+    }
+}
+
+// Full-resolution composite of the half-resolution path: the sky pass, the
+// upsampled cloud luminance and transmittance (clamped as the reference
+// does: luminance >= 0, transmittance in [0, 1]) and dst = lum + sky * T.
+// P holds the full-resolution parameters and the RT.
+__global__ __launch_bounds__(256) void k_composite_half(const RenderParams* __restrict__ Pp) {
+    const RenderParams& P = *Pp;
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= P.W || y >= P.H) return;
+    const PixelRay ray = pixel_ray(P, x, y);
+    const float3 sky = sky_pass(P, ray);
+    float4 lum, trn;
+    catmull_rom2(P.rt_lum, P.rt_trn, P.rt_W, P.rt_H, ((float)x + 0.5f) * P.inv_W, ((float)y + 0.5f) * P.inv_H, lum,
+                 trn);
+    lum.x = fmaxf(lum.x, 0.f); lum.y = fmaxf(lum.y, 0.f); lum.z = fmaxf(lum.z, 0.f);
+    trn.x = clampf(trn.x, 0.f, 1.f); trn.y = clampf(trn.y, 0.f, 1.f); trn.z = clampf(trn.z, 0.f, 1.f);
+    const size_t o = (size_t)y * P.W + x;
+    const float4 comp = make_float4(lum.x + sky.x * trn.x, lum.y + sky.y * trn.y, lum.z + sky.z * trn.z, 1.f);
+    const float4 lum4 = make_float4(lum.x, lum.y, lum.z, 1.f);
+    const float4 tr4 = make_float4(trn.x, trn.y, trn.z, 1.f);
+    if (P.out_f16) {
+        if (P.out_comp) {
+            __half2* q = (__half2*)P.out_comp + 2 * o;
+            q[0] = __floats2half2_rn(comp.x, comp.y); q[1] = __floats2half2_rn(comp.z, comp.w);
+        }
+        if (P.out_lum) {
+            __half2* q = (__half2*)P.out_lum + 2 * o;
+            q[0] = __floats2half2_rn(lum4.x, lum4.y); q[1] = __floats2half2_rn(lum4.z, lum4.w);
+        }
+        if (P.out_trans) {
+            __half2* q = (__half2*)P.out_trans + 2 * o;
+            q[0] = __floats2half2_rn(tr4.x, tr4.y); q[1] = __floats2half2_rn(tr4.z, tr4.w);
+        }
+    } else {
+        if (P.out_comp) ((float4*)P.out_comp)[o] = comp;
+        if (P.out_lum) ((float4*)P.out_lum)[o] = lum4;
+        if (P.out_trans) ((float4*)P.out_trans)[o] = tr4;
+    }
+}
+
 }  // namespace
 
 hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, bool delta, bool train,
-                            hipStream_t s) {
+                            hipStream_t s, bool half) {
     dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
-    const int f = (count ? RF_COUNT : 0) | (delta ? RF_DELTA : 0) | (train ? RF_TRAIN : 0);
+    const int f = (count ? RF_COUNT : 0) | (delta ? RF_DELTA : 0) | (train ? RF_TRAIN : 0) | (half ? RF_HALF : 0);
     switch (f) {
+        case RF_HALF: hipLaunchKernelGGL(k_render<RF_HALF>, grid, block, 0, s, d_params); break;
+        case RF_HALF | RF_DELTA: hipLaunchKernelGGL(k_render<RF_HALF | RF_DELTA>, grid, block, 0, s, d_params); break;
         case 0: hipLaunchKernelGGL(k_render<0>, grid, block, 0, s, d_params); break;
         case RF_COUNT: hipLaunchKernelGGL(k_render<RF_COUNT>, grid, block, 0, s, d_params); break;
         case RF_DELTA: hipLaunchKernelGGL(k_render<RF_DELTA>, grid, block, 0, s, d_params); break;
@@ -542,5 +622,10 @@ hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* 
         else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
         else hipLaunchKernelGGL((k_render_tiles<8, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
     }
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_composite_half(const RenderParams* d_params, int W, int H, hipStream_t s) {
+    hipLaunchKernelGGL(k_composite_half, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, d_params);
     return hipGetLastError();
 }

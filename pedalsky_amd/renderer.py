@@ -207,6 +207,11 @@ class Renderer:
         check(self.lib.ps_set_cloud_extensions(self.ctx, k, n, ptr))
         self._density_delta = density_delta
 
+    def set_half_res(self, enabled: bool) -> None:
+        """Half-resolution cloud march with a Catmull-Rom upsample in the
+        full-resolution composite (the reference's clouds.half_res_render)."""
+        check(self.lib.ps_set_half_res(self.ctx, 1 if enabled else 0))
+
     def set_checkpoint_interval(self, steps: int) -> None:
         check(self.lib.ps_set_checkpoint_interval(self.ctx, steps))
 
