@@ -219,7 +219,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
-    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
@@ -227,6 +227,7 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=16)
     ap.add_argument("--no-calibration", action="store_true", help="skip the atmosphere calibration run")
     ap.add_argument("--no-half-res", action="store_true", help="skip the half-resolution cloud path measurement")
+    ap.add_argument("--no-taa", action="store_true", help="skip the cloud TAA measurement")
     ap.add_argument("--calibration-sets", type=int, default=256)
     ap.add_argument("--calibration-iters", type=int, default=200)
     ap.add_argument("--batch-scenes", type=int, default=64)
@@ -353,9 +354,10 @@ def main() -> int:
                      "hbm_GBps": round(hbm_bytes / render_s / 1e9, 1)},
         "work_per_frame": {k: v / len(used) for k, v in cnt_sum.items()},
     }
-    if not args.no_half_res:
-        log("half-resolution cloud path")
-        r.set_half_res(True)
+    def time_mode(half: bool, taa: bool) -> float:
+        r.set_half_res(half)
+        r.set_taa(taa)
+        r.invalidate_taa_history()
         for k in range(args.warmup):
             frame(k)
         r.synchronize()
@@ -363,13 +365,28 @@ def main() -> int:
         for k in range(args.steps):
             frame(k)
         r.synchronize()
-        hs = (time.perf_counter() - t0) / args.steps
+        dt = (time.perf_counter() - t0) / args.steps
         r.set_half_res(False)
+        r.set_taa(False)
+        return dt
+
+    if not args.no_half_res:
+        log("half-resolution cloud path")
+        hs = time_mode(True, False)
         result["half_res"] = {"ms_per_frame": round(hs * 1e3, 4), "mpix_s": round(W * H / hs / 1e6, 3),
                               "speedup_vs_full": round(ms_per_step / (hs * 1e3), 3),
                               "rt": f"{max(1, W // 2)}x{max(1, H // 2)}",
                               "workload": "the same frames: cloud march into the RT, Catmull-Rom composite "
                                           "with the sky pass at full resolution"}
+    if not args.no_taa:
+        log("cloud TAA")
+        ts = time_mode(False, True)
+        ths = time_mode(True, True)
+        result["taa"] = {"ms_per_frame": round(ts * 1e3, 4), "mpix_s": round(W * H / ts / 1e6, 3),
+                         "overhead_vs_full": round(ts * 1e3 / ms_per_step, 3),
+                         "half_res_ms_per_frame": round(ths * 1e3, 4),
+                         "workload": "the same frames: march into a full-size (and a half-size) RT, TAA resolve "
+                                     "against the history, composite"}
     if not args.no_adjoint:
         log("forward + adjoint")
         result["fwd_adjoint"] = bench_adjoint(r, W, H, args.adjoint_steps, 3)

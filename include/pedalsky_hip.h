@@ -30,6 +30,7 @@ extern "C" {
 #define PS_ABI_VERSION 1
 /* MAX_CLOUD_LAYERS: crates/ps-clouds/src/params.rs:9, cloud_uniforms.wgsl:8 */
 #define PS_MAX_CLOUD_LAYERS 8
+#define PS_MAX_VIEWS 2
 
 typedef int32_t ps_status;
 enum {
@@ -310,6 +311,25 @@ ps_status ps_count_work(ps_ctx* ctx, const float* depth, ps_work_counters* out);
  * values). Off by default. ps_count_work, ps_render_train and batches run
  * at full resolution only (PS_ERR_UNSUPPORTED while it is on). */
 ps_status ps_set_half_res(ps_ctx* ctx, int32_t enabled);
+/* Cloud temporal anti-aliasing (`clouds.temporal_taa`, crates/ps-clouds/src/
+ * lib.rs:128-142 and 773-829; shaders/clouds/cloud_taa.wgsl): with `enabled`
+ * != 0, ps_render and ps_frame resolve each cloud RT (full or half
+ * resolution) against the active view's history before the composite:
+ * reprojection by the frame's inv_view_proj and prev_view_proj, 3x3
+ * neighbourhood clamp, blend weight 1/8 (1 on the first frame of a history).
+ * The caller passes `temporal_taa && !freeze_time` (the reference's gate).
+ * An off -> on transition of a view, a change of the RT size and
+ * ps_invalidate_taa_history discard history. Off by default. The cloud
+ * outputs are the resolved values. ps_render_train and batches return
+ * PS_ERR_UNSUPPORTED while it is on; ps_count_work counts the march only. */
+ps_status ps_set_taa(ps_ctx* ctx, int32_t enabled);
+/* `invalidate_taa_history` (crates/ps-clouds/src/lib.rs:726-734): every
+ * view's next TAA frame starts from the current sample. */
+ps_status ps_invalidate_taa_history(ps_ctx* ctx);
+/* `set_active_view` (crates/ps-clouds/src/lib.rs:119-123, MAX_CLOUD_VIEWS
+ * = 2 at lib.rs:69): the view (0 or 1) whose TAA history the next renders
+ * read and write, for stereo hosts that render each eye in turn. */
+ps_status ps_set_active_view(ps_ctx* ctx, int32_t view);
 
 /* ------------------------------------------------------------------------ */
 /* Many scenes per launch. No reference counterpart: the reference renders

@@ -106,6 +106,9 @@ def load():
         "or_render_pixels": (None, [P, P, i, P, P, P, i]),
         "or_render_ext": (None, [P, P, i, i, i, i, P, P, P, P, i]),
         "or_render_halfres": (None, [P, i, i, i, i, P, P, P, i]),
+        "or_march_rt": (None, [P, i, i, P, P, i]),
+        "or_taa_resolve": (None, [P, i, i, P, P, P, P, d, d, P, P]),
+        "or_composite_rt": (None, [P, i, i, P, P, i, i, i, i, i, P, P, P, i]),
         "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i, d, P]),
         "or_sky_ap_image": (None, [P, P, i, i, i, i, i, i, i, i, i, d, i, i, i, i, P, i]),
         "or_noise_base": (None, [C.c_uint32, i, i, P, i]),
@@ -276,6 +279,41 @@ class Frame:
         lum = np.zeros((h, w, 4))
         trn = np.zeros((h, w, 4))
         load().or_render_halfres(C.byref(self.inputs), x0, y0, w, h, _p(comp), _p(lum), _p(trn), THREADS)
+        return comp, lum, trn
+
+    def march_rt(self, rw, rh):
+        """The cloud march into an rw x rh RT over the frame's viewport:
+        (lum, trn), each (rh, rw, 4)."""
+        lum = np.zeros((rh, rw, 4))
+        trn = np.zeros((rh, rw, 4))
+        load().or_march_rt(C.byref(self.inputs), rw, rh, _p(lum), _p(trn), THREADS)
+        return lum, trn
+
+    def taa_resolve(self, cur_lum, cur_trn, hist_lum, hist_trn, blend, history_valid):
+        """The cloud TAA resolve of one RT with this frame's camera and
+        prev_view_proj: (lum, trn)."""
+        rh, rw = cur_lum.shape[:2]
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (cur_lum, cur_trn, hist_lum, hist_trn)]
+        lum = np.zeros((rh, rw, 4))
+        trn = np.zeros((rh, rw, 4))
+        load().or_taa_resolve(C.byref(self.inputs), rw, rh, *(_p(a) for a in args), float(blend),
+                              float(history_valid), _p(lum), _p(trn))
+        return lum, trn
+
+    def composite_rt(self, rt_lum, rt_trn, upsample, x0=0, y0=0, w=None, h=None):
+        """The full-resolution composite of an RT (upsample: the
+        half-resolution Catmull-Rom composite; otherwise the RT has the
+        frame's size): (comp, lum, trn), each (h, w, 4)."""
+        w = self.w.width if w is None else w
+        h = self.w.height if h is None else h
+        rh, rw = rt_lum.shape[:2]
+        a = np.ascontiguousarray(rt_lum, dtype=np.float64)
+        b = np.ascontiguousarray(rt_trn, dtype=np.float64)
+        comp = np.zeros((h, w, 4))
+        lum = np.zeros((h, w, 4))
+        trn = np.zeros((h, w, 4))
+        load().or_composite_rt(C.byref(self.inputs), rw, rh, _p(a), _p(b), int(bool(upsample)), x0, y0, w, h,
+                               _p(comp), _p(lum), _p(trn), THREADS)
         return comp, lum, trn
 
     def render_pixels(self, xy: np.ndarray):

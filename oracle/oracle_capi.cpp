@@ -8,6 +8,7 @@
 #include <cstring>
 #include "oracle_clouds.hpp"
 #include "oracle_noise.hpp"
+#include "oracle_taa.hpp"
 
 using namespace oracle;
 
@@ -293,6 +294,61 @@ void or_render_halfres(const or_inputs* in, int x0, int y0, int w, int h, double
     for (int i = 0; i < w * h; ++i) {
         double c4[4], l4[4], t4[4];
         render_pixel_half(s, x0 + i % w, y0 + i / w, rl.data(), rt.data(), c4, l4, t4);
+        for (int k = 0; k < 4; ++k) {
+            if (comp) comp[size_t(i) * 4 + k] = c4[k];
+            if (lum) lum[size_t(i) * 4 + k] = l4[k];
+            if (trn) trn[size_t(i) * 4 + k] = t4[k];
+        }
+    }
+}
+
+// The cloud march into an RT of rw x rh texels covering the frame's viewport
+// (rw = W: the full-resolution march; floor(W / 2): the half-resolution
+// one), luminance and transmittance RGBA, rw*rh*4 each.
+void or_march_rt(const or_inputs* in, int rw, int rh, double* lum, double* trn, int nthreads) {
+    Scene<double> s = make_scene<double>(in);
+    s.rt_w = rw;
+    s.rt_h = rh;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < rw * rh; ++i) {
+        V3<double> cl, ct;
+        cloud_march_pixel(s, i % rw, i / rw, cl, ct, (Counters*)nullptr);
+        const size_t o = size_t(i) * 4;
+        lum[o] = cl.x; lum[o + 1] = cl.y; lum[o + 2] = cl.z; lum[o + 3] = 1.0;
+        trn[o] = ct.x; trn[o + 1] = ct.y; trn[o + 2] = ct.z; trn[o + 3] = 1.0;
+    }
+}
+
+// The TAA resolve of a whole rw x rh RT (cloud_taa.wgsl) with the frame's
+// inv_view_proj, camera position and prev_view_proj.
+void or_taa_resolve(const or_inputs* in, int rw, int rh, const double* cur_l, const double* cur_t,
+                    const double* hist_l, const double* hist_t, double blend, double history_valid, double* out_l,
+                    double* out_t) {
+    for (int y = 0; y < rh; ++y)
+        for (int x = 0; x < rw; ++x) {
+            const size_t o = (size_t(y) * rw + x) * 4;
+            taa_resolve_texel(*in->frame, rw, rh, x, y, cur_l, cur_t, hist_l, hist_t, blend, history_valid,
+                              out_l + o, out_t + o);
+        }
+}
+
+// The full-resolution composite of an RT over a w x h region: `upsample` 1 =
+// the half-resolution composite (Catmull-Rom, clamps), 0 = the plain
+// composite of an RT of the frame's own size.
+void or_composite_rt(const or_inputs* in, int rw, int rh, const double* rt_lum, const double* rt_trn, int upsample,
+                     int x0, int y0, int w, int h, double* comp, double* lum, double* trn, int nthreads) {
+    Scene<double> s = make_scene<double>(in);
+    s.rt_w = rw;
+    s.rt_h = rh;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < w * h; ++i) {
+        double c4[4], l4[4], t4[4];
+        if (upsample)
+            render_pixel_half(s, x0 + i % w, y0 + i / w, rt_lum, rt_trn, c4, l4, t4);
+        else
+            render_pixel_rt(s, x0 + i % w, y0 + i / w, rt_lum, rt_trn, c4, l4, t4);
         for (int k = 0; k < 4; ++k) {
             if (comp) comp[size_t(i) * 4 + k] = c4[k];
             if (lum) lum[size_t(i) * 4 + k] = l4[k];

@@ -841,4 +841,21 @@ inline void render_pixel_half(const Scene<double>& s, int ix, int iy, const doub
     trn[0] = t[0]; trn[1] = t[1]; trn[2] = t[2]; trn[3] = 1.0;
 }
 
+// One output pixel of the full-resolution composite over a cloud RT of the
+// frame's own size (cloud_composite.wgsl: the RT texel at the pixel, added
+// as lum + sky * T by the dual-source blend, no clamps): the path the TAA
+// resolve feeds at full resolution.
+inline void render_pixel_rt(const Scene<double>& s, int ix, int iy, const double* rt_lum, const double* rt_trn,
+                            double comp[4], double lum[4], double trn[4]) {
+    double fx = ix + 0.5, fy = iy + 0.5;
+    double ndc_x = (fx / s.f.vw) * 2.0 - 1.0, ndc_y = 1.0 - (fy / s.f.vh) * 2.0;
+    V3<double> sky = sky_pass_pixel(s, ndc_x, ndc_y);
+    const size_t o = (size_t(iy) * int(s.f.vw) + ix) * 4;
+    const double* l = rt_lum + o;
+    const double* t = rt_trn + o;
+    comp[0] = l[0] + sky.x * t[0]; comp[1] = l[1] + sky.y * t[1]; comp[2] = l[2] + sky.z * t[2]; comp[3] = 1.0;
+    lum[0] = l[0]; lum[1] = l[1]; lum[2] = l[2]; lum[3] = 1.0;
+    trn[0] = t[0]; trn[1] = t[1]; trn[2] = t[2]; trn[3] = 1.0;
+}
+
 }  // namespace oracle

@@ -248,6 +248,9 @@ _SIGNATURES = [
     ("ps_set_cloud_extensions", C.c_int32, [_P, C.POINTER(C.c_float), C.c_int32, _P]),
     ("ps_set_checkpoint_interval", C.c_int32, [_P, C.c_int32]),
     ("ps_set_half_res", C.c_int32, [_P, C.c_int32]),
+    ("ps_set_taa", C.c_int32, [_P, C.c_int32]),
+    ("ps_invalidate_taa_history", C.c_int32, [_P]),
+    ("ps_set_active_view", C.c_int32, [_P, C.c_int32]),
     ("ps_render_train", C.c_int32, [_P, _P, C.POINTER(ImageOut)]),
     ("ps_backward", C.c_int32, [_P, _P, _P, _P, C.POINTER(CloudGrads)]),
     ("ps_sky_ap_image", C.c_int32, [_P, _P, C.c_float, _P]),

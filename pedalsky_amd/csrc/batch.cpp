@@ -218,6 +218,7 @@ ps_status ps_batch_create(ps_ctx* c, const ps_scene_desc* scenes, int32_t n, ps_
     if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_create: NULL argument");
     *out = nullptr;
     if (c->half_res) return fail(PS_ERR_UNSUPPORTED, "ps_batch_create: batches render at full resolution");
+    if (c->taa) return fail(PS_ERR_UNSUPPORTED, "ps_batch_create: batches render single frames (TAA off)");
     ps_batch* b = new ps_batch();
     b->c = c;
     auto bail = [&](ps_status s) {

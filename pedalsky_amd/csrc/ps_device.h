@@ -129,8 +129,9 @@ struct RenderParams {
     // through the viewport coordinates (x + 0.5) * rt_sx, (y + 0.5) * rt_sy
     // of the full_W x full_H viewport, and the depth buffer (full size) is
     // read at (int)(x * rt_sx), (int)(y * rt_sy). Full resolution: W =
-    // full_W, rt_sx = rt_sy = 1. The composite of the half-resolution path
-    // reads the RT: rt_lum, rt_trn (rt_W x rt_H float4).
+    // full_W, rt_sx = rt_sy = 1. The composite over an RT (half resolution,
+    // or any frame that goes through the TAA resolve) reads rt_lum, rt_trn
+    // (rt_W x rt_H float4).
     int full_W, full_H;
     double rt_sx, rt_sy;
     const float4* rt_lum;
@@ -166,6 +167,25 @@ struct RenderParams {
     float* g_delta;                   // 128^3 accumulated, or null
     double* g_delta64;                // FP64 accumulator the kernel scatters into (added to g_delta after)
     float* rec;                       // backward records [tile][slot][REC_FIELDS][lane]
+};
+
+// The cloud TAA resolve of one RT (shaders/clouds/cloud_taa.wgsl), passed by
+// value: the current march, the history slot it reads and the slot it
+// writes, all w x h float4; the current frame's inv_view_proj and camera
+// position and the previous frame's view_proj (column-major, FP64).
+struct TaaParams {
+    const float4* cur_lum;
+    const float4* cur_trn;
+    const float4* hist_lum;
+    const float4* hist_trn;
+    float4* out_lum;
+    float4* out_trn;
+    int w, h;
+    float blend;                      // current-sample weight, clamped to [0, 1]
+    int history_valid;
+    double inv_vp[16];
+    double prev_vp[16];
+    double cam[3];
 };
 
 // Per-pixel forward totals stored by the training forward.

@@ -162,6 +162,19 @@ struct ps_ctx {
     bool half_res = false;
     float* rt = nullptr;
     size_t rt_cap = 0;            // floats
+    // cloud TAA: per view, two history slots of luminance and transmittance
+    // (float4, slot-major: lum0, trn0, lum1, trn1) at the RT size
+    struct TaaView {
+        float* hist = nullptr;
+        size_t cap = 0;           // floats
+        bool history_valid = false;
+        bool prev_on = false;
+        int write_slot = 0;
+    };
+    bool taa = false;
+    int active_view = 0;
+    int taa_rt_w = 0, taa_rt_h = 0;   // RT size the histories hold
+    TaaView taa_views[PS_MAX_VIEWS];
 
     // atmosphere differentiation: LUT and tangent-LUT sets for n parameter sets
     int atmo_cap = 0;

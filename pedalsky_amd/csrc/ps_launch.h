@@ -16,12 +16,15 @@ hipError_t ps_launch_luts(const LutParams* d_params, const LutParams& dims, int 
 
 // Single-scene render: `delta` enables the density-perturbation volume,
 // `train` also stores the checkpoints and totals the backward reads.
-// `half`: the cloud RT of the half-resolution path (W x H = the RT size;
-// no sky pass, luminance and transmittance to out_lum / out_trans).
+// `rt`: the march writes the cloud RT (W x H = the RT size; no sky pass,
+// luminance and transmittance to out_lum / out_trans).
 hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, bool delta, bool train,
-                            hipStream_t s, bool half = false);
-// Full-resolution composite of the half-resolution path (W x H viewport).
-hipError_t ps_launch_composite_half(const RenderParams* d_params, int W, int H, hipStream_t s);
+                            hipStream_t s, bool rt = false);
+// Full-resolution composite over the cloud RT (W x H viewport): Catmull-Rom
+// upsample of a half-resolution RT, or the texel of a full-size one.
+hipError_t ps_launch_composite_rt(const RenderParams* d_params, int W, int H, bool upsample, hipStream_t s);
+// Cloud TAA resolve of one RT.
+hipError_t ps_launch_taa_resolve(const TaaParams& p, hipStream_t s);
 // Backward of a training render (the same parameter block, with cotangents
 // and gradient outputs set).
 // With g_delta non-null the perturbation-volume gradient is accumulated in

@@ -31,9 +31,10 @@ namespace {
 // render_tile variants: event counting, the density-perturbation volume, and
 // the training forward (transmittance checkpoints + per-pixel totals for the
 // backward).
-// RF_HALF: the cloud RT of the half-resolution path (no sky pass: the
-// full-resolution composite adds it).
-enum { RF_COUNT = 1, RF_DELTA = 2, RF_TRAIN = 4, RF_HALF = 8 };
+// RF_RT: the march writes the cloud RT (luminance and transmittance) of the
+// half-resolution path or of a frame that goes through the TAA resolve; no
+// sky pass (the full-resolution composite adds it).
+enum { RF_COUNT = 1, RF_DELTA = 2, RF_TRAIN = 4, RF_RT = 8 };
 
 // ---------------------------------------------------------------------------
 // Deferred dense-sample shading.
@@ -195,7 +196,7 @@ template <int FLAGS, int TW>
 __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, int tile_y, WaveQueue& Q) {
     static_assert(TW == 4 || TW == 8 || TW == 16, "wave tiles are 4x16, 8x8 or 16x4 pixels");
     constexpr bool COUNT = (FLAGS & RF_COUNT) != 0, DELTA = (FLAGS & RF_DELTA) != 0, TRAIN = (FLAGS & RF_TRAIN) != 0;
-    constexpr bool HALF = (FLAGS & RF_HALF) != 0;
+    constexpr bool RT = (FLAGS & RF_RT) != 0;
     static_assert(!TRAIN || TW == 8, "the training forward uses 8x8 tiles");
     const int lane = threadIdx.x;
     const int x = tile_x * TW + (lane % TW);
@@ -213,7 +214,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
     const double fx = ray.fx, fy = ray.fy;
     const float omcf = ray.omcf;
-    const float3 sky = HALF ? make_float3(0.f, 0.f, 0.f) : sky_pass(P, ray);
+    const float3 sky = RT ? make_float3(0.f, 0.f, 0.f) : sky_pass(P, ray);
     const float jitter = pixel_jitter(P, xc, yc);
     const double depth_t_max = pixel_depth_tmax(P, ray, xc, yc);
     const double r_cam = P.r0;
@@ -548,21 +549,29 @@ __device__ __forceinline__ void catmull_rom2(const float4* __restrict__ a, const
     }
 }
 
-// Full-resolution composite of the half-resolution path: the sky pass, the
-// upsampled cloud luminance and transmittance (clamped as the reference
-// does: luminance >= 0, transmittance in [0, 1]) and dst = lum + sky * T.
-// P holds the full-resolution parameters and the RT.
-__global__ __launch_bounds__(256) void k_composite_half(const RenderParams* __restrict__ Pp) {
+// Full-resolution composite over a cloud RT: the sky pass plus the RT's
+// luminance and transmittance, dst = lum + sky * T. UPSAMPLE (the
+// half-resolution path, cloud_composite_halfres.wgsl): Catmull-Rom of the
+// RT, luminance clamped >= 0 and transmittance to [0, 1] as the reference
+// does. Otherwise the RT has the frame's size and its texel is read as is
+// (cloud_composite.wgsl). P holds the full-resolution parameters and the RT.
+template <bool UPSAMPLE>
+__global__ __launch_bounds__(256) void k_composite_rt(const RenderParams* __restrict__ Pp) {
     const RenderParams& P = *Pp;
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= P.W || y >= P.H) return;
     const PixelRay ray = pixel_ray(P, x, y);
     const float3 sky = sky_pass(P, ray);
     float4 lum, trn;
-    catmull_rom2(P.rt_lum, P.rt_trn, P.rt_W, P.rt_H, ((float)x + 0.5f) * P.inv_W, ((float)y + 0.5f) * P.inv_H, lum,
-                 trn);
-    lum.x = fmaxf(lum.x, 0.f); lum.y = fmaxf(lum.y, 0.f); lum.z = fmaxf(lum.z, 0.f);
-    trn.x = clampf(trn.x, 0.f, 1.f); trn.y = clampf(trn.y, 0.f, 1.f); trn.z = clampf(trn.z, 0.f, 1.f);
+    if (UPSAMPLE) {
+        catmull_rom2(P.rt_lum, P.rt_trn, P.rt_W, P.rt_H, ((float)x + 0.5f) * P.inv_W, ((float)y + 0.5f) * P.inv_H,
+                     lum, trn);
+        lum.x = fmaxf(lum.x, 0.f); lum.y = fmaxf(lum.y, 0.f); lum.z = fmaxf(lum.z, 0.f);
+        trn.x = clampf(trn.x, 0.f, 1.f); trn.y = clampf(trn.y, 0.f, 1.f); trn.z = clampf(trn.z, 0.f, 1.f);
+    } else {
+        lum = P.rt_lum[(size_t)y * P.rt_W + x];
+        trn = P.rt_trn[(size_t)y * P.rt_W + x];
+    }
     const size_t o = (size_t)y * P.W + x;
     const float4 comp = make_float4(lum.x + sky.x * trn.x, lum.y + sky.y * trn.y, lum.z + sky.z * trn.z, 1.f);
     const float4 lum4 = make_float4(lum.x, lum.y, lum.z, 1.f);
@@ -587,18 +596,108 @@ __global__ __launch_bounds__(256) void k_composite_half(const RenderParams* __re
     }
 }
 
+// ---- cloud_taa.wgsl: temporal resolve of one cloud RT ----------------------------
+// One thread per RT texel in 16 x 16 blocks. The block's current luminance
+// and transmittance with a one-texel halo (edge texels repeated: the
+// reference's ClampToEdge taps) are staged in LDS for the 3x3 box. The
+// reprojection (cloud_taa.wgsl:81-109) runs in FP64: the direction is the
+// difference of the unprojected near-plane point and the camera position,
+// a cancellation that loses ~1e-4 of the direction in FP32 for a camera a
+// few hundred metres from the origin.
+__device__ __forceinline__ float4 bilinear_rt(const float4* __restrict__ t, int w, int h, double u, double v) {
+    const double sx = u * w - 0.5, sy = v * h - 0.5;
+    const double x0d = floor(sx), y0d = floor(sy);
+    const float fx = (float)(sx - x0d), fy = (float)(sy - y0d);
+    const double wm = (double)(w - 1), hm = (double)(h - 1);
+    const int x0 = (int)fmin(fmax(x0d, 0.0), wm), x1 = (int)fmin(fmax(x0d + 1.0, 0.0), wm);
+    const int y0 = (int)fmin(fmax(y0d, 0.0), hm), y1 = (int)fmin(fmax(y0d + 1.0, 0.0), hm);
+    const float4 a = t[(size_t)y0 * w + x0], b = t[(size_t)y0 * w + x1];
+    const float4 c = t[(size_t)y1 * w + x0], d = t[(size_t)y1 * w + x1];
+    const float gx = 1.f - fx, gy = 1.f - fy;
+    return make_float4((a.x * gx + b.x * fx) * gy + (c.x * gx + d.x * fx) * fy,
+                       (a.y * gx + b.y * fx) * gy + (c.y * gx + d.y * fx) * fy,
+                       (a.z * gx + b.z * fx) * gy + (c.z * gx + d.z * fx) * fy,
+                       (a.w * gx + b.w * fx) * gy + (c.w * gx + d.w * fx) * fy);
+}
+
+__device__ __forceinline__ float4 min4(float4 a, float4 b) {
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), fminf(a.w, b.w));
+}
+__device__ __forceinline__ float4 max4(float4 a, float4 b) {
+    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+}
+__device__ __forceinline__ float4 clamp4(float4 v, float4 lo, float4 hi) { return min4(max4(v, lo), hi); }
+
+__global__ __launch_bounds__(256) void k_taa_resolve(const TaaParams p) {
+    __shared__ float4 sl[18][18], st[18][18];
+    const int bx = blockIdx.x * 16, by = blockIdx.y * 16;
+    for (int i = threadIdx.x; i < 18 * 18; i += 256) {
+        const int ly = i / 18, lx = i - ly * 18;
+        const size_t g = (size_t)clampi(by + ly - 1, p.h) * p.w + clampi(bx + lx - 1, p.w);
+        sl[ly][lx] = p.cur_lum[g];
+        st[ly][lx] = p.cur_trn[g];
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x = bx + tx, y = by + ty;
+    if (x >= p.w || y >= p.h) return;
+    const float4 cl = sl[ty + 1][tx + 1], ct = st[ty + 1][tx + 1];
+    float4 ol = cl, ot = ct;
+    const float bw = clampf(p.blend, 0.f, 1.f);
+    if (p.history_valid && bw < 1.f) {
+        const double u = ((double)x + 0.5) / p.w, v = ((double)y + 0.5) / p.h;
+        const double n0 = u * 2.0 - 1.0, n1 = 1.0 - v * 2.0;
+        double hq[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hq[r] = p.inv_vp[r] * n0 + p.inv_vp[4 + r] * n1 + p.inv_vp[8 + r] + p.inv_vp[12 + r];
+        const double iw = 1.0 / hq[3];
+        double d0 = hq[0] * iw - p.cam[0], d1 = hq[1] * iw - p.cam[1], d2 = hq[2] * iw - p.cam[2];
+        const double il = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        d0 *= il; d1 *= il; d2 *= il;
+        const double cx = p.prev_vp[0] * d0 + p.prev_vp[4] * d1 + p.prev_vp[8] * d2;
+        const double cy = p.prev_vp[1] * d0 + p.prev_vp[5] * d1 + p.prev_vp[9] * d2;
+        const double cw = p.prev_vp[3] * d0 + p.prev_vp[7] * d1 + p.prev_vp[11] * d2;
+        double pu = 0.0, pv = 0.0;
+        bool inside = false;
+        if (cw > 1e-4) {
+            pu = cx / cw * 0.5 + 0.5;
+            pv = 1.0 - (cy / cw * 0.5 + 0.5);
+            inside = pu >= 0.0 && pv >= 0.0 && pu <= 1.0 && pv <= 1.0;
+        }
+        const float eb = inside ? bw : 1.f;
+        const float4 hl = bilinear_rt(p.hist_lum, p.w, p.h, pu, pv), ht = bilinear_rt(p.hist_trn, p.w, p.h, pu, pv);
+        float4 lo_l = cl, hi_l = cl, lo_t = ct, hi_t = ct;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                lo_l = min4(lo_l, sl[ty + dy][tx + dx]); hi_l = max4(hi_l, sl[ty + dy][tx + dx]);
+                lo_t = min4(lo_t, st[ty + dy][tx + dx]); hi_t = max4(hi_t, st[ty + dy][tx + dx]);
+            }
+        const float4 a = clamp4(hl, lo_l, hi_l), b = clamp4(ht, lo_t, hi_t);
+        const float ga = 1.f - eb;
+        ol = make_float4(fmaxf(a.x * ga + cl.x * eb, 0.f), fmaxf(a.y * ga + cl.y * eb, 0.f),
+                         fmaxf(a.z * ga + cl.z * eb, 0.f), fmaxf(a.w * ga + cl.w * eb, 0.f));
+        ot = make_float4(clampf(b.x * ga + ct.x * eb, 0.f, 1.f), clampf(b.y * ga + ct.y * eb, 0.f, 1.f),
+                         clampf(b.z * ga + ct.z * eb, 0.f, 1.f), clampf(b.w * ga + ct.w * eb, 0.f, 1.f));
+    }
+    const size_t o = (size_t)y * p.w + x;
+    p.out_lum[o] = ol;
+    p.out_trn[o] = ot;
+}
+
 }  // namespace
 
 hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool count, bool delta, bool train,
-                            hipStream_t s, bool half) {
+                            hipStream_t s, bool rt) {
     dim3 grid((W + 7) / 8, (H + 7) / 8), block(64);
-    const int f = (count ? RF_COUNT : 0) | (delta ? RF_DELTA : 0) | (train ? RF_TRAIN : 0) | (half ? RF_HALF : 0);
+    const int f = (count ? RF_COUNT : 0) | (delta ? RF_DELTA : 0) | (train ? RF_TRAIN : 0) | (rt ? RF_RT : 0);
     switch (f) {
-        case RF_HALF: hipLaunchKernelGGL(k_render<RF_HALF>, grid, block, 0, s, d_params); break;
-        case RF_HALF | RF_DELTA: hipLaunchKernelGGL(k_render<RF_HALF | RF_DELTA>, grid, block, 0, s, d_params); break;
+        case RF_RT: hipLaunchKernelGGL(k_render<RF_RT>, grid, block, 0, s, d_params); break;
+        case RF_RT | RF_DELTA: hipLaunchKernelGGL(k_render<RF_RT | RF_DELTA>, grid, block, 0, s, d_params); break;
         case 0: hipLaunchKernelGGL(k_render<0>, grid, block, 0, s, d_params); break;
-        case RF_COUNT: hipLaunchKernelGGL(k_render<RF_COUNT>, grid, block, 0, s, d_params); break;
         case RF_DELTA: hipLaunchKernelGGL(k_render<RF_DELTA>, grid, block, 0, s, d_params); break;
+        case RF_COUNT: hipLaunchKernelGGL(k_render<RF_COUNT>, grid, block, 0, s, d_params); break;
         case RF_DELTA | RF_COUNT: hipLaunchKernelGGL(k_render<RF_DELTA | RF_COUNT>, grid, block, 0, s, d_params); break;
         case RF_TRAIN: hipLaunchKernelGGL(k_render<RF_TRAIN>, grid, block, 0, s, d_params); break;
         case RF_TRAIN | RF_DELTA: hipLaunchKernelGGL(k_render<RF_TRAIN | RF_DELTA>, grid, block, 0, s, d_params); break;
@@ -625,7 +724,14 @@ hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* 
     return hipGetLastError();
 }
 
-hipError_t ps_launch_composite_half(const RenderParams* d_params, int W, int H, hipStream_t s) {
-    hipLaunchKernelGGL(k_composite_half, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, d_params);
+hipError_t ps_launch_composite_rt(const RenderParams* d_params, int W, int H, bool upsample, hipStream_t s) {
+    const dim3 g((W + 15) / 16, (H + 15) / 16), b(256);
+    if (upsample) hipLaunchKernelGGL(k_composite_rt<true>, g, b, 0, s, d_params);
+    else hipLaunchKernelGGL(k_composite_rt<false>, g, b, 0, s, d_params);
+    return hipGetLastError();
+}
+
+hipError_t ps_launch_taa_resolve(const TaaParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_taa_resolve, dim3((p.w + 15) / 16, (p.h + 15) / 16), dim3(256), 0, s, p);
     return hipGetLastError();
 }

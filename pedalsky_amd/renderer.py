@@ -212,6 +212,18 @@ class Renderer:
         full-resolution composite (the reference's clouds.half_res_render)."""
         check(self.lib.ps_set_half_res(self.ctx, 1 if enabled else 0))
 
+    def set_taa(self, enabled: bool) -> None:
+        """Cloud temporal anti-aliasing (the reference's clouds.temporal_taa;
+        pass temporal_taa and not freeze_time)."""
+        check(self.lib.ps_set_taa(self.ctx, 1 if enabled else 0))
+
+    def invalidate_taa_history(self) -> None:
+        check(self.lib.ps_invalidate_taa_history(self.ctx))
+
+    def set_active_view(self, view: int) -> None:
+        """The view (0 or 1) whose TAA history the next renders use."""
+        check(self.lib.ps_set_active_view(self.ctx, view))
+
     def set_checkpoint_interval(self, steps: int) -> None:
         check(self.lib.ps_set_checkpoint_interval(self.ctx, steps))
 

@@ -1,0 +1,235 @@
+"""GPU parity of the cloud TAA path (the reference's clouds.temporal_taa:
+crates/ps-clouds/src/lib.rs:773-829, shaders/clouds/cloud_taa.wgsl) against
+the FP64 oracle, through the C ABI: frame sequences with a turning camera at
+full and half resolution, history bootstrap and invalidation, RT resizes,
+the two views' separate histories, and the calls that reject the mode.
+
+Bar (same as the plain-path tests): relative radiance error 99.9th
+percentile <= 1e-3, median <= 1e-5; cloud transmittance absolute error
+99.9th percentile <= 1e-3, on every frame of the sequence.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from pedalsky_amd import abi, scene
+from pedalsky_amd.abi import PedalskyError
+from pedalsky_amd.renderer import Renderer
+
+pytestmark = pytest.mark.gpu
+
+W, H = 192, 108
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU (no CPU fallback exists)"
+    r = Renderer(0)
+    yield r
+    r.close()
+
+
+def _sequence(yaws, pitch=12.0, width=W, height=H):
+    """Workloads of consecutive frames: camera yaw per frame, each frame's
+    prev_view_proj = the previous frame's view_proj, ground depth per frame."""
+    base = scene.workload_small(width=width, height=height, steps=48)
+    out, prev = [], None
+    for k, yaw in enumerate(yaws):
+        f = scene.make_frame(width, height, camera=(0.0, 500.0, 0.0), pitch_deg=pitch, yaw_deg=yaw,
+                             sun_zenith_deg=40.0, frame_index=k)
+        if prev is not None:
+            f.prev_view_proj[:] = prev.view_proj[:]
+        w = dataclasses.replace(base)
+        w.frame = f
+        w.depth = scene.ground_depth(f, base.atmosphere.planet_radius_m)
+        out.append(w)
+        prev = f
+    return out
+
+
+def _gpu_frame(r, w):
+    r.set_frame(w.frame)
+    r.set_depth(w.depth)
+    r.workload = w
+    return r.render_numpy()
+
+
+def _noise(r, w):
+    fmt = w.noise_format
+    return [oracle.decode(r.read_noise(k, fmt), fmt)
+            for k in (abi.PS_NOISE_BASE, abi.PS_NOISE_DETAIL, abi.PS_NOISE_CURL)]
+
+
+def _oracle_chain(ws, noise, half):
+    """The oracle's composite for each frame of a TAA sequence that starts
+    from no history."""
+    outs, hist = [], None
+    for w in ws:
+        fr = oracle.Frame(w, *noise)
+        rw, rh = (max(1, w.width // 2), max(1, w.height // 2)) if half else (w.width, w.height)
+        cl, ct = fr.march_rt(rw, rh)
+        if hist is None:
+            rl, rt_ = cl, ct
+        else:
+            rl, rt_ = fr.taa_resolve(cl, ct, hist[0], hist[1], 1.0 / 8.0, 1.0)
+        hist = (rl, rt_)
+        outs.append(fr.composite_rt(rl, rt_, upsample=half))
+    return outs
+
+
+def _check(got, ref, what):
+    comp, _, trn = ref
+    e = oracle.relative_error(got["composite"], comp)
+    p999, med = float(np.percentile(e, 99.9)), float(np.median(e))
+    td = np.abs(got["cloud_transmittance"][..., :3] - trn[..., :3]).max(-1)
+    tp = float(np.percentile(td, 99.9))
+    _log(f"  {what}: composite p99.9 {p999:.3e} median {med:.3e} max {e.max():.3e}; "
+         f"transmittance abs p99.9 {tp:.3e} max {td.max():.3e}")
+    assert p999 <= 1e-3
+    assert med <= 1e-5
+    assert tp <= 1e-3
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_taa_sequence_matches_oracle(renderer, half):
+    r = renderer
+    ws = _sequence([178.0, 178.6, 179.4, 180.0])
+    _log(f"case: TAA sequence {W}x{H}, half resolution {half}, yaw 178 -> 180 deg")
+    r.load_workload(ws[0])
+    noise = _noise(r, ws[0])
+    refs = _oracle_chain(ws, noise, half)
+    r.set_half_res(half)
+    r.set_taa(True)
+    try:
+        r.invalidate_taa_history()
+        gots = [_gpu_frame(r, w) for w in ws]
+    finally:
+        r.set_taa(False)
+        r.set_half_res(False)
+    for k, (g, ref) in enumerate(zip(gots, refs)):
+        _check(g, ref, f"frame {k}")
+    # the resolve is live: the last frame differs from the same frame without history
+    fr = oracle.Frame(ws[-1], *noise)
+    rw, rh = (W // 2, H // 2) if half else (W, H)
+    plain = fr.composite_rt(*fr.march_rt(rw, rh), upsample=half)[0]
+    diff = float(np.abs(refs[-1][0][..., :3] - plain[..., :3]).max() / np.abs(plain[..., :3]).max())
+    gdiff = float(np.abs(gots[-1]["composite"][..., :3] - plain[..., :3]).max() / np.abs(plain[..., :3]).max())
+    _log(f"  history changes the last frame by {diff:.3e} (GPU {gdiff:.3e})")
+    assert diff > 1e-3 and gdiff > 1e-3
+
+
+def test_taa_bootstrap_and_invalidation(renderer):
+    """The first TAA frame, the frame after invalidate_taa_history, after an
+    off -> on toggle and after an RT resize are the plain render."""
+    r = renderer
+    ws = _sequence([100.0, 101.0, 102.0, 103.0])
+    r.load_workload(ws[0])
+    plain = []
+    for w in ws:
+        plain.append(_gpu_frame(r, w))
+    r.set_taa(True)
+    try:
+        r.invalidate_taa_history()
+        a = _gpu_frame(r, ws[0])
+        b = _gpu_frame(r, ws[1])
+        r.invalidate_taa_history()
+        c = _gpu_frame(r, ws[2])
+        r.set_taa(False)
+        _gpu_frame(r, ws[2])
+        r.set_taa(True)
+        d = _gpu_frame(r, ws[3])
+    finally:
+        r.set_taa(False)
+
+    def rel(x, y):
+        return float(np.abs(x["composite"][..., :3] - y["composite"][..., :3]).max()
+                     / np.abs(y["composite"][..., :3]).max())
+    _log(f"  bootstrap {rel(a, plain[0]):.2e}, after invalidate {rel(c, plain[2]):.2e}, "
+         f"after toggle {rel(d, plain[3]):.2e}, with history {rel(b, plain[1]):.2e}")
+    for x, y in ((a, plain[0]), (c, plain[2]), (d, plain[3])):
+        assert rel(x, y) <= 1e-6
+        assert np.abs(x["cloud_transmittance"] - y["cloud_transmittance"]).max() <= 1e-6
+    assert rel(b, plain[1]) > 1e-3
+    # a new RT size discards the history
+    small = _sequence([100.0, 101.0], width=W - 32, height=H - 18)
+    r.workload = small[0]
+    p_small = _gpu_frame(r, small[1])
+    r.set_taa(True)
+    try:
+        r.invalidate_taa_history()
+        _gpu_frame(r, ws[0])
+        e = _gpu_frame(r, small[1])
+    finally:
+        r.set_taa(False)
+    assert rel(e, p_small) <= 1e-6
+
+
+def test_taa_views_keep_separate_histories(renderer):
+    r = renderer
+    wa = _sequence([30.0, 31.0, 32.0])
+    wb = _sequence([250.0, 249.0, 248.0], pitch=5.0)
+    r.load_workload(wa[0])
+    r.set_taa(True)
+    try:
+        r.invalidate_taa_history()
+        alone = [_gpu_frame(r, w) for w in wa]
+        r.set_active_view(1)
+        r.invalidate_taa_history()
+        inter = []
+        for k in range(3):
+            r.set_active_view(0)
+            inter.append(_gpu_frame(r, wa[k]))
+            r.set_active_view(1)
+            _gpu_frame(r, wb[k])
+        # view 1's chain alone, for the same comparison the other way round
+        r.invalidate_taa_history()
+        b_alone = [_gpu_frame(r, w) for w in wb]
+        r.set_active_view(0)
+        r.invalidate_taa_history()
+        for k in range(3):
+            r.set_active_view(1)
+            bk = _gpu_frame(r, wb[k])
+            r.set_active_view(0)
+            _gpu_frame(r, wa[k])
+            assert np.array_equal(bk["composite"], b_alone[k]["composite"])
+    finally:
+        r.set_active_view(0)
+        r.set_taa(False)
+    for k in range(3):
+        assert np.array_equal(inter[k]["composite"], alone[k]["composite"])
+    with pytest.raises(PedalskyError) as ei:
+        r.set_active_view(2)
+    assert ei.value.status == abi.PS_ERR_INVALID_ARGUMENT
+
+
+def test_taa_rejects_single_frame_calls(renderer):
+    r = renderer
+    w = scene.workload_small(width=64, height=32, steps=16)
+    r.load_workload(w)
+    r.bake_luts()
+    base = r.count_work()
+    r.set_taa(True)
+    try:
+        bufs = r.alloc_outputs(64, 32)
+        out = r.image_out(bufs)
+        with pytest.raises(PedalskyError) as ei:
+            r.render_train(out)
+        assert ei.value.status == abi.PS_ERR_UNSUPPORTED
+        from pedalsky_amd.batch import Batch
+        with pytest.raises(PedalskyError) as ei:
+            Batch(r, [w])
+        assert ei.value.status == abi.PS_ERR_UNSUPPORTED
+        c = r.count_work()          # counts the march, which TAA leaves alone
+        assert [getattr(c, f) for f, _ in abi.WorkCounters._fields_] == \
+            [getattr(base, f) for f, _ in abi.WorkCounters._fields_]
+    finally:
+        r.set_taa(False)
