@@ -106,7 +106,7 @@ def load():
         "or_render_pixels": (None, [P, P, i, P, P, P, i]),
         "or_render_ext": (None, [P, P, i, i, i, i, P, P, P, P, i]),
         "or_render_halfres": (None, [P, i, i, i, i, P, P, P, i]),
-        "or_march_rt": (None, [P, i, i, P, P, i]),
+        "or_march_rt": (None, [P, P, i, i, P, P, i]),
         "or_taa_resolve": (None, [P, i, i, P, P, P, P, d, d, P, P]),
         "or_composite_rt": (None, [P, i, i, P, P, i, i, i, i, i, P, P, P, i]),
         "or_vjp_clouds": (i, [P, P, i, i, i, i, P, P, P, P, i, d, P]),
@@ -281,12 +281,15 @@ class Frame:
         load().or_render_halfres(C.byref(self.inputs), x0, y0, w, h, _p(comp), _p(lum), _p(trn), THREADS)
         return comp, lum, trn
 
-    def march_rt(self, rw, rh):
-        """The cloud march into an rw x rh RT over the frame's viewport:
-        (lum, trn), each (rh, rw, 4)."""
+    def march_rt(self, rw, rh, diff=None):
+        """The cloud march into an rw x rh RT over the frame's viewport, with
+        the differentiation extensions of `diff` (a Diff) if given: (lum,
+        trn), each (rh, rw, 4)."""
         lum = np.zeros((rh, rw, 4))
         trn = np.zeros((rh, rw, 4))
-        load().or_march_rt(C.byref(self.inputs), rw, rh, _p(lum), _p(trn), THREADS)
+        st = None if diff is None else diff.struct()
+        load().or_march_rt(C.byref(self.inputs), None if st is None else C.byref(st), rw, rh, _p(lum), _p(trn),
+                           THREADS)
         return lum, trn
 
     def taa_resolve(self, cur_lum, cur_trn, hist_lum, hist_trn, blend, history_valid):

@@ -304,9 +304,12 @@ void or_render_halfres(const or_inputs* in, int x0, int y0, int w, int h, double
 
 // The cloud march into an RT of rw x rh texels covering the frame's viewport
 // (rw = W: the full-resolution march; floor(W / 2): the half-resolution
-// one), luminance and transmittance RGBA, rw*rh*4 each.
-void or_march_rt(const or_inputs* in, int rw, int rh, double* lum, double* trn, int nthreads) {
+// one), luminance and transmittance RGBA, rw*rh*4 each; `d` (or null) holds
+// the differentiation extensions and overrides as in or_render_ext.
+void or_march_rt(const or_inputs* in, const or_diff* d, int rw, int rh, double* lum, double* trn, int nthreads) {
     Scene<double> s = make_scene<double>(in);
+    std::vector<int> track, slot;
+    apply_diff(s, d, track, slot);
     s.rt_w = rw;
     s.rt_h = rh;
     if (nthreads > 0) omp_set_num_threads(nthreads);

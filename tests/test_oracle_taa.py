@@ -172,3 +172,18 @@ def test_rt_march_and_composite_reproduce_the_renders():
     l_h, t_h = fr.march_rt(W // 2, H // 2)
     c3, l3, t3 = fr.composite_rt(l_h, t_h, upsample=True)
     assert np.array_equal(c3, ch) and np.array_equal(l3, lh) and np.array_equal(t3, th)
+
+
+def test_rt_march_with_extensions_reproduces_render_ext():
+    W, H = 64, 36
+    w = scene.workload_adjoint(width=W, height=H)
+    fr = oracle.Frame(w, *_smooth_noise())
+    tgt = scene.adjoint_target(w.layers)
+    diff = oracle.Diff(2, density_scale=tgt.density_scale, extinction_scale=tgt.extinction_scale,
+                       coverage=tgt.coverage, delta=tgt.delta)
+    comp, lum, trn, _ = fr.render_ext(diff, 0, 0, W, H)
+    c2, l2, t2 = fr.composite_rt(*fr.march_rt(W, H, diff), upsample=False)
+    assert np.abs(c2[..., :3] - comp[..., :3]).max() <= 1e-12 * np.abs(comp).max()
+    assert np.abs(t2[..., :3] - trn[..., :3]).max() <= 1e-12
+    plain, _, _, _ = fr.render_region(0, 0, W, H)
+    assert np.abs(plain[..., :3] - comp[..., :3]).max() > 1e-3 * np.abs(comp).max()
