@@ -190,6 +190,11 @@ ps_status enqueue(ps_batch* b) {
 ps_status launch_once(ps_batch* b) {
     ps_status s;
     if ((s = check_shared_inputs(b->c))) return s;
+    // a batch renders full-resolution single frames: the context's
+    // half-resolution and TAA modes, switched on after ps_batch_create, are
+    // refused rather than ignored
+    if (b->c->half_res || b->c->taa)
+        return fail(PS_ERR_UNSUPPORTED, "batch launch: switch half resolution and TAA off first");
     if (b->exec) {
         PS_HIP(hipGraphLaunch(b->exec, b->c->stream));
     } else if ((s = enqueue(b))) {

@@ -228,6 +228,18 @@ def test_taa_rejects_single_frame_calls(renderer):
         with pytest.raises(PedalskyError) as ei:
             Batch(r, [w])
         assert ei.value.status == abi.PS_ERR_UNSUPPORTED
+        r.set_taa(False)
+        b = Batch(r, [w])
+        for mode in ("taa", "half_res"):
+            getattr(r, f"set_{mode}")(True)
+            with pytest.raises(PedalskyError) as ei:
+                b.run()
+            assert ei.value.status == abi.PS_ERR_UNSUPPORTED
+            getattr(r, f"set_{mode}")(False)
+        b.run()
+        r.synchronize()
+        b.close()
+        r.set_taa(True)
         c = r.count_work()          # counts the march, which TAA leaves alone
         assert [getattr(c, f) for f, _ in abi.WorkCounters._fields_] == \
             [getattr(base, f) for f, _ in abi.WorkCounters._fields_]
