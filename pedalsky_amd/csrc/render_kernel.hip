@@ -480,7 +480,9 @@ template <int FLAGS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_render(
     const RenderParams* __restrict__ Pp) {
     __shared__ WaveQueue Qs;
-    render_tile<FLAGS, 8>(*Pp, blockIdx.x, blockIdx.y, Qs);
+    int tx = blockIdx.x, ty = blockIdx.y;
+    xcd_grouped_tile(tx, ty);
+    render_tile<FLAGS, 8>(*Pp, tx, ty, Qs);
 }
 
 // Tiles of many scenes in one launch. Each entry of `tiles` is
