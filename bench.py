@@ -38,7 +38,7 @@ def cpu_baseline(w, base, detail, curl, budget_s: float, threads: int) -> dict:
         oracle.render_f32_frame(w, base, detail, curl, nthreads=threads)
         frames += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or frames >= 8:
+        if el >= budget_s or frames >= 64:
             break
     mpix = frames * w.width * w.height / el / 1e6
     return {"value": round(mpix, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
