@@ -3,7 +3,7 @@
 Bars (relative radiance error = max over RGB of |gpu - oracle| /
 max(|oracle|, 1e-4 * max frame radiance)):
   * LUTs: every texel within 2e-5 (FP64 compute, FP32 storage);
-  * rendered frames: 99.9th percentile <= 1e-3, median
+  * rendered frames: relative radiance error 99.9th percentile <= 1e-3, median
     <= 1e-5;
   * noise bakes: integer hash exact; stored texels within one storage step.
 """
