@@ -196,7 +196,9 @@ typedef struct ps_work_counters {
 } ps_work_counters;
 
 /* Per-kernel device times of the last timed frame sequence, averaged per
- * frame, in milliseconds (HIP events on the context stream). */
+ * frame, in milliseconds (HIP events on the context stream). When a frame
+ * bakes both the sky-view and the aerial-perspective LUTs they run in one
+ * launch: skyview_ms then covers both and aerial_perspective_ms is ~0. */
 typedef struct ps_frame_timing {
     float transmittance_ms;
     float multiscatter_ms;

@@ -340,8 +340,8 @@ __device__ __forceinline__ void inscatter_segment(const LutParams& P, const floa
 // ---- skyview.comp.wgsl:46-174: 32 steps on kSvSeg lanes per texel ----------------
 constexpr int kSvSeg = 8;
 __device__ __forceinline__ void skyview_body(const LutParams& P, const float4* __restrict__ T,
-                                             const float4* __restrict__ MS, float4* __restrict__ out) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+                                             const float4* __restrict__ MS, float4* __restrict__ out, int block) {
+    const int gid = block * (int)blockDim.x + (int)threadIdx.x;
     const int i = gid / kSvSeg, seg = gid % kSvSeg;
     const bool valid = i < P.sw * P.sh;
     const int ic = valid ? i : 0;
@@ -399,8 +399,8 @@ __device__ __forceinline__ void skyview_body(const LutParams& P, const float4* _
 // ---- aerialperspective.comp.wgsl:41-127: 16 steps on kApSeg lanes per froxel ------
 constexpr int kApSeg = 4;
 __device__ __forceinline__ void aerial_body(const LutParams& P, const float4* __restrict__ T,
-                                            const float4* __restrict__ MS, float4* __restrict__ out) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+                                            const float4* __restrict__ MS, float4* __restrict__ out, int block) {
+    const int gid = block * (int)blockDim.x + (int)threadIdx.x;
     const int i = gid / kApSeg, seg = gid % kApSeg;
     const bool valid = i < P.aw * P.ah * P.ad;
     const int ic = valid ? i : 0;
@@ -459,14 +459,28 @@ __global__ __launch_bounds__(64) void k_skyview(const LutParams* __restrict__ Ps
                                                 const float4* __restrict__ MS, float4* __restrict__ SV) {
     const LutParams& P = Ps[blockIdx.y];
     const size_t s = blockIdx.y;
-    skyview_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, SV + s * P.sw * P.sh);
+    skyview_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, SV + s * P.sw * P.sh, blockIdx.x);
 }
 
 __global__ __launch_bounds__(64) void k_aerial(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
                                                const float4* __restrict__ MS, float4* __restrict__ AP) {
     const LutParams& P = Ps[blockIdx.y];
     const size_t s = blockIdx.y;
-    aerial_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, AP + s * P.aw * P.ah * P.ad);
+    aerial_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, AP + s * P.aw * P.ah * P.ad, blockIdx.x);
+}
+
+// Sky-view and aerial-perspective LUTs in one launch (both read only the
+// transmittance and multi-scattering LUTs): blocks [0, nsv) build sky-view
+// texels, the rest AP froxels, so the two short, latency-bound passes
+// overlap instead of running back to back.
+__global__ __launch_bounds__(64) void k_skyview_aerial(const LutParams* __restrict__ Ps, const float4* __restrict__ T,
+                                                       const float4* __restrict__ MS, float4* __restrict__ SV,
+                                                       float4* __restrict__ AP, int nsv) {
+    const LutParams& P = Ps[blockIdx.y];
+    const size_t s = blockIdx.y;
+    const int b = (int)blockIdx.x;
+    if (b < nsv) skyview_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, SV + s * P.sw * P.sh, b);
+    else aerial_body(P, T + s * P.tw * P.th, MS + s * P.mw * P.mh, AP + s * P.aw * P.ah * P.ad, b - nsv);
 }
 
 }  // namespace
@@ -482,12 +496,16 @@ hipError_t ps_launch_luts(const LutParams* d_params, const LutParams& P, int n, 
     if (flags & 2u)
         hipLaunchKernelGGL(k_multiscatter, dim3(P.mw * P.mh, ns), dim3(64 * kMsSeg), 0, s, d_params, T, MS);
     if (ev) hipEventRecord(ev[2], s);
-    if (flags & 4u)
-        hipLaunchKernelGGL(k_skyview, dim3((P.sw * P.sh * kSvSeg + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS, SV);
-    if (ev) hipEventRecord(ev[3], s);
-    if (flags & 8u)
-        hipLaunchKernelGGL(k_aerial, dim3((P.aw * P.ah * P.ad * kApSeg + 63) / 64, ns), dim3(64), 0, s, d_params, T, MS,
-                           AP);
+    const int nsv = (P.sw * P.sh * kSvSeg + 63) / 64, nap = (P.aw * P.ah * P.ad * kApSeg + 63) / 64;
+    if ((flags & 12u) == 12u) {
+        // one launch for both (the sky-view time then covers both)
+        hipLaunchKernelGGL(k_skyview_aerial, dim3(nsv + nap, ns), dim3(64), 0, s, d_params, T, MS, SV, AP, nsv);
+        if (ev) hipEventRecord(ev[3], s);
+    } else {
+        if (flags & 4u) hipLaunchKernelGGL(k_skyview, dim3(nsv, ns), dim3(64), 0, s, d_params, T, MS, SV);
+        if (ev) hipEventRecord(ev[3], s);
+        if (flags & 8u) hipLaunchKernelGGL(k_aerial, dim3(nap, ns), dim3(64), 0, s, d_params, T, MS, AP);
+    }
     if (ev) hipEventRecord(ev[4], s);
     return hipGetLastError();
 }
