@@ -13,27 +13,36 @@ oracle/Makefile) with numpy-friendly functions. The C++ sources restate:
 
 The reference (Rust + wgpu + WGSL) cannot be built or run here and holds no
 golden vectors for this path, so the restatement is pinned by checks in
-tests/test_oracle.py and tests/test_abi.py:
+tests/test_oracle.py, tests/test_oracle_pins.py and tests/test_abi.py:
   optical depth / transmittance LUT   closed form vertical OD, h^2 convergence,
                                       T texel = exp(-OD), uv round trip
   phase functions                     normalisation (HG, Draine, Rayleigh, mix),
                                       Draine -> HG / Rayleigh reductions,
                                       published Jendersie-d'Eon fit values
-  multi-scatter / sky-view / AP       MS = 0 without scattering, sky-view uv
-                                      inverse + azimuthal symmetry + blue
-                                      dominance, AP monotone in depth
+  multi-scatter / sky-view / AP       texel values under constant density
+                                      (F / (1 - F); telescoped in-scatter),
+                                      sky-view uv inverse + azimuthal symmetry
+                                      + blue dominance, AP monotone in depth
   samplers                            scipy.ndimage.map_coordinates(order=1)
   march + integration                 Beer-Lambert constant column, emission-only
-                                      telescoping, coverage/density 0 identity,
-                                      depth clip, NDF shape properties
-  noise hash                          the integer mix (exact)
+                                      telescoping, sun in-scatter step sum and
+                                      continuous limit over a constant-density
+                                      column (light march, Beer-Powder, octaves,
+                                      ambient, AP), LUT-sampled sun transmittance
+                                      and sky ambient, coverage/density 0
+                                      identity, depth clip, NDF shape properties
+  sky pass                            sun disk integrates to 0.8 E
+  noise                               the integer mix (exact), Perlin lattice
+                                      values and gradients, brute-force Worley,
+                                      FBM channel assembly, divergence-free curl
   host synthesis                      coverage remap / type defaults / blue-noise
-                                      histogram / wind profile pins
-Parity unpinned (checked only against the restatement itself): the FBM
-arithmetic of the noise bakes beyond the hash and the tiling/range checks,
-the sun-disk limb darkening of the sky pass, the cone-kernel offsets of the
-cone light march, and the reference hardware sampler's filter-weight and
-f16/UNORM8 rounding.
+                                      placement, histogram and spectrum / wind
+                                      profile pins
+Every pin of tests/test_oracle_pins.py also fails on a planted slip of the
+function it covers (oracle builds with -DORACLE_SLIP=k).
+Parity unpinned (checked only against the restatement itself): the cone-kernel
+offsets of the cone light march, and the reference hardware sampler's
+filter-weight and f16/UNORM8 rounding.
 """
 from __future__ import annotations
 
@@ -90,7 +99,14 @@ def load():
     if not LIB_PATH.exists():
         import subprocess
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
-    lib = C.CDLL(str(LIB_PATH))
+    _lib = open_library(LIB_PATH)
+    return _lib
+
+
+def open_library(path):
+    """Load an oracle build (the checker, or a planted-slip build of the pin
+    tests) and declare its signatures."""
+    lib = C.CDLL(str(path))
     d, i, P = C.c_double, C.c_int, C.c_void_p
     sigs = {
         "or_bake_transmittance": (None, [P, i, i, P, i]),
@@ -127,12 +143,16 @@ def load():
         "or_sample3d_repeat": (None, [P, i, i, i, i, d, d, d, P]),
         "or_sample2d": (None, [P, i, i, i, d, d, i, P]),
         "or_hash3": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "or_perlin_tile": (d, [d, d, d, C.c_uint32, C.c_uint32]),
+        "or_worley_tile": (d, [d, d, d, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "or_perlin2": (d, [d, d, C.c_uint32, C.c_uint32]),
+        "or_sun_transmittance_at": (None, [P, P, i, P]),
+        "or_sky_ambient_at": (None, [P, P, i, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 

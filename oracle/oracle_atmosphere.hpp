@@ -283,7 +283,7 @@ void bake_multiscatter_texel(const Atm<R>& a, const Tex<R, S>& T, R u, R v, R ou
     V3<R> L2 = l2 * norm * (R(1) / (R(4) * R(kPI)));
     V3<R> F = fms * norm * (R(1) / (R(4) * R(kPI)));
     V3<R> fc(clampr<R>(F.x, R(0), R(0.95)), clampr<R>(F.y, R(0), R(0.95)), clampr<R>(F.z, R(0), R(0.95)));
-    V3<R> full = L2 / (V3<R>(R(1)) - fc);
+    V3<R> full = ORACLE_SLIP == 4 ? L2 : L2 / (V3<R>(R(1)) - fc);
     out[0] = full.x; out[1] = full.y; out[2] = full.z; out[3] = R(1);
 }
 
@@ -348,7 +348,7 @@ void bake_skyview_texel(const Frame<R>& f, const Atm<R>& a, const Tex<R, S>& T, 
         R sun_cos = clampr<R>(dot(pi / std::max(r_p, R(1)), sun_dir), R(-1), R(1));
         R ms[4];
         sample2d(MS, sun_cos * R(0.5) + R(0.5), h_norm, false, ms);
-        V3<R> multi = (sr + sm) * V3<R>(ms[0], ms[1], ms[2]);
+        V3<R> multi = (ORACLE_SLIP == 11 ? sigma_t : sr + sm) * V3<R>(ms[0], ms[1], ms[2]);
         V3<R> in_scatter = sun_in + multi;
         V3<R> safe = vmax(sigma_t, V3<R>(R(1e-7)));
         V3<R> s_int = (in_scatter - in_scatter * sample_transmit) / safe;
@@ -385,7 +385,7 @@ void bake_ap_froxel(const Frame<R>& f, const Atm<R>& a, const Tex<R, S>& T, cons
     const int AP_STEPS = 16;
     R u = (R(gx) + R(0.5)) / R(W), v = (R(gy) + R(0.5)) / R(H);
     R ndc_x = u * R(2) - R(1), ndc_y = v * R(2) - R(1);
-    R hv[4] = {ndc_x, -ndc_y, R(1), R(1)}, nh[4];
+    R hv[4] = {ndc_x, ORACLE_SLIP == 12 ? ndc_y : -ndc_y, R(1), R(1)}, nh[4];
     mat4_mul(f.inv_vp, hv, nh);
     V3<R> near_p(nh[0] / nh[3], nh[1] / nh[3], nh[2] / nh[3]);
     V3<R> view_dir = normalize(near_p - f.cam);
@@ -420,7 +420,7 @@ void bake_ap_froxel(const Frame<R>& f, const Atm<R>& a, const Tex<R, S>& T, cons
         R ms[4];
         sample2d(MS, sun_cos * R(0.5) + R(0.5), h_norm, false, ms);
         V3<R> multi = (sr + sm) * V3<R>(ms[0], ms[1], ms[2]);
-        V3<R> in_scatter = (sun_in + multi) * f.E;
+        V3<R> in_scatter = ORACLE_SLIP == 5 ? sun_in + multi : (sun_in + multi) * f.E;
         V3<R> safe = vmax(sigma_t, V3<R>(R(1e-7)));
         V3<R> s_int = (in_scatter - in_scatter * sample_transmit) / safe;
         lum = lum + trans * s_int;

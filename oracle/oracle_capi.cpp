@@ -360,6 +360,25 @@ void or_composite_rt(const or_inputs* in, int rw, int rh, const double* rt_lum, 
     }
 }
 
+// The two LUT-sampled light terms of a dense cloud sample at world position
+// p (n points, p and the outputs n*3): the sun transmittance
+// (cloud_march.wgsl:328-354) and the sky ambient before its strength and
+// height gradient (cloud_march.wgsl:693-700).
+void or_sun_transmittance_at(const or_inputs* in, const double* p, int n, double* out) {
+    Scene<double> s = make_scene<double>(in);
+    for (int i = 0; i < n; ++i) {
+        V3<double> v = sun_transmittance_at(s, V3<double>(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
+void or_sky_ambient_at(const or_inputs* in, const double* p, int n, double* out) {
+    Scene<double> s = make_scene<double>(in);
+    for (int i = 0; i < n; ++i) {
+        V3<double> v = sample_sky_ambient(s, V3<double>(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
+
 // Renders an arbitrary set of n pixels given as (x, y) pairs; outputs n*4.
 void or_render_pixels(const or_inputs* in, const int32_t* xy, int n, double* comp, double* lum, double* trn,
                       int nthreads) {
@@ -575,5 +594,15 @@ void or_sample2d(const float* data, int w, int h, int ch, double u, double v, in
     for (int c = 0; c < ch && c < 4; ++c) out[c] = o[c];
 }
 uint32_t or_hash3(uint32_t x, uint32_t y, uint32_t z, uint32_t seed) { return hash3(x, y, z, seed); }
+// The noise primitives at an arbitrary point (for the lattice and
+// brute-force checks of the bakes).
+double or_perlin_tile(double x, double y, double z, uint32_t freq, uint32_t seed) {
+    return perlin_tile<double>(V3<double>(x, y, z), freq, seed);
+}
+double or_worley_tile(double x, double y, double z, uint32_t freq, uint32_t ox, uint32_t oy, uint32_t oz,
+                      uint32_t seed) {
+    return worley_tile<double>(V3<double>(x, y, z), freq, ox, oy, oz, seed);
+}
+double or_perlin2(double x, double y, uint32_t freq, uint32_t seed) { return perlin2<double>(x, y, freq, seed); }
 
 }  // extern "C"

@@ -362,7 +362,7 @@ inline R march_to_light(const Scene<R>& s, const V3<R>& p, R p_alt, const V3<R>&
     }
     R step = dist_to_top / R(s.c.light_steps);
     for (uint32_t i = 0; i < s.c.light_steps; ++i) {
-        R t = (R(i) + R(0.5)) * step;
+        R t = (R(i) + R(ORACLE_SLIP == 3 ? 1.0 : 0.5)) * step;
         V3<R> pi = p + sun * t;
         R alt = altitude_from_entry(s, pi, r0, cos_view, h0, t);
         if (cnt) cnt->light_steps++;
@@ -457,7 +457,8 @@ inline V3<R> sun_transmittance_at(const Scene<R>& s, const V3<R>& p) {
     R xm = clampr<R>((d - d_min) / std::max(d_max - d_min, R(1)), 0, 1);
     R xr = rho / std::max(big_h, R(1));
     R t[4];
-    sample2d(s.T, xm, xr, false, t);
+    if (ORACLE_SLIP == 15) sample2d(s.T, xr, xm, false, t);
+    else sample2d(s.T, xm, xr, false, t);
     return V3<R>(t[0], t[1], t[2]);
 }
 
@@ -485,7 +486,7 @@ inline void skyview_uv(const Atm<R>& a, const V3<R>& view_dir, const V3<R>& sun,
 template <class R>
 inline V3<R> sample_sky_ambient(const Scene<R>& s, const V3<R>& p) {
     V3<R> c(0, -s.a.planet_r, 0);
-    V3<R> up = normalize(p - c);
+    V3<R> up = normalize(ORACLE_SLIP == 16 ? p : p - c);
     R cam_r = length(p - c);
     R u, v;
     skyview_uv(s.a, up, s.f.sun, cam_r, u, v);
@@ -588,7 +589,7 @@ inline V3<R> sky_pass_pixel(const Scene<R>& s, R ndc_x, R ndc_y) {
     if (cvs > cos_disk) {
         R theta = acos(clampr<R>(cvs, -1, 1));
         R tn = theta / std::max(s.f.sun_ang, R(1e-6));
-        R limb = R(1) - R(0.6) * (R(1) - sqrt(std::max(R(1) - tn * tn, R(0))));
+        R limb = R(1) - R(ORACLE_SLIP == 10 ? 0.4 : 0.6) * (R(1) - sqrt(std::max(R(1) - tn * tn, R(0))));
         R omega = R(kPI) * s.f.sun_ang * s.f.sun_ang;
         V3<R> ld = s.f.E * limb / std::max(omega, R(1e-6));
         V3<R> svis = sample_transmittance_lut(s.a, s.T, cam_atm, sun);
@@ -692,9 +693,9 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
                 if (cnt) cnt->dense_samples++;
                 R od = march_to_light(s, p, alt, sun, L, wsample[0], wind, ctype, cnt);
                 V3<R> beer = vexp(-sigma_t * od);
-                V3<R> powder = V3<R>(1) - vexp(sigma_t * od * R(-2));
+                V3<R> powder = V3<R>(1) - vexp(sigma_t * od * R(ORACLE_SLIP == 2 ? -1 : -2));
                 V3<R> bp = beer * powder * R(2);
-                V3<R> energy = mixv(beer, bp, s.c.powder);
+                V3<R> energy = ORACLE_SLIP == 14 ? mixv(bp, beer, s.c.powder) : mixv(beer, bp, s.c.powder);
                 R phase_h = clampr<R>((alt - L.base) / std::max(L.top - L.base, R(1)), 0, 1);
                 PhaseConsts<R> pc = cloud_phase_constants(s, L, phase_h);
                 V3<R> sun_tr = sun_transmittance_at(s, p);
@@ -703,12 +704,12 @@ inline void cloud_march_pixel(const Scene<R>& s, int ix, int iy, V3<R>& out_lum,
                 for (uint32_t n = 0; n < s.c.octaves; ++n) {
                     if (cnt) cnt->octaves++;
                     R ph = cloud_phase_with(cos_theta, cc, pc);
-                    sun_in = sun_in + s.f.E * sun_tr * (a * ph) * vexp(-sigma_t * od * b);
+                    sun_in = sun_in + s.f.E * sun_tr * (a * ph) * vexp(-sigma_t * od * ((ORACLE_SLIP == 1 && n > 0) ? s.c.ms_b : b));
                     a = a * s.c.ms_a; b = b * s.c.ms_b; cc = cc * s.c.ms_c;
                 }
                 if (s.c.forward_bias > R(0)) sun_in = sun_in * (R(1) + s.c.forward_bias * fb_dir);
                 R h_norm = clampr<R>((alt - L.base) / std::max(L.top - L.base, R(1)), 0, 1);
-                V3<R> amb = sample_sky_ambient(s, p) * s.c.ambient * mixr(R(0.3), R(1), h_norm);
+                V3<R> amb = sample_sky_ambient(s, p) * s.c.ambient * (ORACLE_SLIP == 6 ? mixr(R(1), R(0.3), h_norm) : mixr(R(0.3), R(1), h_norm));
                 V3<R> lightning(0);
                 R lr = s.f.light_radius;
                 if (lr > R(0)) {

@@ -16,6 +16,15 @@
 #include <cstddef>
 #include <algorithm>
 
+// Planted mistakes for the mutation checks of the oracle's pins
+// (tests/test_oracle_pins.py builds this library with -DORACLE_SLIP=k, k > 0,
+// and requires the pin of the function that carries slip k to fail). The
+// default 0 is the restatement itself; every ORACLE_SLIP test below is a
+// compile-time constant that is false in the checker build.
+#ifndef ORACLE_SLIP
+#define ORACLE_SLIP 0
+#endif
+
 namespace oracle {
 
 // Math calls are unqualified so that the forward-mode dual numbers of

@@ -8,7 +8,7 @@
 
 namespace oracle {
 
-// noise_base.comp.wgsl:25-30 (hash3), noise_curl.comp.wgsl:223-228 (hash2)
+// noise_base.comp.wgsl:25-30 (hash3), noise_curl.comp.wgsl:12-17 (hash2)
 inline uint32_t hash3(uint32_t x0, uint32_t y0, uint32_t z0, uint32_t seed) {
     uint32_t x = x0 * 0x27d4eb2du + y0 * 0x165667b1u + z0 * 0x9e3779b9u + seed * 0x85ebca6bu;
     x = (x ^ (x >> 15u)) * 0x2c1b3c6du;
@@ -31,7 +31,7 @@ inline V3<R> rand3(uint32_t x, uint32_t y, uint32_t z, uint32_t seed) {
 
 inline int wrap_freq(int c, int f) { return ((c % f) + f) % f; }
 
-// noise_base.comp.wgsl:45-68 and noise_detail.comp.wgsl:170-192 (hash offset)
+// noise_base.comp.wgsl:45-68 and noise_detail.comp.wgsl:31-53 (hash offset added after the wrap)
 template <class R>
 inline R worley_tile(const V3<R>& p, uint32_t freq, uint32_t ox, uint32_t oy, uint32_t oz, uint32_t seed) {
     R f = R(freq);
@@ -40,13 +40,17 @@ inline R worley_tile(const V3<R>& p, uint32_t freq, uint32_t ox, uint32_t oy, ui
     int cx = int(fl.x), cy = int(fl.y), cz = int(fl.z);
     V3<R> fr = sc - fl;
     R min_d = R(1);
-    for (int dz = -1; dz <= 1; ++dz)
+    for (int dz = -1; dz <= (ORACLE_SLIP == 8 ? 0 : 1); ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
                 uint32_t wx = uint32_t(wrap_freq(cx + dx, int(freq)));
                 uint32_t wy = uint32_t(wrap_freq(cy + dy, int(freq)));
                 uint32_t wz = uint32_t(wrap_freq(cz + dz, int(freq)));
-                V3<R> j = rand3<R>(wx + ox, wy + oy, wz + oz, seed);
+                V3<R> j = ORACLE_SLIP == 9
+                              ? rand3<R>(uint32_t(wrap_freq(cx + dx + int(ox), int(freq))),
+                                         uint32_t(wrap_freq(cy + dy + int(oy), int(freq))),
+                                         uint32_t(wrap_freq(cz + dz + int(oz), int(freq))), seed)
+                              : rand3<R>(wx + ox, wy + oy, wz + oz, seed);
                 V3<R> cc = V3<R>(R(dx), R(dy), R(dz)) + j;
                 R d = length(cc - fr);
                 min_d = std::min(min_d, d);
@@ -85,7 +89,7 @@ inline R perlin_tile(const V3<R>& p, uint32_t freq, uint32_t seed) {
             for (int ix = 0; ix < 2; ++ix) {
                 V3<R> g = perlin_grad_tile<R>(cx + ix, cy + iy, cz + iz, freq, seed);
                 V3<R> d = f - V3<R>(R(ix), R(iy), R(iz));
-                dots[ix + iy * 2 + iz * 4] = dot(g, d);
+                dots[ORACLE_SLIP == 7 ? iy + ix * 2 + iz * 4 : ix + iy * 2 + iz * 4] = dot(g, d);
             }
     R x00 = mixr(dots[0], dots[1], u.x), x10 = mixr(dots[2], dots[3], u.x);
     R x01 = mixr(dots[4], dots[5], u.x), x11 = mixr(dots[6], dots[7], u.x);
@@ -108,12 +112,12 @@ inline void base_texel(int x, int y, int z, uint32_t seed, R out[4]) {
     // remap(perlin, wl - 1, 1, 0, 1) with the noise shader's own remap (:21-23)
     R pw = R(0) + (perlin - (wl - R(1))) * (R(1) - R(0)) / std::max(R(1) - (wl - R(1)), R(1e-5));
     out[0] = clampr<R>(pw, R(0), R(1));
-    out[1] = worley_fbm(p, 2u, 0, 0, 0, seed);
+    out[1] = worley_fbm(p, ORACLE_SLIP == 17 ? 4u : 2u, 0, 0, 0, seed);
     out[2] = worley_fbm(p, 8u, 0, 0, 0, seed);
     out[3] = worley_fbm(p, 14u, 0, 0, 0, seed);
 }
 
-// noise_detail.comp.wgsl:201-211
+// noise_detail.comp.wgsl:62-72
 template <class R>
 inline void detail_texel(int x, int y, int z, uint32_t seed, R out[4]) {
     const R N = R(32);
@@ -124,7 +128,7 @@ inline void detail_texel(int x, int y, int z, uint32_t seed, R out[4]) {
     out[3] = R(0);
 }
 
-// noise_curl.comp.wgsl:230-285
+// noise_curl.comp.wgsl:19-74
 template <class R>
 inline void rand2(uint32_t x, uint32_t y, uint32_t seed, R& a, R& b) {
     uint32_t h0 = hash2(x, y, seed), h1 = hash2(x + 1u, y, seed);
@@ -169,6 +173,7 @@ inline void curl_texel(int x, int y, uint32_t seed, R out[2]) {
     R eps = R(1) / N;
     R dpdx = (potential(px + eps, py, seed) - potential(px - eps, py, seed)) / (R(2) * eps);
     R dpdy = (potential(px, py + eps, seed) - potential(px, py - eps, seed)) / (R(2) * eps);
+    if (ORACLE_SLIP == 13) std::swap(dpdx, dpdy);
     out[0] = clampr<R>(dpdy * R(0.5) + R(0.5), R(0), R(1));
     out[1] = clampr<R>(-dpdx * R(0.5) + R(0.5), R(0), R(1));
 }
