@@ -3,9 +3,13 @@
 One step = one full frame at 1280x720: the four LUT bakes (transmittance,
 multi-scattering, sky-view, 32^3 aerial perspective), then the fused sky
 pass + two-layer cloud march (cumulus 1500-2300 m, cirrus 8000-9000 m;
-128 primary / 6 light steps; FP16 noise) + AP + composite. Consecutive steps
-walk the seeded 16-camera sweep (pitch U[-10, 30] deg, yaw U[0, 360) deg).
-Inputs are resident in HBM before the timed region.
+128 primary / 6 light steps; FP16 noise) + AP + composite, writing the
+composited radiance and the cloud transmittance. Consecutive steps walk the
+seeded 16-camera sweep (pitch U[-10, 30] deg, yaw U[0, 360) deg). The depth
+buffer is all far (no ground: rays below the horizon march the layer shells
+on the far side of the planet); --depth ground times the same frames with
+the planet-ground depth buffer instead. Inputs are resident in HBM before
+the timed region.
 
 Prints one JSON line (see the round contract): value = Mpix/s, roofline of
 the dominant kernel (the render), and a CPU baseline (the oracle's float
@@ -44,7 +48,25 @@ def cpu_baseline(w, base, detail, curl, budget_s: float, threads: int) -> dict:
     return {"value": round(mpix, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
             "sample": f"{frames} whole {w.width}x{w.height} frame(s) of sweep camera 0 (LUT chain + sky + "
                       f"2-layer march + AP + composite), float instantiation of the CPU restatement, "
-                      f"{el:.1f} s"}
+                      f"{el:.1f} s",
+            "host": host_cores()}
+
+
+def host_cores() -> dict:
+    """The host's core counts: what the OS reports, what this process may run
+    on (its affinity mask) and the OpenMP thread limit of the environment."""
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def baseline_threads() -> int:
+    """Every core this process may use, capped by an OpenMP thread limit the
+    environment sets (the GPU box fixes its per-GPU CPU share that way)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(n, 1)
 
 
 def bench_batch(r, n_scenes: int) -> dict:
@@ -231,6 +253,9 @@ def main() -> int:
     ap.add_argument("--calibration-sets", type=int, default=256)
     ap.add_argument("--calibration-iters", type=int, default=200)
     ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--depth", choices=("none", "ground"), default="none",
+                    help="depth buffer of the headline frames: none (all far, the benchmark configuration) or "
+                         "the planet ground")
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -251,22 +276,25 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     W, H = args.width, args.height
-    sweep = [scene.workload_full_frame(i, width=W, height=H) for i in range(16)]
+    sweep = [scene.workload_full_frame(i, width=W, height=H, ground=True) for i in range(16)]
     w0 = sweep[0]
     r = Renderer(0)
     r.load_workload(w0)
-    depths = []
-    for w in sweep:
-        depths.append(torch.from_numpy(w.depth).to("cuda:0"))
-    bufs = r.alloc_outputs(W, H, abi.PS_PIXEL_F32, ("composite",))
+    ground_depths = [torch.from_numpy(w.depth).to("cuda:0") for w in sweep]
+    depths = [None] * 16 if args.depth == "none" else ground_depths
+    bufs = r.alloc_outputs(W, H, abi.PS_PIXEL_F32, ("composite", "cloud_transmittance"))
     out = r.image_out(bufs, abi.PS_PIXEL_F32)
     lib = r.lib
     import ctypes as C
 
-    def frame(k: int) -> None:
+    def dptr(d):
+        return None if d is None else C.c_void_p(d.data_ptr())
+
+    def frame(k: int, ds=None) -> None:
         w = sweep[k % 16]
+        d = (depths if ds is None else ds)[k % 16]
         abi.check(lib.ps_set_frame(r.ctx, C.byref(w.frame)))
-        abi.check(lib.ps_frame(r.ctx, abi.PS_LUT_ALL, C.c_void_p(depths[k % 16].data_ptr()), C.byref(out)))
+        abi.check(lib.ps_frame(r.ctx, abi.PS_LUT_ALL, dptr(d), C.byref(out)))
 
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
@@ -305,7 +333,7 @@ def main() -> int:
         abi.check(lib.ps_set_frame(r.ctx, C.byref(sweep[i].frame)))
         abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
         cw = abi.WorkCounters()
-        abi.check(lib.ps_count_work(r.ctx, C.c_void_p(depths[i].data_ptr()), C.byref(cw)))
+        abi.check(lib.ps_count_work(r.ctx, dptr(depths[i]), C.byref(cw)))
         per_cam[i] = cw
     for i in used:
         cw = per_cam[i]
@@ -316,7 +344,7 @@ def main() -> int:
     flops_per_launch = flops / len(used)
     bytes_per_launch = gbytes / len(used)
     render_s = timing.render_ms * 1e-3
-    hbm_bytes = cost_model.render_hbm_bytes(W, H, out_bytes_per_px=16)
+    hbm_bytes = cost_model.render_hbm_bytes(W, H, out_bytes_per_px=32, depth=args.depth != "none")
     achieved_tflops = flops_per_launch / render_s / 1e12
     achieved_gbps = bytes_per_launch / render_s / 1e9
 
@@ -334,14 +362,17 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded noise bake, PCG32 weather map, 16-camera seeded sweep)",
         "config": {"workload": f"forward frame {W}x{H}: LUT chain + sky + cumulus/cirrus march "
-                               f"128/6 steps + AP 32^3 + composite, FP16 noise",
+                               f"128/6 steps + AP 32^3 + composite, FP16 noise, "
+                               f"{'depth all far (no ground)' if args.depth == 'none' else 'planet-ground depth'}",
                    "width": W, "height": H, "layers": 2, "primary_steps": 128, "light_steps": 6,
-                   "lut": "T 256x64, MS 32x32, SV 192x108, AP 32^3 (FP32)", "variant": "fwd"},
+                   "lut": "T 256x64, MS 32x32, SV 192x108, AP 32^3 (FP32)", "variant": "fwd",
+                   "depth": "all far" if args.depth == "none" else "planet ground",
+                   "outputs": "composite RGBA32F + cloud transmittance RGBA32F"},
         "kernel_ms": {"transmittance": round(timing.transmittance_ms, 5),
                       "multiscatter": round(timing.multiscatter_ms, 5),
-                      "skyview": round(timing.skyview_ms, 5),
-                      "aerial_perspective": round(timing.aerial_perspective_ms, 5),
-                      "render": round(timing.render_ms, 5), "frame_events": round(timing.frame_ms, 5)},
+                      "skyview_aerial": round(timing.skyview_ms + timing.aerial_perspective_ms, 5),
+                      "render": round(timing.render_ms, 5), "frame_events": round(timing.frame_ms, 5),
+                      "note": "sky-view and aerial-perspective LUTs are baked by one fused launch"},
         "roofline": {"bound": "valu", "kernel": "k_render", "achieved": round(achieved_tflops, 3),
                      "peak": cost_model.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / cost_model.PEAK_FP32_VALU_TFLOPS, 4),
@@ -354,6 +385,20 @@ def main() -> int:
                      "hbm_GBps": round(hbm_bytes / render_s / 1e9, 1)},
         "work_per_frame": {k: v / len(used) for k, v in cnt_sum.items()},
     }
+    if args.depth == "none":
+        log("the same frames with the planet-ground depth buffer")
+        for k in range(args.warmup):
+            frame(k, ground_depths)
+        r.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            frame(k, ground_depths)
+        r.synchronize()
+        gs = (time.perf_counter() - t0) / args.steps
+        result["ground_depth"] = {"ms_per_frame": round(gs * 1e3, 4), "mpix_s": round(W * H / gs / 1e6, 3),
+                                  "workload": "the same frames with the depth buffer of an opaque planet "
+                                              "ground (clips the far-side layer shells)"}
+
     def time_mode(half: bool, taa: bool) -> float:
         r.set_half_res(half)
         r.set_taa(taa)
@@ -405,8 +450,8 @@ def main() -> int:
         base = oracle.decode(r.read_noise(abi.PS_NOISE_BASE, fmt), fmt)
         det = oracle.decode(r.read_noise(abi.PS_NOISE_DETAIL, fmt), fmt)
         curl = oracle.decode(r.read_noise(abi.PS_NOISE_CURL, fmt), fmt)
-        threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(w0, base, det, curl, args.cpu_seconds, threads)
+        wc = scene.workload_full_frame(0, width=W, height=H, ground=args.depth != "none")
+        result["cpu_baseline"] = cpu_baseline(wc, base, det, curl, args.cpu_seconds, baseline_threads())
     r.close()
     print(json.dumps(result), flush=True)
     return 0

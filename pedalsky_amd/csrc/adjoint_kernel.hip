@@ -167,10 +167,11 @@ __device__ __forceinline__ LightWalk light_walk(const RenderParams& P, const Lay
     return w;
 }
 
-// Light sample i: position, altitude and quadrature weight.
-__device__ __forceinline__ void light_sample(const RenderParams& P, const LightWalk& w, int i, float px, float py,
-                                             float pz, float alt, float& qx, float& qyy, float& qz, float& a,
-                                             float& wt) {
+// Light sample i: lookup-frame position (from the sample's lookup position
+// q0), altitude (from its world position p) and quadrature weight.
+__device__ __forceinline__ void light_sample(const RenderParams& P, const LightWalk& w, int i, float qx0, float qy0,
+                                             float qz0, float px, float py, float pz, float alt, float& qx,
+                                             float& qyy, float& qz, float& a, float& wt) {
     const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
     if (P.cone) {
         const float off[6][3] = {{0.38051305f, 0.92453449f, -0.02111345f}, {-0.50625799f, -0.03590792f, -0.86163418f},
@@ -179,19 +180,18 @@ __device__ __forceinline__ void light_sample(const RenderParams& P, const LightW
         float t, scale;
         if (i < 5) { t = (i + 0.5f) * w.step; scale = w.step * (i + 1.f) / 5.f; wt = w.step; }
         else { t = w.dist * 3.f; scale = w.step * 3.f; wt = w.step * 3.f; }
-        qx = px + sx * t + off[i][0] * scale;
-        qyy = py + sy * t + off[i][1] * scale;
-        qz = pz + sz * t + off[i][2] * scale;
-        float yy = qyy + P.R;
-        float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+        const float ox = sx * t + off[i][0] * scale, oy = sy * t + off[i][1] * scale, oz = sz * t + off[i][2] * scale;
+        float wx = px + ox, yy = py + oy + P.R, wz = pz + oz;
+        float rr = fsqrt(wx * wx + yy * yy + wz * wz);
         a = alt + t * (w.k + t) * frcp(w.r0c + rr);
+        qx = qx0 + ox; qyy = qy0 + oy; qz = qz0 + oz;
         return;
     }
     float t = (i + 0.5f) * w.step;
-    qx = px + sx * t; qyy = py + sy * t; qz = pz + sz * t;
-    float yy = qyy + P.R;
-    float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+    float wx = px + sx * t, yy = py + sy * t + P.R, wz = pz + sz * t;
+    float rr = fsqrt(wx * wx + yy * yy + wz * wz);
     a = alt + t * (w.k + t) * frcp(w.r0c + rr);
+    qx = qx0 + sx * t; qyy = qy0 + sy * t; qz = qz0 + sz * t;
     wt = w.step;
 }
 
@@ -200,9 +200,9 @@ __device__ __forceinline__ void light_sample(const RenderParams& P, const LightW
 // light sample i sits inside the clamp's linear range (0 < x < 1), where the
 // perturbation volume receives d od / d delta = density_scale * weight.
 template <bool DELTA>
-__device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& L, const NdfC& nc, float px, float py,
-                                           float pz, float alt, float wr, uint32_t ctype, float& dod_ds,
-                                           float& dod_cov, uint32_t& lin) {
+__device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
+                                           float qy0, float qz0, float px, float py, float pz, float alt, float wr,
+                                           uint32_t ctype, float& dod_ds, float& dod_cov, uint32_t& lin) {
     const LightWalk w = light_walk(P, L, px, py, pz, alt);
     const float wc = wr * L.coverage;
     const float cov = clampf(wc, 0.f, 1.f);
@@ -213,7 +213,7 @@ __device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& 
     lin = 0u;
     for (int i = 0; i < w.n; ++i) {
         float qx, qyy, qz, a, wt;
-        light_sample(P, w, i, px, py, pz, alt, qx, qyy, qz, a, wt);
+        light_sample(P, w, i, qx0, qy0, qz0, px, py, pz, alt, qx, qyy, qz, a, wt);
         const float h = (a - L.base) * L.inv_thick;
         float d = 0.f;
         if (!(h < 0.f || h > 1.f) && cov >= 0.01f) {
@@ -239,8 +239,8 @@ __device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& 
 // `lin` (lanes with has). Called with all 64 lanes active; light sample i of
 // every lane is scattered together.
 __device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, float* colacc, const LayerF& L,
-                                                         bool has, float px, float py, float pz, float alt,
-                                                         uint32_t lin, float g) {
+                                                         bool has, float qx0, float qy0, float qz0, float px,
+                                                         float py, float pz, float alt, uint32_t lin, float g) {
     const LightWalk w = light_walk(P, L, px, py, pz, alt);
     const int n = P.cone ? 6 : P.light_steps;
     unsigned long long any = __ballot(has && lin != 0u);
@@ -252,7 +252,7 @@ __device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, 
         float wt = 0.f;
         if (hi) {
             float qx, qyy, qz, a;
-            light_sample(P, w, i, px, py, pz, alt, qx, qyy, qz, a, wt);
+            light_sample(P, w, i, qx0, qy0, qz0, px, py, pz, alt, qx, qyy, qz, a, wt);
             const float h = (a - L.base) * L.inv_thick;
             float dcx, dcy, dcz;
             delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
@@ -272,12 +272,14 @@ __device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, 
 // has drained.
 // ---------------------------------------------------------------------------
 constexpr int kBQCap = 128;  // >= 64 + 63: a full flush plus one step's arrivals
-enum { BQ_PX = 0, BQ_PY, BQ_PZ, BQ_ALT, BQ_WR, BQ_DENS, BQ_TR, BQ_TG, BQ_TB, BQ_T, BQ_STEP, BQ_META, BQ_FIELDS };
-enum { SD_G0 = 0, SD_G1, SD_G2, SD_GTW, SD_GTWN, SD_OMC, SD_FIELDS };
+// BQ_QX..BQ_QZ: the sample's position in its segment's lookup frame (SegGeom)
+enum { BQ_QX = 0, BQ_QY, BQ_QZ, BQ_ALT, BQ_WR, BQ_DENS, BQ_TR, BQ_TG, BQ_TB, BQ_T, BQ_STEP, BQ_META, BQ_FIELDS };
+enum { SD_G0 = 0, SD_G1, SD_G2, SD_GTW, SD_GTWN, SD_OMC, SD_DX, SD_DY, SD_DZ, SD_FIELDS };
 
 struct BackQueue {
     float f[BQ_FIELDS][kBQCap];
-    float seed[SD_FIELDS][64];        // per owner lane: luminance cotangent, depth-weight cotangents, 1 - cos
+    float seed[SD_FIELDS][64];        // per owner lane: luminance cotangent, depth-weight cotangents, 1 - cos,
+                                      // view ray
     float wacc[3 * PS_MAX_LAYERS];    // per-layer gradient sums of the wave
     float colacc[kColSlots];          // perturbation-gradient column block (zero between uses)
 };
@@ -287,14 +289,14 @@ struct BackQueue {
 // light-march and extinction contributions to the wave's layer sums.
 // Light-march scatter work a shaded sample leaves for the wave.
 struct LightScatter {
-    float px, py, pz, alt, g;
+    float qx, qy, qz, px, py, pz, alt, g;
     uint32_t lin;
     int li;
 };
 
 template <bool DELTA>
 __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQueue& Q, int j, float* rec_tile) {
-    const float px = Q.f[BQ_PX][j], py = Q.f[BQ_PY][j], pz = Q.f[BQ_PZ][j];
+    const float qx = Q.f[BQ_QX][j], qyl = Q.f[BQ_QY][j], qz = Q.f[BQ_QZ][j];
     const float alt = Q.f[BQ_ALT][j], wr = Q.f[BQ_WR][j], density = Q.f[BQ_DENS][j];
     const float T[3] = {Q.f[BQ_TR][j], Q.f[BQ_TG][j], Q.f[BQ_TB][j]};
     const float t = Q.f[BQ_T][j], step = Q.f[BQ_STEP][j];
@@ -308,13 +310,16 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     const float g_tw = Q.seed[SD_GTW][owner], g_twn = Q.seed[SD_GTWN][owner];
     const float omcf = Q.seed[SD_OMC][owner];
     const float cosv = 1.f - omcf;
+    // world position from the camera, as the forward's shading computes it
+    const float px = P.cam_f[0] + Q.seed[SD_DX][owner] * t, py = P.cam_f[1] + Q.seed[SD_DY][owner] * t,
+                pz = P.cam_f[2] + Q.seed[SD_DZ][owner] * t;
     const float lw[3] = {0.2126f, 0.7152f, 0.0722f};
     const float qy = py + P.R;
     float gds = 0.f, gcov = 0.f, gk = 0.f;
     // dense sample: light march and shading with their adjoints
     float dod_ds, dod_cov;
     uint32_t lin;
-    const float od = light_adj<DELTA>(P, L, nc, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
+    const float od = light_adj<DELTA>(P, L, nc, qx, qyl, qz, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
     const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
     float st[3] = {0.f, 0.f, 0.f};
     {
@@ -409,7 +414,7 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
     atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
     LightScatter ls;
-    ls.px = px; ls.py = py; ls.pz = pz; ls.alt = alt; ls.lin = lin; ls.li = li;
+    ls.qx = qx; ls.qy = qyl; ls.qz = qz; ls.px = px; ls.py = py; ls.pz = pz; ls.alt = alt; ls.lin = lin; ls.li = li;
     ls.g = g_od * L.density_scale;
     return ls;
 }
@@ -423,8 +428,8 @@ __device__ __forceinline__ void flush_adj(const RenderParams& P, BackQueue& Q, i
     LightScatter ls{};
     if (lane < n) ls = shade_adj<DELTA>(P, Q, lane, rec_tile);
     if (want_delta)
-        light_delta_scatter_wave(P, Q.colacc, P.layers[ls.li].f, lane < n, ls.px, ls.py, ls.pz, ls.alt, ls.lin,
-                                 ls.g);
+        light_delta_scatter_wave(P, Q.colacc, P.layers[ls.li].f, lane < n, ls.qx, ls.qy, ls.qz, ls.px, ls.py,
+                                 ls.pz, ls.alt, ls.lin, ls.g);
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
@@ -534,13 +539,10 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
 
     Q.seed[SD_G0][lane] = G[0]; Q.seed[SD_G1][lane] = G[1]; Q.seed[SD_G2][lane] = G[2];
     Q.seed[SD_GTW][lane] = g_tw; Q.seed[SD_GTWN][lane] = g_twn; Q.seed[SD_OMC][lane] = ray.omcf;
+    Q.seed[SD_DX][lane] = dx; Q.seed[SD_DY][lane] = dy; Q.seed[SD_DZ][lane] = dz;
     wave_sync_lds();
 
     // ---- chunks, last to first ----------------------------------------------------------
-    const float r0c = fmaxf((float)P.r0, 1.f);
-    const float kcam = (float)(2.0 * P.r0 * ray.b / fmax(P.r0, 1.0));
-    const float h0f = (float)P.h0;
-    const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
     const float wscale = P.inv_weather_scale, nscale = P.inv_base_scale * 128.f;
     const float gmax = DELTA ? delta_gmax(P) : 0.f;
     const int steps = P.steps, cps = P.ck_per_seg, ckl = P.ck_len;
@@ -550,7 +552,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
         chunk = (g_last / steps) * cps + (g_last % steps) / ckl;
     }
     int cur_seg = -1, li = 0;
-    float tstart = 0.f, step = 0.f;
+    SegGeom g{};
     LayerF L;
     // perturbation-volume gradients need no volume: at delta = 0 they are
     // the gradients with respect to an additive perturbation
@@ -571,11 +573,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 li = (int)((order >> (3 * seg)) & 7u);
                 const LayerC& LC = P.layers[li];
                 L = LC.f;
-                double ta, tb;
-                layer_interval(LC, P.r0, ray.b, depth_t_max, ta, tb);
-                const double stepd = (tb - ta) / P.steps;
-                tstart = (float)(ta + jitter * stepd);
-                step = (float)stepd;
+                seg_geom(P, LC, ray, depth_t_max, jitter, g);
             }
             s0 = sub * ckl;
             s1 = min(min(steps, s0 + ckl), n_exec - seg * steps);
@@ -589,11 +587,9 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             float t = 0.f, px = 0.f, py = 0.f, pz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
             uint32_t ctype = 0u;
             if (act && s < s1) {
-                t = tstart + (float)s * step;
-                px = camx + dx * t; py = camy + dy * t; pz = camz + dz * t;
-                const float qy = py + P.R;
-                const float rr = fsqrt(px * px + qy * qy + pz * pz);
-                alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+                const float tl = (float)s * g.step;
+                t = g.t0 + tl;
+                seg_sample(g, dx, dy, dz, tl, px, py, pz, alt);
                 const float h = (alt - L.base) * L.inv_thick;
                 const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
                 float wfx, wfy;
@@ -667,16 +663,16 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 if (dense) {
                     const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                    Q.f[BQ_PX][pos] = px; Q.f[BQ_PY][pos] = py; Q.f[BQ_PZ][pos] = pz;
+                    Q.f[BQ_QX][pos] = px; Q.f[BQ_QY][pos] = py; Q.f[BQ_QZ][pos] = pz;
                     Q.f[BQ_ALT][pos] = alt; Q.f[BQ_WR][pos] = wr; Q.f[BQ_DENS][pos] = density;
                     Q.f[BQ_TR][pos] = T[0]; Q.f[BQ_TG][pos] = T[1]; Q.f[BQ_TB][pos] = T[2];
-                    Q.f[BQ_T][pos] = t; Q.f[BQ_STEP][pos] = step;
+                    Q.f[BQ_T][pos] = t; Q.f[BQ_STEP][pos] = g.step;
                     Q.f[BQ_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
                                                         ((uint32_t)k << 12));
                     // step transmittance (the forward's update)
-                    T[0] *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * step);
-                    T[1] *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * step);
-                    T[2] *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * step);
+                    T[0] *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
+                    T[1] *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * g.step);
+                    T[2] *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * g.step);
                 }
                 qn += __popcll(bal);
             }
@@ -714,8 +710,8 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     for (int c = 0; c < 3; ++c) {
                         const float sig = L.sigma_t[c];
                         if (rho * sig > 1e-6f) {
-                            g_rho -= step * sig * A[c];
-                            gk -= rho * step * A[c] * LC.sigma_t0[c];
+                            g_rho -= g.step * sig * A[c];
+                            gk -= rho * g.step * A[c] * LC.sigma_t0[c];
                         }
                     }
                     gds += g_rho * cval;
@@ -723,11 +719,8 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     A[0] += v0; A[1] += v1; A[2] += v2;
                     if (want_delta && cval < 1.f) {
                         const int s = s0 + slot;
-                        const float t = tstart + (float)s * step;
-                        const float px = camx + dx * t, py = camy + dy * t, pz = camz + dz * t;
-                        const float qy = py + P.R;
-                        const float rr = fsqrt(px * px + qy * qy + pz * pz);
-                        const float alt = h0f + t * (kcam + t) * frcp(r0c + rr);
+                        float px, py, pz, alt;
+                        seg_sample(g, dx, dy, dz, (float)s * g.step, px, py, pz, alt);
                         const float h = (alt - L.base) * L.inv_thick;
                         float dcx, dcy, dcz;
                         delta_coords(P, px - L.wind_off_x, pz - L.wind_off_z, h, dcx, dcy, dcz);

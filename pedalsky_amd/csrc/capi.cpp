@@ -102,6 +102,23 @@ double smoothstep_d(double e0, double e1, double x) {
     return t * t * (3.0 - 2.0 * t);
 }
 
+// Smallest common multiple of the periods p[0..n) that is a whole multiple
+// of p[0] (n p[0], n <= 2^20) and at most 4096 km, to a relative 1e-9;
+// 0 when there is none.
+double common_period(const double* p, int n) {
+    for (int k = 1; k <= (1 << 20); ++k) {
+        const double L = k * p[0];
+        if (L > 4.096e6) break;
+        bool ok = true;
+        for (int i = 1; i < n && ok; ++i) {
+            const double q = L / p[i];
+            ok = std::fabs(q - std::nearbyint(q)) <= 1e-9 * q;
+        }
+        if (ok) return L;
+    }
+    return 0.0;
+}
+
 }  // namespace
 
 ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutSet& luts, const float* depth,
@@ -166,6 +183,13 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
     P.inv_detail_scale = (float)(1.0 / std::max((double)cp.detail_scale_m, 1.0));
     P.inv_base_scale = (float)(1.0 / std::max((double)cp.base_scale_m, 1.0));
     P.inv_weather_scale = (float)(1.0 / std::max((double)cp.weather_scale_m, 1.0));
+    {
+        const double base = std::max((double)cp.base_scale_m, 1.0), det = std::max((double)cp.detail_scale_m, 1.0),
+                     wea = std::max((double)cp.weather_scale_m, 1.0);
+        const double xz[4] = {base, det, wea, 32000.0};  // 32 km: the type grid (128 cells of 250 m)
+        P.wrap_xz = common_period(xz, 4);
+        P.wrap_y = common_period(xz, 2);
+    }
     for (int i = 0; i < 3; ++i) P.sigma_s[i] = cp.sigma_s[i];
     P.ms_a = cp.multi_scatter_a; P.ms_b = cp.multi_scatter_b; P.ms_c = cp.multi_scatter_c;
     P.droplet_bias = cp.droplet_diameter_bias;

@@ -99,6 +99,14 @@ struct RenderParams {
     float light_rgb[3], light_radius, light_x, light_z;
     double r0;                        // camera radius |cam - centre|
     double h0;                        // r0 - R
+    // Common periods (m) of every position-periodic lookup of the march:
+    // x and z (base and detail volumes, curl map, weather map, type grid,
+    // perturbation volume) and y (base and detail volumes); 0 when the
+    // scales share no period below 4096 km. A layer segment's sample
+    // positions are taken relative to its entry point minus a whole number
+    // of periods, so they stay small (and exact to a few cm) even for rays
+    // that reach a layer shell on the far side of the planet.
+    double wrap_xz, wrap_y;
     LayerC layers[PS_MAX_LAYERS];
     // resources
     const float4* T; int tw, th;

@@ -293,10 +293,13 @@ __device__ __forceinline__ float erode(const RenderParams& P, const LayerF& L, f
     return (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
 }
 
-// cloud_march.wgsl:543-578 (straight) and 599-666 (cone), FP32.
+// cloud_march.wgsl:543-578 (straight) and 599-666 (cone), FP32. (px, py, pz)
+// is the sample's world position (its radius and local up), (qx, qy, qz)
+// the same point in the segment's lookup frame (the density fetches).
 template <bool DELTA = false>
-__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, const NdfC& nc, float px,
-                                             float py, float pz, float alt, float weather_r, uint32_t ctype, Cnt& cnt) {
+__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
+                                             float qy0, float qz0, float px, float py, float pz, float alt,
+                                             float weather_r, uint32_t ctype, Cnt& cnt) {
     const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
     float cos_sun = fmaxf(sy, 0.05f);
     float dist = fmaxf((L.top - alt) / cos_sun, 1.f);
@@ -316,13 +319,12 @@ __device__ __forceinline__ float march_light(const RenderParams& P, const LayerF
             float t, scale, w;
             if (i < 5) { t = (i + 0.5f) * step; scale = step * (i + 1.f) / 5.f; w = step; }
             else { t = dist * 3.f; scale = step * 3.f; w = step * 3.f; }
-            float qx = px + sx * t + off[i][0] * scale;
-            float qyy = py + sy * t + off[i][1] * scale;
-            float qz = pz + sz * t + off[i][2] * scale;
-            float yy = qyy + P.R;
-            float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+            const float ox = sx * t + off[i][0] * scale, oy = sy * t + off[i][1] * scale,
+                        oz = sz * t + off[i][2] * scale;
+            float wx = px + ox, yy = py + oy + P.R, wz = pz + oz;
+            float rr = fsqrt(wx * wx + yy * yy + wz * wz);
             float a = alt + t * (k + t) * frcp(r0c + rr);
-            float d = density_light<DELTA>(P, L, nc, qx, qyy, qz, a, weather_r, ctype, cov);
+            float d = density_light<DELTA>(P, L, nc, qx0 + ox, qy0 + oy, qz0 + oz, a, weather_r, ctype, cov);
             cnt.light++;
             cnt.light_cov += cov ? 1u : 0u;
             od += d * w;
@@ -332,11 +334,10 @@ __device__ __forceinline__ float march_light(const RenderParams& P, const LayerF
     float step = dist / (float)P.light_steps;
     for (int i = 0; i < P.light_steps; ++i) {
         float t = (i + 0.5f) * step;
-        float qx = px + sx * t, qyy = py + sy * t, qz = pz + sz * t;
-        float yy = qyy + P.R;
-        float rr = fsqrt(qx * qx + yy * yy + qz * qz);
+        float wx = px + sx * t, yy = py + sy * t + P.R, wz = pz + sz * t;
+        float rr = fsqrt(wx * wx + yy * yy + wz * wz);
         float a = alt + t * (k + t) * frcp(r0c + rr);
-        float d = density_light<DELTA>(P, L, nc, qx, qyy, qz, a, weather_r, ctype, cov);
+        float d = density_light<DELTA>(P, L, nc, qx0 + sx * t, qy0 + sy * t, qz0 + sz * t, a, weather_r, ctype, cov);
         cnt.light++;
         cnt.light_cov += cov ? 1u : 0u;
         od += d * step;
@@ -470,6 +471,61 @@ __device__ __forceinline__ PixelRay pixel_ray(const RenderParams& P, int xc, int
     r.omcf = (float)(1.0 - r.cos_vs);
     r.b = P.cam[0] * dxd + (P.cam[1] + P.R) * dyd + P.cam[2] * dzd;
     return r;
+}
+
+// ---------------------------------------------------------------------------
+// One layer segment of a pixel's primary march (cloud_march.wgsl:808-1000),
+// restated relative to the segment. Sample s sits at ray distance t = t0 +
+// s step, t0 = ta + jitter step. Its density lookups use o + d (s step),
+// where o is the first sample's position minus whole common periods of every
+// periodic lookup (the wrap_* periods): the same texels as the world
+// position, with coordinates that stay below a few hundred km. Its altitude
+// is h_e + q / (r_e + sqrt(r_e^2 + q)), q = tl (k_e + tl), tl = s step, from
+// the first sample's altitude h_e, radius r_e and k_e = 2 (p_e - c).d
+// (|p - c|^2 = r_e^2 + tl k_e + tl^2), the reference's altitude_from_entry
+// (cloud_march.wgsl:360-365) anchored at the segment instead of the camera.
+// With world positions from the camera, FP32 rounding at a far-side shell
+// (t ~ 1.3e4 km, rays through the planet when no depth buffer clips them)
+// is ~1 m in both the lookups and the altitude; here it is ~1 cm.
+// ---------------------------------------------------------------------------
+struct SegGeom {
+    float ox, oy, oz;                 // lookup-frame position of the first sample
+    float he, ke, re;                 // its altitude, 2 (p - c).d and radius
+    float t0;                         // its ray distance
+    float step;
+};
+
+__device__ __forceinline__ double wrap_period(double v, double period) {
+    return period > 0.0 ? v - period * rint(v / period) : v;
+}
+
+// Segment geometry of layer LC for a ray (FP64). False when the ray misses
+// the layer's interval.
+__device__ __forceinline__ bool seg_geom(const RenderParams& P, const LayerC& LC, const PixelRay& ray,
+                                         double depth_t_max, float jitter, SegGeom& g) {
+    double ta, tb;
+    if (!layer_interval(LC, P.r0, ray.b, depth_t_max, ta, tb)) return false;
+    const double stepd = (tb - ta) / P.steps;
+    const double t0 = ta + (double)jitter * stepd;
+    const double r2 = P.r0 * P.r0 + t0 * (2.0 * ray.b + t0);
+    const double re = sqrt(fmax(r2, 0.0));
+    g.he = (float)(re - (double)P.R);
+    g.ke = (float)(2.0 * (ray.b + t0));
+    g.re = (float)re;
+    g.ox = (float)wrap_period(P.cam[0] + ray.dxd * t0, P.wrap_xz);
+    g.oy = (float)wrap_period(P.cam[1] + ray.dyd * t0, P.wrap_y);
+    g.oz = (float)wrap_period(P.cam[2] + ray.dzd * t0, P.wrap_xz);
+    g.t0 = (float)t0;
+    g.step = (float)stepd;
+    return true;
+}
+
+// Sample at tl = s step of a segment: lookup position, altitude.
+__device__ __forceinline__ void seg_sample(const SegGeom& g, float dx, float dy, float dz, float tl, float& qx,
+                                           float& qy, float& qz, float& alt) {
+    qx = g.ox + dx * tl; qy = g.oy + dy * tl; qz = g.oz + dz * tl;
+    const float qq = tl * (g.ke + tl);
+    alt = g.he + qq * frcp(g.re + fsqrt(fmaf(g.re, g.re, qq)));
 }
 
 // Sky pass (sky.wgsl:159-220): sky-view LUT, overcast blend, sun disk.

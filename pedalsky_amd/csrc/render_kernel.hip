@@ -49,8 +49,13 @@ enum { RF_COUNT = 1, RF_DELTA = 2, RF_TRAIN = 4, RF_RT = 8 };
 // march because only some of its pixels hit cloud.
 // ---------------------------------------------------------------------------
 constexpr int kQCap = 128;        // >= 64 + 63: a full flush plus one step's arrivals
-constexpr int kQFields = 13;
-enum { QF_PX = 0, QF_PY, QF_PZ, QF_ALT, QF_WR, QF_DENS, QF_TR, QF_TG, QF_TB, QF_T, QF_STEP, QF_OMC, QF_META };
+// A record holds the sample's lookup-frame position (the segment frame of
+// SegGeom), altitude, weather coverage, density, the transmittance in front
+// of it, its ray distance and step, and the owner lane, layer and cloud
+// type. World-frame quantities (local up, lightning distance) come from the
+// owner's view ray, read from the owner lane when the queue is flushed.
+constexpr int kQFields = 12;
+enum { QF_QX = 0, QF_QY, QF_QZ, QF_ALT, QF_WR, QF_DENS, QF_TR, QF_TG, QF_TB, QF_T, QF_STEP, QF_META };
 
 struct WaveQueue {
     float f[kQFields][kQCap];
@@ -63,20 +68,25 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // Shade one queued dense sample (cloud_march.wgsl:886-993) and add its
-// luminance and depth-weight contributions to the owning pixel.
+// luminance and depth-weight contributions to the owning pixel. (odx, ody,
+// odz) is the owner's view ray and omc its 1 - cos(view, sun).
 template <bool DELTA>
-__device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q, int j, Cnt& cnt) {
-    const float px = Q.f[QF_PX][j], py = Q.f[QF_PY][j], pz = Q.f[QF_PZ][j];
+__device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q, int j, Cnt& cnt, float odx,
+                                             float ody, float odz, float omc) {
+    const float qx = Q.f[QF_QX][j], qy = Q.f[QF_QY][j], qz = Q.f[QF_QZ][j];
     const float alt = Q.f[QF_ALT][j], wr = Q.f[QF_WR][j], density = Q.f[QF_DENS][j];
     const float Tr = Q.f[QF_TR][j], Tg = Q.f[QF_TG][j], Tb = Q.f[QF_TB][j];
-    const float t = Q.f[QF_T][j], step = Q.f[QF_STEP][j], omc = Q.f[QF_OMC][j];
+    const float t = Q.f[QF_T][j], step = Q.f[QF_STEP][j];
     const uint32_t meta = __float_as_uint(Q.f[QF_META][j]);
     const int owner = (int)(meta & 63u), li = (int)((meta >> 6) & 7u);
     const uint32_t ctype = (meta >> 9) & 7u;
     const LayerC& LC = P.layers[li];
     const LayerF L = LC.f;
+    // world position from the camera (the local up, sun angle and lightning
+    // distance need only its direction from the planet centre)
+    const float px = P.cam_f[0] + odx * t, py = P.cam_f[1] + ody * t, pz = P.cam_f[2] + odz * t;
     cnt.dense++;
-    const float od = march_light<DELTA>(P, L, ndf_coef(ctype), px, py, pz, alt, wr, ctype, cnt);
+    const float od = march_light<DELTA>(P, L, ndf_coef(ctype), qx, qy, qz, px, py, pz, alt, wr, ctype, cnt);
     const float sig0 = L.sigma_t[0], sig1 = L.sigma_t[1], sig2 = L.sigma_t[2];
     // Beer-Powder (cloud_march.wgsl:888-892); exp(-2x) = exp(-x)^2
     const float e0r = __expf(-sig0 * od), e0g = __expf(-sig1 * od), e0b = __expf(-sig2 * od);
@@ -85,12 +95,12 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
     const float eng = e0g + (2.f * e0g * (1.f - e0g * e0g) - e0g) * pw;
     const float enb = e0b + (2.f * e0b * (1.f - e0b * e0b) - e0b) * pw;
     const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
-    const float qy = py + P.R;
+    const float wy = py + P.R;
     // sun transmittance (cloud_march.wgsl:328-354)
     float str = 0.f, stg = 0.f, stb = 0.f;
     {
         const float hh = fmaxf(alt, 1.f);
-        const float mu = clampf((px * P.sun[0] + qy * P.sun[1] + pz * P.sun[2]) / (P.R + hh), -1.f, 1.f);
+        const float mu = clampf((px * P.sun[0] + wy * P.sun[1] + pz * P.sun[2]) / (P.R + hh), -1.f, 1.f);
         float tu, tv;
         if (trans_uv_from_h(P, hh, mu, tu, tv)) {
             const float4 tt = lut2d(P.T, P.tw, P.th, tu, tv);
@@ -138,7 +148,7 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
         const float rho = sqrtf(fmaxf(alt * (2.f * P.R + alt), 0.f));
         const float zha = (float)PS_PI - atan2f(P.R, rho);
         float u, v;
-        skyview_uv_f(P, px, qy, pz, zha, u, v);
+        skyview_uv_f(P, px, wy, pz, zha, u, v);
         const float4 sv = lut2d(P.SV, P.sw, P.sh, u, v);
         const float g = P.ambient * (0.3f + 0.7f * hn);
         ar = sv.x * P.E[0] * g; ag = sv.y * P.E[1] * g; ab = sv.z * P.E[2] * g;
@@ -168,11 +178,16 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
 }
 
 // Shade the first n (<= 64) queued samples with the whole wave, then move
-// the entries [64, qn) to the front. Called with all 64 lanes active.
+// the entries [64, qn) to the front. Called with all 64 lanes active; (dx,
+// dy, dz) and omc are this lane's own view ray and 1 - cos(view, sun).
 template <bool DELTA>
-__device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q, int lane, int n, int qn, Cnt& cnt) {
+__device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q, int lane, int n, int qn, Cnt& cnt,
+                                            float dx, float dy, float dz, float omc) {
     wave_sync_lds();
-    if (lane < n) shade_record<DELTA>(P, Q, lane, cnt);
+    const int owner = lane < n ? (int)(__float_as_uint(Q.f[QF_META][lane]) & 63u) : lane;
+    const float odx = __shfl(dx, owner), ody = __shfl(dy, owner), odz = __shfl(dz, owner);
+    const float oomc = __shfl(omc, owner);
+    if (lane < n) shade_record<DELTA>(P, Q, lane, cnt, odx, ody, odz, oomc);
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
@@ -190,8 +205,24 @@ __device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q,
     }
 }
 
+// Keep a value the compiler cannot see through, so that per-pixel FP64 work
+// recomputed inside the march is not hoisted out of it (and kept live in
+// registers across the whole march).
+__device__ __forceinline__ int opaque_int(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // One wave tile of TW x (64 / TW) pixels: everything from the view ray to
 // the written output pixels. Called by one full wave.
+//
+// The march runs in wave-uniform layer rounds. Each lane walks its own
+// layer segments in entry order; a round takes the next layer of the first
+// lane that still has one, and every lane whose next segment is that layer
+// marches it, all from step 0 together. The layer index is then uniform, so
+// the layer's constants are scalar loads, and the step counter is uniform.
+// For the usual layer stacks every lane enters the layers in the same order
+// and each layer takes one round.
 template <int FLAGS, int TW>
 __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, int tile_y, WaveQueue& Q) {
     static_assert(TW == 4 || TW == 8 || TW == 16, "wave tiles are 4x16, 8x8 or 16x4 pixels");
@@ -209,176 +240,158 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
 #pragma unroll
     for (int k = 0; k < 5; ++k) Q.acc[k][lane] = 0.f;
 
-    // ---- per-pixel setup: FP64 view ray, sky pass, jitter, depth clip, layer order ----
-    const PixelRay ray = pixel_ray(P, xc, yc);
-    const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
-    const double fx = ray.fx, fy = ray.fy;
-    const float omcf = ray.omcf;
-    const float3 sky = RT ? make_float3(0.f, 0.f, 0.f) : sky_pass(P, ray);
-    const float jitter = pixel_jitter(P, xc, yc);
-    const double depth_t_max = pixel_depth_tmax(P, ray, xc, yc);
-    const double r_cam = P.r0;
-    const double b_ = ray.b;
+    // ---- per-pixel setup: view ray and the entry order of the layer segments ----------
+    float dx, dy, dz, omcf;
     int nseg = 0;
-    const uint32_t order = valid ? layer_order(P, ray, depth_t_max, nseg) : 0u;
+    uint32_t order = 0u;
+    {
+        const PixelRay ray = pixel_ray(P, xc, yc);
+        dx = ray.dx; dy = ray.dy; dz = ray.dz;
+        omcf = ray.omcf;
+        if (valid) order = layer_order(P, ray, pixel_depth_tmax(P, ray, xc, yc), nseg);
+    }
 
-    // ---- wave-uniform primary march ---------------------------------------------------
-    const float r0c = fmaxf((float)P.r0, 1.f);
-    const float kcam = (float)(2.0 * P.r0 * b_ / fmax(P.r0, 1.0));
-    const float h0f = (float)P.h0;
-    const float camx = P.cam_f[0], camy = P.cam_f[1], camz = P.cam_f[2];
+    // ---- primary march in wave-uniform layer rounds ----------------------------------------
     const float wscale = P.inv_weather_scale, nscale = P.inv_base_scale * 128.f;
     const float gmax = DELTA ? delta_gmax(P) : 0.f;
     float tr = 1.f, tg = 1.f, tbb = 1.f;
-    int cur = 0, s = 0, li = 0;
-    float tstart = 0.f, step = 0.f;
-    LayerF L;
-    bool active = nseg > 0;
-    // training forward: steps until the next checkpoint and its chunk index
-    // (counters instead of a division by the runtime interval every step)
-    int ck_left = 0, ck_c = 0;
-    // enter the march's next layer interval (the cur-th in entry order)
-    auto enter = [&](int k) {
-        li = (int)((order >> (3 * k)) & 7u);
+    int next = 0;                 // the lane's next segment (rank in its entry order)
+    bool alive = nseg > 0;        // not yet opaque
+    int qn = 0;                   // queued samples (wave-uniform)
+    for (;;) {
+        const bool want = alive && next < nseg;
+        const unsigned long long wb = __ballot(want);
+        if (wb == 0ull) break;
+        const int my_li = (int)((order >> (3 * next)) & 7u);
+        const int li = __builtin_amdgcn_readlane(my_li, (int)__builtin_ctzll(wb));
+        const bool in_round = want && my_li == li;
+        bool on = in_round;
         const LayerC& LC = P.layers[li];
-        L = LC.f;
-        double ta, tb;
-        layer_interval(LC, r_cam, b_, depth_t_max, ta, tb);
-        const double stepd = (tb - ta) / P.steps;
-        tstart = (float)(ta + jitter * stepd);
-        step = (float)stepd;
-        s = 0;
-        ck_left = 0;
-        ck_c = k * P.ck_per_seg;
-    };
-    if (active) enter(0);
-    int qn = 0;  // queued samples (wave-uniform)
-    while (__ballot(active) != 0ull) {
-        bool dense = false;
-        float t = 0.f, px = 0.f, py = 0.f, pz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
-        uint32_t ctype = 0u;
-        if (active) {
+        const LayerF& L = LC.f;
+        SegGeom g{};
+        if (on) {
+            const int xo = opaque_int(xc), yo = opaque_int(yc);
+            const PixelRay ray = pixel_ray(P, xo, yo);
+            on = seg_geom(P, LC, ray, pixel_depth_tmax(P, ray, xo, yo), pixel_jitter(P, xo, yo), g);
+        }
+        // training checkpoints of this segment: chunk next * ck_per_seg + s / ck_len
+        float* ckp = nullptr;
+        if constexpr (TRAIN)
+            ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + next * P.ck_per_seg) * 192 + lane;
+        NdfC ncl = layer_ndf(L);
+        for (int s = 0; s < P.steps; ++s) {
+            if (__ballot(on) == 0ull) break;
             if constexpr (TRAIN) {
-                // transmittance at the first step of every checkpoint chunk
-                // (steps 0, ck_len, 2 ck_len, ... of segment cur: chunk
-                // cur * ck_per_seg + s / ck_len)
-                if (ck_left == 0) {
-                    float* ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + ck_c) * 192 + lane;
-                    ckp[0] = tr; ckp[64] = tg; ckp[128] = tbb;
-                    ++ck_c;
-                    ck_left = P.ck_len;
+                if (on && s % P.ck_len == 0) {
+                    float* c = ckp + (size_t)(s / P.ck_len) * 192;
+                    c[0] = tr; c[64] = tg; c[128] = tbb;
                 }
-                --ck_left;
             }
-            cnt.primary++;
-            t = tstart + (float)s * step;
-            px = camx + dx * t; py = camy + dy * t; pz = camz + dz * t;
-            const float qy = py + P.R;
-            const float rr = fsqrt(px * px + qy * qy + pz * pz);
-            alt = h0f + t * (kcam + t) * frcp(r0c + rr);
-            const float h = (alt - L.base) * L.inv_thick;
-            // the weather quad and the brick maximum depend on the position
-            // only: both loads are issued before either is used
-            const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
-            float wfx, wfy;
-            const uint32_t woff = weather_tap(P, wx * wscale + 0.5f, wz * wscale + 0.5f, wfx, wfy);
-            const float ax = wx - L.skew_x * h, ay = py, az = wz - L.skew_z * h;
-            const float cx = ax * nscale - 0.5f, cy = ay * nscale - 0.5f, cz = az * nscale - 0.5f;
-            const uint32_t roff = ((((int)floorf(cz) >> 2) & 31) << 10) | ((((int)floorf(cy) >> 2) & 31) << 5) |
-                                  (((int)floorf(cx) >> 2) & 31);
-            const uint2 wq = ld_g<uint2>(P.weather, woff);
-            const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
-            float dcx = 0.f, dcy = 0.f, dcz = 0.f;
-            if constexpr (DELTA) delta_coords(P, wx, wz, h, dcx, dcy, dcz);
-            wr = weather_blend(wq, wfx, wfy);
-            ctype = L.type;
-            NdfC nc = layer_ndf(L);
-            if (P.use_type_grid) {
-                float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
-                const uint32_t gx = (uint32_t)(int)floorf(gu * 128.f) & 127u, gy = (uint32_t)(int)floorf(gv * 128.f) & 127u;
-                const uint32_t g = ld_g<uint8_t>(P.type_grid, gy * 128u + gx);
-                if (g != 255u) { ctype = g; nc = ndf_coef(g); }
-            }
-            // sample_density (cloud_march.wgsl:432-530). The upper gate has a
-            // 1e-5 tolerance: with blue-noise jitter 255/255 the last sample
-            // lies exactly on the top shell (h = 1 in exact arithmetic), which
-            // the literal `h > 1` test would decide by rounding; only the
-            // cumulonimbus anvil is non-zero there.
-            int stage = 0;
-            if (h >= 0.f && h <= 1.f + 1e-5f) {
-                const float cov = clampf(wr * L.coverage, 0.f, 1.f);
-                if (cov >= 0.01f) {
-                    stage = 1;
-                    const float ndfv = ndf_eval(nc, ctype, h, L.anvil);
-                    // with the perturbation volume, skip only when shape + max(delta) <= 0:
-                    // the volume's maximum first, then the brick's (see shape())
-                    float thr = DELTA ? (1.f - cov - gmax) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
-                    float2 db = make_float2(0.f, 0.f);
-                    if (DELTA && ndfv > thr) {
-                        db = delta_bound(P, dcx, dcy, dcz);
-                        thr = (1.f - cov - db.x) - 1e-5f;
-                    }
-                    const float rm = (float)rmb * (1.f / 255.f);
-                    float cloud = -1.f;
-                    if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
-                        const float ds = P.inv_detail_scale;
-                        const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
-                        const float2 b = base_fetch(P.base_rl, cx, cy, cz);
-                        const float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
-                        const float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
-                        cloud = base_cloud * ndfv - (1.f - cov);
-                        if (cloud > 0.f) {
-                            stage = 2;
-                            const float ddx = ax + cu.x * P.curl_off_scale, ddz = az + cu.y * P.curl_off_scale;
-                            const float hf = detail_fetch(P.detail_hf, ddx * ds * 32.f - 0.5f, ay * ds * 32.f - 0.5f,
-                                                          ddz * ds * 32.f - 0.5f);
-                            const float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
-                            const float e = dmod * L.ld;
-                            cloud = (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
+            bool dense = false;
+            float t = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
+            uint32_t ctype = 0u;
+            if (on) {
+                cnt.primary++;
+                const float tl = (float)s * g.step;
+                t = g.t0 + tl;
+                seg_sample(g, dx, dy, dz, tl, qx, qy, qz, alt);
+                const float h = (alt - L.base) * L.inv_thick;
+                // the weather quad and the brick maximum depend on the position
+                // only: both loads are issued before either is used
+                const float wx = qx - L.wind_off_x, wz = qz - L.wind_off_z;
+                float wfx, wfy;
+                const uint32_t woff = weather_tap(P, wx * wscale + 0.5f, wz * wscale + 0.5f, wfx, wfy);
+                const float ax = wx - L.skew_x * h, ay = qy, az = wz - L.skew_z * h;
+                const float cx = ax * nscale - 0.5f, cy = ay * nscale - 0.5f, cz = az * nscale - 0.5f;
+                const uint32_t roff = ((((int)floorf(cz) >> 2) & 31) << 10) | ((((int)floorf(cy) >> 2) & 31) << 5) |
+                                      (((int)floorf(cx) >> 2) & 31);
+                const uint2 wq = ld_g<uint2>(P.weather, woff);
+                const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
+                float dcx = 0.f, dcy = 0.f, dcz = 0.f;
+                if constexpr (DELTA) delta_coords(P, wx, wz, h, dcx, dcy, dcz);
+                wr = weather_blend(wq, wfx, wfy);
+                ctype = L.type;
+                NdfC nc = ncl;
+                if (P.use_type_grid) {
+                    float gu = (wx + 16000.f) / 32000.f, gv = (wz + 16000.f) / 32000.f;
+                    const uint32_t gx = (uint32_t)(int)floorf(gu * 128.f) & 127u, gy = (uint32_t)(int)floorf(gv * 128.f) & 127u;
+                    const uint32_t gt = ld_g<uint8_t>(P.type_grid, gy * 128u + gx);
+                    if (gt != 255u) { ctype = gt; nc = ndf_coef(gt); }
+                }
+                // sample_density (cloud_march.wgsl:432-530). The upper gate has a
+                // 1e-5 tolerance: with blue-noise jitter 255/255 the last sample
+                // lies exactly on the top shell (h = 1 in exact arithmetic), which
+                // the literal `h > 1` test would decide by rounding; only the
+                // cumulonimbus anvil is non-zero there.
+                int stage = 0;
+                if (h >= 0.f && h <= 1.f + 1e-5f) {
+                    const float cov = clampf(wr * L.coverage, 0.f, 1.f);
+                    if (cov >= 0.01f) {
+                        stage = 1;
+                        const float ndfv = ndf_eval(nc, ctype, h, L.anvil);
+                        // with the perturbation volume, skip only when shape + max(delta) <= 0:
+                        // the volume's maximum first, then the brick's (see shape())
+                        float thr = DELTA ? (1.f - cov - gmax) - 1e-5f : (1.f - cov) * (1.f - 1e-5f);
+                        float2 db = make_float2(0.f, 0.f);
+                        if (DELTA && ndfv > thr) {
+                            db = delta_bound(P, dcx, dcy, dcz);
+                            thr = (1.f - cov - db.x) - 1e-5f;
                         }
-                        if constexpr (DELTA) cloud += delta_value(P, db, dcx, dcy, dcz);
+                        const float rm = (float)rmb * (1.f / 255.f);
+                        float cloud = -1.f;
+                        if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
+                            const float ds = P.inv_detail_scale;
+                            const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
+                            const float2 b = base_fetch(P.base_rl, cx, cy, cz);
+                            const float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
+                            const float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
+                            cloud = base_cloud * ndfv - (1.f - cov);
+                            if (cloud > 0.f) {
+                                stage = 2;
+                                const float ddx = ax + cu.x * P.curl_off_scale, ddz = az + cu.y * P.curl_off_scale;
+                                const float hf = detail_fetch(P.detail_hf, ddx * ds * 32.f - 0.5f, ay * ds * 32.f - 0.5f,
+                                                              ddz * ds * 32.f - 0.5f);
+                                const float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
+                                const float e = dmod * L.ld;
+                                cloud = (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
+                            }
+                            if constexpr (DELTA) cloud += delta_value(P, db, dcx, dcy, dcz);
+                        }
+                        density = clampf(cloud, 0.f, 1.f) * L.density_scale;
                     }
-                    density = clampf(cloud, 0.f, 1.f) * L.density_scale;
                 }
+                cnt.covered += stage >= 1 ? 1u : 0u;
+                cnt.eroded += stage >= 2 ? 1u : 0u;
+                dense = density > 1e-3f;
             }
-            cnt.covered += stage >= 1 ? 1u : 0u;
-            cnt.eroded += stage >= 2 ? 1u : 0u;
-            dense = density > 1e-3f;
-        }
-        const unsigned long long bal = __ballot(dense);
-        if (bal != 0ull) {
-            if (dense) {
-                const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                Q.f[QF_PX][pos] = px; Q.f[QF_PY][pos] = py; Q.f[QF_PZ][pos] = pz;
-                Q.f[QF_ALT][pos] = alt; Q.f[QF_WR][pos] = wr; Q.f[QF_DENS][pos] = density;
-                Q.f[QF_TR][pos] = tr; Q.f[QF_TG][pos] = tg; Q.f[QF_TB][pos] = tbb;
-                Q.f[QF_T][pos] = t; Q.f[QF_STEP][pos] = step; Q.f[QF_OMC][pos] = omcf;
-                Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9));
+            const unsigned long long bal = __ballot(dense);
+            if (bal != 0ull) {
+                if (dense) {
+                    const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    Q.f[QF_QX][pos] = qx; Q.f[QF_QY][pos] = qy; Q.f[QF_QZ][pos] = qz;
+                    Q.f[QF_ALT][pos] = alt; Q.f[QF_WR][pos] = wr; Q.f[QF_DENS][pos] = density;
+                    Q.f[QF_TR][pos] = tr; Q.f[QF_TG][pos] = tg; Q.f[QF_TB][pos] = tbb;
+                    Q.f[QF_T][pos] = t; Q.f[QF_STEP][pos] = g.step;
+                    Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9));
+                    tr *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
+                    tg *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * g.step);
+                    tbb *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * g.step);
+                    if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) {
+                        on = false;
+                        alive = false;
+                    }
+                }
+                qn += __popcll(bal);
             }
-            qn += __popcll(bal);
-        }
-        if (active) {
-            if (dense) {
-                tr *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * step);
-                tg *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * step);
-                tbb *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * step);
-            }
-            ++s;
-            if (fmaxf(tr, fmaxf(tg, tbb)) < 0.01f) {
-                active = false;
-            } else if (s >= P.steps) {
-                ++cur;
-                if (cur < nseg) enter(cur);
-                else active = false;
+            if (qn >= 64) {
+                flush_queue<DELTA>(P, Q, lane, 64, qn, cnt, dx, dy, dz, omcf);
+                qn -= 64;
             }
         }
-        if (qn >= 64) {
-            flush_queue<DELTA>(P, Q, lane, 64, qn, cnt);
-            qn -= 64;
-        }
+        if (in_round) ++next;
     }
-    if (qn > 0) flush_queue<DELTA>(P, Q, lane, qn, qn, cnt);
+    if (qn > 0) flush_queue<DELTA>(P, Q, lane, qn, qn, cnt, dx, dy, dz, omcf);
     wave_sync_lds();
     cnt.layer_marches = (uint32_t)nseg;
 
@@ -400,7 +413,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         const float tc = tw / fmaxf(twn, 1e-6f);
         const float ds = fmaxf(tc, 50.f);
         const float apz = clampf(__logf(ds / 50.f) * (float)(1.0 / 7.6009024595420822), 0.f, 1.f);
-        const float u = (float)fx * P.inv_W, v = (float)fy * P.inv_H;
+        const float u = (float)((xc + 0.5) * P.rt_sx) * P.inv_W, v = (float)((yc + 0.5) * P.rt_sy) * P.inv_H;
         // trilinear ClampToEdge on the froxel volume
         const float cx = u * P.aw - 0.5f, cy = v * P.ah - 0.5f, cz = apz * P.ad - 0.5f;
         const float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
@@ -430,8 +443,14 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         olb = lb * ap[3] + ap[2] * opacity;
     }
 
-    // ---- outputs: composite dst = lum + sky * T ------------------------------------------------
+    // ---- outputs: composite dst = lum + sky * T (the sky pass runs here, after
+    // the march, so that nothing of it is live during the march) ---------------
     if (valid) {
+        float3 sky = make_float3(0.f, 0.f, 0.f);
+        if constexpr (!RT) {
+            const int xo = opaque_int(xc), yo = opaque_int(yc);
+            sky = sky_pass(P, pixel_ray(P, xo, yo));
+        }
         const size_t o = (size_t)y * P.W + x;
         const float4 comp = make_float4(olr + sky.x * tr, olg + sky.y * tg, olb + sky.z * tbb, 1.f);
         const float4 lum4 = make_float4(olr, olg, olb, 1.f);
@@ -477,7 +496,10 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
 // values to scratch under it and still runs 3.5 % faster at 1280x720 than at
 // three waves with no spills (more waves hide the gather latency).
 template <int FLAGS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_render(
+#ifndef PS_RENDER_WAVES
+#define PS_RENDER_WAVES 4
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render(
     const RenderParams* __restrict__ Pp) {
     __shared__ WaveQueue Qs;
     int tx = blockIdx.x, ty = blockIdx.y;
@@ -492,7 +514,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 // leaves). Otherwise block b renders entry b. Four waves per SIMD, as
 // k_render (8 % faster on the 64-scene batch than three).
 template <int TW, bool PERSISTENT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
                                                      const uint32_t* __restrict__ tiles, int n_tiles,
                                                      unsigned* __restrict__ next) {
     __shared__ WaveQueue Qs;

@@ -174,3 +174,52 @@ def test_autotune_and_errors(setup):
         b.update(ws3)
     assert err.value.status == abi.PS_ERR_INVALID_ARGUMENT and "resolution" in str(err.value)
     b.close()
+
+
+def test_benchmark_batch_64_scenes(setup):
+    """The batched configuration at its full size: 64 scenes from seed 0xC3
+    (uneven resolutions up to 1920x1080, 1-3 layers of any type, scaled
+    atmospheres, sun zenith to 95 deg), captured in a hipGraph and replayed.
+    Every scene must be bit-identical to its own ps_frame render at 8x8
+    tiles, and a 64x64 window of every scene (its cloudiest) must match the
+    FP64 oracle at the plain-path bar."""
+    import torch
+    r, _ = setup
+    ws = scene.workload_batch(64, 0xC3)
+    for w in ws:
+        w.overcast = ws[0].overcast
+    _log(f"case: 64-scene batch seed 0xC3, {sum(w.width * w.height for w in ws) / 1e6:.1f} Mpix")
+    b = Batch(r, ws, outputs=("composite", "cloud_transmittance"), tuning=abi.Tuning(8, 0, 8, 1))
+    b.capture()
+    for bb in b.bufs:
+        bb["composite"].zero_()
+    torch.cuda.synchronize()
+    b.replay()
+    r.synchronize()
+    torch.cuda.synchronize()
+    got = [bb["composite"].cpu().numpy() for bb in b.bufs]
+    trn = [bb["cloud_transmittance"].cpu().numpy() for bb in b.bufs]
+    b.close()
+    ref = _single(r, ws)
+    same = [bool(np.array_equal(g, e)) for g, e in zip(got, ref)]
+    _log(f"  graph replay bit-identical to per-scene ps_frame: {sum(same)} / {len(same)} scenes")
+    assert all(same)
+    del ref
+    fmt = ws[0].noise_format
+    base = oracle.decode(r.read_noise(abi.PS_NOISE_BASE, fmt), fmt)
+    det = oracle.decode(r.read_noise(abi.PS_NOISE_DETAIL, fmt), fmt)
+    curl = oracle.decode(r.read_noise(abi.PS_NOISE_CURL, fmt), fmt)
+    worst = 0.0
+    for i, w in enumerate(ws):
+        cw = 64
+        cl = (trn[i][..., 1] < 0.95).astype(np.float64)
+        cand = [(x, y) for y in range(0, w.height - cw + 1, 32) for x in range(0, w.width - cw + 1, 32)]
+        x0, y0 = max(cand, key=lambda p: cl[p[1]:p[1] + cw, p[0]:p[0] + cw].mean())
+        comp, _, _, _ = oracle.Frame(w, base, det, curl).render_region(x0, y0, cw, cw)
+        e = oracle.relative_error(got[i][y0:y0 + cw, x0:x0 + cw], comp)
+        p999 = float(np.percentile(e, 99.9))
+        worst = max(worst, p999)
+        if p999 > 1e-3:
+            _log(f"  scene {i} ({w.name}) window ({x0}, {y0}): p99.9 {p999:.3e}")
+        assert p999 <= 1e-3, (i, w.name)
+    _log(f"  every scene's cloudiest 64x64 window vs the oracle: worst p99.9 {worst:.3e}")

@@ -107,9 +107,10 @@ def test_noise_bake_matches_oracle(renderer, fmt):
         assert frac < 2e-2
 
 
-def test_render_small_matches_oracle(renderer):
-    w = scene.workload_small()
-    _log("case: render 320x180 one cumulus layer")
+@pytest.mark.parametrize("ground", [True, False], ids=["ground", "all-far"])
+def test_render_small_matches_oracle(renderer, ground):
+    w = scene.workload_small(ground=ground)
+    _log(f"case: render 320x180 one cumulus layer, {'planet ground depth' if ground else 'no depth buffer'}")
     renderer.load_workload(w)
     out = renderer.render_numpy()
     fr = _oracle_frame(renderer, w)
@@ -125,17 +126,29 @@ def test_render_small_matches_oracle(renderer):
     assert (trn[..., 1] < 0.9).mean() > 0.02
 
 
-def test_render_full_frame_matches_oracle(renderer):
-    w = scene.workload_full_frame(0)
-    _log("case: render 1280x720 cumulus + cirrus, sweep frame 0")
+# Sweep frames of the full-frame workload: with the planet-ground depth
+# buffer, and with none (depth all far: rays below the horizon pass through
+# the planet and march the layer shells on its far side, ~1.3e4 km away).
+FULL_FRAME_CASES = [(0, True), (7, True), (12, True), (0, False), (5, False), (10, False), (15, False)]
+
+
+@pytest.mark.parametrize("index, ground", FULL_FRAME_CASES,
+                         ids=[f"frame{i}-{'ground' if g else 'all-far'}" for i, g in FULL_FRAME_CASES])
+def test_render_full_frame_matches_oracle(renderer, index, ground):
+    w = scene.workload_full_frame(index, ground=ground)
+    _log(f"case: render 1280x720 cumulus + cirrus, sweep frame {index}, "
+         f"{'planet ground depth' if ground else 'no depth buffer'}")
     renderer.load_workload(w)
     out = renderer.render_numpy(which=("composite", "cloud_transmittance"))
     fr = _oracle_frame(renderer, w)
     comp, lum, trn, _ = fr.render_region(0, 0, w.width, w.height)
     p999, med, mx = _rel_stats(out["composite"], comp)
-    _log(f"  composite: p99.9 {p999:.3e} median {med:.3e} max {mx:.3e}")
+    tp = float(np.percentile(np.abs(out["cloud_transmittance"][..., :3] - trn[..., :3]).max(-1), 99.9))
+    _log(f"  composite: p99.9 {p999:.3e} median {med:.3e} max {mx:.3e}; transmittance abs p99.9 {tp:.3e}; "
+         f"cloudy {(trn[..., 1] < 0.9).mean():.3f}")
     assert p999 <= 1e-3
     assert med <= 1e-5
+    assert tp <= 1e-3
 
 
 def test_f16_output_rounds_f32(renderer):
