@@ -68,6 +68,14 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// A value the compiler cannot see through, so that per-pixel FP64 work
+// recomputed inside the chunk loop is not hoisted out of it (and kept live
+// in registers across the whole loop).
+__device__ __forceinline__ int opaque_int(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // Scatter v times the eight trilinear weights of each lane's tap (lanes with
 // `has`) into the FP64 gradient accumulator. Same-address atomics within one
 // instruction serialise and a wave's taps share texels heavily, so the taps
@@ -138,103 +146,6 @@ __device__ __forceinline__ void delta_scatter_columns(double* __restrict__ g, fl
     }
 }
 
-// Geometry of the march toward the sun (cloud_march.wgsl:543-578 straight,
-// 599-666 cone) with the arithmetic of march_light, so sample positions and
-// altitudes match the forward bit for bit.
-struct LightWalk {
-    float dist, k, r0c, step;
-    int n;
-};
-
-__device__ __forceinline__ LightWalk light_walk(const RenderParams& P, const LayerF& L, float px, float py, float pz,
-                                                float alt) {
-    const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
-    LightWalk w;
-    float cos_sun = fmaxf(sy, 0.05f);
-    w.dist = fmaxf((L.top - alt) / cos_sun, 1.f);
-    float r0 = P.R + alt;
-    float qy = py + P.R;
-    w.r0c = fmaxf(r0, 1.f);
-    float cv = (px * sx + qy * sy + pz * sz) * frcp(w.r0c);
-    w.k = 2.f * r0 * cv;
-    if (P.cone) {
-        w.step = w.dist / 5.f;
-        w.n = 6;
-    } else {
-        w.step = w.dist / (float)P.light_steps;
-        w.n = P.light_steps;
-    }
-    return w;
-}
-
-// Light sample i: lookup-frame position (from the sample's lookup position
-// q0), altitude (from its world position p) and quadrature weight.
-__device__ __forceinline__ void light_sample(const RenderParams& P, const LightWalk& w, int i, float qx0, float qy0,
-                                             float qz0, float px, float py, float pz, float alt, float& qx,
-                                             float& qyy, float& qz, float& a, float& wt) {
-    const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
-    if (P.cone) {
-        const float off[6][3] = {{0.38051305f, 0.92453449f, -0.02111345f}, {-0.50625799f, -0.03590792f, -0.86163418f},
-                                 {-0.32509218f, -0.94557439f, 0.01428793f}, {0.09026238f, -0.27376545f, 0.95755165f},
-                                 {0.28128598f, 0.42443639f, -0.86065785f}, {-0.16852403f, 0.14748697f, 0.97460106f}};
-        float t, scale;
-        if (i < 5) { t = (i + 0.5f) * w.step; scale = w.step * (i + 1.f) / 5.f; wt = w.step; }
-        else { t = w.dist * 3.f; scale = w.step * 3.f; wt = w.step * 3.f; }
-        const float ox = sx * t + off[i][0] * scale, oy = sy * t + off[i][1] * scale, oz = sz * t + off[i][2] * scale;
-        float wx = px + ox, yy = py + oy + P.R, wz = pz + oz;
-        float rr = fsqrt(wx * wx + yy * yy + wz * wz);
-        a = alt + t * (w.k + t) * frcp(w.r0c + rr);
-        qx = qx0 + ox; qyy = qy0 + oy; qz = qz0 + oz;
-        return;
-    }
-    float t = (i + 0.5f) * w.step;
-    float wx = px + sx * t, yy = py + sy * t + P.R, wz = pz + sz * t;
-    float rr = fsqrt(wx * wx + yy * yy + wz * wz);
-    a = alt + t * (w.k + t) * frcp(w.r0c + rr);
-    qx = qx0 + sx * t; qyy = qy0 + sy * t; qz = qz0 + sz * t;
-    wt = w.step;
-}
-
-// Light march with its parameter partials: returns the optical depth and
-// d od / d density_scale, d od / d coverage; bit i of `lin` is set when
-// light sample i sits inside the clamp's linear range (0 < x < 1), where the
-// perturbation volume receives d od / d delta = density_scale * weight.
-template <bool DELTA>
-__device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
-                                           float qy0, float qz0, float px, float py, float pz, float alt, float wr,
-                                           uint32_t ctype, float& dod_ds, float& dod_cov, uint32_t& lin) {
-    const LightWalk w = light_walk(P, L, px, py, pz, alt);
-    const float wc = wr * L.coverage;
-    const float cov = clampf(wc, 0.f, 1.f);
-    const float dcov = (wc > 0.f && wc < 1.f) ? wr : 0.f;
-    float od = 0.f;
-    dod_ds = 0.f;
-    dod_cov = 0.f;
-    lin = 0u;
-    for (int i = 0; i < w.n; ++i) {
-        float qx, qyy, qz, a, wt;
-        light_sample(P, w, i, qx0, qy0, qz0, px, py, pz, alt, qx, qyy, qz, a, wt);
-        const float h = (a - L.base) * L.inv_thick;
-        float d = 0.f;
-        if (!(h < 0.f || h > 1.f) && cov >= 0.01f) {
-            float ax, ay, az, cx, cy, cz;
-            advect(P, L, qx, qyy, qz, h, ax, ay, az, cx, cy, cz);
-            float dcx = 0.f, dcy = 0.f, dcz = 0.f;
-            if constexpr (DELTA) delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
-            const float x = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, -1.f, dcx, dcy, dcz);
-            const float c = clampf(x, 0.f, 1.f);
-            d = c * L.density_scale;
-            dod_ds += c * wt;
-            if (x > 0.f && x < 1.f) {
-                dod_cov += L.density_scale * dcov * wt;
-                lin |= 1u << i;
-            }
-        }
-        od += d * wt;
-    }
-    return od;
-}
-
 // Scatter g (= dL/d od * density_scale) over each lane's light samples in
 // `lin` (lanes with has). Called with all 64 lanes active; light sample i of
 // every lane is scattered together.
@@ -287,15 +198,16 @@ struct BackQueue {
 // Shade queue entry j with its adjoint (cloud_march.wgsl:886-993): writes
 // the owner's record (v, dL/drho through the shading) and adds the
 // light-march and extinction contributions to the wave's layer sums.
-// Light-march scatter work a shaded sample leaves for the wave.
+// Light-march scatter work a shaded sample leaves for the wave (the sample's
+// position and layer are read back from its queue entry).
 struct LightScatter {
-    float qx, qy, qz, px, py, pz, alt, g;
+    float g;
     uint32_t lin;
-    int li;
 };
 
 template <bool DELTA>
-__device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQueue& Q, int j, float* rec_tile) {
+__device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQueue& Q, int j, float* rec_tile,
+                                                  const float* lrec_tile) {
     const float qx = Q.f[BQ_QX][j], qyl = Q.f[BQ_QY][j], qz = Q.f[BQ_QZ][j];
     const float alt = Q.f[BQ_ALT][j], wr = Q.f[BQ_WR][j], density = Q.f[BQ_DENS][j];
     const float T[3] = {Q.f[BQ_TR][j], Q.f[BQ_TG][j], Q.f[BQ_TB][j]};
@@ -303,6 +215,7 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     const uint32_t meta = __float_as_uint(Q.f[BQ_META][j]);
     const int owner = (int)(meta & 63u), li = (int)((meta >> 6) & 7u), slot = (int)((meta >> 12) & 31u);
     const uint32_t ctype = (meta >> 9) & 7u;
+    const int kd = (int)((meta >> 17) & 255u);
     const NdfC nc = ndf_coef(ctype);
     const LayerC& LC = P.layers[li];
     const LayerF L = LC.f;
@@ -317,9 +230,17 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     const float qy = py + P.R;
     float gds = 0.f, gcov = 0.f, gk = 0.f;
     // dense sample: light march and shading with their adjoints
-    float dod_ds, dod_cov;
+    // the training forward's light march and partials (samples past its
+    // record cap march again)
+    float od, dod_ds, dod_cov;
     uint32_t lin;
-    const float od = light_adj<DELTA>(P, L, nc, qx, qyl, qz, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
+    if (kd < P.lrec_cap) {
+        const float* r = lrec_tile + (size_t)kd * (LR_FIELDS * 64) + owner;
+        od = r[LR_OD * 64]; dod_ds = r[LR_DDS * 64]; dod_cov = r[LR_DCOV * 64];
+        lin = __float_as_uint(r[LR_LIN * 64]);
+    } else {
+        od = light_adj<DELTA>(P, L, nc, qx, qyl, qz, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
+    }
     const float hn = clampf((alt - L.base) * L.inv_thick, 0.f, 1.f);
     float st[3] = {0.f, 0.f, 0.f};
     {
@@ -414,7 +335,7 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
     atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
     LightScatter ls;
-    ls.qx = qx; ls.qy = qyl; ls.qz = qz; ls.px = px; ls.py = py; ls.pz = pz; ls.alt = alt; ls.lin = lin; ls.li = li;
+    ls.lin = lin;
     ls.g = g_od * L.density_scale;
     return ls;
 }
@@ -423,13 +344,21 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
 // the entries [64, qn) to the front. Called with all 64 lanes active.
 template <bool DELTA>
 __device__ __forceinline__ void flush_adj(const RenderParams& P, BackQueue& Q, int lane, int n, int qn,
-                                          float* rec_tile, bool want_delta) {
+                                          float* rec_tile, const float* lrec_tile, bool want_delta) {
     wave_sync_lds();
     LightScatter ls{};
-    if (lane < n) ls = shade_adj<DELTA>(P, Q, lane, rec_tile);
-    if (want_delta)
-        light_delta_scatter_wave(P, Q.colacc, P.layers[ls.li].f, lane < n, ls.qx, ls.qy, ls.qz, ls.px, ls.py,
-                                 ls.pz, ls.alt, ls.lin, ls.g);
+    if (lane < n) ls = shade_adj<DELTA>(P, Q, lane, rec_tile, lrec_tile);
+    if (want_delta) {
+        // the sample's lookup position, altitude and world position from its entry
+        const int j = lane < n ? lane : 0;
+        const uint32_t meta = __float_as_uint(Q.f[BQ_META][j]);
+        const int owner = (int)(meta & 63u), li = (int)((meta >> 6) & 7u);
+        const float t = Q.f[BQ_T][j];
+        const float px = P.cam_f[0] + Q.seed[SD_DX][owner] * t, py = P.cam_f[1] + Q.seed[SD_DY][owner] * t,
+                    pz = P.cam_f[2] + Q.seed[SD_DZ][owner] * t;
+        light_delta_scatter_wave(P, Q.colacc, P.layers[li].f, lane < n, Q.f[BQ_QX][j], Q.f[BQ_QY][j],
+                                 Q.f[BQ_QZ][j], px, py, pz, Q.f[BQ_ALT][j], ls.lin, ls.g);
+    }
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
@@ -559,6 +488,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     const bool want_delta = P.g_delta != nullptr;
     float* rec_tile = P.rec + tile * (size_t)(ckl * kRecFields * 64);
     float* rec = rec_tile + lane;
+    const float* lrec_tile = P.lrec + tile * (size_t)P.lrec_cap * (LR_FIELDS * 64);
     int qn = 0;  // queued samples (wave-uniform)
 
     while (__ballot(chunk >= 0) != 0ull) {
@@ -566,6 +496,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
         int s0 = 0, s1 = 0;
         float T[3] = {1.f, 1.f, 1.f};
         uint32_t dmask = 0u;
+        int kd = 0;                   // the pixel's dense samples before this one (forward order)
         if (act) {
             const int seg = chunk / cps, sub = chunk % cps;
             if (seg != cur_seg) {
@@ -573,12 +504,16 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 li = (int)((order >> (3 * seg)) & 7u);
                 const LayerC& LC = P.layers[li];
                 L = LC.f;
-                seg_geom(P, LC, ray, depth_t_max, jitter, g);
+                // the FP64 ray again (keeping it live across the chunk loop costs registers)
+                const int xo = opaque_int(xc), yo = opaque_int(yc);
+                const PixelRay ray2 = pixel_ray(P, xo, yo);
+                seg_geom(P, LC, ray2, pixel_depth_tmax(P, ray2, xo, yo), pixel_jitter(P, xo, yo), g);
             }
             s0 = sub * ckl;
             s1 = min(min(steps, s0 + ckl), n_exec - seg * steps);
-            const float* ckp = P.ck + (tile * P.ck_max + chunk) * 192 + lane;
+            const float* ckp = P.ck + (tile * P.ck_max + chunk) * PS_CK_STRIDE + lane;
             T[0] = ckp[0]; T[1] = ckp[64]; T[2] = ckp[128];
+            kd = (int)ckp[192];
         }
         // chunk forward, recomputed from the checkpoint; dense samples queued
         for (int k = 0; k < ckl; ++k) {
@@ -589,7 +524,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             if (act && s < s1) {
                 const float tl = (float)s * g.step;
                 t = g.t0 + tl;
-                seg_sample(g, dx, dy, dz, tl, px, py, pz, alt);
+                seg_sample(P, g, dx, dy, dz, tl, px, py, pz, alt);
                 const float h = (alt - L.base) * L.inv_thick;
                 const float wx = px - L.wind_off_x, wz = pz - L.wind_off_z;
                 float wfx, wfy;
@@ -630,7 +565,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                         if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
                             const float ds_ = P.inv_detail_scale;
                             const float2 cu = curl_fetch(P.curl, ax * ds_ * 128.f - 0.5f, az * ds_ * 128.f - 0.5f);
-                            const float2 b = base_fetch(P.base_rl, cx, cy, cz);
+                            const float2 b = base_fetch(P.base_rl, cx, cy, cz, P.base_rs, P.base_ls);
                             const float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
                             const float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
                             cloud = base_cloud * ndfv - (1.f - cov);
@@ -668,7 +603,8 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     Q.f[BQ_TR][pos] = T[0]; Q.f[BQ_TG][pos] = T[1]; Q.f[BQ_TB][pos] = T[2];
                     Q.f[BQ_T][pos] = t; Q.f[BQ_STEP][pos] = g.step;
                     Q.f[BQ_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
-                                                        ((uint32_t)k << 12));
+                                                        ((uint32_t)k << 12) | ((uint32_t)min(kd, 255) << 17));
+                    ++kd;
                     // step transmittance (the forward's update)
                     T[0] *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
                     T[1] *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * g.step);
@@ -677,12 +613,12 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 qn += __popcll(bal);
             }
             if (qn >= 64) {
-                flush_adj<DELTA>(P, Q, lane, 64, qn, rec_tile, want_delta);
+                flush_adj<DELTA>(P, Q, lane, 64, qn, rec_tile, lrec_tile, want_delta);
                 qn -= 64;
             }
         }
         if (qn > 0) {
-            flush_adj<DELTA>(P, Q, lane, qn, qn, rec_tile, want_delta);
+            flush_adj<DELTA>(P, Q, lane, qn, qn, rec_tile, lrec_tile, want_delta);
             qn = 0;
         }
         // the records written by other lanes are visible after the fence
@@ -720,7 +656,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     if (want_delta && cval < 1.f) {
                         const int s = s0 + slot;
                         float px, py, pz, alt;
-                        seg_sample(g, dx, dy, dz, (float)s * g.step, px, py, pz, alt);
+                        seg_sample(P, g, dx, dy, dz, (float)s * g.step, px, py, pz, alt);
                         const float h = (alt - L.base) * L.inv_thick;
                         float dcx, dcy, dcz;
                         delta_coords(P, px - L.wind_off_x, pz - L.wind_off_z, h, dcx, dcy, dcz);

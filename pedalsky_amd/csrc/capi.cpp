@@ -270,6 +270,7 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
     P.SV = luts.SV; P.sw = dm.skyview_w; P.sh = dm.skyview_h;
     P.AP = luts.AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
     P.base_rl = c->base_rl; P.rmax = c->rmax; P.detail_hf = c->detail_hf; P.curl = c->curl2;
+    base_fold_scales(c->noise_format, P.base_rs, P.base_ls);
     P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
     P.weather_pow2 = (c->ww & (c->ww - 1)) == 0 && (c->wh & (c->wh - 1)) == 0 ? 1 : 0;
     P.ww_log2 = 0;
@@ -499,7 +500,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->d_counters);
     dev_free(c->delta_bnd); dev_free(c->rt);
     for (ps_ctx::TaaView& v : c->taa_views) dev_free(v.hist);
-    dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->g_delta64);
+    dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->lrec); dev_free(c->g_delta64);
     dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
     dev_free(c->aTT); dev_free(c->aMST); dev_free(c->aSVT); dev_free(c->aAPT);
     if (c->atmo_params_h) hipHostFree(c->atmo_params_h);
@@ -876,10 +877,14 @@ ps_status ps_render_train(ps_ctx* c, const float* depth, const ps_image_out* out
     P.ck_len = c->ck_len;
     P.ck_per_seg = (P.steps + c->ck_len - 1) / c->ck_len;
     P.ck_max = std::max(P.n_layers, 1) * P.ck_per_seg;
-    if ((s = grow(&c->ck, &c->ck_cap, tiles * P.ck_max * 192, "transmittance checkpoints"))) return s;
+    if ((s = grow(&c->ck, &c->ck_cap, tiles * P.ck_max * PS_CK_STRIDE, "transmittance checkpoints"))) return s;
     if ((s = grow(&c->px_state, &c->px_cap, tiles * PX_FIELDS * 64, "forward totals"))) return s;
+    if ((s = grow(&c->lrec, &c->lrec_cap, tiles * (size_t)PS_LREC_CAP * LR_FIELDS * 64, "light-march records")))
+        return s;
     P.ck = c->ck;
     P.px_state = c->px_state;
+    P.lrec = c->lrec;
+    P.lrec_cap = PS_LREC_CAP;
     RenderParams* dP = nullptr;
     if ((s = c->render_ring.upload(P, c->stream, &dP))) return s;
     PS_HIP(ps_launch_render(dP, P.W, P.H, false, c->delta != nullptr, true, c->stream));

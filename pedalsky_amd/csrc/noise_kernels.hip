@@ -177,13 +177,25 @@ __global__ __launch_bounds__(256) void k_fold(int format, const void* base, cons
                                               uint32_t* base_rl, float* detail_hf, float2* curl2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 128 * 128 * 128) {
-        float r, g, b, a;
-        load4(base, (size_t)i, format, r, g, b, a);
-        // R and the low-frequency FBM sum the march reads, as 16-bit UNORM
-        // (step 1.5e-5) in the bricked layout
-        float lf = g * 0.625f + b * 0.25f + a * 0.125f;
-        uint32_t qr = (uint32_t)rintf(clampf(r, 0.f, 1.f) * 65535.f);
-        uint32_t ql = (uint32_t)rintf(clampf(lf, 0.f, 1.f) * 65535.f);
+        // R and the low-frequency FBM sum the march reads, as 16-bit fixed
+        // point in the bricked layout, with scales that hold the stored
+        // values exactly where 16 bits can: UNORM8 input R = k/255 as 257 k
+        // (scale 65535) and LF = (5 kG + 2 kB + kA) / 2040 as its numerator
+        // (scale 2040); binary16 input R as R * 32768 (exact for R >= 1/32)
+        // and LF as LF * 65535 (step 1.5e-5). base_fold_scales gives the
+        // scales the march multiplies by.
+        uint32_t qr, ql;
+        if (format == 0) {
+            const uchar4 q = ((const uchar4*)base)[i];
+            qr = (uint32_t)q.x * 257u;
+            ql = 5u * q.y + 2u * q.z + q.w;
+        } else {
+            float r, g, b, a;
+            load4(base, (size_t)i, format, r, g, b, a);
+            const float lf = g * 0.625f + b * 0.25f + a * 0.125f;
+            qr = (uint32_t)rintf(clampf(r, 0.f, 1.f) * 32768.f);
+            ql = (uint32_t)rintf(clampf(lf, 0.f, 1.f) * 65535.f);
+        }
         int x = i & 127, y = (i >> 7) & 127, z = i >> 14;
         base_rl[brick_x(x) + brick_y(y) + brick_z(z)] = qr | (ql << 16);
     }
@@ -209,7 +221,7 @@ __global__ __launch_bounds__(256) void k_fold(int format, const void* base, cons
 // texels [4b, 4b + 4] on each axis (the brick plus the +1 neighbour a
 // trilinear tap whose lower corner lies in the brick also reads), rounded up
 // to a multiple of 1/255.
-__global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_rl, uint8_t* rmax) {
+__global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_rl, uint32_t r_full, uint8_t* rmax) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= 32 * 32 * 32) return;
     int bx = b & 31, by = (b >> 5) & 31, bz = b >> 10;
@@ -220,8 +232,8 @@ __global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_r
                 int x = (bx * 4 + i) & 127, y = (by * 4 + j) & 127, z = (bz * 4 + k) & 127;
                 m = max(m, base_rl[brick_x(x) + brick_y(y) + brick_z(z)] & 0xffffu);
             }
-    // u16 -> u8 rounded up: ceil(m * 255 / 65535)
-    rmax[b] = (uint8_t)min((m * 255u + 65534u) / 65535u, 255u);
+    // fixed point (R = m / r_full) -> u8 rounded up: ceil(m * 255 / r_full)
+    rmax[b] = (uint8_t)min((m * 255u + r_full - 1u) / r_full, 255u);
 }
 
 // Texel (x, y) of the output holds channel 0 of (x, y), (x+1, y) in .x and
@@ -249,7 +261,8 @@ hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail
                                 uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s) {
     hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl, base_rl,
                        detail_hf, curl2);
-    hipLaunchKernelGGL(k_rmax, dim3(32 * 32 * 32 / 64), dim3(64), 0, s, base_rl, rmax);
+    hipLaunchKernelGGL(k_rmax, dim3(32 * 32 * 32 / 64), dim3(64), 0, s, base_rl, format == 0 ? 65535u : 32768u,
+                       rmax);
     return hipGetLastError();
 }
 

@@ -113,6 +113,7 @@ struct RenderParams {
     const float4* SV; int sw, sh;
     const float4* AP; int aw, ah, ad;
     const uint32_t* base_rl;          // 128^3 in 4x4x4 bricks: u16 R | u16 (0.625G + 0.25B + 0.125A) << 16
+    float base_rs, base_ls;           // fixed-point scales of base_rl's R and LF (base_fold_scales)
     const uint8_t* rmax;              // 32^3: per brick, max R over the brick and its +1 halo (ceil to 1/255)
     const float* detail_hf;           // 32^3 (0.625R + 0.25G + 0.125B)
     const float2* curl;               // 128^2 (2*rg - 1)
@@ -158,11 +159,17 @@ struct RenderParams {
     float delta_su;                   // 128 / weather_scale: delta texels per metre
     // ---- training forward (checkpoints) and backward ----
     // Per wave tile (tiles row-major, 64 lanes): transmittance checkpoints
-    // [tile][chunk][rgb][lane] at the first step of every ck_len-step chunk
+    // [tile][chunk][rgb + dense count][lane] at the first step of every ck_len-step chunk
     // of every layer segment, and the per-pixel forward totals
     // [tile][PX_FIELDS][lane].
     float* ck;
     float* px_state;
+    // Light-march results of the training forward, per wave tile and per
+    // pixel's k-th dense sample (k < lrec_cap, in march order):
+    // [tile][k][LR_FIELDS][lane]. The backward reads them instead of
+    // marching toward the sun again; samples beyond the cap march again.
+    float* lrec;
+    int lrec_cap;
     int ck_len;                       // checkpoint interval (primary steps), 1..32
     int ck_per_seg;                   // ceil(steps / ck_len)
     int ck_max;                       // chunk slots per pixel (n_layers * ck_per_seg)
@@ -198,6 +205,14 @@ struct TaaParams {
 
 // Per-pixel forward totals stored by the training forward.
 enum { PX_LR = 0, PX_LG, PX_LB, PX_TR, PX_TG, PX_TB, PX_TW, PX_TWN, PX_NEXEC, PX_FIELDS };
+// A transmittance checkpoint: rgb and the number of dense samples the pixel
+// had before the chunk (as a float), each a row of 64 lanes.
+#define PS_CK_STRIDE 256
+// Stored light-march results per dense sample: optical depth, its partials
+// in density_scale and coverage, and the bit mask of light samples inside
+// the density clamp's linear range.
+enum { LR_OD = 0, LR_DDS, LR_DCOV, LR_LIN, LR_FIELDS };
+#define PS_LREC_CAP 32
 
 // ---------------------------------------------------------------------------
 // Small vector helpers.

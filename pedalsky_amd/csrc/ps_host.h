@@ -155,6 +155,8 @@ struct ps_ctx {
     size_t px_cap = 0;
     float* rec = nullptr;
     size_t rec_cap = 0;
+    float* lrec = nullptr;        // light-march records of the training forward
+    size_t lrec_cap = 0;
     double* g_delta64 = nullptr;  // FP64 accumulator of the perturbation-volume gradient
     RenderParams train{};         // parameter block of the last training forward
     bool train_valid = false;

@@ -50,6 +50,14 @@ hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* 
 // Noise: bake the stored volumes (format 0 = UNORM8, 1 = FP16), then fold
 // them into the render layouts.
 hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* detail, void* curl, hipStream_t s);
+// The factors that turn the folded base volume's fixed-point R and LF
+// channels back into values, for the noise storage format (0 UNORM8, 1
+// binary16).
+inline void base_fold_scales(int format, float& r_scale, float& lf_scale) {
+    r_scale = format == 0 ? 1.f / 65535.f : 1.f / 32768.f;
+    lf_scale = format == 0 ? 1.f / 2040.f : 1.f / 65535.f;
+}
+
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
                                 uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s);
 // Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads

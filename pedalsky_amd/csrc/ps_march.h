@@ -12,14 +12,16 @@ struct Cnt {
              octaves = 0, ap = 0;
 };
 
-// The 16-bit channels are interpolated as integers-in-float and scaled by
-// 1/65535 once after the trilinear blend (filtering is linear).
+// The 16-bit channels are interpolated as integers-in-float and scaled once
+// after the trilinear blend (filtering is linear).
 __device__ __forceinline__ float2 unpack_rl(uint32_t v) {
     return make_float2((float)(v & 0xffffu), (float)(v >> 16));
 }
 
-// Trilinear Repeat fetch of the folded base volume (128^3, bricked, 2 x u16).
-__device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, float cx, float cy, float cz) {
+// Trilinear Repeat fetch of the folded base volume (128^3, bricked, 2 x u16
+// fixed point; rs and ls turn the blended R and LF integers into values).
+__device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, float cx, float cy, float cz, float rs,
+                                             float ls) {
     float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
     float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
@@ -43,8 +45,8 @@ __device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, flo
         float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
         r.y = p0 + (p1 - p0) * fz;
     }
-    r.x *= 1.f / 65535.f;
-    r.y *= 1.f / 65535.f;
+    r.x *= rs;
+    r.y *= ls;
     return r;
 }
 
@@ -244,7 +246,7 @@ __device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, f
         rm = (float)ld_g<uint8_t>(P.rmax, (bz << 10) | (by << 5) | bx) * (1.f / 255.f);
     }
     if ((rm + 1.f) * 0.5f * ndfv <= thr) return -1.f;
-    float2 b = base_fetch(P.base_rl, cx, cy, cz);
+    float2 b = base_fetch(P.base_rl, cx, cy, cz, P.base_rs, P.base_ls);
     float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
     float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
     float v = base_cloud * ndfv - (1.f - cov);
@@ -293,56 +295,131 @@ __device__ __forceinline__ float erode(const RenderParams& P, const LayerF& L, f
     return (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
 }
 
-// cloud_march.wgsl:543-578 (straight) and 599-666 (cone), FP32. (px, py, pz)
-// is the sample's world position (its radius and local up), (qx, qy, qz)
-// the same point in the segment's lookup frame (the density fetches).
-template <bool DELTA = false>
-__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
-                                             float qy0, float qz0, float px, float py, float pz, float alt,
-                                             float weather_r, uint32_t ctype, Cnt& cnt) {
+// Geometry of the march toward the sun (cloud_march.wgsl:543-578 straight,
+// 599-666 cone) with the arithmetic of march_light, so sample positions and
+// altitudes match the forward bit for bit.
+struct LightWalk {
+    float dist, k, r0c, step;
+    int n;
+};
+
+__device__ __forceinline__ LightWalk light_walk(const RenderParams& P, const LayerF& L, float px, float py, float pz,
+                                                float alt) {
     const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
+    LightWalk w;
     float cos_sun = fmaxf(sy, 0.05f);
-    float dist = fmaxf((L.top - alt) / cos_sun, 1.f);
+    w.dist = fmaxf((L.top - alt) / cos_sun, 1.f);
     float r0 = P.R + alt;
     float qy = py + P.R;
-    const float r0c = fmaxf(r0, 1.f);
-    float cv = (px * sx + qy * sy + pz * sz) * frcp(r0c);
-    float k = 2.f * r0 * cv;
-    float od = 0.f;
-    bool cov;
+    w.r0c = fmaxf(r0, 1.f);
+    float cv = (px * sx + qy * sy + pz * sz) * frcp(w.r0c);
+    w.k = 2.f * r0 * cv;
+    if (P.cone) {
+        w.step = w.dist / 5.f;
+        w.n = 6;
+    } else {
+        w.step = w.dist / (float)P.light_steps;
+        w.n = P.light_steps;
+    }
+    return w;
+}
+
+// Light sample i: lookup-frame position (from the sample's lookup position
+// q0), altitude (from its world position p) and quadrature weight.
+__device__ __forceinline__ void light_sample(const RenderParams& P, const LightWalk& w, int i, float qx0, float qy0,
+                                             float qz0, float px, float py, float pz, float alt, float& qx,
+                                             float& qyy, float& qz, float& a, float& wt) {
+    const float sx = P.sun[0], sy = P.sun[1], sz = P.sun[2];
     if (P.cone) {
         const float off[6][3] = {{0.38051305f, 0.92453449f, -0.02111345f}, {-0.50625799f, -0.03590792f, -0.86163418f},
                                  {-0.32509218f, -0.94557439f, 0.01428793f}, {0.09026238f, -0.27376545f, 0.95755165f},
                                  {0.28128598f, 0.42443639f, -0.86065785f}, {-0.16852403f, 0.14748697f, 0.97460106f}};
-        float step = dist / 5.f;
-        for (int i = 0; i < 6; ++i) {
-            float t, scale, w;
-            if (i < 5) { t = (i + 0.5f) * step; scale = step * (i + 1.f) / 5.f; w = step; }
-            else { t = dist * 3.f; scale = step * 3.f; w = step * 3.f; }
-            const float ox = sx * t + off[i][0] * scale, oy = sy * t + off[i][1] * scale,
-                        oz = sz * t + off[i][2] * scale;
-            float wx = px + ox, yy = py + oy + P.R, wz = pz + oz;
-            float rr = fsqrt(wx * wx + yy * yy + wz * wz);
-            float a = alt + t * (k + t) * frcp(r0c + rr);
-            float d = density_light<DELTA>(P, L, nc, qx0 + ox, qy0 + oy, qz0 + oz, a, weather_r, ctype, cov);
-            cnt.light++;
-            cnt.light_cov += cov ? 1u : 0u;
-            od += d * w;
-        }
-        return od;
-    }
-    float step = dist / (float)P.light_steps;
-    for (int i = 0; i < P.light_steps; ++i) {
-        float t = (i + 0.5f) * step;
-        float wx = px + sx * t, yy = py + sy * t + P.R, wz = pz + sz * t;
+        float t, scale;
+        if (i < 5) { t = (i + 0.5f) * w.step; scale = w.step * (i + 1.f) / 5.f; wt = w.step; }
+        else { t = w.dist * 3.f; scale = w.step * 3.f; wt = w.step * 3.f; }
+        const float ox = sx * t + off[i][0] * scale, oy = sy * t + off[i][1] * scale, oz = sz * t + off[i][2] * scale;
+        float wx = px + ox, yy = py + oy + P.R, wz = pz + oz;
         float rr = fsqrt(wx * wx + yy * yy + wz * wz);
-        float a = alt + t * (k + t) * frcp(r0c + rr);
-        float d = density_light<DELTA>(P, L, nc, qx0 + sx * t, qy0 + sy * t, qz0 + sz * t, a, weather_r, ctype, cov);
-        cnt.light++;
-        cnt.light_cov += cov ? 1u : 0u;
-        od += d * step;
+        a = alt + t * (w.k + t) * frcp(w.r0c + rr);
+        qx = qx0 + ox; qyy = qy0 + oy; qz = qz0 + oz;
+        return;
+    }
+    float t = (i + 0.5f) * w.step;
+    float wx = px + sx * t, yy = py + sy * t + P.R, wz = pz + sz * t;
+    float rr = fsqrt(wx * wx + yy * yy + wz * wz);
+    a = alt + t * (w.k + t) * frcp(w.r0c + rr);
+    qx = qx0 + sx * t; qyy = qy0 + sy * t; qz = qz0 + sz * t;
+    wt = w.step;
+}
+
+// The march toward the sun: returns the optical depth and, with PARTIALS,
+// d od / d density_scale, d od / d coverage and the linear-range mask of
+// light_adj. `cnt` (or null) counts the light samples and covered ones.
+template <bool DELTA, bool PARTIALS>
+__device__ __forceinline__ float light_march_impl(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
+                                                  float qy0, float qz0, float px, float py, float pz, float alt,
+                                                  float wr, uint32_t ctype, Cnt* cnt, float& dod_ds, float& dod_cov,
+                                                  uint32_t& lin) {
+    const LightWalk w = light_walk(P, L, px, py, pz, alt);
+    const float wc = wr * L.coverage;
+    const float cov = clampf(wc, 0.f, 1.f);
+    const float dcov = (wc > 0.f && wc < 1.f) ? wr : 0.f;
+    float od = 0.f;
+    dod_ds = 0.f;
+    dod_cov = 0.f;
+    lin = 0u;
+    for (int i = 0; i < w.n; ++i) {
+        float qx, qyy, qz, a, wt;
+        light_sample(P, w, i, qx0, qy0, qz0, px, py, pz, alt, qx, qyy, qz, a, wt);
+        const float h = (a - L.base) * L.inv_thick;
+        float d = 0.f;
+        if (cnt) cnt->light++;
+        if (!(h < 0.f || h > 1.f) && cov >= 0.01f) {
+            if (cnt) cnt->light_cov++;
+            float ax, ay, az, cx, cy, cz;
+            advect(P, L, qx, qyy, qz, h, ax, ay, az, cx, cy, cz);
+            float dcx = 0.f, dcy = 0.f, dcz = 0.f;
+            if constexpr (DELTA) delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
+            const float x = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, -1.f, dcx, dcy, dcz);
+            const float c = clampf(x, 0.f, 1.f);
+            d = c * L.density_scale;
+            if constexpr (PARTIALS) {
+                dod_ds += c * wt;
+                if (x > 0.f && x < 1.f) {
+                    dod_cov += L.density_scale * dcov * wt;
+                    if (i < 32) lin |= 1u << i;
+                }
+            }
+        }
+        od += d * wt;
     }
     return od;
+}
+
+// The light march's optical depth (cloud_march.wgsl:543-578 straight,
+// 599-666 cone) for the forward, counting its work. (px, py, pz) is the
+// sample's world position (its radius and local up), (qx0, qy0, qz0) the same
+// point in the segment's lookup frame (the density fetches).
+template <bool DELTA = false>
+__device__ __forceinline__ float march_light(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
+                                             float qy0, float qz0, float px, float py, float pz, float alt,
+                                             float weather_r, uint32_t ctype, Cnt& cnt) {
+    float a, b;
+    uint32_t c;
+    return light_march_impl<DELTA, false>(P, L, nc, qx0, qy0, qz0, px, py, pz, alt, weather_r, ctype, &cnt, a, b, c);
+}
+
+// The same optical depth (the same arithmetic, so the training forward, the
+// plain forward and the backward agree bit for bit) with its partials in
+// density_scale and coverage; bit i of `lin` is set when light sample i
+// sits inside the density clamp's linear range (0 < x < 1), where the
+// perturbation volume receives d od / d delta = density_scale * weight.
+template <bool DELTA>
+__device__ __forceinline__ float light_adj(const RenderParams& P, const LayerF& L, const NdfC& nc, float qx0,
+                                           float qy0, float qz0, float px, float py, float pz, float alt, float wr,
+                                           uint32_t ctype, float& dod_ds, float& dod_cov, uint32_t& lin) {
+    return light_march_impl<DELTA, true>(P, L, nc, qx0, qy0, qz0, px, py, pz, alt, wr, ctype, nullptr, dod_ds, dod_cov,
+                                         lin);
 }
 
 // Transmittance-LUT coordinates for a point at altitude h whose local up
@@ -481,7 +558,7 @@ __device__ __forceinline__ PixelRay pixel_ray(const RenderParams& P, int xc, int
 // periodic lookup (the wrap_* periods): the same texels as the world
 // position, with coordinates that stay below a few hundred km. Its altitude
 // is h_e + q / (r_e + sqrt(r_e^2 + q)), q = tl (k_e + tl), tl = s step, from
-// the first sample's altitude h_e, radius r_e and k_e = 2 (p_e - c).d
+// the first sample's altitude h_e, radius r_e = R + h_e and k_e = 2 (p_e - c).d
 // (|p - c|^2 = r_e^2 + tl k_e + tl^2), the reference's altitude_from_entry
 // (cloud_march.wgsl:360-365) anchored at the segment instead of the camera.
 // With world positions from the camera, FP32 rounding at a far-side shell
@@ -490,7 +567,7 @@ __device__ __forceinline__ PixelRay pixel_ray(const RenderParams& P, int xc, int
 // ---------------------------------------------------------------------------
 struct SegGeom {
     float ox, oy, oz;                 // lookup-frame position of the first sample
-    float he, ke, re;                 // its altitude, 2 (p - c).d and radius
+    float he, ke;                     // its altitude and 2 (p - c).d (radius r_e = R + h_e)
     float t0;                         // its ray distance
     float step;
 };
@@ -511,7 +588,6 @@ __device__ __forceinline__ bool seg_geom(const RenderParams& P, const LayerC& LC
     const double re = sqrt(fmax(r2, 0.0));
     g.he = (float)(re - (double)P.R);
     g.ke = (float)(2.0 * (ray.b + t0));
-    g.re = (float)re;
     g.ox = (float)wrap_period(P.cam[0] + ray.dxd * t0, P.wrap_xz);
     g.oy = (float)wrap_period(P.cam[1] + ray.dyd * t0, P.wrap_y);
     g.oz = (float)wrap_period(P.cam[2] + ray.dzd * t0, P.wrap_xz);
@@ -520,12 +596,15 @@ __device__ __forceinline__ bool seg_geom(const RenderParams& P, const LayerC& LC
     return true;
 }
 
-// Sample at tl = s step of a segment: lookup position, altitude.
-__device__ __forceinline__ void seg_sample(const SegGeom& g, float dx, float dy, float dz, float tl, float& qx,
-                                           float& qy, float& qz, float& alt) {
+// Sample at tl = s step of a segment: lookup position, altitude. The radius
+// enters only the denominator, where its FP32 rounding (R + h_e, 0.5 m) is a
+// relative 1e-7.
+__device__ __forceinline__ void seg_sample(const RenderParams& P, const SegGeom& g, float dx, float dy, float dz,
+                                           float tl, float& qx, float& qy, float& qz, float& alt) {
     qx = g.ox + dx * tl; qy = g.oy + dy * tl; qz = g.oz + dz * tl;
     const float qq = tl * (g.ke + tl);
-    alt = g.he + qq * frcp(g.re + fsqrt(fmaf(g.re, g.re, qq)));
+    const float re = P.R + g.he;
+    alt = g.he + qq * frcp(re + fsqrt(fmaf(re, re, qq)));
 }
 
 // Sky pass (sky.wgsl:159-220): sky-view LUT, overcast blend, sun disk.

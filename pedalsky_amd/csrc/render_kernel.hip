@@ -69,10 +69,12 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // Shade one queued dense sample (cloud_march.wgsl:886-993) and add its
 // luminance and depth-weight contributions to the owning pixel. (odx, ody,
-// odz) is the owner's view ray and omc its 1 - cos(view, sun).
-template <bool DELTA>
+// odz) is the owner's view ray and omc its 1 - cos(view, sun). TRAIN: the
+// light march also takes its parameter partials, and the results go to the
+// owner's light-march record k (the backward's input) when k < lrec_cap.
+template <bool DELTA, bool TRAIN>
 __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q, int j, Cnt& cnt, float odx,
-                                             float ody, float odz, float omc) {
+                                             float ody, float odz, float omc, float* lrec_tile) {
     const float qx = Q.f[QF_QX][j], qy = Q.f[QF_QY][j], qz = Q.f[QF_QZ][j];
     const float alt = Q.f[QF_ALT][j], wr = Q.f[QF_WR][j], density = Q.f[QF_DENS][j];
     const float Tr = Q.f[QF_TR][j], Tg = Q.f[QF_TG][j], Tb = Q.f[QF_TB][j];
@@ -86,7 +88,20 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
     // distance need only its direction from the planet centre)
     const float px = P.cam_f[0] + odx * t, py = P.cam_f[1] + ody * t, pz = P.cam_f[2] + odz * t;
     cnt.dense++;
-    const float od = march_light<DELTA>(P, L, ndf_coef(ctype), qx, qy, qz, px, py, pz, alt, wr, ctype, cnt);
+    float od;
+    if constexpr (TRAIN) {
+        float dod_ds, dod_cov;
+        uint32_t lin;
+        od = light_adj<DELTA>(P, L, ndf_coef(ctype), qx, qy, qz, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
+        const int k = (int)((meta >> 12) & 255u);
+        if (k < P.lrec_cap) {
+            float* r = lrec_tile + (size_t)k * (LR_FIELDS * 64) + owner;
+            r[LR_OD * 64] = od; r[LR_DDS * 64] = dod_ds; r[LR_DCOV * 64] = dod_cov;
+            r[LR_LIN * 64] = __uint_as_float(lin);
+        }
+    } else {
+        od = march_light<DELTA>(P, L, ndf_coef(ctype), qx, qy, qz, px, py, pz, alt, wr, ctype, cnt);
+    }
     const float sig0 = L.sigma_t[0], sig1 = L.sigma_t[1], sig2 = L.sigma_t[2];
     // Beer-Powder (cloud_march.wgsl:888-892); exp(-2x) = exp(-x)^2
     const float e0r = __expf(-sig0 * od), e0g = __expf(-sig1 * od), e0b = __expf(-sig2 * od);
@@ -180,14 +195,14 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
 // Shade the first n (<= 64) queued samples with the whole wave, then move
 // the entries [64, qn) to the front. Called with all 64 lanes active; (dx,
 // dy, dz) and omc are this lane's own view ray and 1 - cos(view, sun).
-template <bool DELTA>
+template <bool DELTA, bool TRAIN>
 __device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q, int lane, int n, int qn, Cnt& cnt,
-                                            float dx, float dy, float dz, float omc) {
+                                            float dx, float dy, float dz, float omc, float* lrec_tile) {
     wave_sync_lds();
     const int owner = lane < n ? (int)(__float_as_uint(Q.f[QF_META][lane]) & 63u) : lane;
     const float odx = __shfl(dx, owner), ody = __shfl(dy, owner), odz = __shfl(dz, owner);
     const float oomc = __shfl(omc, owner);
-    if (lane < n) shade_record<DELTA>(P, Q, lane, cnt, odx, ody, odz, oomc);
+    if (lane < n) shade_record<DELTA, TRAIN>(P, Q, lane, cnt, odx, ody, odz, oomc, lrec_tile);
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
@@ -258,6 +273,9 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     int next = 0;                 // the lane's next segment (rank in its entry order)
     bool alive = nseg > 0;        // not yet opaque
     int qn = 0;                   // queued samples (wave-uniform)
+    int kd = 0;                   // training forward: the pixel's dense samples so far
+    float* lrec_tile = TRAIN ? P.lrec + (size_t)(tile_y * P.n_tiles_x + tile_x) * P.lrec_cap * (LR_FIELDS * 64)
+                             : nullptr;
     for (;;) {
         const bool want = alive && next < nseg;
         const unsigned long long wb = __ballot(want);
@@ -277,14 +295,15 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         // training checkpoints of this segment: chunk next * ck_per_seg + s / ck_len
         float* ckp = nullptr;
         if constexpr (TRAIN)
-            ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + next * P.ck_per_seg) * 192 + lane;
+            ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + next * P.ck_per_seg) * PS_CK_STRIDE +
+                  lane;
         NdfC ncl = layer_ndf(L);
         for (int s = 0; s < P.steps; ++s) {
             if (__ballot(on) == 0ull) break;
             if constexpr (TRAIN) {
                 if (on && s % P.ck_len == 0) {
-                    float* c = ckp + (size_t)(s / P.ck_len) * 192;
-                    c[0] = tr; c[64] = tg; c[128] = tbb;
+                    float* c = ckp + (size_t)(s / P.ck_len) * PS_CK_STRIDE;
+                    c[0] = tr; c[64] = tg; c[128] = tbb; c[192] = (float)kd;
                 }
             }
             bool dense = false;
@@ -294,7 +313,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                 cnt.primary++;
                 const float tl = (float)s * g.step;
                 t = g.t0 + tl;
-                seg_sample(g, dx, dy, dz, tl, qx, qy, qz, alt);
+                seg_sample(P, g, dx, dy, dz, tl, qx, qy, qz, alt);
                 const float h = (alt - L.base) * L.inv_thick;
                 // the weather quad and the brick maximum depend on the position
                 // only: both loads are issued before either is used
@@ -342,7 +361,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                         if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
                             const float ds = P.inv_detail_scale;
                             const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
-                            const float2 b = base_fetch(P.base_rl, cx, cy, cz);
+                            const float2 b = base_fetch(P.base_rl, cx, cy, cz, P.base_rs, P.base_ls);
                             const float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
                             const float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
                             cloud = base_cloud * ndfv - (1.f - cov);
@@ -373,7 +392,9 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                     Q.f[QF_ALT][pos] = alt; Q.f[QF_WR][pos] = wr; Q.f[QF_DENS][pos] = density;
                     Q.f[QF_TR][pos] = tr; Q.f[QF_TG][pos] = tg; Q.f[QF_TB][pos] = tbb;
                     Q.f[QF_T][pos] = t; Q.f[QF_STEP][pos] = g.step;
-                    Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9));
+                    Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
+                                                        ((uint32_t)min(kd, 255) << 12));
+                    if constexpr (TRAIN) ++kd;
                     tr *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
                     tg *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * g.step);
                     tbb *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * g.step);
@@ -385,13 +406,13 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                 qn += __popcll(bal);
             }
             if (qn >= 64) {
-                flush_queue<DELTA>(P, Q, lane, 64, qn, cnt, dx, dy, dz, omcf);
+                flush_queue<DELTA, TRAIN>(P, Q, lane, 64, qn, cnt, dx, dy, dz, omcf, lrec_tile);
                 qn -= 64;
             }
         }
         if (in_round) ++next;
     }
-    if (qn > 0) flush_queue<DELTA>(P, Q, lane, qn, qn, cnt, dx, dy, dz, omcf);
+    if (qn > 0) flush_queue<DELTA, TRAIN>(P, Q, lane, qn, qn, cnt, dx, dy, dz, omcf, lrec_tile);
     wave_sync_lds();
     cnt.layer_marches = (uint32_t)nseg;
 

@@ -184,10 +184,12 @@ def test_benchmark_batch_64_scenes(setup):
     tiles, and a 64x64 window of every scene (its cloudiest) must match the
     FP64 oracle at the plain-path bar."""
     import torch
-    r, _ = setup
+    r, ws6 = setup
     ws = scene.workload_batch(64, 0xC3)
+    # the single path reads the context's overcast mask (loaded from the
+    # fixture's first scene): every batch scene gets that one
     for w in ws:
-        w.overcast = ws[0].overcast
+        w.overcast = ws6[0].overcast
     _log(f"case: 64-scene batch seed 0xC3, {sum(w.width * w.height for w in ws) / 1e6:.1f} Mpix")
     b = Batch(r, ws, outputs=("composite", "cloud_transmittance"), tuning=abi.Tuning(8, 0, 8, 1))
     b.capture()
