@@ -146,10 +146,85 @@ __device__ __forceinline__ void delta_scatter_columns(double* __restrict__ g, fl
     }
 }
 
+// Perturbation gradients of many taps of a wave accumulate in a per-wave LDS
+// window of 4 x 4 texel columns (x, z) by 32 height rows around the first
+// tap; a tap whose eight corners all fall inside adds there, any other goes
+// straight to the FP64 accumulator. The window is added to the accumulator
+// (one FP64 atomic per non-zero texel) once per batch of taps: a chunk's
+// primary samples, or one light-sample index of a queue flush.
+constexpr int kWinCols = 4, kWinRows = 32, kWinSlots = kWinCols * kWinCols * kWinRows;
+
+struct DeltaWindow {
+    uint32_t x0, z0, y0;              // lowest texel column (x, z) and row
+    bool set;
+};
+
+__device__ __forceinline__ void delta_scatter_window(double* __restrict__ g, float* win, DeltaWindow& w, bool has,
+                                                     const DeltaTap& d, float v) {
+    const uint32_t xi = d.x0, zi = d.z0 >> 14, r0 = d.y0 >> 7, r1 = d.y1 >> 7;
+    const unsigned long long act = __ballot(has);
+    if (act == 0ull) return;
+    if (!w.set) {
+        const int leader = __builtin_ctzll(act);
+        w.x0 = (__builtin_amdgcn_readlane(xi, leader) + 127u) & 127u;
+        w.z0 = (__builtin_amdgcn_readlane(zi, leader) + 127u) & 127u;
+        const uint32_t lr = __builtin_amdgcn_readlane(r0, leader);
+        w.y0 = lr >= kWinRows / 2 ? min(lr - kWinRows / 2, 128u - kWinRows) : 0u;
+        w.set = true;
+    }
+    if (!has) return;
+    const uint32_t lx = (xi - w.x0) & 127u, lz = (zi - w.z0) & 127u;
+    const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
+    if (lx < kWinCols - 1 && lz < kWinCols - 1 && r0 >= w.y0 && r1 - w.y0 < (uint32_t)kWinRows) {
+        // slot = ((z * 4 + x) * 32 + row)
+        const uint32_t c00 = (lz * kWinCols + lx) * kWinRows, c01 = c00 + kWinRows, c10 = c00 + kWinCols * kWinRows,
+                       c11 = c10 + kWinRows;
+        const uint32_t a = r0 - w.y0, b = r1 - w.y0;
+        atomicAdd(win + c00 + a, v * wz0 * wy0 * wx0);
+        atomicAdd(win + c01 + a, v * wz0 * wy0 * wx1);
+        atomicAdd(win + c00 + b, v * wz0 * wy1 * wx0);
+        atomicAdd(win + c01 + b, v * wz0 * wy1 * wx1);
+        atomicAdd(win + c10 + a, v * wz1 * wy0 * wx0);
+        atomicAdd(win + c11 + a, v * wz1 * wy0 * wx1);
+        atomicAdd(win + c10 + b, v * wz1 * wy1 * wx0);
+        atomicAdd(win + c11 + b, v * wz1 * wy1 * wx1);
+    } else {
+        const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
+                                 d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
+        const float wt[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
+                             v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)wt[c]);
+    }
+}
+
+// Add a chunk's window to the accumulator and clear it. Called with all 64
+// lanes active.
+__device__ __forceinline__ void delta_window_flush(double* __restrict__ g, float* win, DeltaWindow& w) {
+    if (!w.set) return;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kWinSlots / 64; ++k) {
+        const int slot = k * 64 + lane;
+        const float sum = win[slot];
+        if (sum != 0.f) {
+            const uint32_t row = (uint32_t)slot % kWinRows, col = (uint32_t)slot / kWinRows;
+            const uint32_t x = (w.x0 + col % kWinCols) & 127u, z = (w.z0 + col / kWinCols) & 127u;
+            atomicAdd(g + ((z << 14) | ((w.y0 + row) << 7) | x), (double)sum);
+            win[slot] = 0.f;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    w.set = false;
+}
+
 // Scatter g (= dL/d od * density_scale) over each lane's light samples in
 // `lin` (lanes with has). Called with all 64 lanes active; light sample i of
 // every lane is scattered together.
-__device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, float* colacc, const LayerF& L,
+__device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, float* win, const LayerF& L,
                                                          bool has, float qx0, float qy0, float qz0, float px,
                                                          float py, float pz, float alt, uint32_t lin, float g) {
     const LightWalk w = light_walk(P, L, px, py, pz, alt);
@@ -169,7 +244,9 @@ __device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, 
             delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
             d = delta_tap(dcx, dcy, dcz);
         }
-        delta_scatter_columns(P.g_delta64, colacc, hi, d, g * wt);
+        DeltaWindow dw{0u, 0u, 0u, false};
+        delta_scatter_window(P.g_delta64, win, dw, hi, d, g * wt);
+        delta_window_flush(P.g_delta64, win, dw);
     }
 }
 
@@ -193,6 +270,7 @@ struct BackQueue {
                                       // view ray
     float wacc[3 * PS_MAX_LAYERS];    // per-layer gradient sums of the wave
     float colacc[kColSlots];          // perturbation-gradient column block (zero between uses)
+    float win[kWinSlots];             // perturbation-gradient window of a chunk (zero between uses)
 };
 
 // Shade queue entry j with its adjoint (cloud_march.wgsl:886-993): writes
@@ -356,7 +434,7 @@ __device__ __forceinline__ void flush_adj(const RenderParams& P, BackQueue& Q, i
         const float t = Q.f[BQ_T][j];
         const float px = P.cam_f[0] + Q.seed[SD_DX][owner] * t, py = P.cam_f[1] + Q.seed[SD_DY][owner] * t,
                     pz = P.cam_f[2] + Q.seed[SD_DZ][owner] * t;
-        light_delta_scatter_wave(P, Q.colacc, P.layers[li].f, lane < n, Q.f[BQ_QX][j], Q.f[BQ_QY][j],
+        light_delta_scatter_wave(P, Q.win, P.layers[li].f, lane < n, Q.f[BQ_QX][j], Q.f[BQ_QY][j],
                                  Q.f[BQ_QZ][j], px, py, pz, Q.f[BQ_ALT][j], ls.lin, ls.g);
     }
     wave_sync_lds();
@@ -386,6 +464,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     const size_t tile = (size_t)tile_y * P.n_tiles_x + tile_x;
     if (lane < 3 * PS_MAX_LAYERS) Q.wacc[lane] = 0.f;
     if (lane < kColSlots) Q.colacc[lane] = 0.f;
+    for (int k = lane; k < kWinSlots; k += 64) Q.win[k] = 0.f;
 
     const PixelRay ray = pixel_ray(P, xc, yc);
     const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
@@ -630,6 +709,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             // scatter can combine lanes
             const LayerC& LC = P.layers[li];
             float gds = 0.f, gk = 0.f, gcov = 0.f;
+            DeltaWindow win{0u, 0u, 0u, false};
             for (int slot = ckl - 1; slot >= 0; --slot) {
                 const bool mine = act && ((dmask >> slot) & 1u);
                 if (__ballot(mine) == 0ull) continue;
@@ -664,8 +744,9 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                         scat = true;
                     }
                 }
-                if (want_delta) delta_scatter_columns(P.g_delta64, Q.colacc, scat, dtap, g_rho * L.density_scale);
+                if (want_delta) delta_scatter_window(P.g_delta64, Q.win, win, scat, dtap, g_rho * L.density_scale);
             }
+            if (want_delta) delta_window_flush(P.g_delta64, Q.win, win);
             if (act) {
                 atomicAdd(&Q.wacc[li], gds);
                 atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
