@@ -286,6 +286,17 @@ ps_status ps_read_lut(ps_ctx* ctx, int32_t which, float* host_out, size_t floats
  * crates/ps-core/src/framebuffer.rs) or NULL for all-far.                  */
 /* ------------------------------------------------------------------------ */
 ps_status ps_render(ps_ctx* ctx, const float* depth, const ps_image_out* out);
+/* ps_render for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation); no
+ * counterpart in the reference. Same output as ps_render bit for bit; it
+ * stages the frame's parameter block in a fixed per-GPU scratch array (one
+ * block, 4 KiB) instead of the context's allocated ring and calls no
+ * allocating routine. Needs baked LUTs (ps_bake_luts) and the plain path:
+ * half resolution, TAA and a perturbation volume return PS_ERR_UNSUPPORTED.
+ * Every call on a GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (several contexts or streams at once) race; calls on one stream are
+ * ordered. */
+ps_status ps_render_noalloc(ps_ctx* ctx, const float* depth, const ps_image_out* out);
 /* One whole frame: ps_bake_luts(flags) then ps_render. */
 ps_status ps_frame(ps_ctx* ctx, uint32_t lut_flags, const float* depth,
                    const ps_image_out* out);

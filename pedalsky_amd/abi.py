@@ -228,6 +228,7 @@ _SIGNATURES = [
     ("ps_bake_luts", C.c_int32, [_P, C.c_uint32]),
     ("ps_read_lut", C.c_int32, [_P, C.c_int32, _P, C.c_size_t]),
     ("ps_render", C.c_int32, [_P, _P, C.POINTER(ImageOut)]),
+    ("ps_render_noalloc", C.c_int32, [_P, _P, C.POINTER(ImageOut)]),
     ("ps_frame", C.c_int32, [_P, C.c_uint32, _P, C.POINTER(ImageOut)]),
     ("ps_time_frames", C.c_int32, [_P, C.c_int32, C.c_uint32, _P, C.POINTER(ImageOut),
                                    C.POINTER(FrameTiming)]),

@@ -312,6 +312,12 @@ ps_status check_ready(const ps_ctx* c, bool for_render) {
     return PS_OK;
 }
 
+}  // namespace
+
+ps_status psi::check_render_ready(const ps_ctx* c) { return check_ready(c, true); }
+
+namespace {
+
 ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
     ps_status s;
     if ((s = check_ready(c, false))) return s;
@@ -479,6 +485,12 @@ ps_status ps_ctx_create(int32_t device, ps_ctx** out_ctx) {
         return fail(PS_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     c->stream = c->own_stream;
+    const ps_status ns = noalloc_init(device);
+    if (ns != PS_OK) {
+        hipStreamDestroy(c->own_stream);
+        delete c;
+        return ns;
+    }
     e = hipMalloc((void**)&c->d_counters, 16 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         hipStreamDestroy(c->own_stream);

@@ -202,5 +202,10 @@ ps_status check_atmosphere(const ps_atmosphere_params* a, const char* who);
 ps_status check_clouds(const ps_cloud_params* p, const ps_cloud_layer* layers, int32_t n, const char* who);
 // Shared inputs every render needs (noise, weather, blue noise).
 ps_status check_shared_inputs(const ps_ctx* c);
+// Everything a render needs: uniforms, shared inputs and baked LUTs.
+ps_status check_render_ready(const ps_ctx* c);
+// Per-GPU set-up of ps_render_noalloc's scratch array (noalloc.hip), once per
+// device with that device current.
+ps_status noalloc_init(int device);
 
 }  // namespace psi
