@@ -2,9 +2,12 @@
 
 The compute lives in ``lib/libpedalsky_hip.so`` (hand-written HIP kernels
 behind the C ABI of ``include/pedalsky_hip.h``). This package holds the
-Python side of that boundary: the ctypes mirror (``abi``), a context handle
-(``renderer``), scene/workload construction (``scene``) and the host mirror
-of the reference's RenderSubsystem interface (``subsystem``).
+Python side of that boundary: the ctypes mirror (``abi``), the context
+handle that mirrors the reference's RenderSubsystem prepare / dispatch calls
+(``renderer``), scene and workload construction (``scene``), many scenes per
+launch (``batch``), the autograd binding of the cloud adjoint (``autograd``),
+the atmosphere calibration (``calibration``), the roofline cost table
+(``cost_model``) and the in-tree build (``build``).
 """
 from . import abi  # noqa: F401
 

@@ -107,6 +107,10 @@ class Renderer:
                                          None if oc is None else oc.ctypes.data))
 
     def set_depth(self, depth: Optional[np.ndarray]) -> None:
+        # work queued on the library's stream may still read the current
+        # buffer: wait for it before torch's allocator may reuse the block
+        if getattr(self, "_depth_dev", None) is not None:
+            self.synchronize()
         if depth is None:
             self._depth_dev = None
         else:
