@@ -314,6 +314,12 @@ ps_status ps_timing_begin(ps_ctx* ctx, int32_t max_frames);
 ps_status ps_timing_end(ps_ctx* ctx, ps_frame_timing* timing);
 /* Re-run the render with event counting on; fills the algorithmic counts. */
 ps_status ps_count_work(ps_ctx* ctx, const float* depth, ps_work_counters* out);
+/* ps_count_work that also records, per 8x8 wave tile (row-major), the
+ * start and end of its work on the GPU's 100 MHz real-time clock into the
+ * host array tile_times (2 entries per tile, capacity entries): the load
+ * balance and tail of the render. No reference counterpart (diagnostics). */
+ps_status ps_count_work_tiles(ps_ctx* ctx, const float* depth, ps_work_counters* out,
+                              uint64_t* tile_times, int32_t capacity);
 /* Half-resolution cloud march (`clouds.half_res_render`,
  * crates/ps-clouds/src/lib.rs:124-127 and 748-760; the march patch of
  * crates/ps-clouds/src/pipeline.rs:277-319; cloud_composite_halfres.wgsl):

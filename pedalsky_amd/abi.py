@@ -235,6 +235,7 @@ _SIGNATURES = [
     ("ps_timing_begin", C.c_int32, [_P, C.c_int32]),
     ("ps_timing_end", C.c_int32, [_P, C.POINTER(FrameTiming)]),
     ("ps_count_work", C.c_int32, [_P, _P, C.POINTER(WorkCounters)]),
+    ("ps_count_work_tiles", C.c_int32, [_P, _P, C.POINTER(WorkCounters), _P, C.c_int32]),
     ("ps_batch_create", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32, C.POINTER(C.c_void_p)]),
     ("ps_batch_update", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32]),
     ("ps_batch_set_tuning", C.c_int32, [_P, C.POINTER(Tuning)]),

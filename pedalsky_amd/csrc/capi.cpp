@@ -367,6 +367,7 @@ ps_status launch_render(ps_ctx* c, const float* depth, const ps_image_out* out, 
         if (taa && !tv.prev_on) tv.history_valid = false;
         tv.prev_on = c->taa;
     }
+    if (count && c->want_tile_times) next.tile_times = c->d_tile_times;
     if (!c->half_res && !taa) {
         // one kernel: sky pass, march and composite (0.7 % faster at 1280x720
         // than the RT march plus a composite kernel, with identical output)
@@ -510,6 +511,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
+    dev_free(c->d_tile_times);
     dev_free(c->delta_bnd); dev_free(c->rt);
     for (ps_ctx::TaaView& v : c->taa_views) dev_free(v.hist);
     dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->lrec); dev_free(c->g_delta64);
@@ -602,10 +604,10 @@ static ps_status alloc_noise(ps_ctx* c, int format) {
     if ((s = dev_alloc(&c->base_store, noise_texel_bytes(format, 4) * 128 * 128 * 128, "base noise"))) return s;
     if ((s = dev_alloc(&c->detail_store, noise_texel_bytes(format, 4) * 32 * 32 * 32, "detail noise"))) return s;
     if ((s = dev_alloc(&c->curl_store, noise_texel_bytes(format, 2) * 128 * 128, "curl noise"))) return s;
-    if ((s = dev_alloc(&c->base_rl, sizeof(uint32_t) * 128 * 128 * 128, "base fold"))) return s;
+    if ((s = dev_alloc(&c->base_rl, 4 * sizeof(uint32_t) * 128 * 128 * 128, "base fold"))) return s;
     if ((s = dev_alloc(&c->rmax, 32 * 32 * 32, "base max bricks"))) return s;
-    if ((s = dev_alloc(&c->detail_hf, sizeof(float) * 32 * 32 * 32, "detail fold"))) return s;
-    if ((s = dev_alloc(&c->curl2, sizeof(float2) * 128 * 128, "curl fold"))) return s;
+    if ((s = dev_alloc(&c->detail_hf, 4 * sizeof(float) * 32 * 32 * 32, "detail fold"))) return s;
+    if ((s = dev_alloc(&c->curl2, 2 * sizeof(float2) * 128 * 128, "curl fold"))) return s;
     return PS_OK;
 }
 
@@ -813,10 +815,30 @@ ps_status ps_time_frames(ps_ctx* c, int32_t frames, uint32_t lut_flags, const fl
 }
 
 ps_status ps_count_work(ps_ctx* c, const float* depth, ps_work_counters* out) {
+    return ps_count_work_tiles(c, depth, out, nullptr, 0);
+}
+
+ps_status ps_count_work_tiles(ps_ctx* c, const float* depth, ps_work_counters* out, uint64_t* tile_times,
+                              int32_t capacity) {
     if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_count_work: NULL argument");
+    if (tile_times && c->have_frame) {
+        const int W = (int)c->sc.frame.viewport_size[0], H = (int)c->sc.frame.viewport_size[1];
+        const size_t tiles = (size_t)((W + 7) / 8) * ((H + 7) / 8);
+        if (capacity < 0 || (size_t)capacity < 2 * tiles)
+            return fail(PS_ERR_INVALID_ARGUMENT, "ps_count_work_tiles: tile_times needs 2 entries per 8x8 tile");
+        ps_status s0;
+        if ((s0 = grow((float**)&c->d_tile_times, &c->tile_times_cap, 4 * tiles, "tile times"))) return s0;
+    }
+    c->want_tile_times = tile_times != nullptr;
     PS_HIP(hipMemsetAsync(c->d_counters, 0, 16 * sizeof(unsigned long long), c->stream));
-    ps_status s;
-    if ((s = launch_render(c, depth, nullptr, true))) return s;
+    ps_status s = launch_render(c, depth, nullptr, true);
+    c->want_tile_times = false;
+    if (s) return s;
+    if (tile_times) {
+        const int W = (int)c->sc.frame.viewport_size[0], H = (int)c->sc.frame.viewport_size[1];
+        const size_t tiles = (size_t)((W + 7) / 8) * ((H + 7) / 8);
+        PS_HIP(hipMemcpyAsync(tile_times, c->d_tile_times, 16 * tiles, hipMemcpyDeviceToHost, c->stream));
+    }
     unsigned long long h[16];
     PS_HIP(hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     PS_HIP(hipStreamSynchronize(c->stream));

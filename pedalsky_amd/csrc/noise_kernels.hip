@@ -173,54 +173,80 @@ __device__ __forceinline__ void load4(const void* src, size_t i, int format, flo
     }
 }
 
+// R and the low-frequency FBM sum the march reads, as 16-bit fixed point,
+// with scales that hold the stored values exactly where 16 bits can: UNORM8
+// input R = k/255 as 257 k (scale 65535) and LF = (5 kG + 2 kB + kA) / 2040 as
+// its numerator (scale 2040); binary16 input R as R * 32768 (exact for
+// R >= 1/32) and LF as LF * 65535 (step 1.5e-5). base_fold_scales gives the
+// scales the march multiplies by.
+__device__ __forceinline__ uint32_t base_q(const void* base, int x, int y, int z, int format) {
+    const size_t i = (size_t)x + ((size_t)y << 7) + ((size_t)z << 14);
+    uint32_t qr, ql;
+    if (format == 0) {
+        const uchar4 q = ((const uchar4*)base)[i];
+        qr = (uint32_t)q.x * 257u;
+        ql = 5u * q.y + 2u * q.z + q.w;
+    } else {
+        float r, g, b, a;
+        load4(base, i, format, r, g, b, a);
+        const float lf = g * 0.625f + b * 0.25f + a * 0.125f;
+        qr = (uint32_t)rintf(clampf(r, 0.f, 1.f) * 32768.f);
+        ql = (uint32_t)rintf(clampf(lf, 0.f, 1.f) * 65535.f);
+    }
+    return qr | (ql << 16);
+}
+
+__device__ __forceinline__ float detail_h(const void* detail, int x, int y, int z, int format) {
+    float r, g, b, a;
+    load4(detail, (size_t)x + ((size_t)y << 5) + ((size_t)z << 10), format, r, g, b, a);
+    return r * 0.625f + g * 0.25f + b * 0.125f;
+}
+
+__device__ __forceinline__ float2 curl_v(const void* curl, int x, int y, int format) {
+    const int i = x + (y << 7);
+    float r, g;
+    if (format == 0) {
+        uchar2 q = ((const uchar2*)curl)[i];
+        r = (float)q.x / 255.f; g = (float)q.y / 255.f;
+    } else {
+        float2 v = __half22float2(((const __half2*)curl)[i]);
+        r = v.x; g = v.y;
+    }
+    return make_float2(r * 2.f - 1.f, g * 2.f - 1.f);
+}
+
+// Folds the baked volumes into the layouts the march reads. Every texel
+// carries the neighbours a filtered tap with that texel as its lower corner
+// also reads (wrapped): base and detail (x, y, z), (x, y+1, z), (x, y, z+1),
+// (x, y+1, z+1); curl (x, y), (x, y+1). A tap is then one 16-byte load per
+// x corner.
 __global__ __launch_bounds__(256) void k_fold(int format, const void* base, const void* detail, const void* curl,
-                                              uint32_t* base_rl, float* detail_hf, float2* curl2) {
+                                              uint4* base_rl, float4* detail_hf, float4* curl2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 128 * 128 * 128) {
-        // R and the low-frequency FBM sum the march reads, as 16-bit fixed
-        // point in the bricked layout, with scales that hold the stored
-        // values exactly where 16 bits can: UNORM8 input R = k/255 as 257 k
-        // (scale 65535) and LF = (5 kG + 2 kB + kA) / 2040 as its numerator
-        // (scale 2040); binary16 input R as R * 32768 (exact for R >= 1/32)
-        // and LF as LF * 65535 (step 1.5e-5). base_fold_scales gives the
-        // scales the march multiplies by.
-        uint32_t qr, ql;
-        if (format == 0) {
-            const uchar4 q = ((const uchar4*)base)[i];
-            qr = (uint32_t)q.x * 257u;
-            ql = 5u * q.y + 2u * q.z + q.w;
-        } else {
-            float r, g, b, a;
-            load4(base, (size_t)i, format, r, g, b, a);
-            const float lf = g * 0.625f + b * 0.25f + a * 0.125f;
-            qr = (uint32_t)rintf(clampf(r, 0.f, 1.f) * 32768.f);
-            ql = (uint32_t)rintf(clampf(lf, 0.f, 1.f) * 65535.f);
-        }
-        int x = i & 127, y = (i >> 7) & 127, z = i >> 14;
-        base_rl[brick_x(x) + brick_y(y) + brick_z(z)] = qr | (ql << 16);
+        const int x = i & 127, y = (i >> 7) & 127, z = i >> 14;
+        const int y1 = (y + 1) & 127, z1 = (z + 1) & 127;
+        base_rl[brick_x(x) + brick_y(y) + brick_z(z)] =
+            make_uint4(base_q(base, x, y, z, format), base_q(base, x, y1, z, format), base_q(base, x, y, z1, format),
+                       base_q(base, x, y1, z1, format));
     }
     if (i < 32 * 32 * 32) {
-        float r, g, b, a;
-        load4(detail, (size_t)i, format, r, g, b, a);
-        detail_hf[i] = r * 0.625f + g * 0.25f + b * 0.125f;
+        const int x = i & 31, y = (i >> 5) & 31, z = i >> 10;
+        const int y1 = (y + 1) & 31, z1 = (z + 1) & 31;
+        detail_hf[i] = make_float4(detail_h(detail, x, y, z, format), detail_h(detail, x, y1, z, format),
+                                   detail_h(detail, x, y, z1, format), detail_h(detail, x, y1, z1, format));
     }
     if (i < 128 * 128) {
-        float r, g;
-        if (format == 0) {
-            uchar2 q = ((const uchar2*)curl)[i];
-            r = (float)q.x / 255.f; g = (float)q.y / 255.f;
-        } else {
-            float2 v = __half22float2(((const __half2*)curl)[i]);
-            r = v.x; g = v.y;
-        }
-        curl2[i] = make_float2(r * 2.f - 1.f, g * 2.f - 1.f);
+        const int x = i & 127, y = i >> 7;
+        const float2 a = curl_v(curl, x, y, format), b = curl_v(curl, x, (y + 1) & 127, format);
+        curl2[i] = make_float4(a.x, a.y, b.x, b.y);
     }
 }
 
 // Conservative per-brick bound for empty-space skipping: the maximum R over
 // texels [4b, 4b + 4] on each axis (the brick plus the +1 neighbour a
 // trilinear tap whose lower corner lies in the brick also reads), rounded up
-// to a multiple of 1/255.
+// to a multiple of 1/255. Reads the first (own) entry of each folded texel.
 __global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_rl, uint32_t r_full, uint8_t* rmax) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= 32 * 32 * 32) return;
@@ -230,7 +256,7 @@ __global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_r
         for (int j = 0; j <= 4; ++j)
             for (int i = 0; i <= 4; ++i) {
                 int x = (bx * 4 + i) & 127, y = (by * 4 + j) & 127, z = (bz * 4 + k) & 127;
-                m = max(m, base_rl[brick_x(x) + brick_y(y) + brick_z(z)] & 0xffffu);
+                m = max(m, base_rl[(brick_x(x) + brick_y(y) + brick_z(z)) * 4u] & 0xffffu);
             }
     // fixed point (R = m / r_full) -> u8 rounded up: ceil(m * 255 / r_full)
     rmax[b] = (uint8_t)min((m * 255u + r_full - 1u) / r_full, 255u);
@@ -259,8 +285,8 @@ hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* det
 
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
                                 uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl, base_rl,
-                       detail_hf, curl2);
+    hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl,
+                       (uint4*)base_rl, (float4*)detail_hf, (float4*)curl2);
     hipLaunchKernelGGL(k_rmax, dim3(32 * 32 * 32 / 64), dim3(64), 0, s, base_rl, format == 0 ? 65535u : 32768u,
                        rmax);
     return hipGetLastError();

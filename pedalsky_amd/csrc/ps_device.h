@@ -112,11 +112,15 @@ struct RenderParams {
     const float4* T; int tw, th;
     const float4* SV; int sw, sh;
     const float4* AP; int aw, ah, ad;
-    const uint32_t* base_rl;          // 128^3 in 4x4x4 bricks: u16 R | u16 (0.625G + 0.25B + 0.125A) << 16
+    // Folded noise, each texel stored with the neighbours a filtered tap also
+    // reads (one 16-byte load per x corner instead of one load per corner):
+    // base_rl: 128^3 texels in 4x4x4 bricks, per texel 4 x (u16 R | u16 LF << 16)
+    // for (x, y, z), (x, y+1, z), (x, y, z+1), (x, y+1, z+1), LF = 0.625G + 0.25B + 0.125A;
+    const uint32_t* base_rl;
     float base_rs, base_ls;           // fixed-point scales of base_rl's R and LF (base_fold_scales)
     const uint8_t* rmax;              // 32^3: per brick, max R over the brick and its +1 halo (ceil to 1/255)
-    const float* detail_hf;           // 32^3 (0.625R + 0.25G + 0.125B)
-    const float2* curl;               // 128^2 (2*rg - 1)
+    const float* detail_hf;           // 32^3, per texel 4 floats 0.625R + 0.25G + 0.125B at the same 4 texels
+    const float2* curl;               // 128^2, per texel 2 float2 (2*rg - 1) at (x, y), (x, y+1)
     const uint2* weather;             // ww*wh texels, each the binary16 2x2 quad (x, y), (x+1, y) | (x, y+1), (x+1, y+1)
     int ww, wh;
     int weather_pow2;                 // ww and wh are powers of two
@@ -132,6 +136,8 @@ struct RenderParams {
     int out_f16;
     int _pad;
     unsigned long long* counters;     // 10 x u64 when counting
+    unsigned long long* tile_times;   // counting: per wave tile (row-major) its start and end on the
+                                      // 100 MHz real-time clock, or null
     // ---- half-resolution cloud march (crates/ps-clouds/src/lib.rs:748-760,
     // pipeline.rs:277-319) ----
     // The march covers W x H pixels (the cloud RT); its view rays pass
@@ -239,6 +245,18 @@ template <>
 __device__ __forceinline__ uint2 ld_g<uint2>(const void* base, uint32_t byte_off) {
     const ps_u32x2 v = *(const PS_GLOBAL ps_u32x2*)((const PS_GLOBAL char*)base + byte_off);
     return make_uint2(v.x, v.y);
+}
+typedef uint32_t ps_u32x4 __attribute__((ext_vector_type(4)));
+typedef float ps_f32x4 __attribute__((ext_vector_type(4)));
+template <>
+__device__ __forceinline__ uint4 ld_g<uint4>(const void* base, uint32_t byte_off) {
+    const ps_u32x4 v = *(const PS_GLOBAL ps_u32x4*)((const PS_GLOBAL char*)base + byte_off);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+template <>
+__device__ __forceinline__ float4 ld_g<float4>(const void* base, uint32_t byte_off) {
+    const ps_f32x4 v = *(const PS_GLOBAL ps_f32x4*)((const PS_GLOBAL char*)base + byte_off);
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 
 // NDF coefficients of cloud types 0..6 (rising edge 1/a; falling part

@@ -157,6 +157,10 @@ struct ps_ctx {
     size_t rec_cap = 0;
     float* lrec = nullptr;        // light-march records of the training forward
     size_t lrec_cap = 0;
+    // ps_count_work_tiles: per-tile start / end times of the counting render
+    unsigned long long* d_tile_times = nullptr;
+    size_t tile_times_cap = 0;    // floats
+    bool want_tile_times = false;
     double* g_delta64 = nullptr;  // FP64 accumulator of the perturbation-volume gradient
     RenderParams train{};         // parameter block of the last training forward
     bool train_valid = false;

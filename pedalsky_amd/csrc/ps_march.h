@@ -20,18 +20,18 @@ __device__ __forceinline__ float2 unpack_rl(uint32_t v) {
 
 // Trilinear Repeat fetch of the folded base volume (128^3, bricked, 2 x u16
 // fixed point; rs and ls turn the blended R and LF integers into values).
+// Each texel holds its (y, z) neighbourhood, so the eight corners are two
+// 16-byte loads (the x0 and x1 texels).
 __device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, float cx, float cy, float cz, float rs,
                                              float ls) {
     float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
     float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    uint32_t x0 = brick_x(ix & 127), x1 = brick_x((ix + 1) & 127);
-    uint32_t y0 = brick_y(iy & 127), y1 = brick_y((iy + 1) & 127);
-    uint32_t z0 = brick_z(iz & 127), z1 = brick_z((iz + 1) & 127);
-    float2 c000 = unpack_rl(ld_g<uint32_t>(v, (z0 + y0 + x0) * 4u)), c100 = unpack_rl(ld_g<uint32_t>(v, (z0 + y0 + x1) * 4u));
-    float2 c010 = unpack_rl(ld_g<uint32_t>(v, (z0 + y1 + x0) * 4u)), c110 = unpack_rl(ld_g<uint32_t>(v, (z0 + y1 + x1) * 4u));
-    float2 c001 = unpack_rl(ld_g<uint32_t>(v, (z1 + y0 + x0) * 4u)), c101 = unpack_rl(ld_g<uint32_t>(v, (z1 + y0 + x1) * 4u));
-    float2 c011 = unpack_rl(ld_g<uint32_t>(v, (z1 + y1 + x0) * 4u)), c111 = unpack_rl(ld_g<uint32_t>(v, (z1 + y1 + x1) * 4u));
+    const uint32_t yz = brick_y(iy & 127) + brick_z(iz & 127);
+    const uint4 q0 = ld_g<uint4>(v, (brick_x(ix & 127) + yz) * 16u);
+    const uint4 q1 = ld_g<uint4>(v, (brick_x((ix + 1) & 127) + yz) * 16u);
+    float2 c000 = unpack_rl(q0.x), c100 = unpack_rl(q1.x), c010 = unpack_rl(q0.y), c110 = unpack_rl(q1.y);
+    float2 c001 = unpack_rl(q0.z), c101 = unpack_rl(q1.z), c011 = unpack_rl(q0.w), c111 = unpack_rl(q1.w);
     float2 r;
     {
         float a0 = c000.x + (c100.x - c000.x) * fx, a1 = c010.x + (c110.x - c010.x) * fx;
@@ -50,32 +50,34 @@ __device__ __forceinline__ float2 base_fetch(const uint32_t* __restrict__ v, flo
     return r;
 }
 
-// Trilinear Repeat fetch of the folded detail volume (32^3 float).
+// Trilinear Repeat fetch of the folded detail volume (32^3; each texel holds
+// its (y, z) neighbourhood: two 16-byte loads).
 __device__ __forceinline__ float detail_fetch(const float* __restrict__ v, float cx, float cy, float cz) {
     float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
     float fx = cx - fx0, fy = cy - fy0, fz = cz - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    uint32_t x0 = ix & 31, x1 = (ix + 1) & 31;
-    uint32_t y0 = (iy & 31) << 5, y1 = ((iy + 1) & 31) << 5;
-    uint32_t z0 = (iz & 31) << 10, z1 = ((iz + 1) & 31) << 10;
-    float c000 = ld_g<float>(v, (z0 + y0 + x0) * 4u), c100 = ld_g<float>(v, (z0 + y0 + x1) * 4u);
-    float c010 = ld_g<float>(v, (z0 + y1 + x0) * 4u), c110 = ld_g<float>(v, (z0 + y1 + x1) * 4u);
-    float c001 = ld_g<float>(v, (z1 + y0 + x0) * 4u), c101 = ld_g<float>(v, (z1 + y0 + x1) * 4u);
-    float c011 = ld_g<float>(v, (z1 + y1 + x0) * 4u), c111 = ld_g<float>(v, (z1 + y1 + x1) * 4u);
+    const uint32_t yz = ((uint32_t)(iy & 31) << 5) + ((uint32_t)(iz & 31) << 10);
+    const float4 q0 = ld_g<float4>(v, (yz + (uint32_t)(ix & 31)) * 16u);
+    const float4 q1 = ld_g<float4>(v, (yz + (uint32_t)((ix + 1) & 31)) * 16u);
+    float c000 = q0.x, c100 = q1.x, c010 = q0.y, c110 = q1.y;
+    float c001 = q0.z, c101 = q1.z, c011 = q0.w, c111 = q1.w;
     float a0 = c000 + (c100 - c000) * fx, a1 = c010 + (c110 - c010) * fx;
     float b0 = c001 + (c101 - c001) * fx, b1 = c011 + (c111 - c011) * fx;
     float p0 = a0 + (a1 - a0) * fy, p1 = b0 + (b1 - b0) * fy;
     return p0 + (p1 - p0) * fz;
 }
 
-// Bilinear Repeat fetch of the curl map (128^2 float2).
+// Bilinear Repeat fetch of the curl map (128^2 float2; each texel holds its
+// y neighbour: two 16-byte loads).
 __device__ __forceinline__ float2 curl_fetch(const float2* __restrict__ v, float cx, float cy) {
     float fx0 = floorf(cx), fy0 = floorf(cy);
     float fx = cx - fx0, fy = cy - fy0;
     int ix = (int)fx0, iy = (int)fy0;
-    uint32_t x0 = ix & 127, x1 = (ix + 1) & 127, y0 = (iy & 127) << 7, y1 = ((iy + 1) & 127) << 7;
-    float2 a = ld_g<float2>(v, (y0 + x0) * 8u), b = ld_g<float2>(v, (y0 + x1) * 8u);
-    float2 c = ld_g<float2>(v, (y1 + x0) * 8u), d = ld_g<float2>(v, (y1 + x1) * 8u);
+    const uint32_t y = (uint32_t)(iy & 127) << 7;
+    const float4 q0 = ld_g<float4>(v, (y + (uint32_t)(ix & 127)) * 16u);
+    const float4 q1 = ld_g<float4>(v, (y + (uint32_t)((ix + 1) & 127)) * 16u);
+    const float2 a = make_float2(q0.x, q0.y), b = make_float2(q1.x, q1.y);
+    const float2 c = make_float2(q0.z, q0.w), d = make_float2(q1.z, q1.w);
     float2 r;
     float t0 = a.x + (b.x - a.x) * fx, t1 = c.x + (d.x - c.x) * fx;
     r.x = t0 + (t1 - t0) * fy;

@@ -26,6 +26,7 @@
 
 #include "ps_march.h"
 
+
 namespace {
 
 // render_tile variants: event counting, the density-perturbation volume, and
@@ -252,6 +253,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     const bool valid = x < P.W && y < P.H;
     const int xc = min(x, P.W - 1), yc = min(y, P.H - 1);
     Cnt cnt;
+    const unsigned long long t_start = COUNT ? __builtin_amdgcn_s_memrealtime() : 0ull;
 #pragma unroll
     for (int k = 0; k < 5; ++k) Q.acc[k][lane] = 0.f;
 
@@ -509,6 +511,12 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
             unsigned long long* c = P.counters;
 #pragma unroll
             for (int k = 0; k < 10; ++k) atomicAdd(c + k, (unsigned long long)v[k]);
+            if (P.tile_times) {
+                const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
+                unsigned long long* tt = P.tile_times + 2 * ((size_t)tile_y * P.n_tiles_x + tile_x);
+                tt[0] = t_start;
+                tt[1] = t_end;
+            }
         }
     }
 }
