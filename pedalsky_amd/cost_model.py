@@ -79,11 +79,14 @@ PEAK_FP32_VALU_TFLOPS = 157.3
 PEAK_FP32_VALU_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
 
-# Inputs one frame reads at least once from HBM: base volume (128^3 x 2 x u16),
-# its brick maxima (32^3 u8), detail (32^3 f32), curl (128^2 x 2 f32), the
-# 512^2 weather quads (4 x binary16) and the four FP32 LUTs of the benchmark
-# dimensions; the per-pixel depth (f32) is added when a depth buffer is read.
-INPUT_BYTES = (128 ** 3 * 4 + 32 ** 3 + 32 ** 3 * 4 + 128 ** 2 * 8 + 512 ** 2 * 8
+# Inputs one frame reads at least once from HBM: the base volume (128^3 texels,
+# R and LF as 2 x u16), its brick bounds (8^3 x u16), detail (32^3 f32), curl
+# (128^2 x 2 f32), the 512^2 weather quads (4 x binary16) and the four FP32
+# LUTs of the benchmark dimensions; the per-pixel depth (f32) is added when a
+# depth buffer is read. The folded copies that store each texel with its filter
+# neighbours hold the same values four (base, detail) and two (curl) times;
+# those repeats are a layout choice, not algorithmic bytes.
+INPUT_BYTES = (128 ** 3 * 4 + 8 ** 3 * 2 + 32 ** 3 * 4 + 128 ** 2 * 8 + 512 ** 2 * 8
                + (256 * 64 + 32 * 32 + 192 * 108 + 32 ** 3) * 16)
 
 

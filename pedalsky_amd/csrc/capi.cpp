@@ -269,7 +269,7 @@ ps_status psi::fill_render_params(const ps_ctx* c, const SceneIn& sc, const LutS
     P.T = luts.T; P.tw = dm.transmittance_w; P.th = dm.transmittance_h;
     P.SV = luts.SV; P.sw = dm.skyview_w; P.sh = dm.skyview_h;
     P.AP = luts.AP; P.aw = dm.ap_w; P.ah = dm.ap_h; P.ad = dm.ap_d;
-    P.base_rl = c->base_rl; P.rmax = c->rmax; P.detail_hf = c->detail_hf; P.curl = c->curl2;
+    P.base_rl = c->base_rl; P.bbound = c->bbound; P.detail_hf = c->detail_hf; P.curl = c->curl2;
     base_fold_scales(c->noise_format, P.base_rs, P.base_ls);
     P.weather = c->weather; P.ww = c->ww; P.wh = c->wh;
     P.weather_pow2 = (c->ww & (c->ww - 1)) == 0 && (c->wh & (c->wh - 1)) == 0 ? 1 : 0;
@@ -508,7 +508,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     hipStreamSynchronize(c->stream);
     dev_free(c->luts.T); dev_free(c->luts.MS); dev_free(c->luts.SV); dev_free(c->luts.AP);
     dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
-    dev_free(c->base_rl); dev_free(c->rmax); dev_free(c->detail_hf); dev_free(c->curl2);
+    dev_free(c->base_rl); dev_free(c->bbound); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
     dev_free(c->d_tile_times);
@@ -605,7 +605,7 @@ static ps_status alloc_noise(ps_ctx* c, int format) {
     if ((s = dev_alloc(&c->detail_store, noise_texel_bytes(format, 4) * 32 * 32 * 32, "detail noise"))) return s;
     if ((s = dev_alloc(&c->curl_store, noise_texel_bytes(format, 2) * 128 * 128, "curl noise"))) return s;
     if ((s = dev_alloc(&c->base_rl, 4 * sizeof(uint32_t) * 128 * 128 * 128, "base fold"))) return s;
-    if ((s = dev_alloc(&c->rmax, 32 * 32 * 32, "base max bricks"))) return s;
+    if ((s = dev_alloc(&c->bbound, sizeof(uint16_t) * PS_BN * PS_BN * PS_BN, "base brick bounds"))) return s;
     if ((s = dev_alloc(&c->detail_hf, 4 * sizeof(float) * 32 * 32 * 32, "detail fold"))) return s;
     if ((s = dev_alloc(&c->curl2, 2 * sizeof(float2) * 128 * 128, "curl fold"))) return s;
     return PS_OK;
@@ -619,7 +619,7 @@ ps_status ps_generate_noise(ps_ctx* c, uint32_t seed, int32_t format) {
     if ((s = alloc_noise(c, format))) return s;
     PS_HIP(ps_launch_noise_bake(seed, format, c->base_store, c->detail_store, c->curl_store, c->stream));
     PS_HIP(ps_launch_noise_fold(format, c->base_store, c->detail_store, c->curl_store, c->base_rl, c->detail_hf,
-                                c->curl2, c->rmax, c->stream));
+                                c->curl2, c->bbound, c->stream));
     c->have_noise = true;
     return PS_OK;
 }
@@ -637,7 +637,7 @@ ps_status ps_upload_noise(ps_ctx* c, int32_t format, const void* base, const voi
     PS_HIP(hipMemcpyAsync(c->curl_store, curl, noise_texel_bytes(format, 2) * 128 * 128, hipMemcpyHostToDevice,
                           c->stream));
     PS_HIP(ps_launch_noise_fold(format, c->base_store, c->detail_store, c->curl_store, c->base_rl, c->detail_hf,
-                                c->curl2, c->rmax, c->stream));
+                                c->curl2, c->bbound, c->stream));
     PS_HIP(hipStreamSynchronize(c->stream));
     c->have_noise = true;
     return PS_OK;

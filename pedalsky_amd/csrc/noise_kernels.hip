@@ -243,23 +243,31 @@ __global__ __launch_bounds__(256) void k_fold(int format, const void* base, cons
     }
 }
 
-// Conservative per-brick bound for empty-space skipping: the maximum R over
-// texels [4b, 4b + 4] on each axis (the brick plus the +1 neighbour a
-// trilinear tap whose lower corner lies in the brick also reads), rounded up
-// to a multiple of 1/255. Reads the first (own) entry of each folded texel.
-__global__ __launch_bounds__(64) void k_rmax(const uint32_t* __restrict__ base_rl, uint32_t r_full, uint8_t* rmax) {
+// Conservative per-brick bounds for empty-space skipping, over texels
+// [kb, kb + k] on each axis (the brick of k = PS_BB texels plus the +1
+// neighbour a trilinear tap whose lower corner lies in the brick also reads):
+// max R rounded up and min LF rounded down, each to a multiple of 1/255, from
+// the first (own) entry of each folded texel. r_full and l_full are the
+// fixed-point values of 1.0.
+__global__ __launch_bounds__(64) void k_bbound(const uint32_t* __restrict__ base_rl, uint32_t r_full, uint32_t l_full,
+                                               uint16_t* bbound) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= 32 * 32 * 32) return;
-    int bx = b & 31, by = (b >> 5) & 31, bz = b >> 10;
-    uint32_t m = 0;
-    for (int k = 0; k <= 4; ++k)
-        for (int j = 0; j <= 4; ++j)
-            for (int i = 0; i <= 4; ++i) {
-                int x = (bx * 4 + i) & 127, y = (by * 4 + j) & 127, z = (bz * 4 + k) & 127;
-                m = max(m, base_rl[(brick_x(x) + brick_y(y) + brick_z(z)) * 4u] & 0xffffu);
+    if (b >= PS_BN * PS_BN * PS_BN) return;
+    int bx = b % PS_BN, by = (b / PS_BN) % PS_BN, bz = b / (PS_BN * PS_BN);
+    uint32_t mr = 0, ml = 0xffffu;
+    for (int k = 0; k <= PS_BB; ++k)
+        for (int j = 0; j <= PS_BB; ++j)
+            for (int i = 0; i <= PS_BB; ++i) {
+                int x = (bx * PS_BB + i) & 127, y = (by * PS_BB + j) & 127, z = (bz * PS_BB + k) & 127;
+                const uint32_t v = base_rl[(brick_x(x) + brick_y(y) + brick_z(z)) * 4u];
+                mr = max(mr, v & 0xffffu);
+                ml = min(ml, v >> 16);
             }
-    // fixed point (R = m / r_full) -> u8 rounded up: ceil(m * 255 / r_full)
-    rmax[b] = (uint8_t)min((m * 255u + r_full - 1u) / r_full, 255u);
+    // fixed point (R = mr / r_full, LF = ml / l_full) -> u8:
+    // ceil(mr * 255 / r_full) and floor(ml * 255 / l_full)
+    const uint32_t qr = min((mr * 255u + r_full - 1u) / r_full, 255u);
+    const uint32_t ql = min(ml * 255u / l_full, 255u);
+    bbound[b] = (uint16_t)(qr | (ql << 8));
 }
 
 // Texel (x, y) of the output holds channel 0 of (x, y), (x+1, y) in .x and
@@ -284,11 +292,12 @@ hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* det
 }
 
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
-                                uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s) {
+                                uint32_t* base_rl, float* detail_hf, float2* curl2, uint16_t* bbound,
+                                hipStream_t s) {
     hipLaunchKernelGGL(k_fold, dim3(128 * 128 * 128 / 256), dim3(256), 0, s, format, base, detail, curl,
                        (uint4*)base_rl, (float4*)detail_hf, (float4*)curl2);
-    hipLaunchKernelGGL(k_rmax, dim3(32 * 32 * 32 / 64), dim3(64), 0, s, base_rl, format == 0 ? 65535u : 32768u,
-                       rmax);
+    hipLaunchKernelGGL(k_bbound, dim3((PS_BN * PS_BN * PS_BN + 63) / 64), dim3(64), 0, s, base_rl,
+                       format == 0 ? 65535u : 32768u, format == 0 ? 2040u : 65535u, bbound);
     return hipGetLastError();
 }
 

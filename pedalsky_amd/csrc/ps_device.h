@@ -118,7 +118,10 @@ struct RenderParams {
     // for (x, y, z), (x, y+1, z), (x, y, z+1), (x, y+1, z+1), LF = 0.625G + 0.25B + 0.125A;
     const uint32_t* base_rl;
     float base_rs, base_ls;           // fixed-point scales of base_rl's R and LF (base_fold_scales)
-    const uint8_t* rmax;              // 32^3: per brick, max R over the brick and its +1 halo (ceil to 1/255)
+    // Empty-space bound of the base shape per brick of PS_BB x PS_BB x PS_BB
+    // texels (PS_BN^3 bricks): u8 max R (rounded up) | u8 min LF (rounded
+    // down) << 8, both in 1/255, over the brick and its +1 halo.
+    const uint16_t* bbound;
     const float* detail_hf;           // 32^3, per texel 4 floats 0.625R + 0.25G + 0.125B at the same 4 texels
     const float2* curl;               // 128^2, per texel 2 float2 (2*rg - 1) at (x, y), (x, y+1)
     const uint2* weather;             // ww*wh texels, each the binary16 2x2 quad (x, y), (x+1, y) | (x, y+1), (x+1, y+1)
@@ -219,6 +222,13 @@ enum { PX_LR = 0, PX_LG, PX_LB, PX_TR, PX_TG, PX_TB, PX_TW, PX_TWN, PX_NEXEC, PX
 // the density clamp's linear range.
 enum { LR_OD = 0, LR_DDS, LR_DCOV, LR_LIN, LR_FIELDS };
 #define PS_LREC_CAP 32
+// Edge of an empty-space brick of the base volume, in texels, and the number
+// of bricks per axis.
+#ifndef PS_BB_LOG2
+#define PS_BB_LOG2 4
+#endif
+#define PS_BB (1 << PS_BB_LOG2)
+#define PS_BN (128 >> PS_BB_LOG2)
 
 // ---------------------------------------------------------------------------
 // Small vector helpers.

@@ -121,7 +121,7 @@ struct ps_ctx {
     int noise_format = -1;
     void *base_store = nullptr, *detail_store = nullptr, *curl_store = nullptr;
     uint32_t* base_rl = nullptr;
-    uint8_t* rmax = nullptr;
+    uint16_t* bbound = nullptr;
     float* detail_hf = nullptr;
     float2* curl2 = nullptr;
     bool have_noise = false;

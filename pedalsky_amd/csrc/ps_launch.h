@@ -59,7 +59,8 @@ inline void base_fold_scales(int format, float& r_scale, float& lf_scale) {
 }
 
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
-                                uint32_t* base_rl, float* detail_hf, float2* curl2, uint8_t* rmax, hipStream_t s);
+                                uint32_t* base_rl, float* detail_hf, float2* curl2, uint16_t* bbound,
+                                hipStream_t s);
 // Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
 // (Repeat wrap) of the coverage plane, still binary16.
 hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s);

@@ -147,21 +147,6 @@ __device__ __forceinline__ float ndf_eval(const NdfC& c, uint32_t t, float h, fl
 
 __device__ __forceinline__ NdfC layer_ndf(const LayerF& L) { return NdfC{L.ndf_ia, L.ndf_e0, L.ndf_ie, L.ndf_a, L.ndf_b}; }
 
-// Base shape through the coverage remap (shared by sample_density and
-// sample_density_light). Returns the pre-erosion cloud value and writes the
-// advected position. `cov` >= 0.01 here, so remap(c, 1-cov, 1, 0, 1) * cov
-// equals c - (1 - cov) exactly.
-//
-// Empty-space skipping. base_cloud = (r + 1 - lf) / (2 - lf) is increasing in
-// r and non-increasing in lf, so base_cloud <= (r_max + 1) / 2 for any LF-FBM
-// value. When (r_max + 1)/2 * ndf <= (1 - cov) the shape is <= 0 and the
-// density is exactly 0 whatever the fetch would return; the bound is checked
-// first with r_max = 1 (no memory access), then with the brick's r_max. The
-// 1e-5 margin covers FP32 rounding of the skipped arithmetic, so skipping
-// never changes a result. Returns -1 when skipped.
-//
-// `rm` is the brick maximum when the caller has already loaded it (negative:
-// load it here).
 // ---------------------------------------------------------------------------
 // Density perturbation volume (differentiation extension). Texel coordinates
 // of a sample at drift-offset position (wx, wz) and normalised height h are
@@ -223,10 +208,36 @@ __device__ __forceinline__ void delta_coords(const RenderParams& P, float wx, fl
     cz = wz * P.delta_su + 63.5f;
 }
 
+// Brick of the empty-space bound that holds the lower corner of a base tap.
+__device__ __forceinline__ uint32_t bbound_index(float cx, float cy, float cz) {
+    const uint32_t bx = ((uint32_t)(int)floorf(cx) >> PS_BB_LOG2) & (PS_BN - 1);
+    const uint32_t by = ((uint32_t)(int)floorf(cy) >> PS_BB_LOG2) & (PS_BN - 1);
+    const uint32_t bz = ((uint32_t)(int)floorf(cz) >> PS_BB_LOG2) & (PS_BN - 1);
+    return (bz * PS_BN + by) * PS_BN + bx;
+}
+
+// False when no tap in the brick can make base_cloud * ndfv exceed thr.
+// base_cloud = (R + 1 - lf) / (2 - lf) grows with R and (R <= 1) falls with
+// lf = clamp(LF + esb / 2, 0, 1), so over the brick it is at most
+// (Rmax + 1 - lm) / (2 - lm), lm = clamp(LFmin + esb / 2, 0, 1). The test is
+// multiplied through by 2 - lm >= 1. An absolute 1e-5 on the bound covers the
+// rounding of the interpolated R and lf (a few ulp of O(1) values); thr
+// itself carries the relative 1e-5 that covers the products.
+__device__ __forceinline__ bool brick_may_exceed(uint32_t bb, float hesb, float ndfv, float thr) {
+    const float rm = (float)(bb & 0xffu) * (1.f / 255.f);
+    const float lm = clampf((float)(bb >> 8) * (1.f / 255.f) + hesb, 0.f, 1.f);
+    return ((rm + 1.f) - lm + 1e-5f) * ndfv > thr * (2.f - lm);
+}
+
+// Base shape through the coverage remap (shared by sample_density and
+// sample_density_light): the pre-erosion cloud value. `cov` >= 0.01 here, so
+// remap(c, 1-cov, 1, 0, 1) * cov equals c - (1 - cov) exactly. Returns -1
+// without touching the base volume when ndf alone, or the brick bound
+// (brick_may_exceed), already keeps the shape <= 0: the density is then 0
+// whatever the fetch would return.
 template <bool DELTA = false>
 __device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, float cx, float cy, float cz,
-                                       float ndfv, float cov, float rm, float dcx = 0.f, float dcy = 0.f,
-                                       float dcz = 0.f) {
+                                       float ndfv, float cov, float dcx = 0.f, float dcy = 0.f, float dcz = 0.f) {
     // with the perturbation volume the bound is shape + max(delta) <= 0
     // (an absolute 1e-5 margin: every term is O(1)), with the volume's
     // maximum first and then the brick's (no smaller, so the first test
@@ -242,12 +253,8 @@ __device__ __forceinline__ float shape(const RenderParams& P, const LayerF& L, f
         thr = (1.f - cov) * (1.f - 1e-5f);
     }
     if (ndfv <= thr) return -1.f;
-    if (rm < 0.f) {
-        const uint32_t bx = ((int)floorf(cx) >> 2) & 31, by = ((int)floorf(cy) >> 2) & 31,
-                       bz = ((int)floorf(cz) >> 2) & 31;
-        rm = (float)ld_g<uint8_t>(P.rmax, (bz << 10) | (by << 5) | bx) * (1.f / 255.f);
-    }
-    if ((rm + 1.f) * 0.5f * ndfv <= thr) return -1.f;
+    if (!brick_may_exceed(ld_g<uint16_t>(P.bbound, bbound_index(cx, cy, cz) * 2u), L.esb * 0.5f, ndfv, thr))
+        return -1.f;
     float2 b = base_fetch(P.base_rl, cx, cy, cz, P.base_rs, P.base_ls);
     float lf = clampf(b.y + L.esb * 0.5f, 0.f, 1.f);
     float base_cloud = (b.x + (1.f - lf)) * frcp(fmaxf(2.f - lf, 1e-5f));
@@ -281,7 +288,7 @@ __device__ __forceinline__ float density_light(const RenderParams& P, const Laye
     advect(P, L, px, py, pz, h, ax, ay, az, cx, cy, cz);
     float dcx = 0.f, dcy = 0.f, dcz = 0.f;
     if constexpr (DELTA) delta_coords(P, px - L.wind_off_x, pz - L.wind_off_z, h, dcx, dcy, dcz);
-    float cloud = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, -1.f, dcx, dcy, dcz);
+    float cloud = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, dcx, dcy, dcz);
     return clampf(cloud, 0.f, 1.f) * L.density_scale;
 }
 
@@ -382,7 +389,7 @@ __device__ __forceinline__ float light_march_impl(const RenderParams& P, const L
             advect(P, L, qx, qyy, qz, h, ax, ay, az, cx, cy, cz);
             float dcx = 0.f, dcy = 0.f, dcz = 0.f;
             if constexpr (DELTA) delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
-            const float x = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, -1.f, dcx, dcy, dcz);
+            const float x = shape<DELTA>(P, L, cx, cy, cz, ndf_eval(nc, ctype, h, L.anvil), cov, dcx, dcy, dcz);
             const float c = clampf(x, 0.f, 1.f);
             d = c * L.density_scale;
             if constexpr (PARTIALS) {

@@ -317,17 +317,16 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                 t = g.t0 + tl;
                 seg_sample(P, g, dx, dy, dz, tl, qx, qy, qz, alt);
                 const float h = (alt - L.base) * L.inv_thick;
-                // the weather quad and the brick maximum depend on the position
+                // the weather quad and the brick bound depend on the position
                 // only: both loads are issued before either is used
                 const float wx = qx - L.wind_off_x, wz = qz - L.wind_off_z;
                 float wfx, wfy;
                 const uint32_t woff = weather_tap(P, wx * wscale + 0.5f, wz * wscale + 0.5f, wfx, wfy);
                 const float ax = wx - L.skew_x * h, ay = qy, az = wz - L.skew_z * h;
                 const float cx = ax * nscale - 0.5f, cy = ay * nscale - 0.5f, cz = az * nscale - 0.5f;
-                const uint32_t roff = ((((int)floorf(cz) >> 2) & 31) << 10) | ((((int)floorf(cy) >> 2) & 31) << 5) |
-                                      (((int)floorf(cx) >> 2) & 31);
+                const uint32_t boff = bbound_index(cx, cy, cz) * 2u;
                 const uint2 wq = ld_g<uint2>(P.weather, woff);
-                const uint32_t rmb = ld_g<uint8_t>(P.rmax, roff);
+                const uint32_t bb = ld_g<uint16_t>(P.bbound, boff);
                 float dcx = 0.f, dcy = 0.f, dcz = 0.f;
                 if constexpr (DELTA) delta_coords(P, wx, wz, h, dcx, dcy, dcz);
                 wr = weather_blend(wq, wfx, wfy);
@@ -358,9 +357,8 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                             db = delta_bound(P, dcx, dcy, dcz);
                             thr = (1.f - cov - db.x) - 1e-5f;
                         }
-                        const float rm = (float)rmb * (1.f / 255.f);
                         float cloud = -1.f;
-                        if (ndfv > thr && (rm + 1.f) * 0.5f * ndfv > thr) {
+                        if (ndfv > thr && brick_may_exceed(bb, L.esb * 0.5f, ndfv, thr)) {
                             const float ds = P.inv_detail_scale;
                             const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
                             const float2 b = base_fetch(P.base_rl, cx, cy, cz, P.base_rs, P.base_ls);
