@@ -30,8 +30,8 @@
 //  * Parameter gradients of the layer being marched are lane registers,
 //    added to per-wave LDS sums after every chunk and to the global
 //    accumulators once per wave. Perturbation-volume gradients (8 trilinear
-//    corners per sample) are accumulated per texel column in LDS and added
-//    to an FP64 accumulator (delta_scatter_columns).
+//    corners per sample) are accumulated in a per-wave LDS window of texels
+//    and added to an FP64 accumulator (delta_scatter_window).
 #include "ps_device.h"
 #include "ps_launch.h"
 
@@ -74,76 +74,6 @@ __device__ __forceinline__ void wave_sync_lds() {
 __device__ __forceinline__ int opaque_int(int v) {
     asm volatile("" : "+v"(v));
     return v;
-}
-
-// Scatter v times the eight trilinear weights of each lane's tap (lanes with
-// `has`) into the FP64 gradient accumulator. Same-address atomics within one
-// instruction serialise and a wave's taps share texels heavily, so the taps
-// are first accumulated by texel column. A wave's taps crowd into few (x, z)
-// texel columns but spread over several height rows (a row is 1/128 of the
-// layer, a few metres), so a round takes the first remaining lane's column
-// and every lane of it whose two rows fall in a kColRows window centred on
-// the leader's; their weights go into a per-wave LDS block of 4 (x, z)
-// corners x kColRows rows with LDS float atomics, and the block is added to
-// the accumulator (one FP64 atomic per non-zero texel) and cleared. Lanes
-// left after kColRounds rounds add their own eight weights. `colacc` is the
-// wave's block, zero on entry and on exit. Called with all 64 lanes active.
-constexpr int kColRows = 9;
-constexpr int kColSlots = 4 * kColRows;
-constexpr int kColRounds = 4;
-
-__device__ __forceinline__ void delta_scatter_columns(double* __restrict__ g, float* colacc, bool has,
-                                                      const DeltaTap& d, float v) {
-    const int lane = threadIdx.x;
-    const uint32_t y0r = d.y0 >> 7, y1r = d.y1 >> 7;
-    const uint32_t col = d.z0 | d.x0;
-    const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
-    unsigned long long act = __ballot(has);
-    for (int it = 0; it < kColRounds && act != 0ull; ++it) {
-        const int leader = __builtin_ctzll(act);
-        const uint32_t kcol = __builtin_amdgcn_readlane(col, leader);
-        const uint32_t ly = __builtin_amdgcn_readlane(y0r, leader);
-        const uint32_t base = ly >= (kColRows - 1) / 2 ? ly - (kColRows - 1) / 2 : 0u;
-        // the leader always qualifies: its rows are base + 4 and base + 5 at most
-        const bool m = has && col == kcol && y0r >= base && y1r - base < (uint32_t)kColRows;
-        if (m) {
-            // slot = (2 * z corner + x corner) * kColRows + row - base
-            const uint32_t r0 = y0r - base, r1 = y1r - base;
-            atomicAdd(colacc + 0 * kColRows + r0, v * wz0 * wy0 * wx0);
-            atomicAdd(colacc + 1 * kColRows + r0, v * wz0 * wy0 * wx1);
-            atomicAdd(colacc + 0 * kColRows + r1, v * wz0 * wy1 * wx0);
-            atomicAdd(colacc + 1 * kColRows + r1, v * wz0 * wy1 * wx1);
-            atomicAdd(colacc + 2 * kColRows + r0, v * wz1 * wy0 * wx0);
-            atomicAdd(colacc + 3 * kColRows + r0, v * wz1 * wy0 * wx1);
-            atomicAdd(colacc + 2 * kColRows + r1, v * wz1 * wy1 * wx0);
-            atomicAdd(colacc + 3 * kColRows + r1, v * wz1 * wy1 * wx1);
-        }
-        wave_sync_lds();
-        // the column's corner offsets (the leader's)
-        const uint32_t x0 = __builtin_amdgcn_readlane(d.x0, leader), x1 = __builtin_amdgcn_readlane(d.x1, leader);
-        const uint32_t z0 = __builtin_amdgcn_readlane(d.z0, leader), z1 = __builtin_amdgcn_readlane(d.z1, leader);
-        static_assert(kColSlots <= 64, "one flush pass");
-        if (lane < kColSlots) {
-            const float sum = colacc[lane];
-            if (sum != 0.f) {
-                const int xz = lane / kColRows, r = lane - xz * kColRows;
-                const uint32_t o = ((xz & 2) ? z1 : z0) + ((base + (uint32_t)r) << 7) + ((xz & 1) ? x1 : x0);
-                atomicAdd(g + o, (double)sum);
-                colacc[lane] = 0.f;
-            }
-        }
-        wave_sync_lds();
-        has = has && !m;
-        act = __ballot(has);
-    }
-    if (has) {
-        const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
-                                 d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
-        const float w[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
-                            v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
-#pragma unroll
-        for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)w[c]);
-    }
 }
 
 // Perturbation gradients of many taps of a wave accumulate in a per-wave LDS
@@ -194,7 +124,7 @@ __device__ __forceinline__ void delta_scatter_window(double* __restrict__ g, flo
         const float wt[8] = {v * wz0 * wy0 * wx0, v * wz0 * wy0 * wx1, v * wz0 * wy1 * wx0, v * wz0 * wy1 * wx1,
                              v * wz1 * wy0 * wx0, v * wz1 * wy0 * wx1, v * wz1 * wy1 * wx0, v * wz1 * wy1 * wx1};
 #pragma unroll
-        for (int c = 0; c < 8; ++c) atomicAdd(g + off[c], (double)wt[c]);
+        for (int c = 0; c < 8; ++c) atomic_add_g(g, off[c], (double)wt[c]);
     }
 }
 
@@ -212,7 +142,7 @@ __device__ __forceinline__ void delta_window_flush(double* __restrict__ g, float
         if (sum != 0.f) {
             const uint32_t row = (uint32_t)slot % kWinRows, col = (uint32_t)slot / kWinRows;
             const uint32_t x = (w.x0 + col % kWinCols) & 127u, z = (w.z0 + col / kWinCols) & 127u;
-            atomicAdd(g + ((z << 14) | ((w.y0 + row) << 7) | x), (double)sum);
+            atomic_add_g(g, (z << 14) | ((w.y0 + row) << 7) | x, (double)sum);
             win[slot] = 0.f;
         }
     }
@@ -269,7 +199,6 @@ struct BackQueue {
     float seed[SD_FIELDS][64];        // per owner lane: luminance cotangent, depth-weight cotangents, 1 - cos,
                                       // view ray
     float wacc[3 * PS_MAX_LAYERS];    // per-layer gradient sums of the wave
-    float colacc[kColSlots];          // perturbation-gradient column block (zero between uses)
     float win[kWinSlots];             // perturbation-gradient window of a chunk (zero between uses)
 };
 
@@ -463,7 +392,6 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     const int xc = min(x, P.W - 1), yc = min(y, P.H - 1);
     const size_t tile = (size_t)tile_y * P.n_tiles_x + tile_x;
     if (lane < 3 * PS_MAX_LAYERS) Q.wacc[lane] = 0.f;
-    if (lane < kColSlots) Q.colacc[lane] = 0.f;
     for (int k = lane; k < kWinSlots; k += 64) Q.win[k] = 0.f;
 
     const PixelRay ray = pixel_ray(P, xc, yc);
@@ -482,6 +410,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     const float Tf[3] = {ps[PX_TR * 64], ps[PX_TG * 64], ps[PX_TB * 64]};
     const float tw = ps[PX_TW * 64], twn = ps[PX_TWN * 64];
     const int n_exec = valid ? (int)ps[PX_NEXEC * 64] : 0;
+    const int n_dense = valid ? (int)ps[PX_NDENSE * 64] : 0;
     const size_t o = ((size_t)yc * P.W + xc) * 4;
     float g_lo[3] = {0.f, 0.f, 0.f}, g_T[3] = {0.f, 0.f, 0.f};
     const float skyv[3] = {sky.x, sky.y, sky.z};
@@ -569,6 +498,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     float* rec = rec_tile + lane;
     const float* lrec_tile = P.lrec + tile * (size_t)P.lrec_cap * (LR_FIELDS * 64);
     int qn = 0;  // queued samples (wave-uniform)
+    int kd_next = n_dense;  // dense samples before the chunk after this one
 
     while (__ballot(chunk >= 0) != 0ull) {
         const bool act = chunk >= 0;
@@ -576,6 +506,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
         float T[3] = {1.f, 1.f, 1.f};
         uint32_t dmask = 0u;
         int kd = 0;                   // the pixel's dense samples before this one (forward order)
+        int kd0 = 0;
         if (act) {
             const int seg = chunk / cps, sub = chunk % cps;
             if (seg != cur_seg) {
@@ -593,14 +524,40 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             const float* ckp = P.ck + (tile * P.ck_max + chunk) * PS_CK_STRIDE + lane;
             T[0] = ckp[0]; T[1] = ckp[64]; T[2] = ckp[128];
             kd = (int)ckp[192];
+            kd0 = kd;
         }
-        // chunk forward, recomputed from the checkpoint; dense samples queued
+        // The chunk's dense samples in forward order, from the training
+        // forward's records when they hold every one of them, else by
+        // marching the chunk again from its checkpoint; they are queued for
+        // the adjoint shading either way.
+        const bool use_rec = act && kd_next <= P.lrec_cap;
+        const int n_rec = kd_next - kd0;
         for (int k = 0; k < ckl; ++k) {
             const int s = s0 + k;
+            const bool more = act && (use_rec ? k < n_rec : s < s1);
+            if (__ballot(more) == 0ull) break;
             bool dense = false;
+            int slot = k;                 // the sample's step within the chunk
             float t = 0.f, px = 0.f, py = 0.f, pz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
             uint32_t ctype = 0u;
-            if (act && s < s1) {
+            if (more && use_rec) {
+                const float* r = lrec_tile + (size_t)(kd0 + k) * (LR_FIELDS * 64) + lane;
+                const uint32_t sr = __float_as_uint(r[LR_SREC * 64]);
+                const int sk = (int)(sr & 0xffffu);
+                ctype = sr >> 24;
+                slot = sk - s0;
+                const float cval = r[LR_CVAL * 64];
+                wr = r[LR_WR * 64];
+                const float tl = (float)sk * g.step;
+                t = g.t0 + tl;
+                seg_sample(P, g, dx, dy, dz, tl, px, py, pz, alt);
+                density = cval * L.density_scale;
+                dense = true;
+                float* rp = rec + (size_t)slot * (kRecFields * 64);
+                rp[REC_C * 64] = cval;
+                rp[REC_Q * 64] = r[LR_QC * 64];
+                dmask |= 1u << slot;
+            } else if (more) {
                 const float tl = (float)s * g.step;
                 t = g.t0 + tl;
                 seg_sample(P, g, dx, dy, dz, tl, px, py, pz, alt);
@@ -680,7 +637,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                     Q.f[BQ_TR][pos] = T[0]; Q.f[BQ_TG][pos] = T[1]; Q.f[BQ_TB][pos] = T[2];
                     Q.f[BQ_T][pos] = t; Q.f[BQ_STEP][pos] = g.step;
                     Q.f[BQ_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
-                                                        ((uint32_t)k << 12) | ((uint32_t)min(kd, 255) << 17));
+                                                        ((uint32_t)slot << 12) | ((uint32_t)min(kd, 255) << 17));
                     ++kd;
                     // step transmittance (the forward's update)
                     T[0] *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
@@ -750,13 +707,14 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
                 atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
                 --chunk;
+                kd_next = kd0;
             }
         }
     }
     wave_sync_lds();
     if (lane < 3 * PS_MAX_LAYERS) {
         const float v = Q.wacc[lane];
-        if (v != 0.f) atomicAdd(P.g_layers + lane, v);
+        if (v != 0.f) atomic_add_g(P.g_layers, (uint32_t)lane, v);
     }
 }
 

@@ -173,10 +173,12 @@ struct RenderParams {
     // [tile][PX_FIELDS][lane].
     float* ck;
     float* px_state;
-    // Light-march results of the training forward, per wave tile and per
-    // pixel's k-th dense sample (k < lrec_cap, in march order):
-    // [tile][k][LR_FIELDS][lane]. The backward reads them instead of
-    // marching toward the sun again; samples beyond the cap march again.
+    // Records of the training forward, per wave tile and per pixel's k-th
+    // dense sample (k < lrec_cap, in march order): [tile][k][LR_FIELDS][lane].
+    // The backward reads the sample (step, shape, coverage partial, weather
+    // value) instead of marching the chunk again, and its light march
+    // instead of marching toward the sun again; a chunk with samples beyond
+    // the cap is marched again.
     float* lrec;
     int lrec_cap;
     int ck_len;                       // checkpoint interval (primary steps), 1..32
@@ -212,16 +214,21 @@ struct TaaParams {
     double cam[3];
 };
 
-// Per-pixel forward totals stored by the training forward.
-enum { PX_LR = 0, PX_LG, PX_LB, PX_TR, PX_TG, PX_TB, PX_TW, PX_TWN, PX_NEXEC, PX_FIELDS };
+// Per-pixel forward totals stored by the training forward (PX_NDENSE: the
+// number of dense samples).
+enum { PX_LR = 0, PX_LG, PX_LB, PX_TR, PX_TG, PX_TB, PX_TW, PX_TWN, PX_NEXEC, PX_NDENSE, PX_FIELDS };
 // A transmittance checkpoint: rgb and the number of dense samples the pixel
 // had before the chunk (as a float), each a row of 64 lanes.
 #define PS_CK_STRIDE 256
-// Stored light-march results per dense sample: optical depth, its partials
-// in density_scale and coverage, and the bit mask of light samples inside
-// the density clamp's linear range.
-enum { LR_OD = 0, LR_DDS, LR_DCOV, LR_LIN, LR_FIELDS };
-#define PS_LREC_CAP 32
+// Stored per dense sample: the light march's optical depth, its partials in
+// density_scale and coverage, and the bit mask of light samples inside the
+// density clamp's linear range; the sample's step (bits 0-15), segment rank
+// (16-23) and cloud type (24-31) as integer bits, its clamped shape, its
+// d density / d coverage factor and its weather value.
+enum { LR_OD = 0, LR_DDS, LR_DCOV, LR_LIN, LR_SREC, LR_CVAL, LR_QC, LR_WR, LR_FIELDS };
+#ifndef PS_LREC_CAP
+#define PS_LREC_CAP 128
+#endif
 // Edge of an empty-space brick of the base volume, in texels, and the number
 // of bricks per axis.
 #ifndef PS_BB_LOG2
@@ -243,6 +250,13 @@ enum { LR_OD = 0, LR_DDS, LR_DCOV, LR_LIN, LR_FIELDS };
 template <class T>
 __device__ __forceinline__ T ld_g(const void* base, uint32_t byte_off) {
     return *(const PS_GLOBAL T*)((const PS_GLOBAL char*)base + byte_off);
+}
+// Relaxed device-scope atomic add through the global address space: the
+// global_atomic form, which (unlike the generic flat form) does not hold the
+// wave's next LDS access until the atomic has completed.
+template <class T>
+__device__ __forceinline__ void atomic_add_g(T* base, uint32_t idx, T v) {
+    __hip_atomic_fetch_add((PS_GLOBAL T*)base + idx, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 typedef float ps_f32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t ps_u32x2 __attribute__((ext_vector_type(2)));

@@ -310,6 +310,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
             }
             bool dense = false;
             float t = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
+            float cval = 0.f, qc = 0.f;   // training forward: the clamped shape and its coverage partial
             uint32_t ctype = 0u;
             if (on) {
                 cnt.primary++;
@@ -357,7 +358,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                             db = delta_bound(P, dcx, dcy, dcz);
                             thr = (1.f - cov - db.x) - 1e-5f;
                         }
-                        float cloud = -1.f;
+                        float cloud = -1.f, dsh = 1.f;
                         if (ndfv > thr && brick_may_exceed(bb, L.esb * 0.5f, ndfv, thr)) {
                             const float ds = P.inv_detail_scale;
                             const float2 cu = curl_fetch(P.curl, ax * ds * 128.f - 0.5f, az * ds * 128.f - 0.5f);
@@ -372,11 +373,19 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                                                               ddz * ds * 32.f - 0.5f);
                                 const float dmod = hf + ((1.f - hf) - hf) * clampf(h * 10.f, 0.f, 1.f);
                                 const float e = dmod * L.ld;
-                                cloud = (cloud - e) * frcp(fmaxf(1.f - e, 1e-5f));
+                                dsh = frcp(fmaxf(1.f - e, 1e-5f));
+                                cloud = (cloud - e) * dsh;
                             }
                             if constexpr (DELTA) cloud += delta_value(P, db, dcx, dcy, dcz);
                         }
                         density = clampf(cloud, 0.f, 1.f) * L.density_scale;
+                        if constexpr (TRAIN) {
+                            // the record the backward reads instead of recomputing
+                            // the step (its d density / d coverage factor as there)
+                            cval = clampf(cloud, 0.f, 1.f);
+                            const float wc = wr * L.coverage;
+                            if (cloud > 0.f && cloud < 1.f && wc > 0.f && wc < 1.f) qc = L.density_scale * dsh * wr;
+                        }
                     }
                 }
                 cnt.covered += stage >= 1 ? 1u : 0u;
@@ -394,7 +403,16 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                     Q.f[QF_T][pos] = t; Q.f[QF_STEP][pos] = g.step;
                     Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
                                                         ((uint32_t)min(kd, 255) << 12));
-                    if constexpr (TRAIN) ++kd;
+                    if constexpr (TRAIN) {
+                        if (kd < P.lrec_cap) {
+                            float* r = lrec_tile + (size_t)kd * (LR_FIELDS * 64) + lane;
+                            r[LR_SREC * 64] = __uint_as_float((uint32_t)s | ((uint32_t)next << 16) | (ctype << 24));
+                            r[LR_CVAL * 64] = cval;
+                            r[LR_QC * 64] = qc;
+                            r[LR_WR * 64] = wr;
+                        }
+                        ++kd;
+                    }
                     tr *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
                     tg *= __expf(-fmaxf(density * L.sigma_t[1], 1e-6f) * g.step);
                     tbb *= __expf(-fmaxf(density * L.sigma_t[2], 1e-6f) * g.step);
@@ -425,6 +443,7 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         ps[PX_TR * 64] = tr; ps[PX_TG * 64] = tg; ps[PX_TB * 64] = tbb;
         ps[PX_TW * 64] = tw; ps[PX_TWN * 64] = twn;
         ps[PX_NEXEC * 64] = (float)cnt.primary;
+        ps[PX_NDENSE * 64] = (float)kd;
     }
     const float t_lum = tr * 0.2126f + tg * 0.7152f + tbb * 0.0722f;
     const float opacity = 1.f - t_lum;
