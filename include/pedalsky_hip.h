@@ -245,6 +245,57 @@ ps_status ps_set_clouds(ps_ctx* ctx, const ps_cloud_params* params,
                         const float* layer_wind_xz);
 
 /* ------------------------------------------------------------------------ */
+/* Sun position and top-of-atmosphere illuminance (host code). Replaces
+ * ps_core::astro::sun_position / sun_position_precise
+ * (crates/ps-core/src/astro/spa.rs:47-185) and the sun half of
+ * WorldState::recompute (crates/ps-core/src/world.rs:95-111).             */
+/* ------------------------------------------------------------------------ */
+/* A UTC calendar instant (chrono DateTime<Utc> at whole seconds). */
+typedef struct ps_utc {
+    int32_t year, month, day, hour, minute, second;
+} ps_utc;
+
+/* SunPositionPrecise (spa.rs:29-45), plus the Julian day it was computed at. */
+typedef struct ps_sun_position {
+    double altitude_deg;             /* topocentric, above the horizon, no refraction */
+    double azimuth_deg;              /* clockwise from north, [0, 360) */
+    double zenith_deg;
+    double distance_au;              /* Earth radius vector R */
+    double right_ascension_deg;      /* geocentric, [0, 360) */
+    double declination_deg;          /* geocentric */
+    double apparent_sidereal_time_deg;
+    double julian_day;
+} ps_sun_position;
+
+/* The WorldState sun fields (world.rs:30-37): SunPosition's FP32 values, the
+ * world direction (+X east, +Y up, +Z south) and the TOA illuminance
+ * 127500 lux / d^2. */
+typedef struct ps_world_sun_state {
+    float altitude_rad;
+    float azimuth_rad;
+    float distance_au;
+    float direction_world[3];
+    float toa_illuminance_lux;
+} ps_world_sun_state;
+
+/* julian_day_utc (spa.rs:194-216). */
+double ps_julian_day(const ps_utc* utc);
+/* delta_t_seconds (spa.rs:222-249): TT - UT for the year, seconds. */
+double ps_delta_t_seconds(int32_t year);
+/* sun_position_precise (spa.rs:60-185). Invalid dates or latitudes outside
+ * [-90, 90] return PS_ERR_INVALID_ARGUMENT. */
+ps_status ps_sun_position_precise(const ps_utc* utc, double latitude_deg, double longitude_deg,
+                                  double elevation_m, ps_sun_position* out);
+/* sun_position + WorldState::recompute's sun direction and TOA illuminance
+ * (world.rs:95-100, altaz_to_world world.rs:106-111). */
+ps_status ps_world_sun(const ps_utc* utc, double latitude_deg, double longitude_deg, double elevation_m,
+                       ps_world_sun_state* out);
+/* FrameUniforms::set_sun (frame_uniforms.rs:98-118) with the illuminance the
+ * synthesised WeatherState carries, TOA lux on every channel
+ * (ps-synthesis/src/state.rs:154). */
+ps_status ps_frame_set_sun(ps_frame_uniforms* frame, const ps_world_sun_state* sun, float angular_radius_rad);
+
+/* ------------------------------------------------------------------------ */
 /* Cloud inputs. Replaces CloudNoise::bake (crates/ps-clouds/src/noise.rs:71-216)
  * and the weather textures ps_synthesis::synthesise uploads
  * (crates/ps-synthesis/src/state.rs:50-185).                               */
@@ -269,6 +320,31 @@ ps_status ps_upload_weather(ps_ctx* ctx, const uint16_t* weather_f16, int32_t w,
                             int32_t h, int32_t channels,
                             const uint8_t* cloud_type_grid_128,
                             const uint8_t* overcast_mask_128);
+
+/* Weather inputs synthesised on the GPU. Replaces WeatherMap::synthesise
+ * (crates/ps-synthesis/src/weather_map.rs:46-101, with coverage_grid.rs:31-61
+ * for the gridded R gate) and CloudTypeGrid::synthesise
+ * (cloud_type_grid.rs:41-92). The grids are the data the scene's
+ * coverage_grid files hold (row-major, grid_w x grid_h, spanning
+ * [-extent/2, extent/2] on x and z). */
+typedef struct ps_weather_synth {
+    int32_t has_layers;              /* scene has cloud layers: R gate 1 (else 0) where there is no grid */
+    float precip_mm_per_h;           /* scene.precipitation.intensity_mm_per_h */
+    float ground_wetness;            /* surface.wetness.ground_wetness */
+    const float* coverage_grid;      /* host f32 in [0, 1], or NULL */
+    const uint8_t* type_grid;        /* host u8 cloud types (0..7; others mean "layer type"), or NULL */
+    int32_t grid_w, grid_h;
+    float grid_extent_m;
+} ps_weather_synth;
+/* Synthesise the 128x128 RGBA16F weather map (R coverage gate, G 0, B
+ * cloud-base offset noise, A precipitation / wetness) and the cloud-type
+ * grid, bit-identical to the reference's, and make them the context's
+ * weather inputs (32 km tile; the march's weather_scale_m places it).
+ * `overcast_mask_128` as in ps_upload_weather. */
+ps_status ps_synthesise_weather(ps_ctx* ctx, const ps_weather_synth* synth, const uint8_t* overcast_mask_128);
+/* Read back the last synthesised map (128*128*4 binary16) and, if
+ * `cloud_types_128` is non-NULL, the cloud-type grid (128*128 u8). */
+ps_status ps_read_weather(ps_ctx* ctx, uint16_t* rgba_f16, uint8_t* cloud_types_128);
 
 /* ------------------------------------------------------------------------ */
 /* The four LUT passes (AtmosphereSubsystem::dispatch_pass,

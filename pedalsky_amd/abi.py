@@ -43,6 +43,35 @@ F4 = C.c_float * 4
 F16 = C.c_float * 16
 
 
+class Utc(C.Structure):
+    """ps_utc: a UTC instant at whole seconds."""
+
+    _fields_ = [(n, C.c_int32) for n in ("year", "month", "day", "hour", "minute", "second")]
+
+
+class SunPosition(C.Structure):
+    """ps_sun_position (SunPositionPrecise, spa.rs:29-45, plus the Julian day)."""
+
+    _fields_ = [(n, C.c_double) for n in (
+        "altitude_deg", "azimuth_deg", "zenith_deg", "distance_au", "right_ascension_deg", "declination_deg",
+        "apparent_sidereal_time_deg", "julian_day")]
+
+
+class WorldSun(C.Structure):
+    """ps_world_sun_state (the WorldState sun fields, world.rs:30-37)."""
+
+    _fields_ = [("altitude_rad", C.c_float), ("azimuth_rad", C.c_float), ("distance_au", C.c_float),
+                ("direction_world", C.c_float * 3), ("toa_illuminance_lux", C.c_float)]
+
+
+class WeatherSynth(C.Structure):
+    """ps_weather_synth (WeatherMap / CloudTypeGrid synthesis inputs)."""
+
+    _fields_ = [("has_layers", C.c_int32), ("precip_mm_per_h", C.c_float), ("ground_wetness", C.c_float),
+                ("coverage_grid", C.POINTER(C.c_float)), ("type_grid", C.POINTER(C.c_uint8)),
+                ("grid_w", C.c_int32), ("grid_h", C.c_int32), ("grid_extent_m", C.c_float)]
+
+
 class FrameUniforms(C.Structure):
     """FrameUniformsGpu (448 B)."""
 
@@ -268,6 +297,14 @@ _SIGNATURES = [
     ("ps_check_layers_overlap", C.c_int32, [C.POINTER(CloudLayer), C.c_int32,
                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("ps_layer_wind", None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float * 2]),
+    ("ps_synthesise_weather", C.c_int32, [_P, C.POINTER(WeatherSynth), C.c_void_p]),
+    ("ps_read_weather", C.c_int32, [_P, C.c_void_p, C.c_void_p]),
+    ("ps_julian_day", C.c_double, [C.POINTER(Utc)]),
+    ("ps_delta_t_seconds", C.c_double, [C.c_int32]),
+    ("ps_sun_position_precise", C.c_int32, [C.POINTER(Utc), C.c_double, C.c_double, C.c_double,
+                                            C.POINTER(SunPosition)]),
+    ("ps_world_sun", C.c_int32, [C.POINTER(Utc), C.c_double, C.c_double, C.c_double, C.POINTER(WorldSun)]),
+    ("ps_frame_set_sun", C.c_int32, [C.POINTER(FrameUniforms), C.POINTER(WorldSun), C.c_float]),
     ("ps_overcast_mask_value", C.c_uint8, [C.POINTER(CloudLayer), C.c_int32]),
     ("ps_generate_blue_noise", None, [_P]),
 ]

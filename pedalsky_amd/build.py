@@ -21,7 +21,7 @@ OBJ_DIR = PKG / "_build"
 INCLUDE = PKG.parent / "include"
 
 SOURCES = ["lut_kernels.hip", "render_kernel.hip", "adjoint_kernel.hip", "atmo_adjoint.hip", "noise_kernels.hip",
-           "noalloc.hip", "capi.cpp", "batch.cpp", "host_synthesis.cpp"]
+           "noalloc.hip", "capi.cpp", "batch.cpp", "host_synthesis.cpp", "astro.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PEDALSKY_OFFLOAD_ARCH", "gfx950")

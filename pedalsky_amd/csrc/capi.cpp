@@ -509,7 +509,7 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->luts.T); dev_free(c->luts.MS); dev_free(c->luts.SV); dev_free(c->luts.AP);
     dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
     dev_free(c->base_rl); dev_free(c->bbound); dev_free(c->detail_hf); dev_free(c->curl2);
-    dev_free(c->weather); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
+    dev_free(c->weather); dev_free(c->weather_synth); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
     dev_free(c->d_tile_times);
     dev_free(c->delta_bnd); dev_free(c->rt);
@@ -699,6 +699,65 @@ ps_status ps_upload_weather(ps_ctx* c, const uint16_t* weather, int32_t w, int32
     c->overcast_active = overcast != nullptr;
     if (overcast) PS_HIP(hipMemcpy(c->overcast, overcast, 128 * 128, hipMemcpyHostToDevice));
     c->have_weather = true;
+    return PS_OK;
+}
+
+ps_status ps_synthesise_weather(ps_ctx* c, const ps_weather_synth* ws, const uint8_t* overcast) {
+    if (!c || !ws) return fail(PS_ERR_INVALID_ARGUMENT, "ps_synthesise_weather: NULL argument");
+    const bool grid = ws->coverage_grid != nullptr, types = ws->type_grid != nullptr;
+    if ((grid || types) && (ws->grid_w < 1 || ws->grid_h < 1 || ws->grid_w > 8192 || ws->grid_h > 8192))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_synthesise_weather: grid size must be 1..8192 per axis");
+    if (grid && !(ws->grid_extent_m > 0.f && std::isfinite(ws->grid_extent_m)))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_synthesise_weather: grid_extent_m must be positive");
+    PS_HIP(hipStreamSynchronize(c->stream));
+    const int n = 128 * 128;
+    if (c->ww != 128 || c->wh != 128) dev_free(c->weather);
+    ps_status s;
+    if ((s = dev_alloc(&c->weather, sizeof(uint2) * n, "weather"))) return s;
+    if ((s = dev_alloc(&c->weather_synth, sizeof(uint16_t) * 4 * n, "synthesised weather"))) return s;
+    if ((s = dev_alloc(&c->type_grid, n, "cloud type grid"))) return s;
+    if ((s = dev_alloc(&c->overcast, n, "overcast mask"))) return s;
+    // precipitation intensity over a 50 mm/h ceiling or ground wetness,
+    // the larger (weather_map.rs:66-74)
+    const float precip = std::fmin(std::fmax(ws->precip_mm_per_h / 50.f, 0.f), 1.f);
+    const float wet = std::fmin(std::fmax(ws->ground_wetness, 0.f), 1.f);
+    const float alpha = std::fmax(precip, wet);
+    const size_t cells = grid || types ? (size_t)ws->grid_w * ws->grid_h : 0;
+    float* d_grid = nullptr;
+    uint8_t* d_types = nullptr;
+    hipError_t e = hipSuccess;
+    if (grid) e = hipMalloc((void**)&d_grid, sizeof(float) * cells);
+    if (e == hipSuccess && grid) e = hipMemcpy(d_grid, ws->coverage_grid, sizeof(float) * cells, hipMemcpyHostToDevice);
+    if (e == hipSuccess && types) e = hipMalloc((void**)&d_types, cells);
+    if (e == hipSuccess && types) e = hipMemcpy(d_types, ws->type_grid, cells, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !types) e = hipMemsetAsync(c->type_grid, 0xff, n, c->stream);  // all sentinel
+    if (e == hipSuccess)
+        e = ps_launch_weather_synth(d_grid, ws->grid_w, ws->grid_h, ws->grid_extent_m, ws->has_layers ? 1.f : 0.f,
+                                    alpha, d_types, c->weather_synth, c->type_grid, c->stream);
+    if (e == hipSuccess) e = ps_launch_weather_fold(c->weather_synth, 128, 128, 4, c->weather, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d_grid) hipFree(d_grid);
+    if (d_types) hipFree(d_types);
+    if (e != hipSuccess) return fail(PS_ERR_DEVICE, std::string("ps_synthesise_weather: ") + hipGetErrorString(e));
+    c->ww = 128; c->wh = 128;
+    // the march consults the grid only when some source cell is a valid type
+    c->type_grid_active = false;
+    for (size_t i = 0; types && i < cells; ++i)
+        if (ws->type_grid[i] <= 7u) { c->type_grid_active = true; break; }
+    c->overcast_active = overcast != nullptr;
+    if (overcast) PS_HIP(hipMemcpy(c->overcast, overcast, n, hipMemcpyHostToDevice));
+    c->have_weather = true;
+    return PS_OK;
+}
+
+ps_status ps_read_weather(ps_ctx* c, uint16_t* rgba_f16, uint8_t* cloud_types) {
+    if (!c || !rgba_f16) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_weather: NULL argument");
+    if (!c->weather_synth) return fail(PS_ERR_NOT_READY, "ps_read_weather: no synthesised weather map");
+    PS_HIP(hipMemcpyAsync(rgba_f16, c->weather_synth, sizeof(uint16_t) * 4 * 128 * 128, hipMemcpyDeviceToHost,
+                          c->stream));
+    if (cloud_types)
+        PS_HIP(hipMemcpyAsync(cloud_types, c->type_grid, 128 * 128, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
     return PS_OK;
 }
 

@@ -282,7 +282,105 @@ __global__ __launch_bounds__(256) void k_weather_fold(const uint16_t* src, int w
     dst[i] = make_uint2(c0(x, y) | (c0(x1, y) << 16), c0(x, y1) | (c0(x1, y1) << 16));
 }
 
+// ---- weather-map synthesis (weather_map.rs:46-170, coverage_grid.rs:31-61,
+// cloud_type_grid.rs:41-92) ------------------------------------------------
+// FP32 arithmetic in the reference's order with no contraction (Rust does
+// not fuse a * b + c), so every texel is bit-identical.
+constexpr int kWeatherSize = 128;
+constexpr float kWeatherExtent = 32000.f;
+
+__device__ __forceinline__ float wm_hash2(int x, int y) {
+    // hash2 (weather_map.rs:160-167)
+    uint32_t h = (uint32_t)x * 2654435761u;
+    h ^= (uint32_t)y * 2246822519u;
+    h ^= h >> 16;
+    h *= 2654435761u;
+    h ^= h >> 13;
+    return fminf(fmaxf((float)h / 4294967296.f, 0.f), 1.f);
+}
+
+__device__ float wm_value_noise(float x, float y) {
+#pragma clang fp contract(off)
+    // value_noise_2d (weather_map.rs:143-158): smoothstep-blended lattice hash
+    const float xi = floorf(x), yi = floorf(y);
+    const float xf = x - xi, yf = y - yi;
+    const int ix = (int)xi, iy = (int)yi;
+    const float h00 = wm_hash2(ix, iy), h10 = wm_hash2(ix + 1, iy);
+    const float h01 = wm_hash2(ix, iy + 1), h11 = wm_hash2(ix + 1, iy + 1);
+    const float sx = xf * xf * (3.f - 2.f * xf);
+    const float sy = yf * yf * (3.f - 2.f * yf);
+    const float bottom = h00 + (h10 - h00) * sx;
+    const float top = h01 + (h11 - h01) * sx;
+    return bottom + (top - bottom) * sy;
+}
+
+__device__ float wm_grid_sample(const float* __restrict__ g, int gw, int gh, float extent, float x, float z) {
+#pragma clang fp contract(off)
+    // LoadedCoverageGrid::sample_world / sample (coverage_grid.rs:31-61):
+    // edge-clamped bilinear over the source grid spanning [-extent/2, extent/2]
+    const float half = extent * 0.5f;
+    const float u = (x + half) / extent, v = (z + half) / extent;
+    const float fx = fminf(fmaxf(u, 0.f), 1.f) * ((float)gw - 1.f);
+    const float fy = fminf(fmaxf(v, 0.f), 1.f) * ((float)gh - 1.f);
+    const uint32_t x0 = (uint32_t)floorf(fx), y0 = (uint32_t)floorf(fy);
+    const uint32_t x1 = min(x0 + 1u, (uint32_t)gw - 1u), y1 = min(y0 + 1u, (uint32_t)gh - 1u);
+    const float tx = fx - (float)x0, ty = fy - (float)y0;
+    const float v00 = g[y0 * gw + x0], v10 = g[y0 * gw + x1], v01 = g[y1 * gw + x0], v11 = g[y1 * gw + x1];
+    const float bottom = v00 + (v10 - v00) * tx;
+    const float top = v01 + (v11 - v01) * tx;
+    return fminf(fmaxf(bottom + (top - bottom) * ty, 0.f), 1.f);
+}
+
+// One texel of the 128x128 RGBA16F weather map: R the coverage gate (the
+// sampled coverage grid, or `gate`), G reserved 0, B the cloud-base offset
+// noise in [-1, 1], A the precipitation / wetness scalar.
+__global__ __launch_bounds__(256) void k_weather_synth(const float* __restrict__ grid, int gw, int gh, float extent,
+                                                       float gate, float alpha, uint16_t* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kWeatherSize * kWeatherSize) return;
+    const int x = i % kWeatherSize, y = i / kWeatherSize;
+    float coverage = gate;
+    if (grid) {
+        // pixel_to_world (weather_map.rs:134-139)
+        const float half = kWeatherExtent * 0.5f;
+        const float fx = ((float)x + 0.5f) / (float)kWeatherSize, fy = ((float)y + 0.5f) / (float)kWeatherSize;
+        coverage = wm_grid_sample(grid, gw, gh, extent, fx * kWeatherExtent - half, fy * kWeatherExtent - half);
+    }
+    const float bias = wm_value_noise((float)x * 0.05f, (float)y * 0.05f) * 2.f - 1.f;
+    ushort4 o;
+    o.x = __half_as_ushort(__float2half_rn(coverage));
+    o.y = 0;
+    o.z = __half_as_ushort(__float2half_rn(bias));
+    o.w = __half_as_ushort(__float2half_rn(alpha));
+    ((ushort4*)out)[i] = o;
+}
+
+// Nearest resample of a source cloud-type grid onto 128x128; values above 7
+// become the sentinel 255 (cloud_type_grid.rs:71-86).
+__global__ __launch_bounds__(256) void k_type_grid_synth(const uint8_t* __restrict__ src, int gw, int gh,
+                                                         uint8_t* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kWeatherSize * kWeatherSize) return;
+    const int x = i % kWeatherSize, y = i / kWeatherSize;
+    const uint32_t sx = min((uint32_t)((float)x / (float)kWeatherSize * (float)gw), (uint32_t)gw - 1u);
+    const uint32_t sy = min((uint32_t)((float)y / (float)kWeatherSize * (float)gh), (uint32_t)gh - 1u);
+    const uint8_t v = src[sy * gw + sx];
+    out[i] = v <= 7 ? v : (uint8_t)255;
+}
+
 }  // namespace
+
+hipError_t ps_launch_weather_synth(const float* grid, int gw, int gh, float extent, float gate, float alpha,
+                                   const uint8_t* type_src, uint16_t* out_rgba, uint8_t* out_types, hipStream_t s) {
+    const int n = kWeatherSize * kWeatherSize;
+    hipLaunchKernelGGL(k_weather_synth, dim3((n + 255) / 256), dim3(256), 0, s, grid, gw, gh, extent, gate, alpha,
+                       out_rgba);
+    if (type_src)
+        hipLaunchKernelGGL(k_type_grid_synth, dim3((n + 255) / 256), dim3(256), 0, s, type_src, gw, gh, out_types);
+    return hipGetLastError();
+}
 
 hipError_t ps_launch_noise_bake(uint32_t seed, int format, void* base, void* detail, void* curl, hipStream_t s) {
     hipLaunchKernelGGL(k_noise_base, dim3(128 * 128 * 128 / 64), dim3(64), 0, s, seed, format, base);

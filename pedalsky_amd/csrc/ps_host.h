@@ -127,6 +127,7 @@ struct ps_ctx {
     bool have_noise = false;
 
     uint2* weather = nullptr;         // binary16 2x2 quads
+    uint16_t* weather_synth = nullptr;  // the last synthesised 128x128 RGBA16F map
     int ww = 0, wh = 0;
     uint8_t* type_grid = nullptr;
     bool type_grid_active = false;

@@ -64,6 +64,11 @@ hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail
 // Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
 // (Repeat wrap) of the coverage plane, still binary16.
 hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s);
+// Weather-map synthesis: the 128x128 RGBA binary16 map (R from the coverage
+// grid when `grid` is non-NULL, else `gate`) and, when `type_src` is non-NULL,
+// the 128x128 cloud-type grid resampled from it (same gw x gh source).
+hipError_t ps_launch_weather_synth(const float* grid, int gw, int gh, float extent, float gate, float alpha,
+                                   const uint8_t* type_src, uint16_t* out_rgba, uint8_t* out_types, hipStream_t s);
 
 // ---- atmosphere differentiation (atmo_adjoint.hip) ----------------------------
 // Overwrite the 14 differentiated coefficients of n LUT parameter blocks from

@@ -106,6 +106,33 @@ class Renderer:
                                          None if tg is None else tg.ctypes.data,
                                          None if oc is None else oc.ctypes.data))
 
+    def synthesise_weather(self, has_layers: bool = True, precip_mm_per_h: float = 0.0,
+                           ground_wetness: float = 0.0, coverage_grid: Optional[np.ndarray] = None,
+                           type_grid: Optional[np.ndarray] = None, grid_extent_m: float = 32000.0,
+                           overcast: Optional[np.ndarray] = None) -> None:
+        """The reference's weather map and cloud-type grid, synthesised on the
+        GPU (ps_synthesise_weather); grids are (grid_h, grid_w) arrays."""
+        shape = coverage_grid.shape if coverage_grid is not None else (
+            type_grid.shape if type_grid is not None else (0, 0))
+        g = None if coverage_grid is None else np.ascontiguousarray(coverage_grid, dtype=np.float32)
+        t = None if type_grid is None else np.ascontiguousarray(type_grid, dtype=np.uint8)
+        if g is not None and t is not None and g.shape != t.shape:
+            raise ValueError("coverage and type grids must have the same size")
+        oc = None if overcast is None else np.ascontiguousarray(overcast, dtype=np.uint8)
+        ws = abi.WeatherSynth(int(bool(has_layers)), float(precip_mm_per_h), float(ground_wetness),
+                              None if g is None else g.ctypes.data_as(C.POINTER(C.c_float)),
+                              None if t is None else t.ctypes.data_as(C.POINTER(C.c_uint8)),
+                              int(shape[1]), int(shape[0]), float(grid_extent_m))
+        check(self.lib.ps_synthesise_weather(self.ctx, C.byref(ws), None if oc is None else oc.ctypes.data))
+
+    def read_weather(self):
+        """(128, 128, 4) uint16 binary16 bits of the synthesised map and the
+        (128, 128) cloud-type grid."""
+        rgba = np.zeros((128, 128, 4), np.uint16)
+        types = np.zeros((128, 128), np.uint8)
+        check(self.lib.ps_read_weather(self.ctx, rgba.ctypes.data, types.ctypes.data))
+        return rgba, types
+
     def set_depth(self, depth: Optional[np.ndarray]) -> None:
         # work queued on the library's stream may still read the current
         # buffer: wait for it before torch's allocator may reuse the block
