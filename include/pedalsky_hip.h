@@ -347,6 +347,27 @@ ps_status ps_synthesise_weather(ps_ctx* ctx, const ps_weather_synth* synth, cons
 ps_status ps_read_weather(ps_ctx* ctx, uint16_t* rgba_f16, uint8_t* cloud_types_128);
 
 /* ------------------------------------------------------------------------ */
+/* Presentation: the frame the reference's swapchain (and its golden images)
+ * holds. Replaces the aerial-perspective application of the ground pass
+ * (shaders/ground/pbr.wgsl:653-666: L = lit * ap.a + ap.rgb at the surface
+ * distance), the cloud composite over it (cloud_composite.wgsl:47-57) and
+ * the tone-map pass (shaders/postprocess/tonemap.wgsl:37-58 with the *Srgb
+ * swapchain encode; crates/ps-postprocess/src/tonemap.rs).                 */
+/* ------------------------------------------------------------------------ */
+enum { PS_TONEMAP_ACES = 0, PS_TONEMAP_CLAMP = 1 };
+typedef struct ps_present_params {
+    float ev100;                     /* photographic EV at ISO 100 (tonemap.wgsl:50) */
+    int32_t tonemap;                 /* PS_TONEMAP_ACES (ACESFilmic) or PS_TONEMAP_CLAMP (Passthrough) */
+} ps_present_params;
+/* `in`: the PS_PIXEL_F32 outputs of the ps_render call for this frame and
+ * depth buffer. `ground_lit`: device W*H RGBA32F radiance of the opaque
+ * surface before aerial perspective (the ground pass's `lit`), or NULL: the
+ * composite is presented as it is. Outputs (device, either may be NULL but
+ * not both): W*H RGBA8 sRGB (alpha 255) and W*H RGBA32F HDR. */
+ps_status ps_present(ps_ctx* ctx, const float* depth, const float* ground_lit, const ps_image_out* in,
+                     const ps_present_params* params, uint8_t* out_srgb8, float* out_hdr);
+
+/* ------------------------------------------------------------------------ */
 /* The four LUT passes (AtmosphereSubsystem::dispatch_pass,
  * crates/ps-atmosphere/src/lib.rs:494-584).                                */
 /* ------------------------------------------------------------------------ */

@@ -25,6 +25,8 @@ PS_NOISE_UNORM8 = 0
 PS_NOISE_FP16 = 1
 PS_PIXEL_F32 = 0
 PS_PIXEL_F16 = 1
+PS_TONEMAP_ACES = 0
+PS_TONEMAP_CLAMP = 1
 
 PS_LUT_TRANSMITTANCE = 1
 PS_LUT_MULTISCATTER = 2
@@ -70,6 +72,12 @@ class WeatherSynth(C.Structure):
     _fields_ = [("has_layers", C.c_int32), ("precip_mm_per_h", C.c_float), ("ground_wetness", C.c_float),
                 ("coverage_grid", C.POINTER(C.c_float)), ("type_grid", C.POINTER(C.c_uint8)),
                 ("grid_w", C.c_int32), ("grid_h", C.c_int32), ("grid_extent_m", C.c_float)]
+
+
+class PresentParams(C.Structure):
+    """ps_present_params (tone-map uniforms, tonemap.wgsl:7-13)."""
+
+    _fields_ = [("ev100", C.c_float), ("tonemap", C.c_int32)]
 
 
 class FrameUniforms(C.Structure):
@@ -299,6 +307,8 @@ _SIGNATURES = [
     ("ps_layer_wind", None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float * 2]),
     ("ps_synthesise_weather", C.c_int32, [_P, C.POINTER(WeatherSynth), C.c_void_p]),
     ("ps_read_weather", C.c_int32, [_P, C.c_void_p, C.c_void_p]),
+    ("ps_present", C.c_int32, [_P, C.c_void_p, C.c_void_p, C.POINTER(ImageOut), C.POINTER(PresentParams),
+                               C.c_void_p, C.c_void_p]),
     ("ps_julian_day", C.c_double, [C.POINTER(Utc)]),
     ("ps_delta_t_seconds", C.c_double, [C.c_int32]),
     ("ps_sun_position_precise", C.c_int32, [C.POINTER(Utc), C.c_double, C.c_double, C.c_double,

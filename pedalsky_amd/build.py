@@ -20,8 +20,8 @@ LIB = LIB_DIR / "libpedalsky_hip.so"
 OBJ_DIR = PKG / "_build"
 INCLUDE = PKG.parent / "include"
 
-SOURCES = ["lut_kernels.hip", "render_kernel.hip", "adjoint_kernel.hip", "atmo_adjoint.hip", "noise_kernels.hip",
-           "noalloc.hip", "capi.cpp", "batch.cpp", "host_synthesis.cpp", "astro.cpp"]
+SOURCES = ["lut_kernels.hip", "render_kernel.hip", "post_kernel.hip", "adjoint_kernel.hip", "atmo_adjoint.hip",
+           "noise_kernels.hip", "noalloc.hip", "capi.cpp", "batch.cpp", "host_synthesis.cpp", "astro.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PEDALSKY_OFFLOAD_ARCH", "gfx950")

@@ -761,6 +761,38 @@ ps_status ps_read_weather(ps_ctx* c, uint16_t* rgba_f16, uint8_t* cloud_types) {
     return PS_OK;
 }
 
+ps_status ps_present(ps_ctx* c, const float* depth, const float* ground_lit, const ps_image_out* in,
+                     const ps_present_params* pp, uint8_t* out_srgb8, float* out_hdr) {
+    if (!c || !in || !pp) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: NULL argument");
+    if (!in->composite) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: in->composite is NULL");
+    if (in->format != PS_PIXEL_F32) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: images must be PS_PIXEL_F32");
+    if (ground_lit && (!depth || !in->cloud_luminance || !in->cloud_transmittance))
+        return fail(PS_ERR_INVALID_ARGUMENT,
+                    "ps_present: a surface radiance needs the depth buffer, cloud_luminance and cloud_transmittance");
+    if (pp->tonemap != PS_TONEMAP_ACES && pp->tonemap != PS_TONEMAP_CLAMP)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: tonemap must be PS_TONEMAP_ACES or PS_TONEMAP_CLAMP");
+    if (!out_srgb8 && !out_hdr) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: no output");
+    if (!std::isfinite(pp->ev100)) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: ev100 must be finite");
+    ps_status s;
+    if ((s = check_ready(c, true))) return s;
+    RenderParams next;
+    if ((s = fill_render_params(c, c->sc, c->luts, depth, in, next))) return s;
+    PresentArgs a{};
+    a.composite = (const float4*)in->composite;
+    a.luminance = (const float4*)in->cloud_luminance;
+    a.transmittance = (const float4*)in->cloud_transmittance;
+    a.ground = (const float4*)ground_lit;
+    a.hdr = (float4*)out_hdr;
+    a.ldr = (uint32_t*)out_srgb8;
+    // tonemap.wgsl:50: exposure = 1 / (1.2 * 2^EV100)
+    a.exposure = 1.f / (1.2f * std::pow(2.f, pp->ev100));
+    a.mode = pp->tonemap;
+    RenderParams* dP = nullptr;
+    if ((s = c->render_ring.upload(next, c->stream, &dP))) return s;
+    PS_HIP(ps_launch_present(dP, next.W, next.H, a, c->stream));
+    return c->render_ring.mark_used(c->stream);
+}
+
 ps_status ps_bake_luts(ps_ctx* c, uint32_t flags) {
     if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_bake_luts: ctx is NULL");
     return launch_luts(c, flags, nullptr);

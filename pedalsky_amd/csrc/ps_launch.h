@@ -63,6 +63,19 @@ hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail
                                 hipStream_t s);
 // Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
 // (Repeat wrap) of the coverage plane, still binary16.
+// Presentation (post_kernel.hip): AP over the surface radiance, composite,
+// tone map. Images are W*H FP32 RGBA; `ground` NULL keeps the composite.
+struct PresentArgs {
+    const float4* composite;
+    const float4* luminance;
+    const float4* transmittance;
+    const float4* ground;
+    float4* hdr;
+    uint32_t* ldr;                    // RGBA8 sRGB, R in the low byte
+    float exposure;
+    int mode;                         // 0 ACES filmic, 1 clamp
+};
+hipError_t ps_launch_present(const RenderParams* d_params, int W, int H, const PresentArgs& a, hipStream_t s);
 hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s);
 // Weather-map synthesis: the 128x128 RGBA binary16 map (R from the coverage
 // grid when `grid` is non-NULL, else `gate`) and, when `type_src` is non-NULL,

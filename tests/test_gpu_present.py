@@ -1,0 +1,107 @@
+"""GPU presentation pass (ps_present): aerial perspective over a surface
+radiance (shaders/ground/pbr.wgsl:653-666), the cloud composite over it
+(cloud_composite.wgsl:47-57) and the tone map to sRGB 8-bit
+(shaders/postprocess/tonemap.wgsl:37-58), against the FP64 oracle fed with
+the same render outputs. The AP froxels come from the oracle's own FP64 LUT
+chain. Bars: HDR relative error p99.9 <= 1e-3; 8-bit values within one step."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import oracle.present as PR
+from pedalsky_amd import abi, scene
+from pedalsky_amd.renderer import Renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU (no CPU fallback exists)"
+    r = Renderer(0)
+    w = scene.workload_small()
+    r.load_workload(w)
+    r.bake_luts()
+    bufs = r.alloc_outputs(w.width, w.height, abi.PS_PIXEL_F32,
+                           ("composite", "cloud_luminance", "cloud_transmittance"))
+    r.render(r.image_out(bufs))
+    r.synchronize()
+    host = {k: v.cpu().numpy().astype(np.float64) for k, v in bufs.items()}
+    # a surface radiance with structure: grey 0.18 albedo under the sun,
+    # modulated per pixel
+    yy, xx = np.mgrid[0:w.height, 0:w.width]
+    pat = 0.35 + 0.65 * (0.5 + 0.5 * np.sin(xx * 0.37) * np.cos(yy * 0.23))
+    lit = np.zeros((w.height, w.width, 4), np.float32)
+    lit[..., 0] = 0.18 / np.pi * 127500.0 * 0.6 * pat
+    lit[..., 1] = 0.9 * lit[..., 0]
+    lit[..., 2] = 0.8 * lit[..., 0]
+    ap0 = oracle.sky_ap_image(w, ground_radiance=0.0)
+    ap1 = oracle.sky_ap_image(w, ground_radiance=1.0)
+    yield r, w, bufs, host, lit, ap0, ap1[..., 0] - ap0[..., 0]
+    r.close()
+
+
+def _present(r, w, bufs, lit, ev100, mode, ground=True):
+    import torch
+    dev_lit = torch.from_numpy(lit).to("cuda:0") if ground else None
+    ldr = torch.zeros((w.height, w.width, 4), dtype=torch.uint8, device="cuda:0")
+    hdr = torch.zeros((w.height, w.width, 4), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    pp = abi.PresentParams(ev100, mode)
+    abi.check(r.lib.ps_present(r.ctx, r.depth_ptr, None if dev_lit is None else dev_lit.data_ptr(),
+                               C.byref(r.image_out(bufs)), C.byref(pp), ldr.data_ptr(), hdr.data_ptr()))
+    r.synchronize()
+    torch.cuda.synchronize()
+    return hdr.cpu().numpy(), ldr.cpu().numpy()
+
+
+@pytest.mark.parametrize("ev100,mode", [(14.0, abi.PS_TONEMAP_ACES), (15.5, abi.PS_TONEMAP_CLAMP)])
+def test_present_matches_oracle(setup, ev100, mode):
+    r, w, bufs, host, lit, ap_rgb, ap_a = setup
+    _log(f"case: present 320x180 surface + clouds, EV100 {ev100}, tonemap {mode}")
+    hdr, ldr = _present(r, w, bufs, lit, ev100, mode)
+    want_hdr, want_ldr = PR.present(host["composite"], host["cloud_luminance"], host["cloud_transmittance"],
+                                    w.depth, lit, ap_rgb, ap_a, ev100, mode)
+    ground = w.depth > 0
+    e = oracle.relative_error(hdr[..., :3], want_hdr)
+    d = np.abs(ldr[..., :3].astype(int) - want_ldr.astype(int))
+    _log(f"  surface pixels {ground.mean():.2f}; HDR p99.9 {np.percentile(e, 99.9):.3e}; "
+         f"8-bit max diff {d.max()}, exact {np.mean(d == 0):.4f}")
+    assert np.percentile(e, 99.9) <= 1e-3
+    assert d.max() <= 1 and np.mean(d == 0) >= 0.99
+    assert np.all(ldr[..., 3] == 255)
+    # sky pixels present the render's composite unchanged
+    assert np.array_equal(hdr[~ground][:, :3], host["composite"][~ground][:, :3].astype(np.float32))
+
+
+def test_present_without_surface_is_the_composite(setup):
+    r, w, bufs, host, lit, _, _ = setup
+    hdr, ldr = _present(r, w, bufs, lit, 14.0, abi.PS_TONEMAP_ACES, ground=False)
+    assert np.array_equal(hdr[..., :3], host["composite"][..., :3].astype(np.float32))
+    want = PR.srgb8(PR.aces(host["composite"][..., :3] * PR.exposure(14.0)))
+    assert np.abs(ldr[..., :3].astype(int) - want.astype(int)).max() <= 1
+
+
+def test_present_errors(setup):
+    r, w, bufs, host, lit, _, _ = setup
+    lib = r.lib
+    pp = abi.PresentParams(14.0, 7)
+    out = r.image_out(bufs)
+    assert lib.ps_present(r.ctx, r.depth_ptr, None, C.byref(out), C.byref(pp), None, None) == \
+        abi.PS_ERR_INVALID_ARGUMENT
+    pp = abi.PresentParams(14.0, 0)
+    assert lib.ps_present(r.ctx, None, bufs["composite"].data_ptr(), C.byref(out), C.byref(pp),
+                          bufs["composite"].data_ptr(), None) == abi.PS_ERR_INVALID_ARGUMENT
+    out16 = r.image_out(bufs)
+    out16.format = abi.PS_PIXEL_F16
+    assert lib.ps_present(r.ctx, r.depth_ptr, None, C.byref(out16), C.byref(pp), None,
+                          bufs["composite"].data_ptr()) == abi.PS_ERR_INVALID_ARGUMENT
