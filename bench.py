@@ -74,7 +74,7 @@ def bench_batch(r, n_scenes: int) -> dict:
     layers of any type, scaled atmospheres) rendered by one autotuned,
     graph-captured batch launch sequence, and the same scenes through the
     plain per-scene path (uniform uploads + LUT chain + render per scene)."""
-    from pedalsky_amd import scene
+    from pedalsky_amd import cost_model, scene
     from pedalsky_amd.batch import Batch, time_sequential
     log(f"batch: generating {n_scenes} scenes")
     ws = scene.workload_batch(n=n_scenes, seed=0xC3)
@@ -88,12 +88,45 @@ def bench_batch(r, n_scenes: int) -> dict:
     ms = b.time(5)
     log(f"batch: {ms:.2f} ms per graph replay; timing the per-scene path")
     seq_ms = time_sequential(r, ws, b.bufs, b.depths, reps=2)
+    flops = batch_flops(r, ws, b.depths)
     b.close()
+    tflops = flops / (ms * 1e-3) / 1e12
     return {"scenes": n_scenes, "megapixels": round(mpix, 3), "tuning": tuning.as_dict(),
             "ms_per_batch": round(ms, 3), "value": round(mpix / (ms * 1e-3), 3), "unit": "Mpix/s",
             "sequential_ms": round(seq_ms, 3), "sequential_mpix_s": round(mpix / (seq_ms * 1e-3), 3),
             "speedup_vs_sequential": round(seq_ms / ms, 3), "graph": True,
+            "roofline": {"bound": "valu", "achieved": round(tflops, 3), "peak": cost_model.PEAK_FP32_VALU_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tflops / cost_model.PEAK_FP32_VALU_TFLOPS, 4),
+                         "peak_unpacked": cost_model.PEAK_FP32_VALU_UNPACKED_TFLOPS,
+                         "frac_unpacked": round(tflops / cost_model.PEAK_FP32_VALU_UNPACKED_TFLOPS, 4),
+                         "flop_per_batch": flops,
+                         "note": "algorithmic FLOP of every scene's LUT chain and render (event counts x the "
+                                 "cost table) over the graph replay time"},
             "note": "every scene bakes its own LUT set; noise and weather map shared"}
+
+
+def batch_flops(r, ws, depths) -> float:
+    """Algorithmic FLOP of one batch: per scene its LUT chain and the event
+    counts of its render (ps_count_work, outside any timed region) x the cost
+    table."""
+    import ctypes as C
+    import numpy as np
+    from pedalsky_amd import abi, cost_model
+    lib = r.lib
+    total = 0.0
+    for w, d in zip(ws, depths):
+        layers = (abi.CloudLayer * max(len(w.layers), 1))(*w.layers)
+        winds = np.ascontiguousarray(w.winds, dtype=np.float32).reshape(-1)
+        wp = (C.c_float * len(winds))(*winds.tolist()) if len(winds) else None
+        abi.check(lib.ps_set_atmosphere(r.ctx, C.byref(w.atmosphere)))
+        abi.check(lib.ps_set_frame(r.ctx, C.byref(w.frame)))
+        abi.check(lib.ps_set_clouds(r.ctx, C.byref(w.clouds), layers, len(w.layers), wp))
+        abi.check(lib.ps_bake_luts(r.ctx, abi.PS_LUT_ALL))
+        cw = abi.WorkCounters()
+        abi.check(lib.ps_count_work(r.ctx, None if d is None else C.c_void_p(d.data_ptr()), C.byref(cw)))
+        total += cost_model.render_flops(cw) + cost_model.lut_flops(r.lut_dims())
+    r.synchronize()
+    return total
 
 
 def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:

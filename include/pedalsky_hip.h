@@ -479,6 +479,8 @@ typedef struct ps_tuning {
                                 0: one block per tile (hardware dispatch) */
     int32_t workers_per_cu;  /* persistent workers per compute unit (1..64) */
     int32_t order;           /* tile order: 0 scene order, 1 larger scenes first */
+    int32_t grab;            /* persistent schedule step: consecutive tiles a worker takes per
+                                counter increment (1..256; 0 means 1) */
 } ps_tuning;
 
 typedef struct ps_batch ps_batch;

@@ -169,7 +169,7 @@ class SceneDesc(C.Structure):
 class Tuning(C.Structure):
     """ps_tuning: launch shape of a batch."""
     _fields_ = [("tile_w", C.c_int32), ("persistent", C.c_int32), ("workers_per_cu", C.c_int32),
-                ("order", C.c_int32)]
+                ("order", C.c_int32), ("grab", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

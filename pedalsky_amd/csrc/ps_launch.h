@@ -44,8 +44,9 @@ hipError_t ps_launch_delta_bounds(const float* delta, float2* bnd, hipStream_t s
 // one-wave workers pull tiles from the counter `next` (zeroed by this call,
 // with a kernel, so the sequence can be captured in a graph); else one block
 // per tile.
+// Persistent workers take `grab` consecutive list entries per counter increment.
 hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* tiles, int n_tiles, int tile_w,
-                                  int persistent, int blocks, unsigned* next, hipStream_t s);
+                                  int persistent, int blocks, int grab, unsigned* next, hipStream_t s);
 
 // Noise: bake the stored volumes (format 0 = UNORM8, 1 = FP16), then fold
 // them into the render layouts.

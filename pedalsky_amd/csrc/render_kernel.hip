@@ -562,18 +562,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WA
 template <int TW, bool PERSISTENT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
                                                      const uint32_t* __restrict__ tiles, int n_tiles,
-                                                     unsigned* __restrict__ next) {
+                                                     int grab, unsigned* __restrict__ next) {
     __shared__ WaveQueue Qs;
     if (PERSISTENT) {
+        // entries [t, end) are this worker's; one counter increment takes
+        // `grab` of them (every worker contends for the one counter, so a
+        // million one-tile increments serialise there)
+        unsigned t = 0, end = 0;
         for (;;) {
-            // readfirstlane keeps the tile index, the entry and hence the
-            // scene's parameter block in scalar registers
-            unsigned t = 0;
-            if (threadIdx.x == 0) t = atomicAdd(next, 1u);
-            t = __builtin_amdgcn_readfirstlane(t);
-            if (t >= (unsigned)n_tiles) break;
+            if (t >= end) {
+                // readfirstlane keeps the tile index, the entry and hence the
+                // scene's parameter block in scalar registers
+                unsigned t0 = 0;
+                if (threadIdx.x == 0) t0 = atomicAdd(next, (unsigned)grab);
+                t = __builtin_amdgcn_readfirstlane(t0);
+                if (t >= (unsigned)n_tiles) break;
+                end = min(t + (unsigned)grab, (unsigned)n_tiles);
+            }
             const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[t]);
             render_tile<0, TW>(Ps[e >> 20], (int)(e & 1023u), (int)((e >> 10) & 1023u), Qs);
+            ++t;
         }
     } else {
         const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
@@ -777,19 +785,19 @@ hipError_t ps_launch_render(const RenderParams* d_params, int W, int H, bool cou
 }
 
 hipError_t ps_launch_render_tiles(const RenderParams* d_params, const uint32_t* tiles, int n_tiles, int tile_w,
-                                  int persistent, int blocks, unsigned* next, hipStream_t s) {
+                                  int persistent, int blocks, int grab, unsigned* next, hipStream_t s) {
     if (n_tiles <= 0) return hipSuccess;
     if (persistent) {
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, next);
         const dim3 g(blocks), b(64);
-        if (tile_w == 4) hipLaunchKernelGGL((k_render_tiles<4, true>), g, b, 0, s, d_params, tiles, n_tiles, next);
-        else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, true>), g, b, 0, s, d_params, tiles, n_tiles, next);
-        else hipLaunchKernelGGL((k_render_tiles<8, true>), g, b, 0, s, d_params, tiles, n_tiles, next);
+        if (tile_w == 4) hipLaunchKernelGGL((k_render_tiles<4, true>), g, b, 0, s, d_params, tiles, n_tiles, grab, next);
+        else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, true>), g, b, 0, s, d_params, tiles, n_tiles, grab, next);
+        else hipLaunchKernelGGL((k_render_tiles<8, true>), g, b, 0, s, d_params, tiles, n_tiles, grab, next);
     } else {
         const dim3 g(n_tiles), b(64);
-        if (tile_w == 4) hipLaunchKernelGGL((k_render_tiles<4, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
-        else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
-        else hipLaunchKernelGGL((k_render_tiles<8, false>), g, b, 0, s, d_params, tiles, n_tiles, next);
+        if (tile_w == 4) hipLaunchKernelGGL((k_render_tiles<4, false>), g, b, 0, s, d_params, tiles, n_tiles, grab, next);
+        else if (tile_w == 16) hipLaunchKernelGGL((k_render_tiles<16, false>), g, b, 0, s, d_params, tiles, n_tiles, grab, next);
+        else hipLaunchKernelGGL((k_render_tiles<8, false>), g, b, 0, s, d_params, tiles, n_tiles, grab, next);
     }
     return hipGetLastError();
 }

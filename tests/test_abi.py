@@ -25,7 +25,7 @@ def test_struct_layouts_match_reference():
         assert C.sizeof(cls) % 16 == 0
     # batch descriptors (static_assert'ed in batch.cpp)
     assert C.sizeof(abi.SceneDesc) == 96 and abi.SceneDesc.out.offset == 64
-    assert C.sizeof(abi.Tuning) == 16
+    assert C.sizeof(abi.Tuning) == 20
     # differentiation outputs: two device pointers
     assert C.sizeof(abi.CloudGrads) == 16 and abi.CloudGrads.density_delta.offset == 8
 

@@ -72,7 +72,8 @@ def test_batch_matches_single_path(setup):
     ref = _single(r, ws)
     for tw in (8, 16, 4):
         for persistent in (0, 1):
-            t = abi.Tuning(tw, persistent, 8, 1)
+            # persistent workers at 16x4 tiles take 3 list entries per counter step
+            t = abi.Tuning(tw, persistent, 8, 1, 3 if tw == 16 else 1)
             b = Batch(r, ws, tuning=t)
             b.run()
             r.synchronize()
@@ -161,10 +162,12 @@ def test_autotune_and_errors(setup):
     b = Batch(r, ws)
     t = b.autotune(reps=1)
     _log(f"  autotuned: {t.as_dict()}")
-    assert t.tile_w in (4, 8, 16) and t.persistent in (0, 1)
+    assert t.tile_w in (4, 8, 16) and t.persistent in (0, 1) and 0 <= t.grab <= 256
     assert b.tuning().as_dict() == t.as_dict()
     lib = r.lib
     bad = abi.Tuning(7, 0, 8, 1)
+    assert lib.ps_batch_set_tuning(b.h, C.byref(bad)) == abi.PS_ERR_INVALID_ARGUMENT
+    bad = abi.Tuning(8, 1, 8, 1, 257)
     assert lib.ps_batch_set_tuning(b.h, C.byref(bad)) == abi.PS_ERR_INVALID_ARGUMENT
     assert lib.ps_batch_replay(b.h) in (abi.PS_OK, abi.PS_ERR_NOT_READY)
     # a different resolution cannot be an update of the same batch
