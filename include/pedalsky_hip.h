@@ -311,6 +311,13 @@ ps_status ps_upload_noise(ps_ctx* ctx, int32_t format, const void* base_rgba,
 ps_status ps_read_noise(ps_ctx* ctx, int32_t which, void* host_out, size_t bytes);
 /* 64x64 blue-noise ranks (R8Unorm, noise.rs:646-704). */
 ps_status ps_upload_blue_noise(ps_ctx* ctx, const uint8_t* ranks_64x64);
+/* The same rank map generated on the GPU into the context (replaces the
+ * CPU generate_blue_noise(64) + texture upload of noise.rs:107-140,
+ * 646-704): the reference's greedy void-and-cluster placement, bit-identical
+ * to ps_generate_blue_noise. Asynchronous on the context's stream. */
+ps_status ps_synthesise_blue_noise(ps_ctx* ctx);
+/* Read the context's blue-noise ranks back (64x64 bytes; synchronises). */
+ps_status ps_read_blue_noise(ps_ctx* ctx, uint8_t* ranks_64x64);
 /* Weather map: w*h texels of `channels` binary16 values; channel 0 is the
  * coverage the march reads (cloud_march.wgsl:872-875). `cloud_type_grid`:
  * 128x128 u8 (255 = use the layer's type) or NULL for all-sentinel.

@@ -261,6 +261,8 @@ _SIGNATURES = [
     ("ps_upload_noise", C.c_int32, [_P, C.c_int32, _P, _P, _P]),
     ("ps_read_noise", C.c_int32, [_P, C.c_int32, _P, C.c_size_t]),
     ("ps_upload_blue_noise", C.c_int32, [_P, _P]),
+    ("ps_synthesise_blue_noise", C.c_int32, [_P]),
+    ("ps_read_blue_noise", C.c_int32, [_P, C.c_void_p]),
     ("ps_upload_weather", C.c_int32, [_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     ("ps_bake_luts", C.c_int32, [_P, C.c_uint32]),
     ("ps_read_lut", C.c_int32, [_P, C.c_int32, _P, C.c_size_t]),

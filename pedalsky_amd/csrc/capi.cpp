@@ -670,6 +670,25 @@ ps_status ps_upload_blue_noise(ps_ctx* c, const uint8_t* ranks) {
     return PS_OK;
 }
 
+ps_status ps_synthesise_blue_noise(ps_ctx* c) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_synthesise_blue_noise: NULL context");
+    ps_status s;
+    if ((s = dev_alloc(&c->blue, 64 * 64, "blue noise"))) return s;
+    BlueNoiseWeights w;
+    psi::blue_noise_weights(w.w);
+    PS_HIP(ps_launch_blue_noise(w, c->blue, c->stream));
+    c->have_blue = true;
+    return PS_OK;
+}
+
+ps_status ps_read_blue_noise(ps_ctx* c, uint8_t* ranks_64x64) {
+    if (!c || !ranks_64x64) return fail(PS_ERR_INVALID_ARGUMENT, "ps_read_blue_noise: NULL argument");
+    if (!c->have_blue) return fail(PS_ERR_NOT_READY, "ps_read_blue_noise: no blue noise uploaded or synthesised");
+    PS_HIP(hipMemcpyAsync(ranks_64x64, c->blue, 64 * 64, hipMemcpyDeviceToHost, c->stream));
+    PS_HIP(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
 ps_status ps_upload_weather(ps_ctx* c, const uint16_t* weather, int32_t w, int32_t h, int32_t channels,
                             const uint8_t* type_grid, const uint8_t* overcast) {
     if (!c || !weather) return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_weather: NULL argument");

@@ -99,6 +99,19 @@ void horizontal_wind(float surface_speed, float dir_to_rad, float alt_m, float& 
 
 }  // namespace
 
+namespace psi {
+
+// The blue-noise splat weight by squared texel distance r2 = 0..72:
+// exp(-r2 / (2 sigma^2)) with sigma = 1.9, in f32 as noise.rs:655-675
+// evaluates it.
+void blue_noise_weights(float* w73) {
+    const float sigma = 1.9f;
+    const float inv_two_sigma2 = 1.0f / (2.0f * sigma * sigma);
+    for (int r2 = 0; r2 < 73; ++r2) w73[r2] = std::exp(-(float)r2 * inv_two_sigma2);
+}
+
+}  // namespace psi
+
 extern "C" {
 
 void ps_default_atmosphere(ps_atmosphere_params* o) {
@@ -257,12 +270,12 @@ void ps_generate_blue_noise(uint8_t* out) {
     // generate_blue_noise(64), crates/ps-clouds/src/noise.rs:646-704
     const int size = 64;
     const int n = size * size;
+    const int radius = 6;
+    float w_r2[73];
+    psi::blue_noise_weights(w_r2);
     std::vector<uint8_t> taken(n, 0);
     std::vector<uint32_t> ranks(n, 0);
     std::vector<float> energy(n, 0.f);
-    const float sigma = 1.9f;
-    const int radius = 6;
-    const float inv_two_sigma2 = 1.0f / (2.0f * sigma * sigma);
     for (int placement = 0; placement < n; ++placement) {
         int best = 0;
         float best_e = std::numeric_limits<float>::infinity();
@@ -273,10 +286,8 @@ void ps_generate_blue_noise(uint8_t* out) {
         int cx = best % size, cy = best / size;
         for (int dy = -radius; dy <= radius; ++dy)
             for (int dx = -radius; dx <= radius; ++dx) {
-                float r2 = (float)(dx * dx + dy * dy);
-                float w = std::exp(-r2 * inv_two_sigma2);
                 int mx = ((cx + dx) % size + size) % size, my = ((cy + dy) % size + size) % size;
-                energy[my * size + mx] += w;
+                energy[my * size + mx] += w_r2[dx * dx + dy * dy];
             }
     }
     const float max_rank = (float)(n - 1);

@@ -370,7 +370,70 @@ __global__ __launch_bounds__(256) void k_type_grid_synth(const uint8_t* __restri
     out[i] = v <= 7 ? v : (uint8_t)255;
 }
 
+// ---- blue-noise rank map (noise.rs:646-704) ---------------------------------
+// The reference's greedy placement: 4096 times, take the untaken texel of
+// lowest energy (the lowest index among equals), give it the next rank, and
+// add a Gaussian of radius 6 around it to the energy field on the 64x64
+// torus. One workgroup of 1024 threads; thread t keeps texels 4t..4t+3 (one
+// row segment) with their energies and ranks in registers. Each placement is
+// a 64-bit minimum over (energy bits, index): energies are non-negative
+// sums, so their IEEE bits order like the values. A texel gains at most one
+// term per placement (the splat is 13 texels wide on a 64-texel torus), in
+// placement order, and the weights come from the host, so the energies, and
+// therefore the ranks, are bit-identical to the sequential generator.
+constexpr int kBlueSize = 64, kBlueCells = kBlueSize * kBlueSize, kBlueRadius = 6;
+
+__global__ __launch_bounds__(1024) void k_blue_noise(BlueNoiseWeights wts, uint8_t* __restrict__ out) {
+    __shared__ unsigned long long wave_min[2][16];
+    __shared__ float w_r2[kBlueWeights];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < kBlueWeights) w_r2[t] = wts.w[t];
+    const int base = t * 4, y = base / kBlueSize, x0 = base % kBlueSize;
+    float e[4] = {0.f, 0.f, 0.f, 0.f};
+    int rank[4] = {-1, -1, -1, -1};
+    __syncthreads();
+    for (int p = 0; p < kBlueCells; ++p) {
+        unsigned long long key = ~0ull;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (rank[k] < 0)
+                key = min(key, ((unsigned long long)__float_as_uint(e[k]) << 32) | (unsigned)(base + k));
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) key = min(key, (unsigned long long)__shfl_xor(key, off));
+        if (lane == 0) wave_min[p & 1][wave] = key;
+        // one barrier per placement: the next placement writes the other
+        // buffer, and the one after waits at this placement's successor's
+        // barrier until every thread has read this one
+        __syncthreads();
+        unsigned long long best = wave_min[p & 1][0];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) best = min(best, wave_min[p & 1][i]);
+        const int b = (int)(uint32_t)best, cx = b % kBlueSize, cy = b / kBlueSize;
+        // torus offset of this row from the placed texel, in [-32, 31]
+        const int dy = ((y - cy + kBlueSize / 2) & (kBlueSize - 1)) - kBlueSize / 2;
+        if (dy >= -kBlueRadius && dy <= kBlueRadius) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int dx = ((x0 + k - cx + kBlueSize / 2) & (kBlueSize - 1)) - kBlueSize / 2;
+                if (dx >= -kBlueRadius && dx <= kBlueRadius) e[k] += w_r2[dx * dx + dy * dy];
+            }
+        }
+        if (b >= base && b < base + 4) rank[b - base] = p;
+    }
+    // rank -> 0..255: (r / (n - 1)) * 255 rounded half away from zero, clamped
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float v = roundf(__fdiv_rn((float)rank[k], (float)(kBlueCells - 1)) * 255.f);
+        out[base + k] = (uint8_t)fminf(fmaxf(v, 0.f), 255.f);
+    }
+}
+
 }  // namespace
+
+hipError_t ps_launch_blue_noise(const BlueNoiseWeights& w, uint8_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_blue_noise, dim3(1), dim3(1024), 0, s, w, out);
+    return hipGetLastError();
+}
 
 hipError_t ps_launch_weather_synth(const float* grid, int gw, int gh, float extent, float gate, float alpha,
                                    const uint8_t* type_src, uint16_t* out_rgba, uint8_t* out_types, hipStream_t s) {

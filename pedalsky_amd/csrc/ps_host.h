@@ -14,6 +14,10 @@ namespace psi {
 // Sets the thread-local message ps_last_error() returns; returns `code`.
 ps_status fail(ps_status code, const std::string& msg);
 
+// Blue-noise splat weights by squared texel distance (73 entries), shared by
+// the host generator and the GPU synthesis (host_synthesis.cpp).
+void blue_noise_weights(float* w73);
+
 #define PS_HIP(call)                                                                           \
     do {                                                                                       \
         hipError_t e_ = (call);                                                                \

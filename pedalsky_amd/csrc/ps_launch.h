@@ -83,6 +83,14 @@ hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channel
 // the 128x128 cloud-type grid resampled from it (same gw x gh source).
 hipError_t ps_launch_weather_synth(const float* grid, int gw, int gh, float extent, float gate, float alpha,
                                    const uint8_t* type_src, uint16_t* out_rgba, uint8_t* out_types, hipStream_t s);
+// 64x64 blue-noise rank map (R8 bytes) from the Gaussian splat weights by
+// squared texel distance, w[r2] = exp(-r2 / (2 * 1.9^2)) as the host computes
+// them (r2 = 0..72).
+constexpr int kBlueWeights = 73;
+struct BlueNoiseWeights {
+    float w[kBlueWeights];
+};
+hipError_t ps_launch_blue_noise(const BlueNoiseWeights& w, uint8_t* out, hipStream_t s);
 
 // ---- atmosphere differentiation (atmo_adjoint.hip) ----------------------------
 // Overwrite the 14 differentiated coefficients of n LUT parameter blocks from

@@ -97,6 +97,16 @@ class Renderer:
         r = np.ascontiguousarray(ranks, dtype=np.uint8)
         check(self.lib.ps_upload_blue_noise(self.ctx, r.ctypes.data))
 
+    def synthesise_blue_noise(self) -> None:
+        """Generate the 64x64 blue-noise ranks on the GPU into the context."""
+        check(self.lib.ps_synthesise_blue_noise(self.ctx))
+
+    def read_blue_noise(self) -> np.ndarray:
+        """The context's (64, 64) blue-noise ranks."""
+        out = np.zeros((64, 64), np.uint8)
+        check(self.lib.ps_read_blue_noise(self.ctx, out.ctypes.data))
+        return out
+
     def upload_weather(self, weather: np.ndarray, type_grid: Optional[np.ndarray] = None,
                        overcast: Optional[np.ndarray] = None) -> None:
         w = np.ascontiguousarray(weather, dtype=np.uint16)
