@@ -355,24 +355,86 @@ ps_status ps_read_weather(ps_ctx* ctx, uint16_t* rgba_f16, uint8_t* cloud_types_
 
 /* ------------------------------------------------------------------------ */
 /* Presentation: the frame the reference's swapchain (and its golden images)
- * holds. Replaces the aerial-perspective application of the ground pass
- * (shaders/ground/pbr.wgsl:653-666: L = lit * ap.a + ap.rgb at the surface
- * distance), the cloud composite over it (cloud_composite.wgsl:47-57) and
- * the tone-map pass (shaders/postprocess/tonemap.wgsl:37-58 with the *Srgb
- * swapchain encode; crates/ps-postprocess/src/tonemap.rs).                 */
+ * holds, from the render's outputs and the depth buffer. Replaces, in frame
+ * order: the ground pass (shaders/ground/pbr.wgsl, crates/ps-ground: GGX /
+ * Smith / Lambert ground with Voronoi albedo, wet surface, puddles with
+ * ripples, snow, cloud-shadowed sun and overcast ambient, then
+ * L = lit * ap.a + ap.rgb at the surface distance, pbr.wgsl:653-666); the
+ * cloud composite over it (cloud_composite.wgsl:47-57); godrays
+ * (shaders/godrays/radial.wgsl + composite.wgsl, crates/ps-godrays: radial
+ * blur toward the sun at half resolution, added to the HDR target); bloom
+ * (shaders/bloom/*.wgsl, crates/ps-bloom: soft-knee bright pass, 3-level
+ * 13-tap / 9-tap tent pyramid, added to the HDR target); and the tone-map
+ * pass (shaders/postprocess/tonemap.wgsl:37-58 with the *Srgb swapchain
+ * encode; crates/ps-postprocess/src/tonemap.rs).                           */
 /* ------------------------------------------------------------------------ */
 enum { PS_TONEMAP_ACES = 0, PS_TONEMAP_CLAMP = 1 };
+/* Ground shading: the caller's surface radiance (`ground_lit`, or none), or
+ * the reference's ground pass evaluated on the depth buffer's surface. */
+enum { PS_GROUND_RADIANCE = 0, PS_GROUND_PBR = 1 };
+/* SurfaceMaterial::as_u32 (crates/ps-core/src/scene.rs:359-368, pbr.wgsl:135-139). */
+enum { PS_MATERIAL_GRASS = 0, PS_MATERIAL_BARE_SOIL = 1, PS_MATERIAL_TARMAC = 2, PS_MATERIAL_SAND = 3,
+       PS_MATERIAL_WATER_EDGE = 4 };
+
+/* SurfaceParams, crates/ps-core/src/weather.rs:159-201 and
+ * shaders/common/uniforms.wgsl:62-79. 64 bytes. */
+typedef struct ps_surface_params {
+    float visibility_m;
+    float temperature_c;
+    float dewpoint_c;
+    float pressure_hpa;
+    float wind_dir_deg;
+    float wind_speed_mps;
+    float ground_wetness;
+    float puddle_coverage;
+    float snow_depth_m;
+    float puddle_start;
+    float precip_intensity_mm_per_h;
+    float precip_kind;
+    float material;                  /* PS_MATERIAL_* as a float */
+    float _pad[3];
+} ps_surface_params;
+
 typedef struct ps_present_params {
-    float ev100;                     /* photographic EV at ISO 100 (tonemap.wgsl:50) */
+    float ev100;                     /* photographic EV at ISO 100 (tonemap.wgsl:50; also the frame's ev100 the
+                                        bloom threshold is relative to, ps-bloom lib.rs:482-483) */
     int32_t tonemap;                 /* PS_TONEMAP_ACES (ACESFilmic) or PS_TONEMAP_CLAMP (Passthrough) */
+    int32_t ground_model;            /* PS_GROUND_RADIANCE or PS_GROUND_PBR */
+    /* GodraysTuning (crates/ps-core/src/config.rs:521-549) and the host's
+     * runtime multiplier (ps-godrays lib.rs:337-339); godrays = 0 skips them */
+    int32_t godrays;
+    uint32_t godrays_samples;
+    float godrays_decay;
+    float godrays_intensity;         /* tuning intensity x intensity_scale */
+    float godrays_bright_threshold;
+    /* BloomTuning (config.rs:624-660); bloom = 0 skips it */
+    int32_t bloom;
+    float bloom_threshold_ev100;
+    float bloom_intensity;
+    float bloom_knee_ev;
+    ps_surface_params surface;       /* read with PS_GROUND_PBR (WeatherState.surface) */
 } ps_present_params;
+/* The reference's defaults: EV100 14, ACES, caller radiance, godrays on
+ * (64 samples, decay 0.96, intensity 0.6, threshold 5000), bloom on
+ * (threshold +2 EV, intensity 1, knee 0.5 EV), SurfaceParams::default(). */
+void ps_default_present_params(ps_present_params* out);
 /* `in`: the PS_PIXEL_F32 outputs of the ps_render call for this frame and
  * depth buffer. `ground_lit`: device W*H RGBA32F radiance of the opaque
  * surface before aerial perspective (the ground pass's `lit`), or NULL: the
- * composite is presented as it is. Outputs (device, either may be NULL but
- * not both): W*H RGBA8 sRGB (alpha 255) and W*H RGBA32F HDR. */
+ * composite is presented as it is; ignored with PS_GROUND_PBR, which shades
+ * the surface itself (flat ground, normal +y, at the depth buffer's
+ * positions). Outputs (device, either may be NULL but not both): W*H RGBA8
+ * sRGB (alpha 255) and W*H RGBA32F HDR (the tone map's input). Godrays and
+ * bloom use scratch images the context keeps at the frame's size. */
 ps_status ps_present(ps_ctx* ctx, const float* depth, const float* ground_lit, const ps_image_out* in,
                      const ps_present_params* params, uint8_t* out_srgb8, float* out_hdr);
+/* Auto-exposure statistics of an HDR image (auto_exposure.comp.wgsl:31-73):
+ * over the pixels on a 4-pixel grid, the sum of log2(max(luminance, 1e-6))
+ * and the pixel count, written to `out_sum_count` (device, 2 floats). */
+ps_status ps_exposure_stats(ps_ctx* ctx, const float* hdr_rgba32f, int32_t w, int32_t h, float* out_sum_count);
+/* The EV100 that maps the mean log luminance to mid-grey
+ * (crates/ps-postprocess/src/auto_exposure.rs:209-212). */
+float ps_ev100_from_exposure_stats(float log_lum_sum, float pixel_count);
 
 /* ------------------------------------------------------------------------ */
 /* The four LUT passes (AtmosphereSubsystem::dispatch_pass,

@@ -27,6 +27,9 @@ PS_PIXEL_F32 = 0
 PS_PIXEL_F16 = 1
 PS_TONEMAP_ACES = 0
 PS_TONEMAP_CLAMP = 1
+PS_GROUND_RADIANCE = 0
+PS_GROUND_PBR = 1
+PS_MATERIAL_GRASS, PS_MATERIAL_BARE_SOIL, PS_MATERIAL_TARMAC, PS_MATERIAL_SAND, PS_MATERIAL_WATER_EDGE = range(5)
 
 PS_LUT_TRANSMITTANCE = 1
 PS_LUT_MULTISCATTER = 2
@@ -74,10 +77,31 @@ class WeatherSynth(C.Structure):
                 ("grid_w", C.c_int32), ("grid_h", C.c_int32), ("grid_extent_m", C.c_float)]
 
 
-class PresentParams(C.Structure):
-    """ps_present_params (tone-map uniforms, tonemap.wgsl:7-13)."""
+class SurfaceParams(C.Structure):
+    """SurfaceParams (64 B, crates/ps-core/src/weather.rs:159-201)."""
 
-    _fields_ = [("ev100", C.c_float), ("tonemap", C.c_int32)]
+    _fields_ = [(n, C.c_float) for n in (
+        "visibility_m", "temperature_c", "dewpoint_c", "pressure_hpa", "wind_dir_deg", "wind_speed_mps",
+        "ground_wetness", "puddle_coverage", "snow_depth_m", "puddle_start", "precip_intensity_mm_per_h",
+        "precip_kind", "material")] + [("_pad", C.c_float * 3)]
+
+
+class PresentParams(C.Structure):
+    """ps_present_params: tone-map uniforms (tonemap.wgsl:7-13), ground model,
+    godrays and bloom tunables, surface parameters. Fields not given are 0
+    (caller radiance, godrays and bloom off)."""
+
+    _fields_ = [("ev100", C.c_float), ("tonemap", C.c_int32), ("ground_model", C.c_int32),
+                ("godrays", C.c_int32), ("godrays_samples", C.c_uint32), ("godrays_decay", C.c_float),
+                ("godrays_intensity", C.c_float), ("godrays_bright_threshold", C.c_float),
+                ("bloom", C.c_int32), ("bloom_threshold_ev100", C.c_float), ("bloom_intensity", C.c_float),
+                ("bloom_knee_ev", C.c_float), ("surface", SurfaceParams)]
+
+
+def default_present_params() -> PresentParams:
+    p = PresentParams()
+    load().ps_default_present_params(C.byref(p))
+    return p
 
 
 class FrameUniforms(C.Structure):
@@ -311,6 +335,9 @@ _SIGNATURES = [
     ("ps_read_weather", C.c_int32, [_P, C.c_void_p, C.c_void_p]),
     ("ps_present", C.c_int32, [_P, C.c_void_p, C.c_void_p, C.POINTER(ImageOut), C.POINTER(PresentParams),
                                C.c_void_p, C.c_void_p]),
+    ("ps_default_present_params", None, [C.POINTER(PresentParams)]),
+    ("ps_exposure_stats", C.c_int32, [_P, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    ("ps_ev100_from_exposure_stats", C.c_float, [C.c_float, C.c_float]),
     ("ps_julian_day", C.c_double, [C.POINTER(Utc)]),
     ("ps_delta_t_seconds", C.c_double, [C.c_int32]),
     ("ps_sun_position_precise", C.c_int32, [C.POINTER(Utc), C.c_double, C.c_double, C.c_double,

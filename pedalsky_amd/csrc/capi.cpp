@@ -30,6 +30,8 @@ static_assert(sizeof(ps_frame_uniforms) == 448, "FrameUniformsGpu is 448 B (fram
 static_assert(sizeof(ps_atmosphere_params) == 144, "AtmosphereParams is 144 B (weather.rs:544-555)");
 static_assert(sizeof(ps_cloud_params) == 112, "CloudParamsGpu is 112 B (params.rs:221-231)");
 static_assert(sizeof(ps_cloud_layer) == 48, "CloudLayerGpu is 48 B (weather.rs:544-555)");
+static_assert(sizeof(ps_surface_params) == 64, "SurfaceParams is 64 B (weather.rs:541-554)");
+static_assert(offsetof(ps_surface_params, material) == 48, "SurfaceParams.material offset");
 static_assert(offsetof(ps_frame_uniforms, inv_view_proj) == 192, "inv_view_proj offset");
 static_assert(offsetof(ps_frame_uniforms, camera_position_world) == 256, "camera offset");
 static_assert(offsetof(ps_frame_uniforms, sun_direction) == 288, "sun_direction offset");
@@ -512,7 +514,8 @@ void ps_ctx_destroy(ps_ctx* c) {
     dev_free(c->weather); dev_free(c->weather_synth); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
     dev_free(c->d_counters);
     dev_free(c->d_tile_times);
-    dev_free(c->delta_bnd); dev_free(c->rt);
+    dev_free(c->delta_bnd); dev_free(c->rt); dev_free(c->post_hdr); dev_free(c->post_rays);
+    for (float*& b : c->post_bloom) dev_free(b);
     for (ps_ctx::TaaView& v : c->taa_views) dev_free(v.hist);
     dev_free(c->ck); dev_free(c->px_state); dev_free(c->rec); dev_free(c->lrec); dev_free(c->g_delta64);
     dev_free(c->atmo_params); dev_free(c->aT); dev_free(c->aMS); dev_free(c->aSV); dev_free(c->aAP);
@@ -780,36 +783,172 @@ ps_status ps_read_weather(ps_ctx* c, uint16_t* rgba_f16, uint8_t* cloud_types) {
     return PS_OK;
 }
 
+void ps_default_present_params(ps_present_params* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->ev100 = 14.f;                          // RenderConfig::default (config.rs:243-244)
+    o->tonemap = PS_TONEMAP_ACES;
+    o->ground_model = PS_GROUND_RADIANCE;
+    o->godrays = 1;                           // GodraysTuning::default (config.rs:542-549)
+    o->godrays_samples = 64;
+    o->godrays_decay = 0.96f;
+    o->godrays_intensity = 0.6f;
+    o->godrays_bright_threshold = 5000.f;
+    o->bloom = 1;                             // BloomTuning::default (config.rs:651-659)
+    o->bloom_threshold_ev100 = 2.f;
+    o->bloom_intensity = 1.f;
+    o->bloom_knee_ev = 0.5f;
+    ps_surface_params& f = o->surface;        // SurfaceParams::default (weather.rs:203-222)
+    f.visibility_m = 30000.f;
+    f.temperature_c = 17.f;
+    f.dewpoint_c = 7.f;
+    f.pressure_hpa = 1018.f;
+    f.wind_dir_deg = 240.f;
+    f.wind_speed_mps = 5.f;
+    f.puddle_start = 0.6f;
+}
+
 ps_status ps_present(ps_ctx* c, const float* depth, const float* ground_lit, const ps_image_out* in,
                      const ps_present_params* pp, uint8_t* out_srgb8, float* out_hdr) {
     if (!c || !in || !pp) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: NULL argument");
     if (!in->composite) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: in->composite is NULL");
     if (in->format != PS_PIXEL_F32) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: images must be PS_PIXEL_F32");
-    if (ground_lit && (!depth || !in->cloud_luminance || !in->cloud_transmittance))
+    if (pp->ground_model != PS_GROUND_RADIANCE && pp->ground_model != PS_GROUND_PBR)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: ground_model must be PS_GROUND_RADIANCE or PS_GROUND_PBR");
+    const bool pbr = pp->ground_model == PS_GROUND_PBR;
+    if ((ground_lit || pbr) && (!depth || !in->cloud_luminance || !in->cloud_transmittance))
         return fail(PS_ERR_INVALID_ARGUMENT,
-                    "ps_present: a surface radiance needs the depth buffer, cloud_luminance and cloud_transmittance");
+                    "ps_present: a shaded surface needs the depth buffer, cloud_luminance and cloud_transmittance");
     if (pp->tonemap != PS_TONEMAP_ACES && pp->tonemap != PS_TONEMAP_CLAMP)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: tonemap must be PS_TONEMAP_ACES or PS_TONEMAP_CLAMP");
     if (!out_srgb8 && !out_hdr) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: no output");
     if (!std::isfinite(pp->ev100)) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: ev100 must be finite");
+    if (pp->godrays && (pp->godrays_samples < 1 || pp->godrays_samples > 1024 || !std::isfinite(pp->godrays_decay) ||
+                        !std::isfinite(pp->godrays_intensity) || !std::isfinite(pp->godrays_bright_threshold)))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: godrays need 1..1024 samples and finite tunables");
+    if (pp->bloom && (!std::isfinite(pp->bloom_threshold_ev100) || !std::isfinite(pp->bloom_intensity) ||
+                      !std::isfinite(pp->bloom_knee_ev)))
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: bloom tunables must be finite");
     ps_status s;
     if ((s = check_ready(c, true))) return s;
     RenderParams next;
     if ((s = fill_render_params(c, c->sc, c->luts, depth, in, next))) return s;
+    const int W = next.W, H = next.H;
+    const ps_frame_uniforms& f = c->sc.frame;
+
+    // godrays: the sun direction projected through view_proj as a direction
+    // (w = 0), in f32 in glam's column order (ps-godrays lib.rs:358-386)
+    bool rays_on = false;
+    float sun_u = 0.f, sun_v = 0.f;
+    if (pp->godrays && f.sun_direction[1] > 0.f) {
+        const float* m = f.view_proj;
+        const float sx = f.sun_direction[0], sy = f.sun_direction[1], sz = f.sun_direction[2];
+        const float cx = ((m[0] * sx + m[4] * sy) + m[8] * sz) + m[12] * 0.f;
+        const float cy = ((m[1] * sx + m[5] * sy) + m[9] * sz) + m[13] * 0.f;
+        const float cw = ((m[3] * sx + m[7] * sy) + m[11] * sz) + m[15] * 0.f;
+        if (cw > 1.0e-6f) {
+            const float nx = cx / cw, ny = cy / cw;
+            if (std::fabs(nx) <= 2.f && std::fabs(ny) <= 2.f) {
+                rays_on = true;
+                // radial.wgsl:71-74
+                sun_u = nx * 0.5f + 0.5f;
+                sun_v = 1.f - (ny * 0.5f + 0.5f);
+            }
+        }
+    }
+    const bool bloom_on = pp->bloom != 0;
+    const bool post = rays_on || bloom_on;
+
     PresentArgs a{};
     a.composite = (const float4*)in->composite;
     a.luminance = (const float4*)in->cloud_luminance;
     a.transmittance = (const float4*)in->cloud_transmittance;
-    a.ground = (const float4*)ground_lit;
-    a.hdr = (float4*)out_hdr;
-    a.ldr = (uint32_t*)out_srgb8;
+    a.ground = pbr ? nullptr : (const float4*)ground_lit;
     // tonemap.wgsl:50: exposure = 1 / (1.2 * 2^EV100)
     a.exposure = 1.f / (1.2f * std::pow(2.f, pp->ev100));
     a.mode = pp->tonemap;
+    a.pbr = pbr ? 1 : 0;
+    if (pbr) {
+        const ps_surface_params& sp = pp->surface;
+        const float mf = sp.material;
+        a.material = mf >= 0.f && mf < 4294967040.f ? (uint32_t)mf : 0u;
+        for (int k = 0; k < 3; ++k) a.albedo_scale[k] = c->sc.atm.ground_albedo[k] / 0.18f;
+        a.wetness = sp.ground_wetness;
+        a.puddle_start = sp.puddle_start;
+        a.puddle_coverage = sp.puddle_coverage;
+        a.precip = sp.precip_intensity_mm_per_h;
+        a.temperature = sp.temperature_c;
+        a.snow_depth = sp.snow_depth_m;
+        a.sim_seconds = f.simulated_seconds;
+    }
+    float4* work = (float4*)out_hdr;
+    if (post && !work) {
+        if ((s = grow(&c->post_hdr, &c->post_hdr_cap, (size_t)W * H * 4, "presentation HDR image"))) return s;
+        work = (float4*)c->post_hdr;
+    }
+    a.hdr = work;
+    a.ldr = post ? nullptr : (uint32_t*)out_srgb8;
     RenderParams* dP = nullptr;
     if ((s = c->render_ring.upload(next, c->stream, &dP))) return s;
-    PS_HIP(ps_launch_present(dP, next.W, next.H, a, c->stream));
-    return c->render_ring.mark_used(c->stream);
+    PS_HIP(ps_launch_present(dP, W, H, a, c->stream));
+    if ((s = c->render_ring.mark_used(c->stream))) return s;
+    if (!post) return PS_OK;
+
+    PostArgs q{};
+    q.hdr = work;
+    q.W = W;
+    q.H = H;
+    q.godrays = rays_on ? 1 : 0;
+    if (rays_on) {
+        q.rw = std::max(W / 2, 1);    // HALF_RES_FACTOR 2 (ps-godrays lib.rs:435-438)
+        q.rh = std::max(H / 2, 1);
+        if ((s = grow(&c->post_rays, &c->post_rays_cap, (size_t)q.rw * q.rh * 4, "godrays image"))) return s;
+        q.rays = (float4*)c->post_rays;
+        q.sun_u = sun_u;
+        q.sun_v = sun_v;
+        q.samples = (int)pp->godrays_samples;
+        q.decay = pp->godrays_decay;
+        q.rays_intensity = pp->godrays_intensity;
+        q.bright_threshold = pp->godrays_bright_threshold;
+    }
+    q.bloom = bloom_on ? 1 : 0;
+    if (bloom_on) {
+        // half / quarter / eighth resolution (ps-bloom lib.rs:439-467)
+        int lw = std::max(W / 2, 1), lh = std::max(H / 2, 1);
+        for (int l = 0; l < 3; ++l) {
+            q.lw[l] = lw;
+            q.lh[l] = lh;
+            if ((s = grow(&c->post_bloom[l], &c->post_bloom_cap[l], (size_t)lw * lh * 4, "bloom level"))) return s;
+            q.levels[l] = (float4*)c->post_bloom[l];
+            lw = std::max(lw / 2, 1);
+            lh = std::max(lh / 2, 1);
+        }
+        // threshold exp2(ev100 + threshold_ev100), knee knee_ev * ln 2
+        // (ps-bloom lib.rs:482-484)
+        q.bloom_threshold = std::exp2(pp->ev100 + pp->bloom_threshold_ev100);
+        q.bloom_knee = std::max(pp->bloom_knee_ev * 0.693147180559945309f, 1e-6f);
+        q.bloom_intensity = pp->bloom_intensity;
+    }
+    q.out_hdr = (float4*)out_hdr;
+    q.ldr = (uint32_t*)out_srgb8;
+    q.exposure = a.exposure;
+    q.mode = a.mode;
+    PS_HIP(ps_launch_post(q, c->stream));
+    return PS_OK;
+}
+
+ps_status ps_exposure_stats(ps_ctx* c, const float* hdr, int32_t w, int32_t h, float* out) {
+    if (!c || !hdr || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_exposure_stats: NULL argument");
+    if (w < 1 || h < 1) return fail(PS_ERR_INVALID_ARGUMENT, "ps_exposure_stats: empty image");
+    PS_HIP(ps_launch_exposure_stats((const float4*)hdr, w, h, out, c->stream));
+    return PS_OK;
+}
+
+float ps_ev100_from_exposure_stats(float log_lum_sum, float pixel_count) {
+    // auto_exposure.rs:209-212: mid-grey 0.18 times the tone map's 1.2
+    const float avg_log = log_lum_sum / pixel_count;
+    const float avg_lum = std::max(std::exp2(avg_log), 1e-6f);
+    return std::log2(avg_lum / 0.216f);
 }
 
 ps_status ps_bake_luts(ps_ctx* c, uint32_t flags) {

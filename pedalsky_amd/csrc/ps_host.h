@@ -173,6 +173,14 @@ struct ps_ctx {
     bool half_res = false;
     float* rt = nullptr;
     size_t rt_cap = 0;            // floats
+    // presentation scratch (float4 images): the working HDR image when the
+    // caller keeps none, the godrays RT and the three bloom levels
+    float* post_hdr = nullptr;
+    size_t post_hdr_cap = 0;      // floats
+    float* post_rays = nullptr;
+    size_t post_rays_cap = 0;
+    float* post_bloom[3] = {nullptr, nullptr, nullptr};
+    size_t post_bloom_cap[3] = {0, 0, 0};
     // cloud TAA: per view, two history slots of luminance and transmittance
     // (float4, slot-major: lum0, trn0, lum1, trn1) at the RT size
     struct TaaView {

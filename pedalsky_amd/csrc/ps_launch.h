@@ -62,10 +62,10 @@ inline void base_fold_scales(int format, float& r_scale, float& lf_scale) {
 hipError_t ps_launch_noise_fold(int format, const void* base, const void* detail, const void* curl,
                                 uint32_t* base_rl, float* detail_hf, float2* curl2, uint16_t* bbound,
                                 hipStream_t s);
-// Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
-// (Repeat wrap) of the coverage plane, still binary16.
-// Presentation (post_kernel.hip): AP over the surface radiance, composite,
-// tone map. Images are W*H FP32 RGBA; `ground` NULL keeps the composite.
+// Presentation (post_kernel.hip). Images are W*H FP32 RGBA.
+// Ground shading (surface pixels of the depth buffer): `ground` radiance, or
+// the ground pass itself with `pbr`; `ground` NULL and `pbr` 0 keep the
+// composite. The AP, the cloud composite and (with `ldr`) the tone map follow.
 struct PresentArgs {
     const float4* composite;
     const float4* luminance;
@@ -75,8 +75,43 @@ struct PresentArgs {
     uint32_t* ldr;                    // RGBA8 sRGB, R in the low byte
     float exposure;
     int mode;                         // 0 ACES filmic, 1 clamp
+    // ground pass (pbr.wgsl fs_main) inputs
+    int pbr;
+    uint32_t material;
+    float albedo_scale[3];            // world ground_albedo / 0.18
+    float wetness, puddle_start, puddle_coverage, precip, temperature, snow_depth;
+    float sim_seconds;
 };
 hipError_t ps_launch_present(const RenderParams* d_params, int W, int H, const PresentArgs& a, hipStream_t s);
+// Post-process chain on the HDR image `hdr` (W x H, updated in place):
+// godrays (when `godrays`: radial blur into `rays`, rw x rh, toward the sun
+// at `sun_uv`, then added) and bloom (when `bloom`: bright pass, pyramid in
+// `levels` of sizes lw[i] x lh[i], i < 3), then the final add of the bloom,
+// the optional HDR copy to `out_hdr` and the tone map to `ldr`.
+struct PostArgs {
+    float4* hdr;
+    int W, H;
+    int godrays;
+    float4* rays;
+    int rw, rh;
+    float sun_u, sun_v;
+    int samples;
+    float decay, rays_intensity, bright_threshold;
+    int bloom;
+    float4* levels[3];
+    int lw[3], lh[3];
+    float bloom_threshold, bloom_knee, bloom_intensity;
+    float4* out_hdr;
+    uint32_t* ldr;
+    float exposure;
+    int mode;
+};
+hipError_t ps_launch_post(const PostArgs& a, hipStream_t s);
+// Auto-exposure statistics (sum of log2 luminance, pixel count) of a W x H
+// image into out[0..1].
+hipError_t ps_launch_exposure_stats(const float4* hdr, int W, int H, float* out, hipStream_t s);
+// Weather: binary16 channel 0 of a w*h*channels map -> per-texel 2x2 quads
+// (Repeat wrap) of the coverage plane, still binary16.
 hipError_t ps_launch_weather_fold(const uint16_t* src, int w, int h, int channels, uint2* dst, hipStream_t s);
 // Weather-map synthesis: the 128x128 RGBA binary16 map (R from the coverage
 // grid when `grid` is non-NULL, else `gate`) and, when `type_src` is non-NULL,

@@ -277,6 +277,24 @@ def ground_depth(frame: abi.FrameUniforms, planet_radius: float = 6_360_000.0) -
     return distance_depth(frame, ground_distance(frame, planet_radius))
 
 
+def flat_ground_distance(frame: abi.FrameUniforms, half_extent: float = 100_000.0) -> np.ndarray:
+    """Per pixel, the view distance to the reference's startup ground mesh:
+    the plane y = 0 within |x|, |z| <= half_extent (QUAD_HALF_EXTENT_M,
+    crates/ps-ground/src/lib.rs:151, 365-407); inf where the ray misses it.
+    float64 (H, W)."""
+    cam, d = pixel_rays(frame)
+    down = d[..., 1] < 0
+    t = np.where(down, -cam[1] / np.where(down, d[..., 1], -1.0), np.inf)
+    p = cam + d * np.where(np.isfinite(t), t, 0.0)[..., None]
+    inside = (np.abs(p[..., 0]) <= half_extent) & (np.abs(p[..., 2]) <= half_extent)
+    return np.where(down & inside, t, np.inf)
+
+
+def flat_ground_depth(frame: abi.FrameUniforms, half_extent: float = 100_000.0) -> np.ndarray:
+    """Reverse-Z NDC depth the startup ground quad writes. float32 (H, W)."""
+    return distance_depth(frame, flat_ground_distance(frame, half_extent))
+
+
 def pixel_rays(frame: abi.FrameUniforms):
     """Camera position and the unit view ray through each pixel centre
     (math.wgsl:23-33 in FP64): (cam (3,), d (H, W, 3))."""
