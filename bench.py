@@ -225,6 +225,50 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
                     "L2 cotangent + ps_backward (layer and volume gradients), all on one stream"}
 
 
+def bench_present(r, W: int, H: int, steps: int) -> dict:
+    """Presentation of a rendered 1280x720 frame (ps_present): the ground pass
+    on the flat startup ground, AP and cloud composite, godrays (64 samples,
+    sun on screen), the 3-level bloom pyramid, ACES tone map to sRGB8 and the
+    HDR copy. Timed on its own (the frame's render is not in the timed
+    region), with HIP events around the launches."""
+    import ctypes as C
+    import torch
+    from pedalsky_amd import abi, scene
+    w = scene.workload_small(width=W, height=H, sun_zenith=75.0, pitch=10.0)
+    w.depth = scene.flat_ground_depth(w.frame)
+    r.load_workload(w)
+    bufs = r.alloc_outputs(W, H, abi.PS_PIXEL_F32, ("composite", "cloud_luminance", "cloud_transmittance"))
+    out = r.image_out(bufs)
+    r.frame(out)
+    r.synchronize()
+    ldr = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda:0")
+    hdr = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda:0")
+    pp = abi.default_present_params()
+    pp.ground_model = abi.PS_GROUND_PBR
+    stream = torch.cuda.Stream()  # default priority
+    r.set_stream(stream.cuda_stream)
+    res = {"workload": f"{W}x{H} rendered frame (one cumulus layer, sun 15 deg up and on screen), ground pass on "
+                       f"the flat ground, godrays 64 samples, bloom, ACES + sRGB8 and the HDR copy"}
+    for name, rays, bloom in (("ms_ground_ap_tonemap", 0, 0), ("ms_full_chain", 1, 1)):
+        pp.godrays, pp.bloom = rays, bloom
+
+        def call():
+            abi.check(r.lib.ps_present(r.ctx, r.depth_ptr, None, C.byref(out), C.byref(pp), ldr.data_ptr(),
+                                       hdr.data_ptr()))
+        with torch.cuda.stream(stream):
+            for _ in range(5):
+                call()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                call()
+            e1.record(stream)
+            e1.synchronize()
+        res[name] = round(e0.elapsed_time(e1) / steps, 4)
+    r.set_stream(None)
+    return res
+
+
 def bench_calibration(r, n_sets: int, iters: int) -> dict:
     """Atmosphere inverse calibration: `n_sets` seeded starting points
     (group multipliers U[0.5, 1.5], PCG32 seed 0xC5) fitted by Adam (lr 1e-2
@@ -283,6 +327,7 @@ def main() -> int:
     ap.add_argument("--no-calibration", action="store_true", help="skip the atmosphere calibration run")
     ap.add_argument("--no-half-res", action="store_true", help="skip the half-resolution cloud path measurement")
     ap.add_argument("--no-taa", action="store_true", help="skip the cloud TAA measurement")
+    ap.add_argument("--no-present", action="store_true", help="skip the presentation-chain measurement")
     ap.add_argument("--calibration-sets", type=int, default=256)
     ap.add_argument("--calibration-iters", type=int, default=200)
     ap.add_argument("--batch-scenes", type=int, default=64)
@@ -465,6 +510,10 @@ def main() -> int:
                          "half_res_ms_per_frame": round(ths * 1e3, 4),
                          "workload": "the same frames: march into a full-size (and a half-size) RT, TAA resolve "
                                      "against the history, composite"}
+    if not args.no_present:
+        log("presentation chain")
+        result["present"] = bench_present(r, W, H, args.steps)
+        r.load_workload(w0)
     if not args.no_adjoint:
         log("forward + adjoint")
         result["fwd_adjoint"] = bench_adjoint(r, W, H, args.adjoint_steps, 3)
