@@ -363,7 +363,7 @@ ps_status ps_read_weather(ps_ctx* ctx, uint16_t* rgba_f16, uint8_t* cloud_types_
  * cloud composite over it (cloud_composite.wgsl:47-57); godrays
  * (shaders/godrays/radial.wgsl + composite.wgsl, crates/ps-godrays: radial
  * blur toward the sun at half resolution, added to the HDR target); bloom
- * (shaders/bloom/*.wgsl, crates/ps-bloom: soft-knee bright pass, 3-level
+ * (shaders/bloom/{bright_pass,downsample,upsample}.wgsl, crates/ps-bloom: soft-knee bright pass, 3-level
  * 13-tap / 9-tap tent pyramid, added to the HDR target); and the tone-map
  * pass (shaders/postprocess/tonemap.wgsl:37-58 with the *Srgb swapchain
  * encode; crates/ps-postprocess/src/tonemap.rs).                           */
