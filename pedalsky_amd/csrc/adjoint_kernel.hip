@@ -76,48 +76,128 @@ __device__ __forceinline__ int opaque_int(int v) {
     return v;
 }
 
-// Perturbation gradients of many taps of a wave accumulate in a per-wave LDS
-// window of 4 x 4 texel columns (x, z) by 32 height rows around the first
-// tap; a tap whose eight corners all fall inside adds there, any other goes
-// straight to the FP64 accumulator. The window is added to the accumulator
-// (one FP64 atomic per non-zero texel) once per batch of taps: a chunk's
-// primary samples, or one light-sample index of a queue flush.
-constexpr int kWinCols = 4, kWinRows = 32, kWinSlots = kWinCols * kWinCols * kWinRows;
+// Perturbation gradients of many taps of a wave accumulate in per-wave LDS
+// windows of 4 x 4 texel columns (x, z) by 32 height rows: the first window
+// is placed around the first tap of a batch, the second around the first
+// tap the first one misses. A tap whose eight corners all fall inside a
+// window adds there with LDS float atomics; any other adds its eight weights
+// straight to the FP64 accumulator. The windows are added to the
+// accumulator (one FP64 atomic per non-zero texel) once per batch of taps: a
+// chunk's primary samples, or one light-sample index of a queue flush. A tap
+// in a window costs about one global atomic (the taps of a batch share
+// texels), a direct tap eight: each global atomic lane is a separate memory
+// transaction. A window is stored row-major (16 columns per row) and
+// records which 8-row blocks its taps touched, so that a flush reads only
+// those blocks (128 contiguous slots each).
+constexpr int kWinCols = 4, kWinRows = 32, kWinSize = kWinCols * kWinCols * kWinRows, kWins = 2,
+              kWinSlots = kWins * kWinSize, kWinBlockRows = 8;
 
 struct DeltaWindow {
-    uint32_t x0, z0, y0;              // lowest texel column (x, z) and row
-    bool set;
+    uint32_t x0[kWins], z0[kWins], y0[kWins];  // per window: lowest texel column (x, z) and row
+    uint32_t blocks[kWins];                    // per window: 8-row blocks touched (bit q: rows 8q..8q+7)
+    int n;                                     // windows placed (wave-uniform)
 };
 
+__device__ __forceinline__ DeltaWindow no_window() {
+    DeltaWindow w;
+#pragma unroll
+    for (int k = 0; k < kWins; ++k) w.x0[k] = w.z0[k] = w.y0[k] = w.blocks[k] = 0u;
+    w.n = 0;
+    return w;
+}
+
+// Window k around the tap of lane `leader`: one column before it in x and z,
+// half the rows below it (clamped to the volume).
+__device__ __forceinline__ void place_window(DeltaWindow& w, int k, uint32_t xi, uint32_t zi, uint32_t r0,
+                                             int leader) {
+    w.x0[k] = (__builtin_amdgcn_readlane(xi, leader) + 127u) & 127u;
+    w.z0[k] = (__builtin_amdgcn_readlane(zi, leader) + 127u) & 127u;
+    const uint32_t lr = __builtin_amdgcn_readlane(r0, leader);
+    w.y0[k] = lr >= kWinRows / 2 ? min(lr - kWinRows / 2, 128u - kWinRows) : 0u;
+}
+
+__device__ __forceinline__ bool in_window(uint32_t x0, uint32_t z0, uint32_t y0, uint32_t xi, uint32_t zi,
+                                          uint32_t r0, uint32_t r1) {
+    const uint32_t lx = (xi - x0) & 127u, lz = (zi - z0) & 127u;
+    return lx < kWinCols - 1 && lz < kWinCols - 1 && r0 >= y0 && r1 - y0 < (uint32_t)kWinRows;
+}
+
+// Add the windows' touched blocks to the accumulator and clear them. Called
+// with all 64 lanes active.
+__device__ __forceinline__ void delta_window_flush(double* __restrict__ g, float* win, DeltaWindow& w) {
+    if (w.n == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kWins; ++k) {
+        if (k >= w.n) break;
+        float* wk = win + k * kWinSize;
+        uint32_t bm = w.blocks[k];
+        while (bm != 0u) {
+            const int q = __builtin_ctz(bm);
+            bm &= bm - 1u;
+#pragma unroll
+            for (int j = 0; j < kWinBlockRows * kWinCols * kWinCols / 64; ++j) {
+                const int slot = q * (kWinBlockRows * kWinCols * kWinCols) + j * 64 + lane;
+                const float sum = wk[slot];
+                if (sum != 0.f) {
+                    const uint32_t row = (uint32_t)slot >> 4, col = (uint32_t)slot & 15u;
+                    const uint32_t x = (w.x0[k] + col % kWinCols) & 127u, z = (w.z0[k] + col / kWinCols) & 127u;
+                    atomic_add_g(g, (z << 14) | ((w.y0[k] + row) << 7) | x, (double)sum);
+                    wk[slot] = 0.f;
+                }
+            }
+        }
+        w.blocks[k] = 0u;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    w.n = 0;
+}
+
+// Scatter v times the trilinear weights of each lane's tap d (lanes with
+// has). Called with all 64 lanes active.
 __device__ __forceinline__ void delta_scatter_window(double* __restrict__ g, float* win, DeltaWindow& w, bool has,
                                                      const DeltaTap& d, float v) {
     const uint32_t xi = d.x0, zi = d.z0 >> 14, r0 = d.y0 >> 7, r1 = d.y1 >> 7;
     const unsigned long long act = __ballot(has);
     if (act == 0ull) return;
-    if (!w.set) {
-        const int leader = __builtin_ctzll(act);
-        w.x0 = (__builtin_amdgcn_readlane(xi, leader) + 127u) & 127u;
-        w.z0 = (__builtin_amdgcn_readlane(zi, leader) + 127u) & 127u;
-        const uint32_t lr = __builtin_amdgcn_readlane(r0, leader);
-        w.y0 = lr >= kWinRows / 2 ? min(lr - kWinRows / 2, 128u - kWinRows) : 0u;
-        w.set = true;
+    if (w.n == 0) {
+        place_window(w, 0, xi, zi, r0, __builtin_ctzll(act));
+        w.n = 1;
+    }
+    const bool in0 = has && in_window(w.x0[0], w.z0[0], w.y0[0], xi, zi, r0, r1);
+    const unsigned long long miss = __ballot(has && !in0);
+    if (miss != 0ull && w.n == 1) {
+        place_window(w, 1, xi, zi, r0, __builtin_ctzll(miss));
+        w.n = 2;
+    }
+    const bool in1 = has && !in0 && w.n == 2 && in_window(w.x0[1], w.z0[1], w.y0[1], xi, zi, r0, r1);
+    const uint32_t x0 = in0 ? w.x0[0] : w.x0[1], z0 = in0 ? w.z0[0] : w.z0[1], y0 = in0 ? w.y0[0] : w.y0[1];
+    const uint32_t a = r0 - y0, b = r1 - y0;  // window rows of the tap (meaningful when in0 || in1)
+    // the 8-row blocks the window taps touch
+#pragma unroll
+    for (int q = 0; q < kWinRows / kWinBlockRows; ++q) {
+        const bool tq = (a >> 3) == (uint32_t)q || (b >> 3) == (uint32_t)q;
+        if (__ballot(in0 && tq) != 0ull) w.blocks[0] |= 1u << q;
+        if (__ballot(in1 && tq) != 0ull) w.blocks[1] |= 1u << q;
     }
     if (!has) return;
-    const uint32_t lx = (xi - w.x0) & 127u, lz = (zi - w.z0) & 127u;
     const float wx1 = d.fx, wx0 = 1.f - d.fx, wy1 = d.fy, wy0 = 1.f - d.fy, wz1 = d.fz, wz0 = 1.f - d.fz;
-    if (lx < kWinCols - 1 && lz < kWinCols - 1 && r0 >= w.y0 && r1 - w.y0 < (uint32_t)kWinRows) {
-        // slot = ((z * 4 + x) * 32 + row)
-        const uint32_t c00 = (lz * kWinCols + lx) * kWinRows, c01 = c00 + kWinRows, c10 = c00 + kWinCols * kWinRows,
-                       c11 = c10 + kWinRows;
-        const uint32_t a = r0 - w.y0, b = r1 - w.y0;
-        atomicAdd(win + c00 + a, v * wz0 * wy0 * wx0);
-        atomicAdd(win + c01 + a, v * wz0 * wy0 * wx1);
-        atomicAdd(win + c00 + b, v * wz0 * wy1 * wx0);
-        atomicAdd(win + c01 + b, v * wz0 * wy1 * wx1);
-        atomicAdd(win + c10 + a, v * wz1 * wy0 * wx0);
-        atomicAdd(win + c11 + a, v * wz1 * wy0 * wx1);
-        atomicAdd(win + c10 + b, v * wz1 * wy1 * wx0);
-        atomicAdd(win + c11 + b, v * wz1 * wy1 * wx1);
+    if (in0 || in1) {
+        const uint32_t lx = (xi - x0) & 127u, lz = (zi - z0) & 127u;
+        // slot = window * kWinSize + row * 16 + (z * 4 + x)
+        const uint32_t c = (in0 ? 0u : (uint32_t)kWinSize) + lz * kWinCols + lx;
+        const uint32_t ra = c + a * (kWinCols * kWinCols), rb = c + b * (kWinCols * kWinCols);
+        atomicAdd(win + ra, v * wz0 * wy0 * wx0);
+        atomicAdd(win + ra + 1, v * wz0 * wy0 * wx1);
+        atomicAdd(win + rb, v * wz0 * wy1 * wx0);
+        atomicAdd(win + rb + 1, v * wz0 * wy1 * wx1);
+        atomicAdd(win + ra + kWinCols, v * wz1 * wy0 * wx0);
+        atomicAdd(win + ra + kWinCols + 1, v * wz1 * wy0 * wx1);
+        atomicAdd(win + rb + kWinCols, v * wz1 * wy1 * wx0);
+        atomicAdd(win + rb + kWinCols + 1, v * wz1 * wy1 * wx1);
     } else {
         const uint32_t off[8] = {d.z0 + d.y0 + d.x0, d.z0 + d.y0 + d.x1, d.z0 + d.y1 + d.x0, d.z0 + d.y1 + d.x1,
                                  d.z1 + d.y0 + d.x0, d.z1 + d.y0 + d.x1, d.z1 + d.y1 + d.x0, d.z1 + d.y1 + d.x1};
@@ -126,29 +206,6 @@ __device__ __forceinline__ void delta_scatter_window(double* __restrict__ g, flo
 #pragma unroll
         for (int c = 0; c < 8; ++c) atomic_add_g(g, off[c], (double)wt[c]);
     }
-}
-
-// Add a chunk's window to the accumulator and clear it. Called with all 64
-// lanes active.
-__device__ __forceinline__ void delta_window_flush(double* __restrict__ g, float* win, DeltaWindow& w) {
-    if (!w.set) return;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const int lane = threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kWinSlots / 64; ++k) {
-        const int slot = k * 64 + lane;
-        const float sum = win[slot];
-        if (sum != 0.f) {
-            const uint32_t row = (uint32_t)slot % kWinRows, col = (uint32_t)slot / kWinRows;
-            const uint32_t x = (w.x0 + col % kWinCols) & 127u, z = (w.z0 + col / kWinCols) & 127u;
-            atomic_add_g(g, (z << 14) | ((w.y0 + row) << 7) | x, (double)sum);
-            win[slot] = 0.f;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    w.set = false;
 }
 
 // Scatter g (= dL/d od * density_scale) over each lane's light samples in
@@ -174,7 +231,7 @@ __device__ __forceinline__ void light_delta_scatter_wave(const RenderParams& P, 
             delta_coords(P, qx - L.wind_off_x, qz - L.wind_off_z, h, dcx, dcy, dcz);
             d = delta_tap(dcx, dcy, dcz);
         }
-        DeltaWindow dw{0u, 0u, 0u, false};
+        DeltaWindow dw = no_window();
         delta_scatter_window(P.g_delta64, win, dw, hi, d, g * wt);
         delta_window_flush(P.g_delta64, win, dw);
     }
@@ -199,7 +256,7 @@ struct BackQueue {
     float seed[SD_FIELDS][64];        // per owner lane: luminance cotangent, depth-weight cotangents, 1 - cos,
                                       // view ray
     float wacc[3 * PS_MAX_LAYERS];    // per-layer gradient sums of the wave
-    float win[kWinSlots];             // perturbation-gradient window of a chunk (zero between uses)
+    float win[kWinSlots];             // perturbation-gradient windows of a batch (zero between uses)
 };
 
 // Shade queue entry j with its adjoint (cloud_march.wgsl:886-993): writes
@@ -664,7 +721,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             // scatter can combine lanes
             const LayerC& LC = P.layers[li];
             float gds = 0.f, gk = 0.f, gcov = 0.f;
-            DeltaWindow win{0u, 0u, 0u, false};
+            DeltaWindow win = no_window();
             for (int slot = ckl - 1; slot >= 0; --slot) {
                 const bool mine = act && ((dmask >> slot) & 1u);
                 if (__ballot(mine) == 0ull) continue;
