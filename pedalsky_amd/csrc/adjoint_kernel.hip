@@ -299,9 +299,9 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     float od, dod_ds, dod_cov;
     uint32_t lin;
     if (kd < P.lrec_cap) {
-        const float* r = lrec_tile + (size_t)kd * (LR_FIELDS * 64) + owner;
-        od = r[LR_OD * 64]; dod_ds = r[LR_DDS * 64]; dod_cov = r[LR_DCOV * 64];
-        lin = __float_as_uint(r[LR_LIN * 64]);
+        const float4 r = *(const float4*)(lrec_tile + ((size_t)kd * 64 + owner) * LR_FIELDS);
+        od = r.x; dod_ds = r.y; dod_cov = r.z;
+        lin = __float_as_uint(r.w);
     } else {
         od = light_adj<DELTA>(P, L, nc, qx, qyl, qz, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
     }
@@ -578,9 +578,9 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             }
             s0 = sub * ckl;
             s1 = min(min(steps, s0 + ckl), n_exec - seg * steps);
-            const float* ckp = P.ck + (tile * P.ck_max + chunk) * PS_CK_STRIDE + lane;
-            T[0] = ckp[0]; T[1] = ckp[64]; T[2] = ckp[128];
-            kd = (int)ckp[192];
+            const float4 ck = *(const float4*)(P.ck + (tile * P.ck_max + chunk) * PS_CK_STRIDE + lane * 4);
+            T[0] = ck.x; T[1] = ck.y; T[2] = ck.z;
+            kd = (int)ck.w;
             kd0 = kd;
         }
         // The chunk's dense samples in forward order, from the training
@@ -598,13 +598,14 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
             float t = 0.f, px = 0.f, py = 0.f, pz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
             uint32_t ctype = 0u;
             if (more && use_rec) {
-                const float* r = lrec_tile + (size_t)(kd0 + k) * (LR_FIELDS * 64) + lane;
-                const uint32_t sr = __float_as_uint(r[LR_SREC * 64]);
+                // the sample half of the record: step word, clamped shape, coverage partial, weather
+                const float4 r = *(const float4*)(lrec_tile + ((size_t)(kd0 + k) * 64 + lane) * LR_FIELDS + 4);
+                const uint32_t sr = __float_as_uint(r.x);
                 const int sk = (int)(sr & 0xffffu);
                 ctype = sr >> 24;
                 slot = sk - s0;
-                const float cval = r[LR_CVAL * 64];
-                wr = r[LR_WR * 64];
+                const float cval = r.y;
+                wr = r.w;
                 const float tl = (float)sk * g.step;
                 t = g.t0 + tl;
                 seg_sample(P, g, dx, dy, dz, tl, px, py, pz, alt);
@@ -612,7 +613,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 dense = true;
                 float* rp = rec + (size_t)slot * (kRecFields * 64);
                 rp[REC_C * 64] = cval;
-                rp[REC_Q * 64] = r[LR_QC * 64];
+                rp[REC_Q * 64] = r.z;
                 dmask |= 1u << slot;
             } else if (more) {
                 const float tl = (float)s * g.step;

@@ -168,13 +168,14 @@ struct RenderParams {
     float delta_su;                   // 128 / weather_scale: delta texels per metre
     // ---- training forward (checkpoints) and backward ----
     // Per wave tile (tiles row-major, 64 lanes): transmittance checkpoints
-    // [tile][chunk][rgb + dense count][lane] at the first step of every ck_len-step chunk
+    // [tile][chunk][lane][rgb + dense count] at the first step of every ck_len-step chunk
     // of every layer segment, and the per-pixel forward totals
     // [tile][PX_FIELDS][lane].
     float* ck;
     float* px_state;
     // Records of the training forward, per wave tile and per pixel's k-th
-    // dense sample (k < lrec_cap, in march order): [tile][k][LR_FIELDS][lane].
+    // dense sample (k < lrec_cap, in march order): [tile][k][lane][LR_FIELDS]
+    // (two 16-byte halves: the light march, then the sample).
     // The backward reads the sample (step, shape, coverage partial, weather
     // value) instead of marching the chunk again, and its light march
     // instead of marching toward the sun again; a chunk with samples beyond
@@ -218,7 +219,7 @@ struct TaaParams {
 // number of dense samples).
 enum { PX_LR = 0, PX_LG, PX_LB, PX_TR, PX_TG, PX_TB, PX_TW, PX_TWN, PX_NEXEC, PX_NDENSE, PX_FIELDS };
 // A transmittance checkpoint: rgb and the number of dense samples the pixel
-// had before the chunk (as a float), each a row of 64 lanes.
+// had before the chunk (as a float), one 16-byte vector per lane.
 #define PS_CK_STRIDE 256
 // Stored per dense sample: the light march's optical depth, its partials in
 // density_scale and coverage, and the bit mask of light samples inside the

@@ -55,11 +55,15 @@ constexpr int kQCap = 128;        // >= 64 + 63: a full flush plus one step's ar
 // of it, its ray distance and step, and the owner lane, layer and cloud
 // type. World-frame quantities (local up, lightning distance) come from the
 // owner's view ray, read from the owner lane when the queue is flushed.
-constexpr int kQFields = 12;
-enum { QF_QX = 0, QF_QY, QF_QZ, QF_ALT, QF_WR, QF_DENS, QF_TR, QF_TG, QF_TB, QF_T, QF_STEP, QF_META };
-
+// The training forward's queue also carries the sample's record fields
+// (step word, clamped shape, coverage partial), written with the light
+// march's record when the sample is shaded.
+enum { QF_QX = 0, QF_QY, QF_QZ, QF_ALT, QF_WR, QF_DENS, QF_TR, QF_TG, QF_TB, QF_T, QF_STEP, QF_META, QF_SREC, QF_CVAL,
+       QF_QC };
+template <bool TRAIN>
 struct WaveQueue {
-    float f[kQFields][kQCap];
+    static constexpr int kFields = TRAIN ? 15 : 12;
+    float f[kFields][kQCap];
     float acc[5][64];             // per-pixel luminance rgb, sum(w t), sum(w)
 };
 
@@ -74,7 +78,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 // light march also takes its parameter partials, and the results go to the
 // owner's light-march record k (the backward's input) when k < lrec_cap.
 template <bool DELTA, bool TRAIN>
-__device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q, int j, Cnt& cnt, float odx,
+__device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue<TRAIN>& Q, int j, Cnt& cnt, float odx,
                                              float ody, float odz, float omc, float* lrec_tile) {
     const float qx = Q.f[QF_QX][j], qy = Q.f[QF_QY][j], qz = Q.f[QF_QZ][j];
     const float alt = Q.f[QF_ALT][j], wr = Q.f[QF_WR][j], density = Q.f[QF_DENS][j];
@@ -96,9 +100,9 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
         od = light_adj<DELTA>(P, L, ndf_coef(ctype), qx, qy, qz, px, py, pz, alt, wr, ctype, dod_ds, dod_cov, lin);
         const int k = (int)((meta >> 12) & 255u);
         if (k < P.lrec_cap) {
-            float* r = lrec_tile + (size_t)k * (LR_FIELDS * 64) + owner;
-            r[LR_OD * 64] = od; r[LR_DDS * 64] = dod_ds; r[LR_DCOV * 64] = dod_cov;
-            r[LR_LIN * 64] = __uint_as_float(lin);
+            float4* r = (float4*)(lrec_tile + ((size_t)k * 64 + owner) * LR_FIELDS);
+            r[0] = make_float4(od, dod_ds, dod_cov, __uint_as_float(lin));
+            r[1] = make_float4(Q.f[QF_SREC][j], Q.f[QF_CVAL][j], Q.f[QF_QC][j], wr);
         }
     } else {
         od = march_light<DELTA>(P, L, ndf_coef(ctype), qx, qy, qz, px, py, pz, alt, wr, ctype, cnt);
@@ -197,7 +201,7 @@ __device__ __forceinline__ void shade_record(const RenderParams& P, WaveQueue& Q
 // the entries [64, qn) to the front. Called with all 64 lanes active; (dx,
 // dy, dz) and omc are this lane's own view ray and 1 - cos(view, sun).
 template <bool DELTA, bool TRAIN>
-__device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q, int lane, int n, int qn, Cnt& cnt,
+__device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue<TRAIN>& Q, int lane, int n, int qn, Cnt& cnt,
                                             float dx, float dy, float dz, float omc, float* lrec_tile) {
     wave_sync_lds();
     const int owner = lane < n ? (int)(__float_as_uint(Q.f[QF_META][lane]) & 63u) : lane;
@@ -207,15 +211,16 @@ __device__ __forceinline__ void flush_queue(const RenderParams& P, WaveQueue& Q,
     wave_sync_lds();
     const int rest = qn - n;
     if (rest > 0) {
-        float v[kQFields];
+        constexpr int NF = WaveQueue<TRAIN>::kFields;
+        float v[NF];
         if (lane < rest) {
 #pragma unroll
-            for (int k = 0; k < kQFields; ++k) v[k] = Q.f[k][64 + lane];
+            for (int k = 0; k < NF; ++k) v[k] = Q.f[k][64 + lane];
         }
         wave_sync_lds();
         if (lane < rest) {
 #pragma unroll
-            for (int k = 0; k < kQFields; ++k) Q.f[k][lane] = v[k];
+            for (int k = 0; k < NF; ++k) Q.f[k][lane] = v[k];
         }
         wave_sync_lds();
     }
@@ -240,7 +245,8 @@ __device__ __forceinline__ int opaque_int(int v) {
 // For the usual layer stacks every lane enters the layers in the same order
 // and each layer takes one round.
 template <int FLAGS, int TW>
-__device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, int tile_y, WaveQueue& Q) {
+__device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, int tile_y,
+                                            WaveQueue<(FLAGS & RF_TRAIN) != 0>& Q) {
     static_assert(TW == 4 || TW == 8 || TW == 16, "wave tiles are 4x16, 8x8 or 16x4 pixels");
     constexpr bool COUNT = (FLAGS & RF_COUNT) != 0, DELTA = (FLAGS & RF_DELTA) != 0, TRAIN = (FLAGS & RF_TRAIN) != 0;
     constexpr bool RT = (FLAGS & RF_RT) != 0;
@@ -298,15 +304,13 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
         float* ckp = nullptr;
         if constexpr (TRAIN)
             ckp = P.ck + ((size_t)(tile_y * P.n_tiles_x + tile_x) * P.ck_max + next * P.ck_per_seg) * PS_CK_STRIDE +
-                  lane;
+                  lane * 4;
         NdfC ncl = layer_ndf(L);
         for (int s = 0; s < P.steps; ++s) {
             if (__ballot(on) == 0ull) break;
             if constexpr (TRAIN) {
-                if (on && s % P.ck_len == 0) {
-                    float* c = ckp + (size_t)(s / P.ck_len) * PS_CK_STRIDE;
-                    c[0] = tr; c[64] = tg; c[128] = tbb; c[192] = (float)kd;
-                }
+                if (on && s % P.ck_len == 0)
+                    *(float4*)(ckp + (size_t)(s / P.ck_len) * PS_CK_STRIDE) = make_float4(tr, tg, tbb, (float)kd);
             }
             bool dense = false;
             float t = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, alt = 0.f, wr = 0.f, density = 0.f;
@@ -404,13 +408,9 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
                     Q.f[QF_META][pos] = __uint_as_float((uint32_t)lane | ((uint32_t)li << 6) | (ctype << 9) |
                                                         ((uint32_t)min(kd, 255) << 12));
                     if constexpr (TRAIN) {
-                        if (kd < P.lrec_cap) {
-                            float* r = lrec_tile + (size_t)kd * (LR_FIELDS * 64) + lane;
-                            r[LR_SREC * 64] = __uint_as_float((uint32_t)s | ((uint32_t)next << 16) | (ctype << 24));
-                            r[LR_CVAL * 64] = cval;
-                            r[LR_QC * 64] = qc;
-                            r[LR_WR * 64] = wr;
-                        }
+                        Q.f[QF_SREC][pos] = __uint_as_float((uint32_t)s | ((uint32_t)next << 16) | (ctype << 24));
+                        Q.f[QF_CVAL][pos] = cval;
+                        Q.f[QF_QC][pos] = qc;
                         ++kd;
                     }
                     tr *= __expf(-fmaxf(density * L.sigma_t[0], 1e-6f) * g.step);
@@ -547,7 +547,7 @@ template <int FLAGS>
 #endif
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render(
     const RenderParams* __restrict__ Pp) {
-    __shared__ WaveQueue Qs;
+    __shared__ WaveQueue<(FLAGS & RF_TRAIN) != 0> Qs;
     int tx = blockIdx.x, ty = blockIdx.y;
     xcd_grouped_tile(tx, ty);
     render_tile<FLAGS, 8>(*Pp, tx, ty, Qs);
@@ -563,7 +563,7 @@ template <int TW, bool PERSISTENT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
                                                      const uint32_t* __restrict__ tiles, int n_tiles,
                                                      int grab, unsigned* __restrict__ next) {
-    __shared__ WaveQueue Qs;
+    __shared__ WaveQueue<false> Qs;
     if (PERSISTENT) {
         // entries [t, end) are this worker's; one counter increment takes
         // `grab` of them (every worker contends for the one counter, so a
