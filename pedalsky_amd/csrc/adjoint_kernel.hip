@@ -39,8 +39,12 @@
 
 namespace {
 
-constexpr int kRecFields = 6;
-enum { REC_V0 = 0, REC_V1, REC_V2, REC_C, REC_GRHO, REC_Q };
+// Per chunk slot and lane, two 16-byte halves: the shading's results (the
+// sample's luminance times its cotangent, dL/drho through the shading),
+// written when the queue is shaded, and the sample (clamped shape, d density
+// / d coverage factor), written when it is queued.
+constexpr int kRecFields = 8;
+enum { REC_V0 = 0, REC_V1, REC_V2, REC_GRHO, REC_C, REC_Q };
 
 // (1 - (1 + x) e^-x) / x^2, the x-derivative factor of (1 - e^-x) / x:
 // d/dx[(1 - e^-x)/x] = -psi(x). Series below x = 1/2 (the direct form
@@ -392,9 +396,7 @@ __device__ __forceinline__ LightScatter shade_adj(const RenderParams& P, BackQue
     }
     gds += g_od * dod_ds;
     gcov += g_od * dod_cov;
-    float* rp = rec_tile + (size_t)slot * (kRecFields * 64) + owner;
-    rp[REC_V0 * 64] = v[0]; rp[REC_V1 * 64] = v[1]; rp[REC_V2 * 64] = v[2];
-    rp[REC_GRHO * 64] = g_rho;
+    *(float4*)(rec_tile + ((size_t)slot * 64 + owner) * kRecFields) = make_float4(v[0], v[1], v[2], g_rho);
     atomicAdd(&Q.wacc[li], gds);
     atomicAdd(&Q.wacc[PS_MAX_LAYERS + li], gk);
     atomicAdd(&Q.wacc[2 * PS_MAX_LAYERS + li], gcov);
@@ -552,7 +554,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
     // the gradients with respect to an additive perturbation
     const bool want_delta = P.g_delta != nullptr;
     float* rec_tile = P.rec + tile * (size_t)(ckl * kRecFields * 64);
-    float* rec = rec_tile + lane;
+    float* rec = rec_tile + lane * kRecFields;
     const float* lrec_tile = P.lrec + tile * (size_t)P.lrec_cap * (LR_FIELDS * 64);
     int qn = 0;  // queued samples (wave-uniform)
     int kd_next = n_dense;  // dense samples before the chunk after this one
@@ -611,9 +613,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 seg_sample(P, g, dx, dy, dz, tl, px, py, pz, alt);
                 density = cval * L.density_scale;
                 dense = true;
-                float* rp = rec + (size_t)slot * (kRecFields * 64);
-                rp[REC_C * 64] = cval;
-                rp[REC_Q * 64] = r.z;
+                *(float2*)(rec + (size_t)slot * (kRecFields * 64) + REC_C) = make_float2(cval, r.z);
                 dmask |= 1u << slot;
             } else if (more) {
                 const float tl = (float)s * g.step;
@@ -679,9 +679,7 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 }
                 dense = density > 1e-3f;
                 if (dense) {
-                    float* rp = rec + (size_t)k * (kRecFields * 64);
-                    rp[REC_C * 64] = cval;
-                    rp[REC_Q * 64] = q;
+                    *(float2*)(rec + (size_t)k * (kRecFields * 64) + REC_C) = make_float2(cval, q);
                     dmask |= 1u << k;
                 }
             }
@@ -731,9 +729,11 @@ __device__ __forceinline__ void backward_tile(const RenderParams& P, int tile_x,
                 bool scat = false;
                 if (mine) {
                     const float* rp = rec + (size_t)slot * (kRecFields * 64);
-                    const float v0 = rp[REC_V0 * 64], v1 = rp[REC_V1 * 64], v2 = rp[REC_V2 * 64];
-                    cval = rp[REC_C * 64];
-                    const float g_rho_s = rp[REC_GRHO * 64], q = rp[REC_Q * 64];
+                    const float4 ra = *(const float4*)rp;
+                    const float2 rb = *(const float2*)(rp + REC_C);
+                    const float v0 = ra.x, v1 = ra.y, v2 = ra.z;
+                    cval = rb.x;
+                    const float g_rho_s = ra.w, q = rb.y;
                     const float rho = cval * L.density_scale;
                     g_rho = g_rho_s;
                     for (int c = 0; c < 3; ++c) {

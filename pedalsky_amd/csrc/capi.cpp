@@ -1190,7 +1190,7 @@ ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_lumina
         return fail(PS_ERR_UNSUPPORTED, "ps_backward: perturbation-volume gradients support at most 32 light steps");
     const size_t tiles = (size_t)((P.W + 7) / 8) * ((P.H + 7) / 8);
     ps_status s;
-    if ((s = grow(&c->rec, &c->rec_cap, tiles * (size_t)P.ck_len * 6 * 64, "backward records"))) return s;
+    if ((s = grow(&c->rec, &c->rec_cap, tiles * (size_t)P.ck_len * 8 * 64, "backward records"))) return s;
     if (!grads->layers) {
         if ((s = dev_alloc(&c->d_counters, 16 * sizeof(unsigned long long), "counters"))) return s;
     }

@@ -193,7 +193,7 @@ struct RenderParams {
     float* g_layers;                  // 3 x 8 accumulated: density_scale, extinction_scale, coverage
     float* g_delta;                   // 128^3 accumulated, or null
     double* g_delta64;                // FP64 accumulator the kernel scatters into (added to g_delta after)
-    float* rec;                       // backward records [tile][slot][REC_FIELDS][lane]
+    float* rec;                       // backward records [tile][slot][lane][8]
 };
 
 // The cloud TAA resolve of one RT (shaders/clouds/cloud_taa.wgsl), passed by
