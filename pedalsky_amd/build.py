@@ -36,8 +36,13 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 # approximations (<= 2.5 ulp) one or two. The adjoint recomputes the march
 # and must round exactly as the forward does, so it shares the flag. The LUT
 # kernels are FP64 and the noise bakes keep the WGSL's IEEE FP32 semantics.
-PER_FILE = {"render_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
-            "adjoint_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+#
+# The march kernels are also compiled without machine-level loop-invariant
+# code motion: it hoists constant materialisations out of the step loops and
+# keeps them live across the whole tile, which costs ~24 VGPRs (and so a wave
+# per SIMD, or scratch). Arithmetic is unchanged by it.
+PER_FILE = {"render_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-mllvm", "-disable-machine-licm"],
+            "adjoint_kernel.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-mllvm", "-disable-machine-licm"]}
 
 
 def _compile(src: str) -> Path:

@@ -538,14 +538,21 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
     }
 }
 
-// Four waves per SIMD: a 128-VGPR budget. The plain render spills a few
-// values to scratch under it and still runs 3.5 % faster at 1280x720 than at
-// three waves with no spills (more waves hide the gather latency).
+// Waves per SIMD: more waves hide more of the gather latency. This file is
+// compiled without machine-level loop-invariant code motion, which would
+// otherwise hoist constant materialisations out of the march loops and keep
+// them in registers for the whole tile (120 VGPRs instead of 96 for the
+// plain render). The plain render (and the RT march, and the batched tile
+// kernel) then fits five waves per SIMD (a 96-VGPR budget) with no scratch;
+// the perturbation-volume, training and counting variants carry more state
+// and keep four (128 VGPRs), where they need no scratch either.
 template <int FLAGS>
-#ifndef PS_RENDER_WAVES
-#define PS_RENDER_WAVES 4
-#endif
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render(
+constexpr int render_waves() {
+    return (FLAGS & ~RF_RT) == 0 ? 5 : 4;
+}
+
+template <int FLAGS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(render_waves<FLAGS>(), 8))) void k_render(
     const RenderParams* __restrict__ Pp) {
     __shared__ WaveQueue<(FLAGS & RF_TRAIN) != 0> Qs;
     int tx = blockIdx.x, ty = blockIdx.y;
@@ -557,10 +564,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WA
 // scene << 20 | tile_y << 10 | tile_x. Persistent: every one-wave block
 // pulls the next entry from the counter until the list is exhausted (the
 // exit test is wave-uniform and the counter only grows, so every wave
-// leaves). Otherwise block b renders entry b. Four waves per SIMD, as
-// k_render (8 % faster on the 64-scene batch than three).
+// leaves). Otherwise block b renders entry b. Five waves per SIMD, as the
+// plain k_render.
 template <int TW, bool PERSISTENT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PS_RENDER_WAVES, 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(render_waves<0>(), 8))) void k_render_tiles(const RenderParams* __restrict__ Ps,
                                                      const uint32_t* __restrict__ tiles, int n_tiles,
                                                      int grab, unsigned* __restrict__ next) {
     __shared__ WaveQueue<false> Qs;
