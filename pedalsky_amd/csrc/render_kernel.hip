@@ -543,12 +543,14 @@ __device__ __forceinline__ void render_tile(const RenderParams& P, int tile_x, i
 // otherwise hoist constant materialisations out of the march loops and keep
 // them in registers for the whole tile (120 VGPRs instead of 96 for the
 // plain render). The plain render (and the RT march, and the batched tile
-// kernel) then fits five waves per SIMD (a 96-VGPR budget) with no scratch;
-// the perturbation-volume, training and counting variants carry more state
-// and keep four (128 VGPRs), where they need no scratch either.
+// kernel) then fits five waves per SIMD (a 96-VGPR budget) with no scratch.
+// The training forward with the perturbation volume (the differentiation
+// workload's) also runs five, spilling ~30 VGPRs, 1.5 % faster than four
+// without spills; the other perturbation-volume, training and counting
+// variants keep four (128 VGPRs), where they need no scratch.
 template <int FLAGS>
 constexpr int render_waves() {
-    return (FLAGS & ~RF_RT) == 0 ? 5 : 4;
+    return (FLAGS & ~RF_RT) == 0 || FLAGS == (RF_TRAIN | RF_DELTA) ? 5 : 4;
 }
 
 template <int FLAGS>
