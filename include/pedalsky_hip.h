@@ -243,6 +243,12 @@ ps_status ps_set_frame(ps_ctx* ctx, const ps_frame_uniforms* frame);
 ps_status ps_set_clouds(ps_ctx* ctx, const ps_cloud_params* params,
                         const ps_cloud_layer* layers, int32_t n_layers,
                         const float* layer_wind_xz);
+/* CloudsSubsystem::set_march_steps (crates/ps-clouds/src/lib.rs:701-712):
+ * the primary and light march step counts of the cloud parameters set by
+ * ps_set_clouds, clamped to [8, 256] and [1, 16] as the reference clamps
+ * them; they apply from the next render on (PS_ERR_NOT_READY before
+ * ps_set_clouds). */
+ps_status ps_set_march_steps(ps_ctx* ctx, uint32_t cloud_steps, uint32_t light_steps);
 
 /* ------------------------------------------------------------------------ */
 /* Sun position and top-of-atmosphere illuminance (host code). Replaces
@@ -574,6 +580,38 @@ ps_status ps_batch_time(ps_batch* batch, int32_t reps, float* ms_per_run);
 void ps_batch_destroy(ps_batch* batch);
 /* One-shot: create, run with `tuning` (NULL = default), wait, destroy. */
 ps_status ps_render_batch(ps_ctx* ctx, const ps_scene_desc* scenes, int32_t n, const ps_tuning* tuning);
+
+/* Step schedule of a batch: primary and light march steps per resolution
+ * class (the batch's scenes grouped by viewport size, classes ordered by
+ * pixel count, largest first). Each class's counts are what
+ * CloudsSubsystem::set_march_steps (crates/ps-clouds/src/lib.rs:701-712)
+ * would set for every scene of that size, clamped as it clamps them; 0 keeps
+ * each scene's own counts. The reference has no batch: its host loop sets
+ * one view's counts per frame to trade march detail for frame time. */
+#define PS_MAX_STEP_CLASSES 8
+typedef struct ps_step_schedule {
+    int32_t n_classes;                           /* distinct viewport sizes (<= 8) */
+    int32_t width[PS_MAX_STEP_CLASSES];
+    int32_t height[PS_MAX_STEP_CLASSES];
+    int32_t cloud_steps[PS_MAX_STEP_CLASSES];    /* [8, 256], or 0: the scenes' own */
+    int32_t light_steps[PS_MAX_STEP_CLASSES];    /* [1, 16], or 0: the scenes' own */
+    float batch_ms;                              /* autotune: device time of one run with it */
+} ps_step_schedule;
+ps_status ps_batch_get_step_schedule(const ps_batch* batch, ps_step_schedule* schedule);
+/* Apply a schedule whose classes are the batch's (as get returns them);
+ * counts are clamped. A captured graph stays valid (the counts travel with
+ * the staged scene parameters). */
+ps_status ps_batch_set_step_schedule(ps_batch* batch, const ps_step_schedule* schedule);
+/* Choose the schedule for a device-time budget per batch run, as the
+ * reference's adaptive-quality loop does for one view: starting from the
+ * scenes' own counts (clamped), while a run (timed `reps` times with the
+ * current launch shape) exceeds `budget_ms`, lower the primary steps of the
+ * class with the most march work (pixels x steps over its scenes) by one
+ * rung of 256 224 192 160 128 112 96 80 64 56 48 40 32 24 16 12 8; when every
+ * class is at 8, lower the light steps likewise (16 12 10 8 6 5 4 3 2 1).
+ * The chosen schedule stays applied and is written to `best` (with its
+ * time); a budget below the all-minimum time ends there. */
+ps_status ps_batch_autotune_steps(ps_batch* batch, float budget_ms, int32_t reps, ps_step_schedule* best);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiation. No reference counterpart: the reference only renders

@@ -190,6 +190,23 @@ class SceneDesc(C.Structure):
     ]
 
 
+PS_MAX_STEP_CLASSES = 8
+
+
+class StepSchedule(C.Structure):
+    """ps_step_schedule: primary / light march steps per resolution class
+    of a batch (0 = each scene's own)."""
+    _fields_ = [("n_classes", C.c_int32), ("width", C.c_int32 * PS_MAX_STEP_CLASSES),
+                ("height", C.c_int32 * PS_MAX_STEP_CLASSES), ("cloud_steps", C.c_int32 * PS_MAX_STEP_CLASSES),
+                ("light_steps", C.c_int32 * PS_MAX_STEP_CLASSES), ("batch_ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        n = int(self.n_classes)
+        return {"classes": [{"size": f"{self.width[k]}x{self.height[k]}", "cloud_steps": int(self.cloud_steps[k]),
+                             "light_steps": int(self.light_steps[k])} for k in range(n)],
+                "batch_ms": round(float(self.batch_ms), 3)}
+
+
 class Tuning(C.Structure):
     """ps_tuning: launch shape of a batch."""
     _fields_ = [("tile_w", C.c_int32), ("persistent", C.c_int32), ("workers_per_cu", C.c_int32),
@@ -279,6 +296,7 @@ _SIGNATURES = [
     ("ps_ctx_get_lut_dims", C.c_int32, [_P, C.POINTER(LutDims)]),
     ("ps_set_atmosphere", C.c_int32, [_P, C.POINTER(AtmosphereParams)]),
     ("ps_set_frame", C.c_int32, [_P, C.POINTER(FrameUniforms)]),
+    ("ps_set_march_steps", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("ps_set_clouds", C.c_int32, [_P, C.POINTER(CloudParams), C.POINTER(CloudLayer), C.c_int32,
                                   C.POINTER(C.c_float)]),
     ("ps_generate_noise", C.c_int32, [_P, C.c_uint32, C.c_int32]),
@@ -310,6 +328,9 @@ _SIGNATURES = [
     ("ps_batch_time", C.c_int32, [_P, C.c_int32, C.POINTER(C.c_float)]),
     ("ps_batch_destroy", None, [_P]),
     ("ps_render_batch", C.c_int32, [_P, C.POINTER(SceneDesc), C.c_int32, C.POINTER(Tuning)]),
+    ("ps_batch_get_step_schedule", C.c_int32, [_P, C.POINTER(StepSchedule)]),
+    ("ps_batch_set_step_schedule", C.c_int32, [_P, C.POINTER(StepSchedule)]),
+    ("ps_batch_autotune_steps", C.c_int32, [_P, C.c_float, C.c_int32, C.POINTER(StepSchedule)]),
     ("ps_set_cloud_extensions", C.c_int32, [_P, C.POINTER(C.c_float), C.c_int32, _P]),
     ("ps_set_checkpoint_interval", C.c_int32, [_P, C.c_int32]),
     ("ps_set_half_res", C.c_int32, [_P, C.c_int32]),

@@ -79,6 +79,21 @@ class Batch:
         check(self.lib.ps_batch_autotune(self.h, reps, C.byref(t)))
         return t
 
+    def step_schedule(self) -> abi.StepSchedule:
+        t = abi.StepSchedule()
+        check(self.lib.ps_batch_get_step_schedule(self.h, C.byref(t)))
+        return t
+
+    def set_step_schedule(self, t: abi.StepSchedule) -> None:
+        check(self.lib.ps_batch_set_step_schedule(self.h, C.byref(t)))
+
+    def autotune_steps(self, budget_ms: float, reps: int = 2) -> abi.StepSchedule:
+        """Step counts per resolution class for a device-time budget per
+        run (ps_batch_autotune_steps); the schedule stays applied."""
+        t = abi.StepSchedule()
+        check(self.lib.ps_batch_autotune_steps(self.h, float(budget_ms), reps, C.byref(t)))
+        return t
+
     def run(self) -> None:
         check(self.lib.ps_batch_run(self.h))
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -36,6 +37,12 @@ struct ps_batch {
     int tiles_w = 0, tiles_order = -1;
     unsigned* d_next = nullptr;
     ps_tuning tune{8, 0, 8, 1, 1};
+    // step schedule: resolution class of each scene, the classes' sizes and
+    // their primary / light step counts (0: each scene's own)
+    std::vector<int> cls;
+    int n_sizes = 0;
+    int cls_w[PS_MAX_STEP_CLASSES]{}, cls_h[PS_MAX_STEP_CLASSES]{};
+    int cls_cloud[PS_MAX_STEP_CLASSES]{}, cls_light[PS_MAX_STEP_CLASSES]{};
     int n_cu = 256;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -121,18 +128,55 @@ ps_status wait_idle(ps_batch* b) {
     return PS_OK;
 }
 
+// Resolution classes: distinct viewport sizes, most pixels first (ties:
+// wider first). More than PS_MAX_STEP_CLASSES sizes leave the schedule off.
+void build_classes(ps_batch* b) {
+    std::vector<std::pair<int, int>> sizes;
+    for (int i = 0; i < b->n; ++i) {
+        const std::pair<int, int> wh{b->W[i], b->H[i]};
+        if (std::find(sizes.begin(), sizes.end(), wh) == sizes.end()) sizes.push_back(wh);
+    }
+    std::stable_sort(sizes.begin(), sizes.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& c) {
+        const long pa = (long)a.first * a.second, pc = (long)c.first * c.second;
+        return pa != pc ? pa > pc : a.first > c.first;
+    });
+    b->n_sizes = (int)sizes.size();
+    b->cls.assign(b->n, -1);
+    for (int k = 0; k < PS_MAX_STEP_CLASSES; ++k) b->cls_w[k] = b->cls_h[k] = b->cls_cloud[k] = b->cls_light[k] = 0;
+    if (b->n_sizes > PS_MAX_STEP_CLASSES) return;
+    for (int k = 0; k < b->n_sizes; ++k) {
+        b->cls_w[k] = sizes[k].first;
+        b->cls_h[k] = sizes[k].second;
+    }
+    for (int i = 0; i < b->n; ++i)
+        for (int k = 0; k < b->n_sizes; ++k)
+            if (b->W[i] == b->cls_w[k] && b->H[i] == b->cls_h[k]) b->cls[i] = k;
+}
+
+// A scene's march step counts under the schedule (its own where its class
+// has none).
+void scene_steps(const ps_batch* b, int i, uint32_t& cloud, uint32_t& light) {
+    cloud = b->sc[i].clouds.cloud_steps;
+    light = b->sc[i].clouds.light_steps;
+    const int k = b->cls[i];
+    if (k >= 0 && b->cls_cloud[k] > 0) cloud = (uint32_t)b->cls_cloud[k];
+    if (k >= 0 && b->cls_light[k] > 0) light = (uint32_t)b->cls_light[k];
+}
+
 // Fold every scene into the pinned staging blocks the launch sequence copies.
 ps_status stage_params(ps_batch* b) {
     const ps_lut_dims& d = b->c->dims;
     for (int i = 0; i < b->n; ++i) {
-        fill_lut_params(b->sc[i], d, b->h_lp[i]);
+        SceneIn si = b->sc[i];
+        scene_steps(b, i, si.clouds.cloud_steps, si.clouds.light_steps);
+        fill_lut_params(si, d, b->h_lp[i]);
         LutSet L;
         L.T = b->T + i * lut_texels(d, 0);
         L.MS = b->MS + i * lut_texels(d, 1);
         L.SV = b->SV + i * lut_texels(d, 2);
         L.AP = b->AP + i * lut_texels(d, 3);
         ps_status s;
-        if ((s = fill_render_params(b->c, b->sc[i], L, b->depth[i], &b->out[i], b->h_rp[i]))) return s;
+        if ((s = fill_render_params(b->c, si, L, b->depth[i], &b->out[i], b->h_rp[i]))) return s;
         if (b->own_overcast[i]) b->h_rp[i].overcast = b->d_overcast + (size_t)i * 128 * 128;
     }
     // per-scene masks: the batch is idle here (create, or update after wait_idle)
@@ -254,6 +298,7 @@ ps_status ps_batch_create(ps_ctx* c, const ps_scene_desc* scenes, int32_t n, ps_
         if (e == hipSuccess) b->n_cu = std::max(prop.multiProcessorCount, 1);
     }
     if (e != hipSuccess) return bail(fail(PS_ERR_DEVICE, std::string("ps_batch_create: ") + hipGetErrorString(e)));
+    build_classes(b);
     if ((s = stage_params(b))) return bail(s);
     if ((s = build_tiles(b))) return bail(s);
     *out = b;
@@ -343,13 +388,13 @@ ps_status ps_batch_autotune(ps_batch* b, int32_t reps, ps_tuning* best_out) {
     if (!b) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_autotune: batch is NULL");
     if (reps < 1 || reps > 100) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_autotune: reps must be in [1, 100]");
     // hardware dispatch at each tile shape and order; then persistent workers,
-    // 16 per CU (the four waves per SIMD hardware dispatch holds) and 12,
-    // with schedule steps of 1, 2 and 4 list entries per counter increment
-    // (longer steps leave uneven tails: 64 entries cost 44 % on the 64-scene
+    // 20 per CU (the five waves per SIMD the tile kernel holds) and 16, with
+    // schedule steps of 1, 2 and 4 list entries per counter increment (longer
+    // steps leave uneven tails: 64 entries cost 44 % on the 64-scene
     // benchmark batch)
     const ps_tuning cands[] = {
-        {8, 0, 8, 1, 1},    {8, 0, 8, 0, 1},    {16, 0, 8, 1, 1},  {4, 0, 8, 1, 1},   {16, 1, 16, 1, 1},
-        {16, 1, 16, 1, 2},  {16, 1, 16, 1, 4},  {8, 1, 16, 1, 1},  {8, 1, 16, 1, 2},  {16, 1, 12, 1, 1},
+        {8, 0, 8, 1, 1},    {8, 0, 8, 0, 1},    {16, 0, 8, 1, 1},  {4, 0, 8, 1, 1},   {16, 1, 20, 1, 1},
+        {16, 1, 20, 1, 2},  {16, 1, 20, 1, 4},  {8, 1, 20, 1, 1},  {8, 1, 20, 1, 2},  {16, 1, 16, 1, 2},
     };
     const bool had_graph = b->exec != nullptr;
     ps_tuning best = b->tune;
@@ -368,6 +413,94 @@ ps_status ps_batch_autotune(ps_batch* b, int32_t reps, ps_tuning* best_out) {
     if ((s = ps_batch_set_tuning(b, &best))) return s;
     if (had_graph && (s = ps_batch_capture(b))) return s;
     if (best_out) *best_out = best;
+    return PS_OK;
+}
+
+ps_status ps_batch_get_step_schedule(const ps_batch* b, ps_step_schedule* out) {
+    if (!b || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_get_step_schedule: NULL argument");
+    if (b->n_sizes > PS_MAX_STEP_CLASSES)
+        return fail(PS_ERR_UNSUPPORTED, "ps_batch_get_step_schedule: the batch has more than 8 viewport sizes");
+    std::memset(out, 0, sizeof(*out));
+    out->n_classes = b->n_sizes;
+    for (int k = 0; k < b->n_sizes; ++k) {
+        out->width[k] = b->cls_w[k];
+        out->height[k] = b->cls_h[k];
+        out->cloud_steps[k] = b->cls_cloud[k];
+        out->light_steps[k] = b->cls_light[k];
+    }
+    return PS_OK;
+}
+
+ps_status ps_batch_set_step_schedule(ps_batch* b, const ps_step_schedule* t) {
+    if (!b || !t) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_set_step_schedule: NULL argument");
+    if (b->n_sizes > PS_MAX_STEP_CLASSES)
+        return fail(PS_ERR_UNSUPPORTED, "ps_batch_set_step_schedule: the batch has more than 8 viewport sizes");
+    if (t->n_classes != b->n_sizes)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_set_step_schedule: class count differs from the batch's");
+    for (int k = 0; k < b->n_sizes; ++k) {
+        if (t->width[k] != b->cls_w[k] || t->height[k] != b->cls_h[k])
+            return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_set_step_schedule: class " + std::to_string(k) +
+                                                     " is not the batch's (sizes in ps_batch_get_step_schedule order)");
+        if (t->cloud_steps[k] < 0 || t->light_steps[k] < 0)
+            return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_set_step_schedule: negative step count");
+    }
+    ps_status s;
+    if ((s = wait_idle(b))) return s;
+    for (int k = 0; k < b->n_sizes; ++k) {
+        // set_march_steps' clamps (lib.rs:709-712); 0 keeps the scenes' own
+        b->cls_cloud[k] = t->cloud_steps[k] > 0 ? std::clamp(t->cloud_steps[k], 8, 256) : 0;
+        b->cls_light[k] = t->light_steps[k] > 0 ? std::clamp(t->light_steps[k], 1, 16) : 0;
+    }
+    return stage_params(b);
+}
+
+ps_status ps_batch_autotune_steps(ps_batch* b, float budget_ms, int32_t reps, ps_step_schedule* best) {
+    if (!b) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_autotune_steps: batch is NULL");
+    if (!(budget_ms > 0.f)) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_autotune_steps: budget_ms must be > 0");
+    if (reps < 1 || reps > 100) return fail(PS_ERR_INVALID_ARGUMENT, "ps_batch_autotune_steps: reps must be in [1, 100]");
+    if (b->n_sizes > PS_MAX_STEP_CLASSES)
+        return fail(PS_ERR_UNSUPPORTED, "ps_batch_autotune_steps: the batch has more than 8 viewport sizes");
+    static const int kCloudRungs[] = {256, 224, 192, 160, 128, 112, 96, 80, 64, 56, 48, 40, 32, 24, 16, 12, 8};
+    static const int kLightRungs[] = {16, 12, 10, 8, 6, 5, 4, 3, 2, 1};
+    auto next_rung = [](const int* r, int n, int v) {
+        for (int j = 0; j < n; ++j)
+            if (r[j] < v) return r[j];
+        return v;
+    };
+    ps_step_schedule t{};
+    ps_status s;
+    if ((s = ps_batch_get_step_schedule(b, &t))) return s;
+    // start: every class at the largest of its scenes' own counts, clamped
+    std::vector<double> px(b->n_sizes, 0.0);
+    for (int k = 0; k < b->n_sizes; ++k) t.cloud_steps[k] = t.light_steps[k] = 0;
+    for (int i = 0; i < b->n; ++i) {
+        const int k = b->cls[i];
+        px[k] += (double)b->W[i] * b->H[i];
+        t.cloud_steps[k] = std::max(t.cloud_steps[k], (int)std::clamp(b->sc[i].clouds.cloud_steps, 8u, 256u));
+        t.light_steps[k] = std::max(t.light_steps[k], (int)std::clamp(b->sc[i].clouds.light_steps, 1u, 16u));
+    }
+    float ms = 0.f;
+    for (;;) {
+        if ((s = ps_batch_set_step_schedule(b, &t))) return s;
+        if ((s = ps_batch_time(b, 1, &ms))) return s;  // warm-up
+        if ((s = ps_batch_time(b, reps, &ms))) return s;
+        if (ms <= budget_ms) break;
+        // the class with the most march work whose primary steps can drop
+        int kc = -1;
+        for (int k = 0; k < b->n_sizes; ++k)
+            if (t.cloud_steps[k] > 8 && (kc < 0 || px[k] * t.cloud_steps[k] > px[kc] * t.cloud_steps[kc])) kc = k;
+        if (kc >= 0) {
+            t.cloud_steps[kc] = next_rung(kCloudRungs, 17, t.cloud_steps[kc]);
+            continue;
+        }
+        int kl = -1;
+        for (int k = 0; k < b->n_sizes; ++k)
+            if (t.light_steps[k] > 1 && (kl < 0 || px[k] * t.light_steps[k] > px[kl] * t.light_steps[kl])) kl = k;
+        if (kl < 0) break;  // everything at the minimum
+        t.light_steps[kl] = next_rung(kLightRungs, 10, t.light_steps[kl]);
+    }
+    t.batch_ms = ms;
+    if (best) *best = t;
     return PS_OK;
 }
 

@@ -597,6 +597,14 @@ ps_status ps_set_clouds(ps_ctx* c, const ps_cloud_params* p, const ps_cloud_laye
     return PS_OK;
 }
 
+ps_status ps_set_march_steps(ps_ctx* c, uint32_t cloud_steps, uint32_t light_steps) {
+    if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_set_march_steps: ctx is NULL");
+    if (!c->have_clouds) return fail(PS_ERR_NOT_READY, "ps_set_march_steps: call ps_set_clouds first");
+    c->sc.clouds.cloud_steps = std::clamp(cloud_steps, 8u, 256u);
+    c->sc.clouds.light_steps = std::clamp(light_steps, 1u, 16u);
+    return PS_OK;
+}
+
 static ps_status alloc_noise(ps_ctx* c, int format) {
     if (c->noise_format != format) {
         PS_HIP(hipStreamSynchronize(c->stream));

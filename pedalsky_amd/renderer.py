@@ -77,6 +77,10 @@ class Renderer:
             w = (C.c_float * len(wa))(*wa.tolist())
         check(self.lib.ps_set_clouds(self.ctx, C.byref(params), arr, n, w))
 
+    def set_march_steps(self, cloud_steps: int, light_steps: int) -> None:
+        """CloudsSubsystem::set_march_steps: clamped to [8, 256] / [1, 16]."""
+        check(self.lib.ps_set_march_steps(self.ctx, int(cloud_steps), int(light_steps)))
+
     # inputs -------------------------------------------------------------------
     def generate_noise(self, seed: int = 0, fmt: int = abi.PS_NOISE_FP16) -> None:
         check(self.lib.ps_generate_noise(self.ctx, seed, fmt))

@@ -26,6 +26,8 @@ def test_struct_layouts_match_reference():
     # batch descriptors (static_assert'ed in batch.cpp)
     assert C.sizeof(abi.SceneDesc) == 96 and abi.SceneDesc.out.offset == 64
     assert C.sizeof(abi.Tuning) == 20
+    # batch step schedule: count, four arrays of 8 int32, the measured time
+    assert C.sizeof(abi.StepSchedule) == 136 and abi.StepSchedule.batch_ms.offset == 132
     # differentiation outputs: two device pointers
     assert C.sizeof(abi.CloudGrads) == 16 and abi.CloudGrads.density_delta.offset == 8
 
@@ -48,6 +50,12 @@ def test_errors_without_device(lib, gpu_available):
     assert st != abi.PS_OK
     assert lib.ps_last_error()  # message present
     assert lib.ps_ctx_create(0, None) == abi.PS_ERR_INVALID_ARGUMENT
+    # the step knobs refuse a missing context or batch before touching a device
+    assert lib.ps_set_march_steps(None, 64, 6) == abi.PS_ERR_INVALID_ARGUMENT
+    t = abi.StepSchedule()
+    assert lib.ps_batch_get_step_schedule(None, C.byref(t)) == abi.PS_ERR_INVALID_ARGUMENT
+    assert lib.ps_batch_set_step_schedule(None, C.byref(t)) == abi.PS_ERR_INVALID_ARGUMENT
+    assert lib.ps_batch_autotune_steps(None, C.c_float(10.0), 1, C.byref(t)) == abi.PS_ERR_INVALID_ARGUMENT
 
 
 def test_default_params_match_reference(lib):
