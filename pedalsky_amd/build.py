@@ -79,6 +79,27 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+HOST_TEST_SRC = PKG.parent / "tests" / "host" / "subsystem_frame.cpp"
+HOST_TEST_BIN = PKG.parent / "tests" / "host" / "_build" / "subsystem_frame"
+
+
+def build_host_test(verbose: bool = False) -> Path:
+    """Compile the driver of the C++ host layer (include/pedalsky_subsystems.hpp)
+    that tests/test_host_subsystems.py runs, linked against the library."""
+    HOST_TEST_BIN.parent.mkdir(parents=True, exist_ok=True)
+    tmp = HOST_TEST_BIN.with_suffix(".tmp")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}", "-Wall", str(HOST_TEST_SRC),
+           "-o", str(tmp), f"-L{LIB_DIR}", "-lpedalsky_hip", f"-Wl,-rpath,{LIB_DIR}",
+           "-Wl,-rpath,$ORIGIN/../../../pedalsky_amd/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, HOST_TEST_BIN)
+    if verbose:
+        print(f"built {HOST_TEST_BIN}")
+    return HOST_TEST_BIN
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     oracle_dir = PKG.parent / "oracle"
