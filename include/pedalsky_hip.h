@@ -469,7 +469,13 @@ ps_status ps_render(ps_ctx* ctx, const float* depth, const ps_image_out* out);
  * GPU (several contexts or streams at once) race; calls on one stream are
  * ordered. */
 ps_status ps_render_noalloc(ps_ctx* ctx, const float* depth, const ps_image_out* out);
-/* One whole frame: ps_bake_luts(flags) then ps_render. */
+/* One whole frame: ps_bake_luts(flags) then ps_render. When `flags` rebakes
+ * every LUT, the bake goes into a second LUT set on a side stream while the
+ * previous frame still renders from the current one, and the sets swap
+ * (ps_read_lut and later renders see this frame's LUTs): consecutive frames
+ * overlap their LUT chain with the previous render. Results are identical
+ * to the unpipelined sequence; under graph capture of the context's stream
+ * the bake stays on that stream. */
 ps_status ps_frame(ps_ctx* ctx, uint32_t lut_flags, const float* depth,
                    const ps_image_out* out);
 /* Run `frames` frames back to back on the context stream and report average
@@ -646,6 +652,10 @@ ps_status ps_set_checkpoint_interval(ps_ctx* ctx, int32_t steps);
  * per-pixel forward totals. The backward recomputes each chunk from its
  * checkpoint. */
 ps_status ps_render_train(ps_ctx* ctx, const float* depth, const ps_image_out* out);
+/* ps_bake_luts(lut_flags) then ps_render_train, with the LUT chain pipelined
+ * as ps_frame pipelines it (a step of an optimisation loop bakes its LUTs
+ * while the previous step's backward still runs). */
+ps_status ps_frame_train(ps_ctx* ctx, uint32_t lut_flags, const float* depth, const ps_image_out* out);
 /* Gradient outputs of ps_backward; both are ADDED to (the caller clears
  * them). */
 typedef struct ps_cloud_grads {

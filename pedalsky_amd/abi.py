@@ -338,6 +338,7 @@ _SIGNATURES = [
     ("ps_invalidate_taa_history", C.c_int32, [_P]),
     ("ps_set_active_view", C.c_int32, [_P, C.c_int32]),
     ("ps_render_train", C.c_int32, [_P, _P, C.POINTER(ImageOut)]),
+    ("ps_frame_train", C.c_int32, [_P, C.c_uint32, _P, C.POINTER(ImageOut)]),
     ("ps_backward", C.c_int32, [_P, _P, _P, _P, C.POINTER(CloudGrads)]),
     ("ps_sky_ap_image", C.c_int32, [_P, _P, C.c_float, _P]),
     ("ps_atmosphere_grad", C.c_int32, [_P, C.c_int32, _P, _P, C.c_float, _P, _P, _P, _P]),

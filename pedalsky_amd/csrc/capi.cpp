@@ -320,7 +320,7 @@ ps_status psi::check_render_ready(const ps_ctx* c) { return check_ready(c, true)
 
 namespace {
 
-ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
+ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev, hipStream_t st) {
     ps_status s;
     if ((s = check_ready(c, false))) return s;
     if ((s = ensure_luts(c))) return s;
@@ -332,9 +332,9 @@ ps_status launch_luts(ps_ctx* c, uint32_t flags, hipEvent_t* ev) {
         k |= 2u;
     }
     LutParams* dP = nullptr;
-    if ((s = c->lut_ring.upload(P, c->stream, &dP))) return s;
-    PS_HIP(ps_launch_luts(dP, P, 1, k, c->luts.T, c->luts.MS, c->luts.SV, c->luts.AP, c->stream, ev));
-    if ((s = c->lut_ring.mark_used(c->stream))) return s;
+    if ((s = c->lut_ring.upload(P, st, &dP))) return s;
+    PS_HIP(ps_launch_luts(dP, P, 1, k, c->luts.T, c->luts.MS, c->luts.SV, c->luts.AP, st, ev));
+    if ((s = c->lut_ring.mark_used(st))) return s;
     c->lut_baked |= (k & 15u);
     c->luts_valid = c->lut_baked == 15u;
     return PS_OK;
@@ -508,7 +508,12 @@ void ps_ctx_destroy(ps_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    if (c->lut_stream) hipStreamSynchronize(c->lut_stream);
     dev_free(c->luts.T); dev_free(c->luts.MS); dev_free(c->luts.SV); dev_free(c->luts.AP);
+    dev_free(c->luts_back.T); dev_free(c->luts_back.MS); dev_free(c->luts_back.SV); dev_free(c->luts_back.AP);
+    if (c->lut_stream) hipStreamDestroy(c->lut_stream);
+    if (c->ev_luts_done) hipEventDestroy(c->ev_luts_done);
+    if (c->ev_render_issued) hipEventDestroy(c->ev_render_issued);
     dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
     dev_free(c->base_rl); dev_free(c->bbound); dev_free(c->detail_hf); dev_free(c->curl2);
     dev_free(c->weather); dev_free(c->weather_synth); dev_free(c->type_grid); dev_free(c->overcast); dev_free(c->blue);
@@ -550,7 +555,10 @@ ps_status ps_ctx_set_lut_dims(ps_ctx* c, const ps_lut_dims* d) {
     if ((size_t)d->ap_w * d->ap_h * d->ap_d > (size_t)1 << 24)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_ctx_set_lut_dims: AP volume too large");
     PS_HIP(hipStreamSynchronize(c->stream));
+    if (c->lut_stream) PS_HIP(hipStreamSynchronize(c->lut_stream));
     dev_free(c->luts.T); dev_free(c->luts.MS); dev_free(c->luts.SV); dev_free(c->luts.AP);
+    psi::LutSet& b = c->luts_back;
+    dev_free(b.T); dev_free(b.MS); dev_free(b.SV); dev_free(b.AP);
     c->dims = *d;
     c->lut_baked = 0;
     c->luts_valid = false;
@@ -961,7 +969,7 @@ float ps_ev100_from_exposure_stats(float log_lum_sum, float pixel_count) {
 
 ps_status ps_bake_luts(ps_ctx* c, uint32_t flags) {
     if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_bake_luts: ctx is NULL");
-    return launch_luts(c, flags, nullptr);
+    return launch_luts(c, flags, nullptr, c->stream);
 }
 
 ps_status ps_read_lut(ps_ctx* c, int32_t which, float* host_out, size_t floats) {
@@ -988,15 +996,76 @@ ps_status ps_render(ps_ctx* c, const float* depth, const ps_image_out* out) {
     return launch_render(c, depth, out, false);
 }
 
+// Whether ps_frame may bake into the second LUT set on the side stream:
+// every LUT is rebaked (the second set then holds nothing stale), and the
+// main stream is not being captured into a graph (the side stream's work
+// would not be part of the capture).
+static bool can_pipeline(ps_ctx* c, uint32_t flags) {
+    if ((flags & 15u) != 15u) return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &st) != hipSuccess) return false;
+    return st == hipStreamCaptureStatusNone;
+}
+
+static ps_status ensure_pipeline(ps_ctx* c) {
+    if (!c->lut_stream) {
+        // default priority, like every other stream of the library
+        PS_HIP(hipStreamCreateWithFlags(&c->lut_stream, hipStreamNonBlocking));
+        PS_HIP(hipEventCreateWithFlags(&c->ev_luts_done, hipEventDisableTiming));
+        PS_HIP(hipEventCreateWithFlags(&c->ev_render_issued, hipEventDisableTiming));
+    }
+    const ps_lut_dims& d = c->dims;
+    ps_status s;
+    psi::LutSet& b = c->luts_back;
+    if ((s = dev_alloc(&b.T, sizeof(float4) * d.transmittance_w * d.transmittance_h, "transmittance"))) return s;
+    if ((s = dev_alloc(&b.MS, sizeof(float4) * d.multiscatter_w * d.multiscatter_h, "multiscatter"))) return s;
+    if ((s = dev_alloc(&b.SV, sizeof(float4) * d.skyview_w * d.skyview_h, "skyview"))) return s;
+    if ((s = dev_alloc(&b.AP, sizeof(float4) * d.ap_w * d.ap_h * d.ap_d, "aerial perspective"))) return s;
+    return PS_OK;
+}
+
+// Per frame of in-loop timing, seven events: the starts of the four LUT
+// bakes and their end (on the stream that bakes), then the render's start
+// and end on the main stream.
+static constexpr size_t kFrameEvents = 7;
+
+// The LUT chain of a frame (ps_frame, ps_frame_train): pipelined into the
+// second set when every LUT is rebaked, else baked on the main stream.
+static ps_status bake_frame_luts(ps_ctx* c, uint32_t lut_flags, hipEvent_t* ev) {
+    ps_status s;
+    if (can_pipeline(c, lut_flags)) {
+        // bake this frame's LUTs into the second set on the side stream
+        // while the previous frame renders from the current one, then swap.
+        // The set being overwritten was last read by work queued on the main
+        // stream before the previous pipelined render (or never).
+        if ((s = check_ready(c, false))) return s;
+        if ((s = ensure_pipeline(c))) return s;
+        std::swap(c->luts, c->luts_back);
+        if (c->render_issued_recorded) PS_HIP(hipStreamWaitEvent(c->lut_stream, c->ev_render_issued, 0));
+        if ((s = launch_luts(c, lut_flags, ev, c->lut_stream))) {
+            std::swap(c->luts, c->luts_back);  // the set the last frame used stays current
+            return s;
+        }
+        PS_HIP(hipEventRecord(c->ev_luts_done, c->lut_stream));
+        PS_HIP(hipStreamWaitEvent(c->stream, c->ev_luts_done, 0));
+        PS_HIP(hipEventRecord(c->ev_render_issued, c->stream));
+        c->render_issued_recorded = true;
+    } else if ((s = launch_luts(c, lut_flags, ev, c->stream))) {
+        return s;
+    }
+    return PS_OK;
+}
+
 ps_status ps_frame(ps_ctx* c, uint32_t lut_flags, const float* depth, const ps_image_out* out) {
     if (!c) return fail(PS_ERR_INVALID_ARGUMENT, "ps_frame: ctx is NULL");
     ps_status s;
     hipEvent_t* ev = nullptr;
-    if (c->timing && c->timing_n < c->timing_max) ev = &c->events[(size_t)c->timing_n * 6];
-    if ((s = launch_luts(c, lut_flags, ev))) return s;
+    if (c->timing && c->timing_n < c->timing_max) ev = &c->events[(size_t)c->timing_n * kFrameEvents];
+    if ((s = bake_frame_luts(c, lut_flags, ev))) return s;
+    if (ev) PS_HIP(hipEventRecord(ev[5], c->stream));
     if ((s = launch_render(c, depth, out, false))) return s;
     if (ev) {
-        PS_HIP(hipEventRecord(ev[5], c->stream));
+        PS_HIP(hipEventRecord(ev[6], c->stream));
         c->timing_n++;
     }
     return PS_OK;
@@ -1015,15 +1084,16 @@ static ps_status reduce_timing(ps_ctx* c, int frames, ps_frame_timing* timing) {
     PS_HIP(hipStreamSynchronize(c->stream));
     double acc[5] = {0, 0, 0, 0, 0};
     for (int f = 0; f < frames; ++f) {
-        hipEvent_t* ev = &c->events[(size_t)f * 6];
+        hipEvent_t* ev = &c->events[(size_t)f * kFrameEvents];
         for (int k = 0; k < 5; ++k) {
+            // LUT bakes ev[k] -> ev[k + 1]; the render ev[5] -> ev[6]
             float ms = 0.f;
-            PS_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            PS_HIP(hipEventElapsedTime(&ms, ev[k < 4 ? k : 5], ev[k < 4 ? k + 1 : 6]));
             acc[k] += ms;
         }
     }
     float total = 0.f;
-    PS_HIP(hipEventElapsedTime(&total, c->events[0], c->events[(size_t)(frames - 1) * 6 + 5]));
+    PS_HIP(hipEventElapsedTime(&total, c->events[0], c->events[(size_t)(frames - 1) * kFrameEvents + 6]));
     timing->transmittance_ms = (float)(acc[0] / frames);
     timing->multiscatter_ms = (float)(acc[1] / frames);
     timing->skyview_ms = (float)(acc[2] / frames);
@@ -1040,7 +1110,7 @@ ps_status ps_timing_begin(ps_ctx* c, int32_t max_frames) {
     if (max_frames < 1 || max_frames > 65536)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_timing_begin: max_frames must be in [1, 65536]");
     ps_status s;
-    if ((s = ensure_events(c, (size_t)max_frames * 6))) return s;
+    if ((s = ensure_events(c, (size_t)max_frames * kFrameEvents))) return s;
     c->timing = true;
     c->timing_max = max_frames;
     c->timing_n = 0;
@@ -1061,12 +1131,13 @@ ps_status ps_time_frames(ps_ctx* c, int32_t frames, uint32_t lut_flags, const fl
     if (frames < 1 || frames > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "ps_time_frames: frames must be in [1, 4096]");
     ps_status s;
     if (c->timing) return fail(PS_ERR_NOT_READY, "ps_time_frames: in-loop timing is active");
-    if ((s = ensure_events(c, (size_t)frames * 6))) return s;
+    if ((s = ensure_events(c, (size_t)frames * kFrameEvents))) return s;
     for (int f = 0; f < frames; ++f) {
-        hipEvent_t* ev = &c->events[(size_t)f * 6];
-        if ((s = launch_luts(c, lut_flags, ev))) return s;
-        if ((s = launch_render(c, depth, out, false))) return s;
+        hipEvent_t* ev = &c->events[(size_t)f * kFrameEvents];
+        if ((s = launch_luts(c, lut_flags, ev, c->stream))) return s;
         PS_HIP(hipEventRecord(ev[5], c->stream));
+        if ((s = launch_render(c, depth, out, false))) return s;
+        PS_HIP(hipEventRecord(ev[6], c->stream));
     }
     return reduce_timing(c, frames, timing);
 }
@@ -1183,6 +1254,15 @@ ps_status ps_render_train(ps_ctx* c, const float* depth, const ps_image_out* out
     c->train = P;
     c->train_valid = true;
     return PS_OK;
+}
+
+ps_status ps_frame_train(ps_ctx* c, uint32_t lut_flags, const float* depth, const ps_image_out* out) {
+    if (!c || !out) return fail(PS_ERR_INVALID_ARGUMENT, "ps_frame_train: NULL argument");
+    if (c->half_res) return fail(PS_ERR_UNSUPPORTED, "ps_frame_train: the training forward runs at full resolution");
+    if (c->taa) return fail(PS_ERR_UNSUPPORTED, "ps_frame_train: the training forward renders single frames (TAA off)");
+    ps_status s;
+    if ((s = bake_frame_luts(c, lut_flags, nullptr))) return s;
+    return ps_render_train(c, depth, out);
 }
 
 ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_luminance, const float* g_transmittance,

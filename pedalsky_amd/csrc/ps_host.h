@@ -121,6 +121,15 @@ struct ps_ctx {
     psi::LutSet luts;
     uint32_t lut_baked = 0;  // LUTs baked since the last atmosphere / size change
     bool luts_valid = false;
+    // Frame pipelining (ps_frame rebaking every LUT): frame k bakes a second
+    // LUT set on a side stream while frame k-1 still renders from the
+    // current one, then the sets swap. ev_render_issued marks the main
+    // stream just before the last pipelined render: everything that could
+    // read the set about to be overwritten was queued before it.
+    psi::LutSet luts_back;
+    hipStream_t lut_stream = nullptr;
+    hipEvent_t ev_luts_done = nullptr, ev_render_issued = nullptr;
+    bool render_issued_recorded = false;
 
     int noise_format = -1;
     void *base_store = nullptr, *detail_store = nullptr, *curl_store = nullptr;
