@@ -667,7 +667,10 @@ typedef struct ps_cloud_grads {
  * FP32 images (alpha ignored) of dL/d composite, dL/d cloud_luminance and
  * dL/d cloud_transmittance, each NULL for zero. The context's LUTs, inputs
  * and perturbation volume must be those of the training forward.
- * Perturbation-volume gradients need light_steps <= 32 (or the cone march). */
+ * Perturbation-volume gradients need light_steps <= 32 (or the cone march).
+ * Changing the extensions, the checkpoint interval, the LUT sizes, the noise
+ * format or the weather-map size discards the training forward (then
+ * PS_ERR_NOT_READY). */
 ps_status ps_backward(ps_ctx* ctx, const float* g_composite, const float* g_luminance,
                       const float* g_transmittance, const ps_cloud_grads* grads);
 

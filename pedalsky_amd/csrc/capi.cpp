@@ -562,6 +562,7 @@ ps_status ps_ctx_set_lut_dims(ps_ctx* c, const ps_lut_dims* d) {
     c->dims = *d;
     c->lut_baked = 0;
     c->luts_valid = false;
+    c->train_valid = false;  // its parameters point into the freed sets
     return ensure_luts(c);
 }
 
@@ -618,6 +619,7 @@ static ps_status alloc_noise(ps_ctx* c, int format) {
         PS_HIP(hipStreamSynchronize(c->stream));
         dev_free(c->base_store); dev_free(c->detail_store); dev_free(c->curl_store);
         c->noise_format = format;
+        c->train_valid = false;  // its parameters point into the freed stores
     }
     ps_status s;
     if ((s = dev_alloc(&c->base_store, noise_texel_bytes(format, 4) * 128 * 128 * 128, "base noise"))) return s;
@@ -714,7 +716,10 @@ ps_status ps_upload_weather(ps_ctx* c, const uint16_t* weather, int32_t w, int32
     if (w < 1 || h < 1 || w > 8192 || h > 8192 || channels < 1 || channels > 4)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_upload_weather: need 1 <= w, h <= 8192 and 1 <= channels <= 4");
     PS_HIP(hipStreamSynchronize(c->stream));
-    if (c->ww != w || c->wh != h) dev_free(c->weather);
+    if (c->ww != w || c->wh != h) {
+        dev_free(c->weather);
+        c->train_valid = false;  // its parameters point into the freed map
+    }
     ps_status s;
     if ((s = dev_alloc(&c->weather, sizeof(uint2) * w * h, "weather"))) return s;
     if ((s = dev_alloc(&c->type_grid, 128 * 128, "cloud type grid"))) return s;
@@ -749,7 +754,10 @@ ps_status ps_synthesise_weather(ps_ctx* c, const ps_weather_synth* ws, const uin
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_synthesise_weather: grid_extent_m must be positive");
     PS_HIP(hipStreamSynchronize(c->stream));
     const int n = 128 * 128;
-    if (c->ww != 128 || c->wh != 128) dev_free(c->weather);
+    if (c->ww != 128 || c->wh != 128) {
+        dev_free(c->weather);
+        c->train_valid = false;
+    }
     ps_status s;
     if ((s = dev_alloc(&c->weather, sizeof(uint2) * n, "weather"))) return s;
     if ((s = dev_alloc(&c->weather_synth, sizeof(uint16_t) * 4 * n, "synthesised weather"))) return s;
@@ -1270,7 +1278,8 @@ ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_lumina
     if (!c || !grads) return fail(PS_ERR_INVALID_ARGUMENT, "ps_backward: NULL argument");
     if (!c->train_valid)
         return fail(PS_ERR_NOT_READY, "ps_backward: no training forward to differentiate (call ps_render_train; "
-                                      "changing the extensions or the checkpoint interval invalidates it)");
+                                      "changing the extensions, the checkpoint interval, the LUT sizes, the noise "
+                                      "format or the weather-map size invalidates it)");
     if (!grads->layers && !grads->density_delta)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_backward: no gradient output requested");
     RenderParams P = c->train;
@@ -1299,6 +1308,12 @@ ps_status ps_backward(ps_ctx* c, const float* g_composite, const float* g_lumina
     RenderParams* dP = nullptr;
     if ((s = c->render_ring.upload(P, c->stream, &dP))) return s;
     PS_HIP(ps_launch_render_backward(dP, P.W, P.H, P.delta != nullptr, P.g_delta, c->g_delta64, c->stream));
+    if (c->lut_stream && P.T == c->luts_back.T) {
+        // the training forward's LUT set has since become the set the next
+        // pipelined frame bakes into: that bake must wait for this backward
+        PS_HIP(hipEventRecord(c->ev_render_issued, c->stream));
+        c->render_issued_recorded = true;
+    }
     return c->render_ring.mark_used(c->stream);
 }
 

@@ -556,3 +556,39 @@ def test_backward_error_paths(renderer):
     assert lib.ps_backward(renderer.ctx, None, None, None, C.byref(g)) == abi.PS_ERR_INVALID_ARGUMENT
     bad = (C.c_float * 1)(-1.0)
     assert lib.ps_set_cloud_extensions(renderer.ctx, bad, 1, None) == abi.PS_ERR_INVALID_ARGUMENT
+
+
+def test_reallocated_inputs_discard_the_training_forward(renderer):
+    """The training forward's parameters point into the context's LUT sets,
+    noise stores and weather map: a change that reallocates one of them (LUT
+    sizes, noise format, weather-map size) discards it, so ps_backward
+    answers NOT_READY instead of reading freed memory."""
+    import torch
+    lib = renderer.lib
+    w = scene.workload_small()
+    _log("case: backward after a LUT-size change, a noise-format change, a weather-map resize")
+    g_layers = torch.zeros(24, dtype=torch.float32, device="cuda:0")
+    grads = abi.CloudGrads(g_layers.data_ptr(), None)
+    W, H = w.width, w.height
+    g_comp = torch.ones(H * W * 4, dtype=torch.float32, device="cuda:0")
+
+    def backward():
+        st = lib.ps_backward(renderer.ctx, g_comp.data_ptr(), None, None, C.byref(grads))
+        renderer.synchronize()
+        return st
+
+    other_fmt = abi.PS_NOISE_UNORM8 if w.noise_format != abi.PS_NOISE_UNORM8 else abi.PS_NOISE_FP16
+    changes = [
+        ("LUT sizes", lambda: renderer.set_lut_dims(renderer.lut_dims())),
+        ("noise format", lambda: renderer.generate_noise(0, other_fmt)),
+        ("weather-map size", lambda: renderer.upload_weather(np.zeros((64, 48, 4), np.uint16))),
+    ]
+    for name, change in changes:
+        _load(renderer, w)
+        _render(renderer, train=True)
+        assert backward() == abi.PS_OK, name
+        change()
+        assert backward() == abi.PS_ERR_NOT_READY, name
+        _log(f"  {name}: training forward discarded")
+    _load(renderer, w)
+    assert torch.isfinite(g_layers).all()

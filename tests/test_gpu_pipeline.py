@@ -152,3 +152,52 @@ def test_pipelined_training_steps_match(rend):
         gd = float(np.abs(g1 - g0).max() / max(np.abs(g0).max(), 1e-30))
         _log(f"  step {k}: forward identical {same}, layer gradients max rel diff {gd:.2e}")
         assert same and gd <= 1e-6
+
+
+def test_backward_after_later_pipelined_frames(rend):
+    """A backward issued after the next pipelined frame reads the LUT set
+    its training forward used, and the frame after that (which rebakes that
+    set on the side stream) waits for it: the gradients equal those of a
+    backward issued straight after its training forward, and the later
+    frames equal bake + render."""
+    import ctypes as C
+    import torch
+    r, w = rend
+    frames = _frames(w, 3)
+    dev = "cuda:0"
+    _log("case: ps_frame_train, ps_frame (new sun), ps_backward of the first, ps_frame (new sun)")
+
+    def grads_buffers():
+        return torch.zeros(24, dtype=torch.float32, device=dev)
+
+    r.set_frame(frames[0])
+    b0 = r.alloc_outputs(w.width, w.height, abi.PS_PIXEL_F32, ("composite",))
+    r.bake_luts()
+    r.render_train(r.image_out(b0))
+    g = torch.ones_like(b0["composite"])
+    lg_ref = grads_buffers()
+    torch.cuda.synchronize()
+    r.backward(g_composite=g, layer_grads=lg_ref)
+    ref_later = _run(r, w, frames[1:], ["bake+render"] * 2)
+
+    r.set_frame(frames[0])
+    b0p = r.alloc_outputs(w.width, w.height, abi.PS_PIXEL_F32, ("composite",))
+    depth = C.c_void_p(r.depth_ptr) if r.depth_ptr else None
+    abi.check(r.lib.ps_frame_train(r.ctx, abi.PS_LUT_ALL, depth, C.byref(r.image_out(b0p))))
+    later = [r.alloc_outputs(w.width, w.height, abi.PS_PIXEL_F32, ("composite", "cloud_transmittance"))
+             for _ in range(2)]
+    r.set_frame(frames[1])
+    r.frame(r.image_out(later[0]))
+    lg = grads_buffers()
+    torch.cuda.synchronize()
+    r.backward(g_composite=g, layer_grads=lg)
+    r.set_frame(frames[2])
+    r.frame(r.image_out(later[1]))
+    r.synchronize()
+    torch.cuda.synchronize()
+    gd = float((lg - lg_ref).abs().max() / lg_ref.abs().max().clamp_min(1e-30))
+    _log(f"  layer gradients max rel diff {gd:.2e}")
+    assert np.array_equal(b0p["composite"].cpu().numpy(), b0["composite"].cpu().numpy())
+    assert gd <= 1e-6
+    for k, (e, bufs) in enumerate(zip(ref_later, later)):
+        assert all(np.array_equal(bufs[n].cpu().numpy(), e[n]) for n in e), k
