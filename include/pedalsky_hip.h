@@ -375,6 +375,12 @@ ps_status ps_read_weather(ps_ctx* ctx, uint16_t* rgba_f16, uint8_t* cloud_types_
  * encode; crates/ps-postprocess/src/tonemap.rs).                           */
 /* ------------------------------------------------------------------------ */
 enum { PS_TONEMAP_ACES = 0, PS_TONEMAP_CLAMP = 1 };
+/* The 8-bit output's target: the windowed app's *Srgb swapchain (the store
+ * encodes the tone map's linear output to sRGB, tonemap.wgsl:57), or the
+ * Rgba8Unorm target of the reference's headless harness, which stores the
+ * linear output as it is (crates/ps-app/src/test_harness.rs:56-58, :72) —
+ * the frames its golden images hold. */
+enum { PS_TARGET_SRGB8 = 0, PS_TARGET_UNORM8 = 1 };
 /* Ground shading: the caller's surface radiance (`ground_lit`, or none), or
  * the reference's ground pass evaluated on the depth buffer's surface. */
 enum { PS_GROUND_RADIANCE = 0, PS_GROUND_PBR = 1 };
@@ -419,6 +425,7 @@ typedef struct ps_present_params {
     float bloom_intensity;
     float bloom_knee_ev;
     ps_surface_params surface;       /* read with PS_GROUND_PBR (WeatherState.surface) */
+    int32_t target;                  /* PS_TARGET_SRGB8 (default) or PS_TARGET_UNORM8 */
 } ps_present_params;
 /* The reference's defaults: EV100 14, ACES, caller radiance, godrays on
  * (64 samples, decay 0.96, intensity 0.6, threshold 5000), bloom on
@@ -430,7 +437,8 @@ void ps_default_present_params(ps_present_params* out);
  * composite is presented as it is; ignored with PS_GROUND_PBR, which shades
  * the surface itself (flat ground, normal +y, at the depth buffer's
  * positions). Outputs (device, either may be NULL but not both): W*H RGBA8
- * sRGB (alpha 255) and W*H RGBA32F HDR (the tone map's input). Godrays and
+ * (alpha 255) encoded for `target`, and W*H RGBA32F HDR (the tone map's
+ * input). Godrays and
  * bloom use scratch images the context keeps at the frame's size. */
 ps_status ps_present(ps_ctx* ctx, const float* depth, const float* ground_lit, const ps_image_out* in,
                      const ps_present_params* params, uint8_t* out_srgb8, float* out_hdr);
@@ -736,6 +744,13 @@ void ps_layer_wind(float surface_speed_mps, float wind_dir_deg, float base_m,
 /* Uniform top-down overcast mask value, density_mask.rs:134-165 with the
  * CPU NDF of ps-synthesis/src/ndf.rs:20-58. */
 uint8_t ps_overcast_mask_value(const ps_cloud_layer* layers, int32_t n);
+/* TopDownMask::synthesise (crates/ps-synthesis/src/density_mask.rs:47-83):
+ * the 128x128 overcast mask over 32 km for a scene with a coverage grid
+ * (grid_h rows of grid_w host floats over grid_extent_m, the scene's
+ * [clouds.coverage_grid]); grid NULL gives the uniform mask of
+ * ps_overcast_mask_value. */
+void ps_overcast_mask(const ps_cloud_layer* layers, int32_t n, const float* grid, int32_t grid_w, int32_t grid_h,
+                      float grid_extent_m, uint8_t* out_128x128);
 /* generate_blue_noise(64), crates/ps-clouds/src/noise.rs:646-704. */
 void ps_generate_blue_noise(uint8_t* out_64x64);
 

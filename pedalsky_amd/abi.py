@@ -27,6 +27,8 @@ PS_PIXEL_F32 = 0
 PS_PIXEL_F16 = 1
 PS_TONEMAP_ACES = 0
 PS_TONEMAP_CLAMP = 1
+PS_TARGET_SRGB8 = 0
+PS_TARGET_UNORM8 = 1
 PS_GROUND_RADIANCE = 0
 PS_GROUND_PBR = 1
 PS_MATERIAL_GRASS, PS_MATERIAL_BARE_SOIL, PS_MATERIAL_TARMAC, PS_MATERIAL_SAND, PS_MATERIAL_WATER_EDGE = range(5)
@@ -95,7 +97,7 @@ class PresentParams(C.Structure):
                 ("godrays", C.c_int32), ("godrays_samples", C.c_uint32), ("godrays_decay", C.c_float),
                 ("godrays_intensity", C.c_float), ("godrays_bright_threshold", C.c_float),
                 ("bloom", C.c_int32), ("bloom_threshold_ev100", C.c_float), ("bloom_intensity", C.c_float),
-                ("bloom_knee_ev", C.c_float), ("surface", SurfaceParams)]
+                ("bloom_knee_ev", C.c_float), ("surface", SurfaceParams), ("target", C.c_int32)]
 
 
 def default_present_params() -> PresentParams:
@@ -367,6 +369,7 @@ _SIGNATURES = [
     ("ps_world_sun", C.c_int32, [C.POINTER(Utc), C.c_double, C.c_double, C.c_double, C.POINTER(WorldSun)]),
     ("ps_frame_set_sun", C.c_int32, [C.POINTER(FrameUniforms), C.POINTER(WorldSun), C.c_float]),
     ("ps_overcast_mask_value", C.c_uint8, [C.POINTER(CloudLayer), C.c_int32]),
+    ("ps_overcast_mask", None, [C.POINTER(CloudLayer), C.c_int32, _P, C.c_int32, C.c_int32, C.c_float, _P]),
     ("ps_generate_blue_noise", None, [_P]),
 ]
 

@@ -845,6 +845,8 @@ ps_status ps_present(ps_ctx* c, const float* depth, const float* ground_lit, con
                     "ps_present: a shaded surface needs the depth buffer, cloud_luminance and cloud_transmittance");
     if (pp->tonemap != PS_TONEMAP_ACES && pp->tonemap != PS_TONEMAP_CLAMP)
         return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: tonemap must be PS_TONEMAP_ACES or PS_TONEMAP_CLAMP");
+    if (pp->target != PS_TARGET_SRGB8 && pp->target != PS_TARGET_UNORM8)
+        return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: target must be PS_TARGET_SRGB8 or PS_TARGET_UNORM8");
     if (!out_srgb8 && !out_hdr) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: no output");
     if (!std::isfinite(pp->ev100)) return fail(PS_ERR_INVALID_ARGUMENT, "ps_present: ev100 must be finite");
     if (pp->godrays && (pp->godrays_samples < 1 || pp->godrays_samples > 1024 || !std::isfinite(pp->godrays_decay) ||
@@ -890,7 +892,7 @@ ps_status ps_present(ps_ctx* c, const float* depth, const float* ground_lit, con
     a.ground = pbr ? nullptr : (const float4*)ground_lit;
     // tonemap.wgsl:50: exposure = 1 / (1.2 * 2^EV100)
     a.exposure = 1.f / (1.2f * std::pow(2.f, pp->ev100));
-    a.mode = pp->tonemap;
+    a.mode = pp->tonemap | (pp->target == PS_TARGET_UNORM8 ? 2 : 0);
     a.pbr = pbr ? 1 : 0;
     if (pbr) {
         const ps_surface_params& sp = pp->surface;

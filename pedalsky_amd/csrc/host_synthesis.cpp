@@ -4,6 +4,7 @@
 // blue-noise rank map. All arithmetic is f32 like the Rust originals.
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #include <limits>
@@ -248,22 +249,62 @@ void ps_layer_wind(float surface_speed_mps, float wind_dir_deg, float base_m, fl
     out_xz[1] = lv[y0] + (lv[y1] - lv[y0]) * fy;
 }
 
-uint8_t ps_overcast_mask_value(const ps_cloud_layer* layers, int32_t n) {
-    // density_mask.rs:134-165 (uniform case, each layer's own coverage)
-    float total = 0.f;
-    for (int i = 0; i < n; ++i) {
-        const ps_cloud_layer& L = layers[i];
-        float thickness = std::fmax(L.top_m - L.base_m, 1.f);
-        uint32_t n_steps = (uint32_t)std::fmax(std::ceil(thickness / 100.f), 1.f);
-        float dh = 1.f / (float)n_steps;
-        float sum = 0.f;
-        for (uint32_t k = 0; k < n_steps; ++k) sum += cpu_ndf(((float)k + 0.5f) * dh, L.cloud_type);
-        float mean = sum / (float)n_steps;
-        total += mean * (thickness / 1000.f) * L.coverage * L.density_scale;
-    }
+// One layer's column density at a coverage (density_mask.rs:155-166): the
+// mean NDF over 100 m height steps x thickness in km x coverage x density.
+static float layer_column(const ps_cloud_layer& L, float coverage) {
+    float thickness = std::fmax(L.top_m - L.base_m, 1.f);
+    uint32_t n_steps = (uint32_t)std::fmax(std::ceil(thickness / 100.f), 1.f);
+    float dh = 1.f / (float)n_steps;
+    float sum = 0.f;
+    for (uint32_t k = 0; k < n_steps; ++k) sum += cpu_ndf(((float)k + 0.5f) * dh, L.cloud_type);
+    float mean = sum / (float)n_steps;
+    return mean * (thickness / 1000.f) * coverage * L.density_scale;
+}
+
+// Column total / 10 in [0, 1] as the mask's u8 (density_mask.rs:64, 134-151).
+static uint8_t mask_u8(float total) {
     float col = std::fmin(std::fmax(total / 10.f, 0.f), 1.f);
     float v = std::fmin(std::fmax(col * 255.f, 0.f), 255.f);
     return (uint8_t)v;
+}
+
+uint8_t ps_overcast_mask_value(const ps_cloud_layer* layers, int32_t n) {
+    // density_mask.rs:134-140 (uniform case, each layer's own coverage)
+    float total = 0.f;
+    for (int i = 0; i < n; ++i) total += layer_column(layers[i], layers[i].coverage);
+    return mask_u8(total);
+}
+
+void ps_overcast_mask(const ps_cloud_layer* layers, int32_t n, const float* grid, int32_t grid_w, int32_t grid_h,
+                      float grid_extent_m, uint8_t* out) {
+    // TopDownMask::synthesise (density_mask.rs:47-83): with a coverage grid,
+    // texel (x, y) of the 128^2 mask over 32 km takes the grid's bilinear,
+    // edge-clamped coverage at its centre (LoadedCoverageGrid::sample_world,
+    // coverage_grid.rs:31-61) for every layer; without one, the uniform value.
+    const int size = 128;
+    if (!grid || grid_w < 1 || grid_h < 1) {
+        std::memset(out, ps_overcast_mask_value(layers, n), size * size);
+        return;
+    }
+    const float extent = 32000.f, half = extent * 0.5f, ghalf = grid_extent_m * 0.5f;
+    for (int y = 0; y < size; ++y)
+        for (int x = 0; x < size; ++x) {
+            const float gx = ((float)x + 0.5f) / (float)size * extent - half;
+            const float gy = ((float)y + 0.5f) / (float)size * extent - half;
+            const float u = (gx + ghalf) / grid_extent_m, v = (gy + ghalf) / grid_extent_m;
+            const float fx = std::fmin(std::fmax(u, 0.f), 1.f) * ((float)grid_w - 1.f);
+            const float fy = std::fmin(std::fmax(v, 0.f), 1.f) * ((float)grid_h - 1.f);
+            const uint32_t x0 = (uint32_t)std::floor(fx), y0 = (uint32_t)std::floor(fy);
+            const uint32_t x1 = std::min(x0 + 1u, (uint32_t)grid_w - 1u), y1 = std::min(y0 + 1u, (uint32_t)grid_h - 1u);
+            const float tx = fx - (float)x0, ty = fy - (float)y0;
+            const float v00 = grid[y0 * grid_w + x0], v10 = grid[y0 * grid_w + x1];
+            const float v01 = grid[y1 * grid_w + x0], v11 = grid[y1 * grid_w + x1];
+            const float bottom = v00 + (v10 - v00) * tx, top = v01 + (v11 - v01) * tx;
+            const float cov = std::fmin(std::fmax(bottom + (top - bottom) * ty, 0.f), 1.f);
+            float total = 0.f;
+            for (int i = 0; i < n; ++i) total += layer_column(layers[i], cov);
+            out[y * size + x] = mask_u8(total);
+        }
 }
 
 void ps_generate_blue_noise(uint8_t* out) {

@@ -73,11 +73,20 @@ __device__ __forceinline__ uint32_t srgb8(float c) {
     return (uint32_t)fminf(floorf(e * 255.f + 0.5f), 255.f);
 }
 
+// Linear -> 8-bit UNORM as an Rgba8Unorm target stores it, rounded to
+// nearest.
+__device__ __forceinline__ uint32_t unorm8(float c) {
+    return (uint32_t)floorf(fminf(fmaxf(c, 0.f), 1.f) * 255.f + 0.5f);
+}
+
+// mode bit 0: clamp instead of ACES; bit 1: Rgba8Unorm target instead of
+// the sRGB encode.
 __device__ __forceinline__ uint32_t tonemap_rgba8(float4 hdr, float exposure, int mode) {
     float r = hdr.x * exposure, g = hdr.y * exposure, b = hdr.z * exposure;
-    if (mode == 0) {
+    if ((mode & 1) == 0) {
         r = aces(r); g = aces(g); b = aces(b);
     }
+    if (mode & 2) return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (255u << 24);
     return srgb8(r) | (srgb8(g) << 8) | (srgb8(b) << 16) | (255u << 24);
 }
 

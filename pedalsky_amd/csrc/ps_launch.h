@@ -74,7 +74,7 @@ struct PresentArgs {
     float4* hdr;
     uint32_t* ldr;                    // RGBA8 sRGB, R in the low byte
     float exposure;
-    int mode;                         // 0 ACES filmic, 1 clamp
+    int mode;                         // bit 0: clamp (else ACES filmic); bit 1: Rgba8Unorm target (else sRGB)
     // ground pass (pbr.wgsl fs_main) inputs
     int pbr;
     uint32_t material;

@@ -248,10 +248,19 @@ def blue_noise() -> np.ndarray:
     return out
 
 
-def overcast_mask(layers: Sequence[abi.CloudLayer]) -> np.ndarray:
+def overcast_mask(layers: Sequence[abi.CloudLayer], grid: Optional[np.ndarray] = None,
+                  grid_extent_m: float = 32000.0) -> np.ndarray:
+    """The 128x128 top-down overcast mask: uniform, or over a (h, w)
+    coverage grid (ps_overcast_mask)."""
     arr = (abi.CloudLayer * max(len(layers), 1))(*layers)
-    v = abi.load().ps_overcast_mask_value(arr, len(layers))
-    return np.full(128 * 128, v, dtype=np.uint8)
+    if grid is None:
+        v = abi.load().ps_overcast_mask_value(arr, len(layers))
+        return np.full(128 * 128, v, dtype=np.uint8)
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    out = np.zeros(128 * 128, dtype=np.uint8)
+    abi.load().ps_overcast_mask(arr, len(layers), g.ctypes.data, g.shape[1], g.shape[0], float(grid_extent_m),
+                                out.ctypes.data)
+    return out
 
 
 def ground_distance(frame: abi.FrameUniforms, planet_radius: float = 6_360_000.0) -> np.ndarray:

@@ -94,3 +94,45 @@ def test_entry_points_exported():
     lib = abi.load()
     for name in ("ps_synthesise_weather", "ps_read_weather"):
         assert hasattr(lib, name)
+
+
+def _cumulus(cov):
+    from pedalsky_amd import scene
+    return scene.synth_layer(0, 1500.0, 2300.0, cov, density_scale=1.0)
+
+
+def test_overcast_mask_reference_cases():
+    """TopDownMask::synthesise, the reference's tests (density_mask.rs:
+    187-246): no layers give zero, cumulus gives a non-zero mask growing with
+    coverage, and a 4x4 grid with full coverage on its left half and none on
+    the right gives a mask that is higher on the left and near zero on the
+    right."""
+    from pedalsky_amd import scene
+    assert not scene.overcast_mask([]).any()
+    assert scene.overcast_mask([_cumulus(0.5)])[0] > 0
+    assert scene.overcast_mask([_cumulus(0.9)])[0] > scene.overcast_mask([_cumulus(0.2)])[0]
+    grid = np.array([[1.0 if (i % 4) < 2 else 0.0 for i in range(4 * r, 4 * r + 4)] for r in range(4)], np.float32)
+    m = scene.overcast_mask([_cumulus(0.5)], grid, 32000.0).reshape(128, 128)
+    left, right = int(m[64, 32]), int(m[64, 96])
+    assert left > right and right < 8
+
+
+def test_overcast_mask_grid_is_the_uniform_mask_at_constant_coverage():
+    """The grid's coverage replaces each layer's own (synthesis-remapped)
+    coverage: a constant grid holding a layer's coverage gives, at every
+    texel, that layer's uniform mask; outside the grid's extent the edge
+    values hold (ClampToEdge)."""
+    from pedalsky_amd import scene
+    for c in (0.0, 0.35, 1.0):
+        for layer in (_cumulus(c), scene.synth_layer(5, 8000.0, 9000.0, c, density_scale=0.3)):
+            want = scene.overcast_mask([layer])[0]
+            got = scene.overcast_mask([layer], np.full((7, 5), layer.coverage, np.float32), 20000.0)
+            assert (got == want).all(), (c, want, np.unique(got))
+    # a grid over 8 km whose columns step from 0 to 1: beyond its extent the
+    # mask takes the edge columns' values
+    layers = [_cumulus(1.0)]
+    g = np.tile(np.linspace(0.0, 1.0, 8, dtype=np.float32), (8, 1))
+    m = scene.overcast_mask(layers, g, 8000.0).reshape(128, 128)
+    full = scene.overcast_mask(layers, np.ones((2, 2), np.float32), 8000.0)[0]
+    assert (m[:, :40] == 0).all() and (m[:, 88:] == full).all()
+    assert (np.diff(m[64].astype(int)) >= 0).all()
