@@ -175,6 +175,8 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
     lg = torch.zeros(24, dtype=torch.float32, device=dev)
     dg = torch.zeros(128 ** 3, dtype=torch.float32, device=dev)
     g = torch.zeros_like(target)
+    # dL/d composite of sum((composite - target)^2) = 2 composite - 2 target, one kernel per step
+    neg2_target = target * -2.0
     lib = r.lib
     grads = abi.CloudGrads(lg.data_ptr(), dg.data_ptr())
     depth = C.c_void_p(r.depth_ptr)
@@ -187,14 +189,14 @@ def bench_adjoint(r, W: int, H: int, steps: int, warmup: int) -> dict:
 
     def fwd_adj():
         abi.check(lib.ps_frame_train(r.ctx, abi.PS_LUT_ALL, depth, C.byref(out)))
-        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)  # the backward reads RGB only
+        torch.add(neg2_target, bufs["composite"], alpha=2.0, out=g)  # the backward reads RGB only
         abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads)))
 
     grads_layers = abi.CloudGrads(lg.data_ptr(), None)
 
     def fwd_adj_layers():
         abi.check(lib.ps_frame_train(r.ctx, abi.PS_LUT_ALL, depth, C.byref(out)))
-        torch.mul(torch.sub(bufs["composite"], target), 2.0, out=g)  # the backward reads RGB only
+        torch.add(neg2_target, bufs["composite"], alpha=2.0, out=g)  # the backward reads RGB only
         abi.check(lib.ps_backward(r.ctx, C.c_void_p(g.data_ptr()), None, None, C.byref(grads_layers)))
 
     def fwd():
