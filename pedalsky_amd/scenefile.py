@@ -88,8 +88,11 @@ def atmosphere(sf: SceneFile) -> abi.AtmosphereParams:
     return a
 
 
-def surface_params(sf: SceneFile) -> abi.SurfaceParams:
-    """state.rs:80-104."""
+def surface_params(sf: SceneFile, wet_surface: bool = False) -> abi.SurfaceParams:
+    """state.rs:80-104, as the ground subsystem uploads them: with the
+    wet_surface switch off (pedalsky.toml:47, the golden harness's config)
+    it zeroes the wetness and the puddle coverage (ps-ground/src/lib.rs:
+    742-749); snow stays."""
     s = sf.surface
     wet = s.get("wetness", {})
     pr = sf.data.get("precipitation", {})
@@ -107,6 +110,9 @@ def surface_params(sf: SceneFile) -> abi.SurfaceParams:
     out.precip_intensity_mm_per_h = pr.get("intensity_mm_per_h", 0.0)
     out.precip_kind = float(PRECIP_KINDS[pr.get("type", "None")])
     out.material = float(MATERIALS[s.get("material", "Grass")])
+    if not wet_surface:
+        out.ground_wetness = 0.0
+        out.puddle_coverage = 0.0
     return out
 
 

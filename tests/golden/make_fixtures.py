@@ -5,8 +5,10 @@ The reference's golden-image regression (crates/ps-app/tests/golden.rs:
 and compares each frame with a PNG held in tests/golden/. This script copies
 those inputs and expected outputs as data, for the scenes whose frames are
 made of the ground, atmosphere and cloud passes and the post-process chain
-only: the thunderstorm (lightning) and hail (precipitation particles) frames
-are left out. Run it in a tree where the reference is readable:
+only: the hail frame (the precipitation subsystem's particles, enabled for
+that scene alone, golden.rs:96-102) is left out. The thunderstorm frame is
+kept: besides this path it holds one thin lightning bolt drawn by the
+lightning subsystem, a few hundred pixels. Run it in a tree where the reference is readable:
 
     python tests/golden/make_fixtures.py /root/reference
 """
@@ -23,6 +25,7 @@ SCENES = [
     ("winter_overcast_snow", "2026-01-08T12:00:00Z", 14.0, 60.0),
     ("twilight_civil", "2026-12-21T08:05:00Z", 8.0, 5.0),
     ("mountain_wave_clouds", "2026-03-15T13:00:00Z", 14.0, 5.0),
+    ("thunderstorm", "2026-08-16T16:00:00Z", 14.0, 5.0),
 ]
 COVERAGE_GRIDS = ["mountain_wave_lozenges_128x128.bin"]
 
