@@ -50,13 +50,14 @@ def setup():
     r.close()
 
 
-def _present(r, w, bufs, lit, ev100, mode, ground=True):
+def _present(r, w, bufs, lit, ev100, mode, ground=True, target=abi.PS_TARGET_SRGB8):
     import torch
     dev_lit = torch.from_numpy(lit).to("cuda:0") if ground else None
     ldr = torch.zeros((w.height, w.width, 4), dtype=torch.uint8, device="cuda:0")
     hdr = torch.zeros((w.height, w.width, 4), dtype=torch.float32, device="cuda:0")
     torch.cuda.synchronize()
     pp = abi.PresentParams(ev100, mode)
+    pp.target = target
     abi.check(r.lib.ps_present(r.ctx, r.depth_ptr, None if dev_lit is None else dev_lit.data_ptr(),
                                C.byref(r.image_out(bufs)), C.byref(pp), ldr.data_ptr(), hdr.data_ptr()))
     r.synchronize()
@@ -91,6 +92,19 @@ def test_present_without_surface_is_the_composite(setup):
     assert np.abs(ldr[..., :3].astype(int) - want.astype(int)).max() <= 1
 
 
+def test_present_unorm8_target(setup):
+    """PS_TARGET_UNORM8, the headless harness's Rgba8Unorm output
+    (test_harness.rs:56-58): the tone map's linear values rounded to 8 bits,
+    no sRGB encode; the HDR output is unchanged."""
+    r, w, bufs, host, lit, _, _ = setup
+    hdr, ldr = _present(r, w, bufs, lit, 14.0, abi.PS_TONEMAP_ACES, ground=False, target=abi.PS_TARGET_UNORM8)
+    assert np.array_equal(hdr[..., :3], host["composite"][..., :3].astype(np.float32))
+    lin = np.clip(PR.aces(host["composite"][..., :3] * PR.exposure(14.0)), 0.0, 1.0)
+    want = np.floor(lin * 255.0 + 0.5)
+    assert np.abs(ldr[..., :3].astype(int) - want.astype(int)).max() <= 1
+    assert (ldr[..., 3] == 255).all()
+
+
 def test_present_errors(setup):
     r, w, bufs, host, lit, _, _ = setup
     lib = r.lib
@@ -99,6 +113,10 @@ def test_present_errors(setup):
     assert lib.ps_present(r.ctx, r.depth_ptr, None, C.byref(out), C.byref(pp), None, None) == \
         abi.PS_ERR_INVALID_ARGUMENT
     pp = abi.PresentParams(14.0, 0)
+    pp.target = 2
+    assert lib.ps_present(r.ctx, r.depth_ptr, None, C.byref(out), C.byref(pp), bufs["composite"].data_ptr(),
+                          None) == abi.PS_ERR_INVALID_ARGUMENT
+    pp.target = abi.PS_TARGET_SRGB8
     assert lib.ps_present(r.ctx, None, bufs["composite"].data_ptr(), C.byref(out), C.byref(pp),
                           bufs["composite"].data_ptr(), None) == abi.PS_ERR_INVALID_ARGUMENT
     out16 = r.image_out(bufs)
